@@ -1,0 +1,128 @@
+"""Host-side description of one CAVIAR single-domain system.
+
+These dataclasses mirror the settings the reference's CASL objects hold for
+the short-range + Ewald path, with the same names and defaults:
+
+* ``domain box`` (domain.cpp:206-266): lower/upper bounds, boundary_condition;
+* ``atom_data basic`` (atom_data.cpp:78-306): ghost_cutoff, per-type mass and
+  charge, owned atoms (id, type, molecule index, position, velocity);
+* ``neighborlist basic`` (neighborlist.cpp:193-251, neighborlist.h:134);
+* ``force_field lj_cell_list`` (lj_cell_list.cpp:33-76);
+* ``force_field electrostatic_ewald_r`` (electrostatic_ewald_r.cpp:31-84);
+* ``force_field electrostatic_ewald_k`` (electrostatic_ewald_k.cpp:31-139).
+
+A :class:`System` is consumed by :class:`caviar_amd.Engine` (GPU path).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+
+@dataclass
+class NeighborSettings:
+    """``neighborlist basic`` (neighborlist.cpp:193-251)."""
+    cutoff: float
+    cutoff_extra_coef: float = 1.12347          # neighborlist.h:134
+    from_cells: bool = True                      # make_verlet_list_from_cell_list
+    build_cells: bool = True                     # build_cell_list
+    all_atom_test: bool = False
+    rebuild_test: bool = False
+
+    @property
+    def cutoff_extra(self) -> float:
+        return self.cutoff * self.cutoff_extra_coef
+
+
+@dataclass
+class LJSettings:
+    """``force_field lj_cell_list`` (lj_cell_list.cpp:33-76).
+
+    ``epsilon[i][j]`` / ``sigma[i][j]`` are the full type matrices (the
+    reference does not symmetrise them, preprocessors.h:568-578)."""
+    cutoff: float
+    epsilon: np.ndarray
+    sigma: np.ndarray
+
+
+@dataclass
+class EwaldRSettings:
+    """``force_field electrostatic_ewald_r`` (electrostatic_ewald_r.cpp:31-84)."""
+    alpha: float = 1.0
+    k_electrostatic: float = 1.0
+    lambda_: Optional[np.ndarray] = None         # per type pair, same-molecule pairs only
+
+
+@dataclass
+class EwaldKSettings:
+    """``force_field electrostatic_ewald_k`` (electrostatic_ewald_k.cpp:31-139)."""
+    alpha: float
+    k_electrostatic: float = 1.0
+    kx_max: int = 1
+    ky_max: int = 1
+    kz_max: int = 1
+    epsilon_dipole: Optional[float] = None       # None: no dipole term
+    slab_normal_axis: Optional[int] = None       # None: no slab geometry
+    slab_lz: float = 1.0
+
+    @staticmethod
+    def cube(alpha: float, k_max: int, k_electrostatic: float = 1.0) -> "EwaldKSettings":
+        return EwaldKSettings(alpha=alpha, k_electrostatic=k_electrostatic,
+                              kx_max=k_max, ky_max=k_max, kz_max=k_max)
+
+
+@dataclass
+class System:
+    lower: np.ndarray
+    upper: np.ndarray
+    bc: np.ndarray
+    ghost_cutoff: float
+    mass: np.ndarray
+    charge: np.ndarray
+    pos: np.ndarray
+    vel: np.ndarray
+    types: np.ndarray
+    ids: Optional[np.ndarray] = None
+    mol: Optional[np.ndarray] = None
+    neighbor: Optional[NeighborSettings] = None
+    lj: Optional[LJSettings] = None
+    ewald_r: Optional[EwaldRSettings] = None
+    ewald_k: Optional[EwaldKSettings] = None
+    meta: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        self.lower = np.asarray(self.lower, dtype=np.float64).reshape(3)
+        self.upper = np.asarray(self.upper, dtype=np.float64).reshape(3)
+        self.bc = np.asarray(self.bc, dtype=np.int32).reshape(3)
+        self.mass = np.asarray(self.mass, dtype=np.float64).reshape(-1)
+        self.charge = np.asarray(self.charge, dtype=np.float64).reshape(-1)
+        self.pos = np.ascontiguousarray(self.pos, dtype=np.float64).reshape(-1, 3)
+        self.vel = np.ascontiguousarray(self.vel, dtype=np.float64).reshape(-1, 3)
+        self.types = np.ascontiguousarray(self.types, dtype=np.int32).reshape(-1)
+        if self.ids is None:
+            self.ids = np.arange(len(self.pos), dtype=np.int32)
+        self.ids = np.ascontiguousarray(self.ids, dtype=np.int32)
+        if self.mol is not None:
+            self.mol = np.ascontiguousarray(self.mol, dtype=np.int32)
+        if len(self.charge) != len(self.mass):
+            raise ValueError("mass and charge must cover the same types")
+        if self.types.size and (self.types.min() < 0 or self.types.max() >= len(self.mass)):
+            raise ValueError("atom type outside the declared types")
+
+    @property
+    def n(self) -> int:
+        return len(self.pos)
+
+    @property
+    def ntypes(self) -> int:
+        return len(self.mass)
+
+    @property
+    def size(self) -> np.ndarray:
+        return self.upper - self.lower
+
+    def copy(self) -> "System":
+        import copy
+        return copy.deepcopy(self)

@@ -1,0 +1,1043 @@
+/* CPU oracle for the CAVIAR short-range + Ewald force path (TEST INFRASTRUCTURE).
+ *
+ * Plain C99, FP64, compiled with -ffp-contract=off so that every expression is
+ * rounded in the order the reference's C++ expressions are (x86-64 g++ -O3
+ * without -march emits no FMA).  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load this library, and only as the checker.
+ *
+ * Citations are reference file:line (paths below src/caviar/objects/ unless
+ * given in full).
+ */
+#include "caviar_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include <time.h>
+
+#define ORC_4PI 12.56637061435917295384    /* macro_constants.h:24 */
+#define ORC_2PI 6.28318530717958647692     /* macro_constants.h:22 */
+#define ORC_PIS_INV 0.56418958354775628695 /* macro_constants.h:28 */
+
+struct orc_sys {
+  double lo[3], hi[3], L[3];
+  int bc[3];
+  double ghost_cutoff;
+  int nt;
+  double *mass, *minv, *charge;
+  int n;
+  int *id, *type, *mol;
+  double *pos, *vel, *acc, *acc_old;
+  int g, gcap;
+  double *gpos;
+  int *gsrc;
+  /* neighbour list */
+  double nl_cutoff, nl_coef, rce, thr_sq, cinv;
+  int from_cells, build_cells, all_atom_test, rebuild_test, nl_on;
+  int nb[3], nbins;
+  int *st_off, *st;            /* de-duplicated 27-stencil per bin, linear indices */
+  int *bin_off, *bin_items;    /* cell list CSR, insertion order */
+  long *nl_off;
+  int *nl_items;
+  long nl_cap;
+  double *pos_old;
+  int n_old;
+  double *gpos_old;
+  int g_old;
+  /* Lj_cell_list */
+  int lj_on;
+  double lj_rc;
+  double *eps, *sig;
+  /* Electrostatic_ewald_r */
+  int er_on, lambda_set;
+  double er_alpha, er_ke;
+  double *lambda;
+  /* Electrostatic_ewald_k */
+  int ek_on, ek_init, kx, ky, kz, dipole, slab_axis, nk;
+  double ek_alpha, ek_ke, eps_dipole, slab_lz, lxyz_inv, dip_coef;
+  double dip_sum[3], dip_field[3];
+  double *kvec, *fk, *sk;
+  double virial;
+};
+
+static int g_threads = 1;
+void orc_set_threads(int nthreads) { g_threads = nthreads > 0 ? nthreads : 1; }
+
+orc_sys *orc_create(void)
+{
+  orc_sys *s = (orc_sys *)calloc(1, sizeof(orc_sys));
+  s->nl_coef = 1.12347; /* neighborlist.h:134 */
+  s->slab_axis = -1;
+  return s;
+}
+
+void orc_destroy(orc_sys *s)
+{
+  if (!s)
+    return;
+  void *ptrs[] = {s->mass, s->minv, s->charge, s->id, s->type, s->mol, s->pos, s->vel,
+                  s->acc, s->acc_old, s->gpos, s->gsrc, s->st_off, s->st, s->bin_off,
+                  s->bin_items, s->nl_off, s->nl_items, s->pos_old, s->gpos_old, s->eps,
+                  s->sig, s->lambda, s->kvec, s->fk, s->sk};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
+    free(ptrs[i]);
+  free(s);
+}
+
+static void *dup_arr(const void *src, size_t bytes)
+{
+  void *p = malloc(bytes ? bytes : 1);
+  if (src && bytes)
+    memcpy(p, src, bytes);
+  else
+    memset(p, 0, bytes ? bytes : 1);
+  return p;
+}
+
+void orc_set_box(orc_sys *s, const double lo[3], const double hi[3], const int bc[3])
+{
+  for (int d = 0; d < 3; ++d) {
+    s->lo[d] = lo[d];
+    s->hi[d] = hi[d];
+    s->L[d] = hi[d] - lo[d]; /* domain.cpp:470-477 size_global = upper - lower */
+    s->bc[d] = bc[d];
+  }
+}
+
+void orc_set_ghost_cutoff(orc_sys *s, double gc) { s->ghost_cutoff = gc; }
+
+void orc_set_types(orc_sys *s, int nt, const double *mass, const double *charge)
+{
+  free(s->mass);
+  free(s->minv);
+  free(s->charge);
+  s->nt = nt;
+  s->mass = (double *)dup_arr(mass, sizeof(double) * nt);
+  s->charge = (double *)dup_arr(charge, sizeof(double) * nt);
+  s->minv = (double *)malloc(sizeof(double) * (nt ? nt : 1));
+  for (int t = 0; t < nt; ++t) /* atom_data.cpp:236-239 */
+    s->minv[t] = (mass[t] == 0.0) ? 0.0 : 1.0 / mass[t];
+}
+
+void orc_set_atoms(orc_sys *s, int n, const int *id, const int *type, const int *mol,
+                   const double *pos, const double *vel)
+{
+  free(s->id);
+  free(s->type);
+  free(s->mol);
+  free(s->pos);
+  free(s->vel);
+  free(s->acc);
+  free(s->acc_old);
+  s->n = n;
+  s->id = (int *)dup_arr(id, sizeof(int) * n);
+  s->type = (int *)dup_arr(type, sizeof(int) * n);
+  s->mol = (int *)malloc(sizeof(int) * (n ? n : 1));
+  for (int i = 0; i < n; ++i)
+    s->mol[i] = mol ? mol[i] : -1; /* atom_data.cpp:509 */
+  s->pos = (double *)dup_arr(pos, sizeof(double) * 3 * n);
+  s->vel = (double *)dup_arr(vel, sizeof(double) * 3 * n);
+  s->acc = (double *)dup_arr(NULL, sizeof(double) * 3 * n);
+  s->acc_old = (double *)dup_arr(NULL, sizeof(double) * 3 * n);
+}
+
+void orc_set_neighbor(orc_sys *s, double cutoff, double coef, int from_cells, int build_cells,
+                      int all_atom_test, int rebuild_test)
+{
+  s->nl_on = 1;
+  s->nl_cutoff = cutoff;
+  s->nl_coef = coef;
+  s->from_cells = from_cells;
+  /* make_verlet_list_from_cell_list implies build_cell_list (neighborlist.cpp:216-220) */
+  s->build_cells = build_cells || from_cells;
+  s->all_atom_test = all_atom_test;
+  s->rebuild_test = rebuild_test;
+}
+
+void orc_set_lj(orc_sys *s, double rc, const double *eps, const double *sigma)
+{
+  s->lj_on = 1;
+  s->lj_rc = rc;
+  free(s->eps);
+  free(s->sig);
+  s->eps = (double *)dup_arr(eps, sizeof(double) * s->nt * s->nt);
+  s->sig = (double *)dup_arr(sigma, sizeof(double) * s->nt * s->nt);
+}
+
+void orc_set_ewald_r(orc_sys *s, double alpha, double ke, const double *lambda)
+{
+  s->er_on = 1;
+  s->er_alpha = alpha;
+  s->er_ke = ke;
+  free(s->lambda);
+  s->lambda = NULL;
+  s->lambda_set = lambda != NULL;
+  if (lambda)
+    s->lambda = (double *)dup_arr(lambda, sizeof(double) * s->nt * s->nt);
+}
+
+void orc_set_ewald_k(orc_sys *s, double alpha, double ke, int kx, int ky, int kz, int dipole,
+                     double eps_dipole, int slab_axis, double slab_lz)
+{
+  s->ek_on = 1;
+  s->ek_init = 0;
+  s->ek_alpha = alpha;
+  s->ek_ke = ke;
+  s->kx = kx;
+  s->ky = ky;
+  s->kz = kz;
+  s->dipole = dipole;
+  s->eps_dipole = eps_dipole;
+  s->slab_axis = slab_axis;
+  s->slab_lz = slab_lz;
+}
+
+/* ---------------------------------------------------------------- domain -- */
+
+/* exchange_owned_single_md_domain.cpp:31-59 with Domain::fix_position
+ * (domain.cpp:418-468): one +-L shift per periodic axis, strict comparisons;
+ * returns whether any atom was shifted. */
+int orc_wrap(orc_sys *s)
+{
+  int wrapped = 0;
+  for (int i = 0; i < s->n; ++i) {
+    double *p = s->pos + 3 * i;
+    for (int d = 0; d < 3; ++d) {
+      if (s->bc[d] != 1)
+        continue;
+      if (p[d] < s->lo[d]) {
+        wrapped = 1;
+        p[d] += s->L[d];
+      } else if (p[d] > s->hi[d]) {
+        wrapped = 1;
+        p[d] -= s->L[d];
+      }
+    }
+  }
+  return wrapped;
+}
+
+static void push_ghost(orc_sys *s, int src, double x, double y, double z)
+{
+  if (s->g == s->gcap) {
+    s->gcap = s->gcap ? 2 * s->gcap : 1024;
+    s->gpos = (double *)realloc(s->gpos, sizeof(double) * 3 * s->gcap);
+    s->gsrc = (int *)realloc(s->gsrc, sizeof(int) * s->gcap);
+  }
+  s->gpos[3 * s->g + 0] = x;
+  s->gpos[3 * s->g + 1] = y;
+  s->gpos[3 * s->g + 2] = z;
+  s->gsrc[s->g] = src;
+  s->g++;
+}
+
+/* exchange_ghost_single_md_domain.cpp:26-215: for owned i ascending, images
+ * in the order x, y, z, xy, xz, yz, xyz; image = pos - s*L with s=-1 below
+ * lower+ghost_cutoff and s=+1 above upper-ghost_cutoff (strict). */
+void orc_ghosts(orc_sys *s)
+{
+  s->g = 0;
+  const double xl = s->lo[0] + s->ghost_cutoff, xu = s->hi[0] - s->ghost_cutoff;
+  const double yl = s->lo[1] + s->ghost_cutoff, yu = s->hi[1] - s->ghost_cutoff;
+  const double zl = s->lo[2] + s->ghost_cutoff, zu = s->hi[2] - s->ghost_cutoff;
+  for (int i = 0; i < s->n; ++i) {
+    const double *p = s->pos + 3 * i;
+    int xv = p[0] < xl ? -1 : (p[0] > xu ? 1 : 0);
+    int yv = p[1] < yl ? -1 : (p[1] > yu ? 1 : 0);
+    int zv = p[2] < zl ? -1 : (p[2] > zu ? 1 : 0);
+    xv *= s->bc[0];
+    yv *= s->bc[1];
+    zv *= s->bc[2];
+    const double gx = p[0] - xv * s->L[0], gy = p[1] - yv * s->L[1], gz = p[2] - zv * s->L[2];
+    if (xv)
+      push_ghost(s, i, gx, p[1], p[2]);
+    if (yv)
+      push_ghost(s, i, p[0], gy, p[2]);
+    if (zv)
+      push_ghost(s, i, p[0], p[1], gz);
+    if (xv && yv)
+      push_ghost(s, i, gx, gy, p[2]);
+    if (xv && zv)
+      push_ghost(s, i, gx, p[1], gz);
+    if (yv && zv)
+      push_ghost(s, i, p[0], gy, gz);
+    if (xv && yv && zv)
+      push_ghost(s, i, gx, gy, gz);
+  }
+}
+
+/* ------------------------------------------------------------ neighbours -- */
+
+static inline double dot3(const double *a) { return a[0] * a[0] + a[1] * a[1] + a[2] * a[2]; }
+
+static inline const double *apos(const orc_sys *s, int j)
+{
+  return j < s->n ? s->pos + 3 * j : s->gpos + 3 * (j - s->n);
+}
+
+static inline int atype(const orc_sys *s, int j) { return j < s->n ? s->type[j] : s->type[s->gsrc[j - s->n]]; }
+static inline int aid(const orc_sys *s, int j) { return j < s->n ? s->id[j] : s->id[s->gsrc[j - s->n]]; }
+static inline int amol(const orc_sys *s, int j) { return j < s->n ? s->mol[j] : s->mol[s->gsrc[j - s->n]]; }
+
+/* make_neigh_bin (neighborlist.cpp:581-697): 27 candidates x-outer, z-inner,
+ * clamped on non-periodic axes, wrapped on periodic ones, de-duplicated
+ * keeping first occurrences. */
+static void make_stencil(orc_sys *s)
+{
+  const int nbx = s->nb[0], nby = s->nb[1], nbz = s->nb[2];
+  s->nbins = nbx * nby * nbz;
+  int *cnt = (int *)calloc(s->nbins, sizeof(int));
+  int *cand = (int *)malloc(sizeof(int) * 27 * s->nbins);
+  for (int x = 0; x < nbx; ++x)
+    for (int y = 0; y < nby; ++y)
+      for (int z = 0; z < nbz; ++z) {
+        int lo[3] = {x - 1, y - 1, z - 1}, hi[3] = {x + 1, y + 1, z + 1};
+        for (int d = 0; d < 3; ++d)
+          if (s->bc[d] == 0) {
+            if (lo[d] == -1)
+              lo[d] = 0;
+            if (hi[d] == s->nb[d])
+              hi[d] = s->nb[d] - 1;
+          }
+        const int k = x + nbx * y + nbx * nby * z;
+        int *c = cand + 27 * k;
+        for (int a = lo[0]; a <= hi[0]; ++a)
+          for (int b = lo[1]; b <= hi[1]; ++b)
+            for (int e = lo[2]; e <= hi[2]; ++e) {
+              int v[3] = {a, b, e};
+              for (int d = 0; d < 3; ++d) {
+                if (v[d] == s->nb[d])
+                  v[d] = 0;
+                if (v[d] == -1)
+                  v[d] = s->nb[d] - 1;
+              }
+              const int lin = v[0] + nbx * v[1] + nbx * nby * v[2];
+              int dup = 0;
+              for (int q = 0; q < cnt[k]; ++q)
+                dup |= c[q] == lin;
+              if (!dup)
+                c[cnt[k]++] = lin;
+            }
+      }
+  free(s->st_off);
+  free(s->st);
+  s->st_off = (int *)malloc(sizeof(int) * (s->nbins + 1));
+  s->st = (int *)malloc(sizeof(int) * 27 * s->nbins);
+  int off = 0;
+  for (int k = 0; k < s->nbins; ++k) {
+    s->st_off[k] = off;
+    memcpy(s->st + off, cand + 27 * k, sizeof(int) * cnt[k]);
+    off += cnt[k];
+  }
+  s->st_off[s->nbins] = off;
+  free(cnt);
+  free(cand);
+}
+
+/* Neighborlist::init (neighborlist.cpp:313-377). */
+static void nl_init(orc_sys *s)
+{
+  s->rce = s->nl_cutoff * s->nl_coef;
+  s->thr_sq = 0.25 * (s->rce - s->nl_cutoff) * (s->rce - s->nl_cutoff);
+  free(s->pos_old);
+  s->pos_old = (double *)calloc(3 * (size_t)(s->n ? s->n : 1), sizeof(double));
+  s->n_old = s->n;
+  if (!s->build_cells)
+    return;
+  s->cinv = 1.0 / s->rce;
+  for (int d = 0; d < 3; ++d)
+    s->nb[d] = (int)ceil(s->L[d] * s->cinv);
+  make_stencil(s);
+}
+
+/* binlist_index (neighborlist.cpp:408-430): floor, clamped to the grid. */
+static void bin_index3(const orc_sys *s, const double *p, int ind[3])
+{
+  for (int d = 0; d < 3; ++d) {
+    int v = (int)floor((p[d] - s->lo[d]) * s->cinv);
+    if (v < 0)
+      v = 0;
+    if (v >= s->nb[d])
+      v = s->nb[d] - 1;
+    ind[d] = v;
+  }
+}
+
+static int bin_of(const orc_sys *s, const double *p)
+{
+  int ind[3];
+  bin_index3(s, p, ind);
+  return ind[0] + s->nb[0] * ind[1] + s->nb[0] * s->nb[1] * ind[2];
+}
+
+/* update_cell_list (neighborlist.cpp:439-472): owned i ascending then ghosts n+g. */
+static void update_cell_list(orc_sys *s)
+{
+  const int tot = s->n + s->g;
+  int *bin = (int *)malloc(sizeof(int) * (tot ? tot : 1));
+  free(s->bin_off);
+  s->bin_off = (int *)calloc(s->nbins + 1, sizeof(int));
+  for (int j = 0; j < tot; ++j) {
+    bin[j] = bin_of(s, apos(s, j));
+    s->bin_off[bin[j] + 1]++;
+  }
+  for (int k = 0; k < s->nbins; ++k)
+    s->bin_off[k + 1] += s->bin_off[k];
+  free(s->bin_items);
+  s->bin_items = (int *)malloc(sizeof(int) * (tot ? tot : 1));
+  int *cur = (int *)malloc(sizeof(int) * (s->nbins ? s->nbins : 1));
+  memcpy(cur, s->bin_off, sizeof(int) * s->nbins);
+  for (int j = 0; j < tot; ++j)
+    s->bin_items[cur[bin[j]]++] = j;
+  free(cur);
+  free(bin);
+}
+
+/* update_is_needed (neighborlist.cpp:91-139). */
+static int update_is_needed(const orc_sys *s)
+{
+  if (s->rebuild_test)
+    return 1;
+  if (s->n != s->n_old || s->g != s->g_old)
+    return 1;
+  for (int i = 0; i < s->n; ++i) {
+    double d[3] = {s->pos[3 * i] - s->pos_old[3 * i], s->pos[3 * i + 1] - s->pos_old[3 * i + 1],
+                   s->pos[3 * i + 2] - s->pos_old[3 * i + 2]};
+    if (dot3(d) > s->thr_sq)
+      return 1;
+  }
+  for (int i = 0; i < s->g; ++i) {
+    double d[3] = {s->gpos[3 * i] - s->gpos_old[3 * i], s->gpos[3 * i + 1] - s->gpos_old[3 * i + 1],
+                   s->gpos[3 * i + 2] - s->gpos_old[3 * i + 2]};
+    if (dot3(d) > s->thr_sq)
+      return 1;
+  }
+  return 0;
+}
+
+static void nl_push(orc_sys *s, long *w, int j)
+{
+  if (*w == s->nl_cap) {
+    s->nl_cap = s->nl_cap ? 2 * s->nl_cap : 4096;
+    s->nl_items = (int *)realloc(s->nl_items, sizeof(int) * s->nl_cap);
+  }
+  s->nl_items[(*w)++] = j;
+}
+
+static void save_old(orc_sys *s)
+{
+  free(s->pos_old);
+  free(s->gpos_old);
+  s->pos_old = (double *)dup_arr(s->pos, sizeof(double) * 3 * s->n);
+  s->gpos_old = (double *)dup_arr(s->gpos, sizeof(double) * 3 * s->g);
+  s->n_old = s->n;
+  s->g_old = s->g;
+}
+
+/* all_atom_test_function (neighborlist.cpp:161-191): every owned j>i and every
+ * ghost, no distance test; pos_old is not refreshed (the callers return early). */
+static void all_atom_list(orc_sys *s)
+{
+  free(s->nl_off);
+  s->nl_off = (long *)malloc(sizeof(long) * (s->n + 1));
+  long w = 0;
+  for (int i = 0; i < s->n; ++i) {
+    s->nl_off[i] = w;
+    for (int j = i + 1; j < s->n; ++j)
+      nl_push(s, &w, j);
+    for (int g = 0; g < s->g; ++g)
+      nl_push(s, &w, g + s->n);
+  }
+  s->nl_off[s->n] = w;
+}
+
+/* update_verlet_list (neighborlist.cpp:257-307): brute force, j ascending,
+ * strict dr*dr < cutoff_extra^2 with dr = pos_j - pos_i. */
+static void verlet_brute(orc_sys *s)
+{
+  if (s->all_atom_test) {
+    all_atom_list(s);
+    return;
+  }
+  const double rc2 = s->rce * s->rce;
+  free(s->nl_off);
+  s->nl_off = (long *)malloc(sizeof(long) * (s->n + 1));
+  long w = 0;
+  for (int i = 0; i < s->n; ++i) {
+    s->nl_off[i] = w;
+    const double *pi = s->pos + 3 * i;
+    for (int j = i + 1; j < s->n; ++j) {
+      const double *pj = s->pos + 3 * j;
+      double d[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+      if (dot3(d) < rc2)
+        nl_push(s, &w, j);
+    }
+    for (int g = 0; g < s->g; ++g) {
+      const double *pj = s->gpos + 3 * g;
+      double d[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+      if (dot3(d) < rc2)
+        nl_push(s, &w, g + s->n);
+    }
+  }
+  s->nl_off[s->n] = w;
+  save_old(s);
+}
+
+/* update_verlet_list_from_cell_list (neighborlist.cpp:515-573): stencil order,
+ * owned j must be > i, dr = pos_i - pos_j, strict test. */
+static void verlet_cells(orc_sys *s)
+{
+  if (s->all_atom_test) {
+    all_atom_list(s);
+    return;
+  }
+  const double rc2 = s->rce * s->rce;
+  free(s->nl_off);
+  s->nl_off = (long *)malloc(sizeof(long) * (s->n + 1));
+  long w = 0;
+  for (int i = 0; i < s->n; ++i) {
+    s->nl_off[i] = w;
+    const double *pi = s->pos + 3 * i;
+    const int b = bin_of(s, pi);
+    for (int q = s->st_off[b]; q < s->st_off[b + 1]; ++q) {
+      const int c = s->st[q];
+      for (int t = s->bin_off[c]; t < s->bin_off[c + 1]; ++t) {
+        const int j = s->bin_items[t];
+        if (j < s->n && j <= i)
+          continue;
+        const double *pj = apos(s, j);
+        double d[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
+        if (dot3(d) < rc2)
+          nl_push(s, &w, j);
+      }
+    }
+  }
+  s->nl_off[s->n] = w;
+  save_old(s);
+}
+
+/* Neighborlist::build (neighborlist.cpp:66-88). */
+int orc_build(orc_sys *s, int update)
+{
+  if (!s->nl_on)
+    return 0;
+  if (s->build_cells)
+    update_cell_list(s);
+  if (!update)
+    update = update_is_needed(s);
+  if (update) {
+    if (s->from_cells)
+      verlet_cells(s);
+    else
+      verlet_brute(s);
+    return 1;
+  }
+  return 0;
+}
+
+/* ----------------------------------------------------------- force fields -- */
+
+void orc_reset_acc(orc_sys *s) { memset(s->acc, 0, sizeof(double) * 3 * s->n); } /* atom_data.cpp:308-316 */
+
+/* Lj_cell_list::calculate_acceleration (lj_cell_list.cpp:85-175): 27-cell walk,
+ * owned i<j and every ghost, skip dr^2 > rc^2, shifted force coefficient,
+ * Newton-3 on owned partners only.  Contributes nothing to the virial. */
+void orc_lj_acc(orc_sys *s)
+{
+  if (!s->build_cells)
+    return; /* Lj_cell_list walks the cell list only */
+  const double rc = s->lj_rc, rc2 = rc * rc;
+  for (int i = 0; i < s->n; ++i) {
+    const double *pi = s->pos + 3 * i;
+    const int ti = s->type[i];
+    const double mi = s->minv[ti];
+    const int b = bin_of(s, pi);
+    for (int q = s->st_off[b]; q < s->st_off[b + 1]; ++q) {
+      const int c = s->st[q];
+      for (int t = s->bin_off[c]; t < s->bin_off[c + 1]; ++t) {
+        const int j = s->bin_items[t];
+        const int ghost = j >= s->n;
+        if (!ghost && !(i < j))
+          continue;
+        const double *pj = apos(s, j);
+        const int tj = atype(s, j);
+        const double mj = s->minv[tj];
+        double dr[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+        const double dr2 = dot3(dr);
+        if (dr2 > rc2)
+          continue;
+        const double e = s->eps[ti * s->nt + tj], sg = s->sig[ti * s->nt + tj];
+        const double rcsi = 1 / (rc * rc);
+        const double rhoc2 = sg * sg * rcsi;
+        const double rhoc6 = rhoc2 * rhoc2 * rhoc2;
+        const double rhoc12 = rhoc6 * rhoc6;
+        const double dsi = 1 / dr2;
+        const double rho2 = sg * sg * dsi;
+        const double rho6 = rho2 * rho2 * rho2;
+        const double rho12 = rho6 * rho6;
+        const double coef = 4 * e *
+                            (-12 * rho12 * dsi + 6 * rho6 * dsi + +12 * rhoc12 * rcsi -
+                             6 * rhoc6 * rcsi);
+        double f[3] = {coef * dr[0], coef * dr[1], coef * dr[2]};
+        for (int d = 0; d < 3; ++d)
+          s->acc[3 * i + d] += f[d] * mi;
+        if (!ghost)
+          for (int d = 0; d < 3; ++d)
+            s->acc[3 * j + d] -= f[d] * mj;
+      }
+    }
+  }
+}
+
+/* Electrostatic_ewald_r::calculate_acceleration (electrostatic_ewald_r.cpp:115-227):
+ * every Verlet-list entry (no cutoff test), exact-zero separation skipped,
+ * virial -coef*r^2 when id_i < id_j. */
+void orc_ewald_r_acc(orc_sys *s)
+{
+  const double alpha = s->er_alpha, a2 = alpha * alpha, ke = s->er_ke;
+  double vir = 0;
+  for (int i = 0; i < s->n; ++i) {
+    const double *pi = s->pos + 3 * i;
+    const int ti = s->type[i];
+    const double qi = s->charge[ti], mi = s->minv[ti];
+    const int idi = s->id[i], moli = s->mol[i];
+    for (long t = s->nl_off[i]; t < s->nl_off[i + 1]; ++t) {
+      const int j = s->nl_items[t];
+      const int ghost = j >= s->n;
+      const double *pj = apos(s, j);
+      const int tj = atype(s, j);
+      double lam = 1.0;
+      if (s->lambda_set && moli == amol(s, j)) {
+        if (s->lambda[ti * s->nt + tj] == 0.0)
+          continue;
+        lam *= s->lambda[ti * s->nt + tj];
+      }
+      double r[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+      if (r[0] == 0 && r[1] == 0 && r[2] == 0)
+        continue;
+      const double qj = s->charge[tj];
+      const double r2 = dot3(r);
+      const double rn = sqrt(r2);
+      const double sum = (2 * alpha * ORC_PIS_INV * exp(-a2 * r2) + erfc(alpha * rn) / rn) * (1.0 / r2);
+      const double coef = -lam * ke * qi * qj * sum;
+      if (idi < aid(s, j))
+        vir += -coef * r2;
+      double f[3] = {coef * r[0], coef * r[1], coef * r[2]};
+      for (int d = 0; d < 3; ++d)
+        s->acc[3 * i + d] += f[d] * mi;
+      if (!ghost) {
+        const double mj = s->minv[tj];
+        for (int d = 0; d < 3; ++d)
+          s->acc[3 * j + d] -= f[d] * mj;
+      }
+    }
+  }
+  s->virial += vir;
+}
+
+/* Electrostatic_ewald_k::initialize / make_k_vectors (electrostatic_ewald_k/
+ * initialize.cpp:31-92,184-222): lexicographic kx,ky,kz in [-kmax,kmax] without
+ * 0, but only the first (2kx)(2ky)(2kz) vectors are used by every consumer. */
+static void ek_init(orc_sys *s)
+{
+  double lx = s->L[0], ly = s->L[1], lz = s->L[2];
+  if (s->slab_axis == 0)
+    lx = s->slab_lz;
+  if (s->slab_axis == 1)
+    ly = s->slab_lz;
+  if (s->slab_axis == 2)
+    lz = s->slab_lz;
+  const double lxi = 1.0 / lx, lyi = 1.0 / ly, lzi = 1.0 / lz;
+  s->lxyz_inv = 1.0 / (lx * ly * lz);
+  s->nk = (2 * s->kx) * (2 * s->ky) * (2 * s->kz);
+  const int nfull = (2 * s->kx + 1) * (2 * s->ky + 1) * (2 * s->kz + 1) - 1;
+  free(s->kvec);
+  free(s->fk);
+  free(s->sk);
+  s->kvec = (double *)malloc(sizeof(double) * 3 * (nfull ? nfull : 1));
+  s->fk = (double *)malloc(sizeof(double) * (nfull ? nfull : 1));
+  s->sk = (double *)calloc(2 * (size_t)(s->nk ? s->nk : 1), sizeof(double));
+  const double a2 = s->ek_alpha * s->ek_alpha, f4a2 = 1.0 / (4.0 * a2);
+  int w = 0;
+  for (int kx = -s->kx; kx <= s->kx; ++kx)
+    for (int ky = -s->ky; ky <= s->ky; ++ky)
+      for (int kz = -s->kz; kz <= s->kz; ++kz) {
+        double k[3] = {ORC_2PI * kx * lxi, ORC_2PI * ky * lyi, ORC_2PI * kz * lzi};
+        const double k2 = dot3(k);
+        if (k2 == 0)
+          continue;
+        const double k2i = 1.0 / k2;
+        memcpy(s->kvec + 3 * w, k, sizeof k);
+        s->fk[w] = k2i * exp(-k2 * f4a2);
+        ++w;
+      }
+  if (s->dipole)
+    s->dip_coef = s->ek_ke * ORC_4PI * s->lxyz_inv / (1 + 2.0 * s->eps_dipole);
+  s->ek_init = 1;
+}
+
+/* calculate_dipole_sum + calculate_potential_k_coef_cmplx
+ * (electrostatic_ewald_k/initialize.cpp:94-146). */
+static void ek_structure_factor(orc_sys *s, int k0, int k1)
+{
+  if (s->dipole) {
+    s->dip_sum[0] = s->dip_sum[1] = s->dip_sum[2] = 0;
+    for (int j = 0; j < s->n; ++j) {
+      const double q = s->charge[s->type[j]];
+      for (int d = 0; d < 3; ++d)
+        s->dip_sum[d] += q * s->pos[3 * j + d];
+    }
+    for (int d = 0; d < 3; ++d)
+      s->dip_field[d] = -s->dip_sum[d] * s->dip_coef;
+  }
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int k = k0; k < k1; ++k) {
+    const double *kv = s->kvec + 3 * k;
+    double complex rho = 0;
+    for (int j = 0; j < s->n; ++j) {
+      const double q = s->charge[s->type[j]];
+      const double *p = s->pos + 3 * j;
+      const double th = kv[0] * p[0] + kv[1] * p[1] + kv[2] * p[2];
+      rho += q * (cos(th) + I * sin(th));
+    }
+    s->sk[2 * k] = creal(rho);
+    s->sk[2 * k + 1] = cimag(rho);
+  }
+}
+
+/* Electrostatic_ewald_k::calculate_acceleration (electrostatic_ewald_k.cpp:161-234).
+ * The reference loops k outer, i inner; per atom the k terms are added in the
+ * same k order here.  With epsilon_dipole the dipole force is added once per
+ * k-vector, as the reference's loop does (electrostatic_ewald_k.cpp:216-219). */
+static void ek_force_pass(orc_sys *s, int i0, int i1)
+{
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int i = i0; i < i1; ++i) {
+    const int ti = s->type[i];
+    const double qi = s->charge[ti], mi = s->minv[ti];
+    const double *p = s->pos + 3 * i;
+    double a[3] = {s->acc[3 * i], s->acc[3 * i + 1], s->acc[3 * i + 2]};
+    for (int k = 0; k < s->nk; ++k) {
+      const double *kv = s->kvec + 3 * k;
+      const double sr = s->sk[2 * k], si = -s->sk[2 * k + 1];
+      const double th = kv[0] * p[0] + kv[1] * p[1] + kv[2] * p[2];
+      const double c = cos(th), sn = sin(th);
+      const double im = sr * sn + si * c;
+      const double w = ORC_4PI * s->fk[k];
+      double sk3[3] = {w * kv[0] * im, w * kv[1] * im, w * kv[2] * im};
+      const double cc = s->ek_ke * qi * s->lxyz_inv;
+      for (int d = 0; d < 3; ++d) {
+        double f = cc * sk3[d];
+        if (s->dipole)
+          f += qi * s->dip_field[d];
+        a[d] += f * mi;
+      }
+    }
+    s->acc[3 * i] = a[0];
+    s->acc[3 * i + 1] = a[1];
+    s->acc[3 * i + 2] = a[2];
+  }
+}
+
+void orc_ewald_k_acc(orc_sys *s)
+{
+  if (!s->ek_init)
+    ek_init(s);
+  ek_structure_factor(s, 0, s->nk);
+  ek_force_pass(s, 0, s->n);
+}
+
+static double now_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+void orc_ewald_k_sample(orc_sys *s, int k0, int k1, int i0, int i1, double *t_s, double *t_f)
+{
+  if (!s->ek_init)
+    ek_init(s);
+  double t0 = now_s();
+  ek_structure_factor(s, k0, k1);
+  double t1 = now_s();
+  ek_force_pass(s, i0, i1);
+  double t2 = now_s();
+  *t_s = t1 - t0;
+  *t_f = t2 - t1;
+}
+
+void orc_forces(orc_sys *s)
+{
+  orc_reset_acc(s);
+  if (s->lj_on)
+    orc_lj_acc(s);
+  if (s->er_on)
+    orc_ewald_r_acc(s);
+  if (s->ek_on)
+    orc_ewald_k_acc(s);
+}
+
+/* ---------------------------------------------------------------- driver -- */
+
+int orc_setup(orc_sys *s)
+{
+  s->virial = 0;
+  if (s->nl_on)
+    nl_init(s);
+  orc_wrap(s);
+  orc_ghosts(s);
+  orc_build(s, 1);
+  orc_forces(s);
+  return 0;
+}
+
+/* integrate_velocity_verlet (md_simulator.cpp:555-617). */
+int orc_step(orc_sys *s, double dt)
+{
+  s->virial = 0; /* Md_simulator::step resets the virial (md_simulator.cpp:408) */
+  for (int i = 0; i < 3 * s->n; ++i)
+    if (isnan(s->acc[i]))
+      return 1;
+  for (int i = 0; i < s->n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      const int q = 3 * i + d;
+      s->pos[q] += s->vel[q] * dt + 0.5 * s->acc[q] * dt * dt;
+      s->acc_old[q] = s->acc[q];
+    }
+  /* re_calculate_acc (md_simulator.cpp:361-374) */
+  int update = orc_wrap(s);
+  orc_ghosts(s);
+  orc_build(s, update);
+  orc_forces(s);
+  for (int q = 0; q < 3 * s->n; ++q)
+    s->vel[q] += 0.5 * dt * (s->acc[q] + s->acc_old[q]);
+  return 0;
+}
+
+/* ------------------------------------------------------ energies, queries -- */
+
+/* Electrostatic_ewald_r::potential(const Vector&) (electrostatic_ewald_r/
+ * potential.cpp:30-86) and field (field.cpp:31-96): every owned or ghost atom
+ * in the 27 cells around r's bin, no cutoff, exact-zero separation skipped. */
+static void point_r(orc_sys *s, const double *r, double *field, double *pot)
+{
+  const double alpha = s->er_alpha, a2 = alpha * alpha;
+  double phi = 0, e[3] = {0, 0, 0};
+  const int b = bin_of(s, r);
+  for (int q = s->st_off[b]; q < s->st_off[b + 1]; ++q) {
+    const int c = s->st[q];
+    for (int t = s->bin_off[c]; t < s->bin_off[c + 1]; ++t) {
+      const int j = s->bin_items[t];
+      const double *pj = apos(s, j);
+      const double qj = s->charge[atype(s, j)];
+      double d[3] = {r[0] - pj[0], r[1] - pj[1], r[2] - pj[2]};
+      if (d[0] == 0 && d[1] == 0 && d[2] == 0)
+        continue;
+      const double d2 = dot3(d), dn = sqrt(d2);
+      phi += qj * erfc(alpha * dn) / dn;
+      const double sr = 2 * alpha * ORC_PIS_INV * exp(-a2 * d2) + erfc(alpha * dn) / dn;
+      for (int k = 0; k < 3; ++k)
+        e[k] += qj * (sr * (d[k] / d2));
+    }
+  }
+  if (pot)
+    *pot = phi * s->er_ke;
+  if (field)
+    for (int k = 0; k < 3; ++k)
+      field[k] = e[k] * s->er_ke;
+}
+
+double orc_energy_r(orc_sys *s)
+{
+  if (!s->build_cells)
+    return NAN; /* the cell walk needs build_cell_list, as in the reference */
+  update_cell_list(s);
+  double e = 0;
+  for (int j = 0; j < s->n; ++j) {
+    double phi;
+    point_r(s, s->pos + 3 * j, NULL, &phi);
+    e += s->charge[s->type[j]] * phi;
+  }
+  return 0.5 * e; /* energy.cpp:30-51 */
+}
+
+void orc_point_r(orc_sys *s, int np, const double *pts, double *field, double *pot)
+{
+  if (!s->build_cells)
+    return;
+  update_cell_list(s);
+  for (int p = 0; p < np; ++p)
+    point_r(s, pts + 3 * p, field ? field + 3 * p : NULL, pot ? pot + p : NULL);
+}
+
+/* k_space_potential(const Vector&) (electrostatic_ewald_k/potential.cpp:107-129)
+ * and k_space_field (field.cpp:80-115), plus the dipole field when enabled. */
+static void point_k(const orc_sys *s, const double *r, double *field, double *pot)
+{
+  double sum = 0, fx = 0, fy = 0, fz = 0;
+  for (int k = 0; k < s->nk; ++k) {
+    const double *kv = s->kvec + 3 * k;
+    const double th = kv[0] * r[0] + kv[1] * r[1] + kv[2] * r[2];
+    const double c = cos(th), sn = sin(th);
+    const double sr = s->sk[2 * k], si = s->sk[2 * k + 1];
+    /* Re(S * exp(-i th)) */
+    sum += s->fk[k] * (sr * c + si * sn);
+    /* Im(f * conj(S) * exp(i th)) */
+    const double im = s->fk[k] * (sr * sn - si * c);
+    fx += kv[0] * im;
+    fy += kv[1] * im;
+    fz += kv[2] * im;
+  }
+  const double c0 = s->ek_ke * s->lxyz_inv * ORC_4PI;
+  if (pot)
+    *pot = ORC_4PI * s->lxyz_inv * sum * s->ek_ke;
+  if (field) {
+    field[0] = c0 * fx;
+    field[1] = c0 * fy;
+    field[2] = c0 * fz;
+    if (s->dipole)
+      for (int d = 0; d < 3; ++d)
+        field[d] += s->dip_field[d];
+  }
+}
+
+void orc_point_k(orc_sys *s, int np, const double *pts, double *field, double *pot)
+{
+  for (int p = 0; p < np; ++p)
+    point_k(s, pts + 3 * p, field ? field + 3 * p : NULL, pot ? pot + p : NULL);
+}
+
+double orc_self_energy(orc_sys *s)
+{
+  double sum = 0;
+  for (int j = 0; j < s->n; ++j) {
+    const double q = s->charge[s->type[j]];
+    sum += q * q;
+  }
+  return -ORC_PIS_INV * s->ek_alpha * sum * s->ek_ke; /* energy.cpp:31-50 */
+}
+
+/* Electrostatic_ewald_k::energy (electrostatic_ewald_k/energy.cpp:61-116); valid
+ * right after a force evaluation at the same positions (S(k) is refreshed there). */
+double orc_energy_k(orc_sys *s)
+{
+  double e = 0;
+  for (int j = 0; j < s->n; ++j) {
+    double phi;
+    point_k(s, s->pos + 3 * j, NULL, &phi);
+    e += s->charge[s->type[j]] * phi;
+  }
+  e = 0.5 * e + orc_self_energy(s);
+  if (s->dipole)
+    e += 0.5 * s->dip_coef * dot3(s->dip_sum);
+  return e;
+}
+
+/* Shifted-force LJ energy consistent with the reference force (lj_cell_list.cpp:
+ * 143-155): U(r) = 4e[(s/r)^12 - (s/r)^6 + g_c r^2/2] - (same at rc), g_c = 12s^12/rc^14
+ * - 6s^6/rc^8.  The reference has no LJ energy; this pairs with its force. */
+double orc_energy_lj(orc_sys *s)
+{
+  if (!s->build_cells)
+    return NAN;
+  update_cell_list(s);
+  const double rc = s->lj_rc, rc2 = rc * rc;
+  double e = 0;
+  for (int i = 0; i < s->n; ++i) {
+    const double *pi = s->pos + 3 * i;
+    const int ti = s->type[i];
+    const int b = bin_of(s, pi);
+    for (int q = s->st_off[b]; q < s->st_off[b + 1]; ++q) {
+      const int c = s->st[q];
+      for (int t = s->bin_off[c]; t < s->bin_off[c + 1]; ++t) {
+        const int j = s->bin_items[t];
+        const int ghost = j >= s->n;
+        if (!ghost && !(i < j))
+          continue;
+        const double *pj = apos(s, j);
+        const int tj = atype(s, j);
+        double dr[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+        const double r2 = dot3(dr);
+        if (r2 > rc2)
+          continue;
+        const double ep = s->eps[ti * s->nt + tj], sg = s->sig[ti * s->nt + tj];
+        const double s2 = sg * sg, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        const double gc = 12 * s12 / pow(rc, 14) - 6 * s6 / pow(rc, 8);
+        const double u = s12 / pow(r2, 6) - s6 / pow(r2, 3) + 0.5 * gc * r2;
+        const double uc = s12 / pow(rc2, 6) - s6 / pow(rc2, 3) + 0.5 * gc * rc2;
+        /* a ghost pair is seen from both owned ends: count half each time */
+        e += (ghost ? 0.5 : 1.0) * 4 * ep * (u - uc);
+      }
+    }
+  }
+  return e;
+}
+
+/* --------------------------------------------------------------- getters -- */
+
+int orc_n(const orc_sys *s) { return s->n; }
+
+static double *arr_of(const orc_sys *s, int what)
+{
+  switch (what) {
+  case 0:
+    return s->pos;
+  case 1:
+    return s->vel;
+  case 2:
+    return s->acc;
+  default:
+    return s->acc_old;
+  }
+}
+
+void orc_get(const orc_sys *s, int what, double *out) { memcpy(out, arr_of(s, what), sizeof(double) * 3 * s->n); }
+void orc_set(orc_sys *s, int what, const double *in) { memcpy(arr_of(s, what), in, sizeof(double) * 3 * s->n); }
+int orc_num_ghosts(const orc_sys *s) { return s->g; }
+
+void orc_get_ghosts(const orc_sys *s, double *gpos, int *gsrc)
+{
+  if (gpos)
+    memcpy(gpos, s->gpos, sizeof(double) * 3 * s->g);
+  if (gsrc)
+    memcpy(gsrc, s->gsrc, sizeof(int) * s->g);
+}
+
+long orc_num_pairs(const orc_sys *s) { return s->nl_off ? s->nl_off[s->n] : 0; }
+
+void orc_get_pairs(const orc_sys *s, int *ij)
+{
+  long w = 0;
+  for (int i = 0; i < s->n; ++i)
+    for (long t = s->nl_off[i]; t < s->nl_off[i + 1]; ++t) {
+      ij[2 * w] = i;
+      ij[2 * w + 1] = s->nl_items[t];
+      ++w;
+    }
+}
+
+void orc_get_nbins(const orc_sys *s, int nb[3])
+{
+  nb[0] = s->nb[0];
+  nb[1] = s->nb[1];
+  nb[2] = s->nb[2];
+}
+
+double orc_virial(const orc_sys *s) { return s->virial; }
+void orc_reset_virial(orc_sys *s) { s->virial = 0; }
+int orc_num_kvectors(const orc_sys *s) { return s->nk; }
+
+void orc_get_kvectors(const orc_sys *s, double *kvec, double *fk)
+{
+  if (kvec)
+    memcpy(kvec, s->kvec, sizeof(double) * 3 * s->nk);
+  if (fk)
+    memcpy(fk, s->fk, sizeof(double) * s->nk);
+}
+
+void orc_get_sk(const orc_sys *s, double *sk) { memcpy(sk, s->sk, sizeof(double) * 2 * s->nk); }

@@ -1,0 +1,95 @@
+/* CPU oracle for the CAVIAR short-range + Ewald force path (TEST INFRASTRUCTURE).
+ *
+ * A plain-C, FP64 restatement of the reference algorithm, used only as the
+ * checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * The product path (caviar_amd) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference tree).  Pins (what checks each function against an
+ * independent answer) are listed in tests/test_oracle.py.
+ *
+ * Layout follows the reference: AoS xyz doubles, owned atoms 0..n-1, ghost
+ * atoms addressed as n+g in neighbour lists (neighborlist.cpp:287,470).
+ */
+#ifndef CAVIAR_ORACLE_H
+#define CAVIAR_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_sys orc_sys;
+
+orc_sys *orc_create(void);
+void orc_destroy(orc_sys *s);
+void orc_set_threads(int nthreads);
+
+/* Domain box + boundary flags (domain.cpp:206-266,470-477). */
+void orc_set_box(orc_sys *s, const double lo[3], const double hi[3], const int bc[3]);
+/* Atom_data::ghost_cutoff (atom_data.cpp:87-91). */
+void orc_set_ghost_cutoff(orc_sys *s, double gc);
+/* add_type_mass / add_type_charge (atom_data.cpp:206-245); mass 0 -> mass_inv 0. */
+void orc_set_types(orc_sys *s, int nt, const double *mass, const double *charge);
+/* Owned atoms (atom_data.cpp:487-521).  mol may be NULL (-> -1 like add_atom). */
+void orc_set_atoms(orc_sys *s, int n, const int *id, const int *type, const int *mol,
+                   const double *pos, const double *vel);
+/* neighborlist basic: cutoff, cutoff_extra_coef, make_verlet_list_from_cell_list
+ * (from_cells), build_cell_list, all_atom_test, rebuild_test (neighborlist.cpp:193-251). */
+void orc_set_neighbor(orc_sys *s, double cutoff, double coef, int from_cells, int build_cells,
+                      int all_atom_test, int rebuild_test);
+/* force_field lj_cell_list (lj_cell_list.cpp:33-76); eps/sigma are nt*nt row-major [ti][tj]. */
+void orc_set_lj(orc_sys *s, double rc, const double *eps, const double *sigma);
+/* force_field electrostatic_ewald_r (electrostatic_ewald_r.cpp:31-110); lambda nt*nt or NULL. */
+void orc_set_ewald_r(orc_sys *s, double alpha, double ke, const double *lambda);
+/* force_field electrostatic_ewald_k (electrostatic_ewald_k.cpp:31-139).
+ * dipole!=0 enables epsilon_dipole; slab_axis -1 = no slab geometry. */
+void orc_set_ewald_k(orc_sys *s, double alpha, double ke, int kx, int ky, int kz, int dipole,
+                     double eps_dipole, int slab_axis, double slab_lz);
+
+/* Md_simulator::setup (md_simulator.cpp:504-549): neighbour init, wrap, ghosts,
+ * build(true), reset acceleration, all force fields. */
+int orc_setup(orc_sys *s);
+/* One velocity-Verlet step (md_simulator.cpp:555-617).  Returns 0, or 1 when an
+ * acceleration is NaN before the drift (md_simulator.cpp:584-590; state untouched). */
+int orc_step(orc_sys *s, double dt);
+/* reset_owned_acceleration + every enabled force field at the current list. */
+void orc_forces(orc_sys *s);
+/* Individual stages, exposed for the parity tests. */
+int orc_wrap(orc_sys *s);            /* exchange_owned_single_md_domain */
+void orc_ghosts(orc_sys *s);         /* exchange_ghost_single_md_domain */
+int orc_build(orc_sys *s, int update); /* Neighborlist::build */
+void orc_reset_acc(orc_sys *s);
+void orc_lj_acc(orc_sys *s);
+void orc_ewald_r_acc(orc_sys *s);
+void orc_ewald_k_acc(orc_sys *s);
+/* Bounded-sample timing of Ewald_k for the CPU baseline: S(k) over all atoms
+ * for k in [k0,k1), and the force pass for atoms [i0,i1) over all k. */
+void orc_ewald_k_sample(orc_sys *s, int k0, int k1, int i0, int i1, double *t_s, double *t_f);
+
+/* Getters. what: 0 pos, 1 vel, 2 acc, 3 acc_old (3n doubles). */
+int orc_n(const orc_sys *s);
+void orc_get(const orc_sys *s, int what, double *out);
+void orc_set(orc_sys *s, int what, const double *in);
+int orc_num_ghosts(const orc_sys *s);
+void orc_get_ghosts(const orc_sys *s, double *gpos, int *gsrc);
+long orc_num_pairs(const orc_sys *s);
+void orc_get_pairs(const orc_sys *s, int *ij);          /* reference order, 2*np ints */
+void orc_get_nbins(const orc_sys *s, int nb[3]);
+double orc_virial(const orc_sys *s);
+void orc_reset_virial(orc_sys *s);
+int orc_num_kvectors(const orc_sys *s);
+void orc_get_kvectors(const orc_sys *s, double *kvec, double *fk);
+void orc_get_sk(const orc_sys *s, double *sk);          /* 2*nk (re,im) */
+
+/* Energies and point queries (off the MD path). */
+double orc_energy_r(orc_sys *s);      /* Electrostatic_ewald_r::energy, P7 cell walk */
+double orc_energy_k(orc_sys *s);      /* Electrostatic_ewald_k::energy (k + self [+ dipole]) */
+double orc_self_energy(orc_sys *s);
+double orc_energy_lj(orc_sys *s);     /* shifted-force LJ energy (no reference counterpart) */
+void orc_point_r(orc_sys *s, int np, const double *pts, double *field, double *pot);
+void orc_point_k(orc_sys *s, int np, const double *pts, double *field, double *pot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,289 @@
+"""Write the golden fixtures under tests/golden/ by running the reference.
+
+Run in the development container only (it needs /root/reference):
+
+    make -f oracle/ref.mk -j8          # builds oracle/_ref/ref_harness
+    python tests/golden/make_golden.py
+
+For every case it writes the input as an XYZ file plus a CASL script in the
+reference's own syntax (neighborlist basic / force_field ... / md_simulator
+basic), runs ``oracle/_ref/ref_harness`` (the reference library built from its
+sources, linked with a small dumping harness) and stores inputs and the
+reference's outputs, printed with 17 significant digits, as a compressed
+``.npz`` fixture.  Fixtures are data: positions, settings and the reference's
+results; no reference source travels with them.
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from caviar_amd import configs  # noqa: E402
+from caviar_amd.system import (EwaldKSettings, EwaldRSettings, LJSettings,  # noqa: E402
+                               NeighborSettings, System)
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+B2 = "/root/reference/benchmark/b2-ewald"
+
+
+def R(x) -> str:
+    return repr(float(x))
+
+
+def casl_for(sysm: System, xyz: str, run_steps: int = 0, dt: float = 0.001) -> str:
+    lo, hi = sysm.lower, sysm.upper
+    L = []
+    L.append("domain box dom")
+    L.append(f"dom xmin {R(lo[0])} xmax {R(hi[0])}")
+    L.append(f"dom ymin {R(lo[1])} ymax {R(hi[1])}")
+    L.append(f"dom zmin {R(lo[2])} zmax {R(hi[2])}")
+    L.append(f"dom boundary_condition {sysm.bc[0]} {sysm.bc[1]} {sysm.bc[2]}")
+    L.append("dom generate")
+    L.append("atom_data basic adata")
+    L.append(f"adata ghost_cutoff {R(sysm.ghost_cutoff)}")
+    L.append("adata k_b 1.0")
+    L.append("adata set_domain dom")
+    for t in range(sysm.ntypes):
+        L.append(f"adata add_type_mass {t} {R(sysm.mass[t])}")
+        L.append(f"adata add_type_charge {t} {R(sysm.charge[t])}")
+    L.append(f'adata add_xyz_data_file file_name "{xyz}" read_velocity')
+    nl = sysm.neighbor
+    ffs = []
+    if nl is not None:
+        L.append("neighborlist basic neigh")
+        L.append("neigh set_atom_data adata")
+        L.append("neigh set_domain dom")
+        L.append(f"neigh cutoff {R(nl.cutoff)}")
+        L.append(f"neigh cutoff_extra_coef {R(nl.cutoff_extra_coef)}")
+        if nl.from_cells:
+            L.append("neigh make_verlet_list_from_cell_list")
+        if nl.build_cells:
+            L.append("neigh build_cell_list")
+        if nl.all_atom_test:
+            L.append("neigh all_atom_test")
+        if nl.rebuild_test:
+            L.append("neigh rebuild_test")
+    nt = sysm.ntypes
+    if sysm.lj is not None:
+        L.append("force_field lj_cell_list f_lj")
+        L.append(f"f_lj cutoff {R(sysm.lj.cutoff)}")
+        for i in range(nt):
+            for j in range(nt):
+                L.append(f"f_lj epsilon {i} {j} {R(float(sysm.lj.epsilon[i][j]))}")
+                L.append(f"f_lj sigma {i} {j} {R(float(sysm.lj.sigma[i][j]))}")
+        L.append("f_lj set_neighborlist neigh")
+        L.append("f_lj set_atom_data adata")
+        ffs.append("f_lj")
+    if sysm.ewald_r is not None:
+        e = sysm.ewald_r
+        L.append("force_field electrostatic_ewald_r f_er")
+        L.append("f_er set_atom_data adata")
+        L.append("f_er set_neighborlist neigh")
+        L.append(f"f_er alpha {R(e.alpha)}")
+        L.append(f"f_er k_electrostatic {R(e.k_electrostatic)}")
+        if e.lambda_ is not None:
+            for i in range(nt):
+                for j in range(nt):
+                    L.append(f"f_er lambda {i} {j} {R(float(e.lambda_[i][j]))}")
+        ffs.append("f_er")
+    if sysm.ewald_k is not None:
+        e = sysm.ewald_k
+        L.append("force_field electrostatic_ewald_k f_ek")
+        L.append("f_ek set_atom_data adata")
+        L.append("f_ek set_domain dom")
+        L.append(f"f_ek alpha {R(e.alpha)}")
+        L.append(f"f_ek k_electrostatic {R(e.k_electrostatic)}")
+        L.append(f"f_ek kx_max {e.kx_max}")
+        L.append(f"f_ek ky_max {e.ky_max}")
+        L.append(f"f_ek kz_max {e.kz_max}")
+        if e.epsilon_dipole is not None:
+            L.append(f"f_ek epsilon_dipole {R(e.epsilon_dipole)}")
+        ffs.append("f_ek")
+    L.append("md_simulator basic sim")
+    L.append("sim set_atom_data adata")
+    for f in ffs:
+        L.append(f"sim add_force_field {f}")
+    if nl is not None:
+        L.append("sim add_neighborlist neigh")
+    L.append("sim initial_step 0")
+    L.append(f"sim final_step {run_steps}")
+    L.append(f"sim dt {R(dt)}")
+    if run_steps:
+        L.append("sim run")
+    return "\n".join(L) + "\n"
+
+
+def write_xyz(path: str, sysm: System) -> None:
+    with open(path, "w") as f:
+        f.write(f"{sysm.n}\nfixture\n")
+        for t, p, v in zip(sysm.types, sysm.pos, sysm.vel):
+            f.write(f"{t} {R(p[0])} {R(p[1])} {R(p[2])} {R(v[0])} {R(v[1])} {R(v[2])}\n")
+
+
+def parse(out: str) -> dict:
+    toks = open(out).read().split("\n")
+    res = {}
+    k = 0
+    while k < len(toks):
+        line = toks[k].strip()
+        k += 1
+        if not line:
+            continue
+        head = line.split()
+        tag = head[0]
+        if tag in ("pos", "vel", "acc", "ghost_pos"):
+            n = int(head[1])
+            res[tag] = np.array([[float(x) for x in toks[k + i].split()] for i in range(n)]).reshape(n, 3)
+            k += n
+        elif tag in ("ghost_id",):
+            n = int(head[1])
+            res[tag] = np.array([int(toks[k + i]) for i in range(n)], dtype=np.int32)
+            k += n
+        elif tag == "pairs":
+            n = int(head[1])
+            res[tag] = np.array([[int(x) for x in toks[k + i].split()] for i in range(n)],
+                                dtype=np.int32).reshape(n, 2)
+            k += n
+        elif tag == "sk":
+            n = int(head[1])
+            a = np.array([[float(x) for x in toks[k + i].split()] for i in range(n)]).reshape(n, 2)
+            res[tag] = a
+            k += n
+        elif tag in ("field_r", "field_k"):
+            n = int(head[1])
+            res[tag] = np.array([[float(x) for x in toks[k + i].split()] for i in range(n)]).reshape(n, 4)
+            k += n
+        elif tag == "no_bins":
+            res[tag] = np.array([int(x) for x in head[1:4]], dtype=np.int32)
+        elif tag == "N":
+            res["N"] = int(head[1])
+        else:
+            res[tag] = float(head[1])
+    return res
+
+
+def settings_json(s: System) -> str:
+    d = {"lower": s.lower.tolist(), "upper": s.upper.tolist(), "bc": s.bc.tolist(),
+         "ghost_cutoff": s.ghost_cutoff, "mass": s.mass.tolist(), "charge": s.charge.tolist()}
+    if s.neighbor:
+        d["neighbor"] = vars(s.neighbor)
+    if s.lj:
+        d["lj"] = {"cutoff": s.lj.cutoff, "epsilon": np.asarray(s.lj.epsilon).tolist(),
+                   "sigma": np.asarray(s.lj.sigma).tolist()}
+    if s.ewald_r:
+        d["ewald_r"] = {"alpha": s.ewald_r.alpha, "k_electrostatic": s.ewald_r.k_electrostatic,
+                        "lambda_": None if s.ewald_r.lambda_ is None else
+                        np.asarray(s.ewald_r.lambda_).tolist()}
+    if s.ewald_k:
+        d["ewald_k"] = vars(s.ewald_k)
+    return json.dumps(d)
+
+
+def run_case(name: str, sysm: System, mode: str, run_steps: int = 0, dt: float = 0.001) -> None:
+    with tempfile.TemporaryDirectory() as td:
+        xyz = os.path.join(td, "in.xyz")
+        write_xyz(xyz, sysm)
+        casl = os.path.join(td, "in.casl")
+        with open(casl, "w") as f:
+            f.write(casl_for(sysm, "in.xyz", run_steps, dt))
+        out = os.path.join(td, "out.txt")
+        r = subprocess.run([HARNESS, casl, mode, out], cwd=td, capture_output=True, text=True)
+        if r.returncode != 0 or not os.path.exists(out):
+            print(r.stdout[-2000:], r.stderr[-2000:])
+            raise RuntimeError(f"harness failed for {name}")
+        res = parse(out)
+    arrays = {f"in_{k}": v for k, v in (("pos", sysm.pos), ("vel", sysm.vel),
+                                         ("types", sysm.types))}
+    arrays["settings"] = np.array(settings_json(sysm))
+    arrays["mode"] = np.array(mode)
+    arrays["run_steps"] = np.array(run_steps)
+    arrays["dt"] = np.array(dt)
+    for k, v in res.items():
+        arrays[f"ref_{k}"] = np.asarray(v)
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **arrays)
+    print(f"{name}: n={sysm.n} ghosts={len(res.get('ghost_pos', []))} "
+          f"pairs={len(res.get('pairs', []))} -> {os.path.getsize(path)} B")
+
+
+def b2_input():
+    txt = open(os.path.join(B2, "espressopp", "ewald.espressopp")).read()
+    parts = re.findall(r"\{(\d+) ([-\d.e]+) ([-\d.e]+) ([-\d.e]+) ([-\d.e]+) (\d+)\}", txt)
+    pos = np.array([[float(p[1]), float(p[2]), float(p[3])] for p in parts])
+    q = np.array([float(p[4]) for p in parts])
+    exp = open(os.path.join(B2, "caviar", "ewald-n100-expected-result.casl")).read()
+    energy = float(re.search(r'total_energy:"\s*echo\s*"([-\d.e]+)"', exp).group(1))
+    forces = np.array([[float(x) for x in m.split()] for m in
+                       re.findall(r'echo\s*"\{\s*([-\d.e\s]+?)\s*\}\s*"', exp)])
+    return pos, q, energy, forces
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        raise SystemExit("build oracle/_ref/ref_harness first: make -f oracle/ref.mk -j8")
+    # 1. The reference's own known-answer fixture (benchmark/b2-ewald), kept as
+    #    published data next to the reference's run on the same input.
+    pos, q, e_pub, f_pub = b2_input()
+    np.savez_compressed(os.path.join(HERE, "b2_published.npz"), pos=pos, q=q, energy=e_pub,
+                        forces=f_pub)
+    b2 = configs.b2_ewald(pos, q, kmax=25)
+    run_case("b2_setup", b2, "setup+er_energy+ek_energy+field")
+
+    # 2. Plain LJ fcc (the LJ workload's recipe at 4x4x4 cells), setup and 30 steps.
+    lj = configs.lj_fcc(ncell=4, seed=3)
+    run_case("lj256_setup", lj, "setup")
+    run_case("lj256_run30", lj, "run", run_steps=30, dt=0.005)
+    # 3. The full LJ workload (N=4000), setup only: pair list, ghosts, forces.
+    run_case("lj4000_setup", configs.lj_fcc(ncell=10, seed=1), "setup")
+
+    # 4. Charged LJ + Ewald at reduced N, cell list and brute-force list.
+    ion = configs.ions(n=600, kmax=5, seed=11)
+    run_case("ions600_setup", ion, "setup+er_energy+ek_energy+sk+field")
+    run_case("ions600_run20", ion, "run", run_steps=20, dt=0.002)
+    ionb = ion.copy()
+    ionb.neighbor.from_cells = False
+    ionb.neighbor.build_cells = True
+    run_case("ions600_brute_setup", ionb, "setup")
+    # the ion workload recipe at N=2000 (kmax 8)
+    run_case("ions2000_setup", configs.ions(n=2000, kmax=8, seed=5), "setup+er_energy+ek_energy")
+
+    # 5. Non-periodic z and an anisotropic box with unequal kmax per axis.
+    aniso = configs.ions(n=400, kmax=4, seed=21)
+    aniso.upper = aniso.upper * np.array([1.0, 1.3, 0.8])
+    aniso.pos = aniso.pos * np.array([1.0, 1.3, 0.8])
+    aniso.bc = np.array([1, 1, 0], dtype=np.int32)
+    aniso.ewald_k.kx_max, aniso.ewald_k.ky_max, aniso.ewald_k.kz_max = 4, 3, 5
+    run_case("aniso400_setup", aniso, "setup+er_energy+ek_energy")
+
+    # 6. A small dense box (L < 2 cutoff_extra): many images per atom.
+    small = configs.ions(n=200, rho=0.8, kmax=3, seed=31, dmin=0.7)
+    run_case("small200_setup", small, "setup+ek_energy")
+    run_case("small200_run10", small, "run", run_steps=10, dt=0.002)
+
+    # 7. Ewald_r lambda (all atoms share molecule index -1) and the Ewald_k dipole term.
+    lam = configs.ions(n=300, kmax=3, seed=41)
+    lam.ewald_r.lambda_ = np.array([[0.5, 1.0], [1.0, 0.0]])
+    lam.ewald_k.epsilon_dipole = 2.0
+    run_case("lambda_dipole300_setup", lam, "setup+ek_energy")
+
+    # 8. all_atom_test list mode and rebuild_test.
+    aat = configs.ions(n=150, kmax=2, seed=51)
+    aat.neighbor.all_atom_test = True
+    run_case("alltest150_setup", aat, "setup")
+    rbt = configs.ions(n=300, kmax=3, seed=61)
+    rbt.neighbor.rebuild_test = True
+    run_case("rebuild300_run10", rbt, "run", run_steps=10, dt=0.002)
+
+
+if __name__ == "__main__":
+    main()

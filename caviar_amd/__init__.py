@@ -1,0 +1,269 @@
+"""caviar_amd: MI355X-native short-range + Ewald force path of CAVIAR.
+
+The compute path is the HIP library ``caviar_amd/libcaviar_amd.so`` (gfx950),
+reached through its C ABI (``include/caviar_amd.h``) with ctypes.  There is no
+CPU fallback: importing works without a GPU, but creating an :class:`Engine`
+fails loudly when the library or a device is missing.
+
+The Python classes mirror the reference's objects for this path (see
+``caviar_amd.objects`` for the CASL-facing names); :class:`Engine` is one
+device-resident single-domain system.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .system import (EwaldKSettings, EwaldRSettings, LJSettings, NeighborSettings,  # noqa: F401
+                     System)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcaviar_amd.so")
+
+CAV_FP64 = 0
+CAV_MIXED = 1
+_ARRAYS = {"pos": 0, "vel": 1, "acc": 2, "acc_old": 3}
+_ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "call out of order",
+           4: "NaN acceleration", 5: "capacity"}
+
+_lib = None
+
+
+class CaviarError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"caviar_amd error {code} ({_ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class NaNAccelerationError(CaviarError):
+    pass
+
+
+def load_library():
+    """Load libcaviar_amd.so (built in-tree by ``caviar_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m caviar_amd.build` "
+                           "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    D = ctypes.c_double
+    L = ctypes.c_long
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "cav_create": [I, ctypes.POINTER(P)],
+        "cav_destroy": [P],
+        "cav_set_precision": [P, I],
+        "cav_set_box": [P, dp, dp, ip],
+        "cav_set_ghost_cutoff": [P, D],
+        "cav_set_types": [P, I, dp, dp],
+        "cav_set_neighbor": [P, D, D, I, I, I, I],
+        "cav_set_lj": [P, D, dp, dp],
+        "cav_set_ewald_r": [P, D, D, dp],
+        "cav_set_ewald_k": [P, D, D, I, I, I, I, D, I, D],
+        "cav_upload_atoms": [P, I, ip, ip, ip, dp, dp],
+        "cav_download": [P, I, dp],
+        "cav_upload": [P, I, dp],
+        "cav_num_atoms": [P, ip],
+        "cav_setup": [P],
+        "cav_step": [P, D, I],
+        "cav_recalculate": [P, I],
+        "cav_neighbor_build": [P, I, ip],
+        "cav_num_ghosts": [P, ip],
+        "cav_get_ghosts": [P, dp, ip],
+        "cav_num_pairs": [P, ctypes.POINTER(L)],
+        "cav_get_pairs": [P, ip, L, ctypes.POINTER(L)],
+        "cav_virial": [P, dp],
+        "cav_num_kvectors": [P, ip],
+        "cav_get_sk": [P, dp],
+        "cav_energy": [P, dp, dp, dp],
+        "cav_point_field": [P, I, I, dp, dp, dp],
+        "cav_list_stats": [P, dp, dp],
+        "cav_kernel_time": [P, ctypes.c_char_p, dp, ip],
+        "cav_enable_timing": [P, I],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = I
+    lib.cav_last_error.argtypes = [P]
+    lib.cav_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list:
+    """Names of the C-ABI entry points declared in include/caviar_amd.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "caviar_amd.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(cav_[a-z_]+)\s*\(", txt)))
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Engine:
+    """One single-domain CAVIAR system resident on one GPU.
+
+    Mirrors the reference's Md_simulator + Atom_data + Neighborlist + force-field
+    objects for the LJ / Ewald path (md_simulator.cpp:361-617).
+    """
+
+    def __init__(self, system: System, precision: str = "fp64", device: int = 0,
+                 timing: bool = False):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.cav_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise CaviarError(rc, "cav_create failed (no usable HIP device?)")
+        self.h = h
+        self.system = system
+        self.n = system.n
+        self._check(self.lib.cav_set_precision(self.h, CAV_FP64 if precision == "fp64" else CAV_MIXED))
+        self.precision = precision
+        s = system
+        self._check(self.lib.cav_set_box(self.h, _dp(s.lower.copy()), _dp(s.upper.copy()), _ip(s.bc.copy())))
+        self._check(self.lib.cav_set_ghost_cutoff(self.h, float(s.ghost_cutoff)))
+        self._check(self.lib.cav_set_types(self.h, s.ntypes, _dp(s.mass.copy()), _dp(s.charge.copy())))
+        nt = s.ntypes
+        if s.neighbor is not None:
+            nl = s.neighbor
+            self._check(self.lib.cav_set_neighbor(self.h, nl.cutoff, nl.cutoff_extra_coef,
+                                                  int(nl.from_cells), int(nl.build_cells),
+                                                  int(nl.all_atom_test), int(nl.rebuild_test)))
+        if s.lj is not None:
+            eps = np.ascontiguousarray(s.lj.epsilon, dtype=np.float64).reshape(nt * nt)
+            sig = np.ascontiguousarray(s.lj.sigma, dtype=np.float64).reshape(nt * nt)
+            self._check(self.lib.cav_set_lj(self.h, s.lj.cutoff, _dp(eps), _dp(sig)))
+        if s.ewald_r is not None:
+            lam = s.ewald_r.lambda_
+            lam = None if lam is None else np.ascontiguousarray(lam, dtype=np.float64).reshape(nt * nt)
+            self._check(self.lib.cav_set_ewald_r(self.h, s.ewald_r.alpha, s.ewald_r.k_electrostatic,
+                                                 _dp(lam)))
+        if s.ewald_k is not None:
+            e = s.ewald_k
+            self._check(self.lib.cav_set_ewald_k(
+                self.h, e.alpha, e.k_electrostatic, e.kx_max, e.ky_max, e.kz_max,
+                int(e.epsilon_dipole is not None), float(e.epsilon_dipole or 0.0),
+                -1 if e.slab_normal_axis is None else int(e.slab_normal_axis), float(e.slab_lz)))
+        ids = np.ascontiguousarray(s.ids, dtype=np.int32)
+        typ = np.ascontiguousarray(s.types, dtype=np.int32)
+        mol = None if s.mol is None else np.ascontiguousarray(s.mol, dtype=np.int32)
+        pos = np.ascontiguousarray(s.pos, dtype=np.float64)
+        vel = np.ascontiguousarray(s.vel, dtype=np.float64)
+        self._check(self.lib.cav_upload_atoms(self.h, s.n, _ip(ids), _ip(typ), _ip(mol), _dp(pos), _dp(vel)))
+        if timing:
+            self._check(self.lib.cav_enable_timing(self.h, 1))
+
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self.lib.cav_last_error(self.h).decode()
+            if rc == 4:
+                raise NaNAccelerationError(rc, msg)
+            raise CaviarError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cav_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- MD path
+    def setup(self):
+        self._check(self.lib.cav_setup(self.h))
+
+    def step(self, dt: float, nsteps: int = 1):
+        self._check(self.lib.cav_step(self.h, float(dt), int(nsteps)))
+
+    def recalculate(self, force_rebuild: bool = False):
+        self._check(self.lib.cav_recalculate(self.h, int(force_rebuild)))
+
+    def neighbor_build(self, update: bool) -> bool:
+        r = ctypes.c_int()
+        self._check(self.lib.cav_neighbor_build(self.h, int(update), ctypes.byref(r)))
+        return bool(r.value)
+
+    # ---- state
+    def get(self, what: str) -> np.ndarray:
+        out = np.empty((self.n, 3), dtype=np.float64)
+        self._check(self.lib.cav_download(self.h, _ARRAYS[what], _dp(out)))
+        return out
+
+    def set(self, what: str, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.n, 3)
+        self._check(self.lib.cav_upload(self.h, _ARRAYS[what], _dp(a)))
+
+    def ghosts(self):
+        g = ctypes.c_int()
+        self._check(self.lib.cav_num_ghosts(self.h, ctypes.byref(g)))
+        gp = np.empty((g.value, 3))
+        gs = np.empty(g.value, dtype=np.int32)
+        if g.value:
+            self._check(self.lib.cav_get_ghosts(self.h, _dp(gp), _ip(gs)))
+        return gp, gs
+
+    def pairs(self) -> np.ndarray:
+        npairs = ctypes.c_long()
+        self._check(self.lib.cav_num_pairs(self.h, ctypes.byref(npairs)))
+        out = np.empty((max(npairs.value, 1), 2), dtype=np.int32)
+        got = ctypes.c_long()
+        self._check(self.lib.cav_get_pairs(self.h, _ip(out), npairs.value, ctypes.byref(got)))
+        return out[:got.value]
+
+    def virial(self) -> float:
+        v = ctypes.c_double()
+        self._check(self.lib.cav_virial(self.h, ctypes.byref(v)))
+        return v.value
+
+    def sk(self) -> np.ndarray:
+        nk = ctypes.c_int()
+        self._check(self.lib.cav_num_kvectors(self.h, ctypes.byref(nk)))
+        out = np.empty((nk.value, 2))
+        self._check(self.lib.cav_get_sk(self.h, _dp(out)))
+        return out[:, 0] + 1j * out[:, 1]
+
+    def energy(self, lj=True, r=True, k=True):
+        vals = [ctypes.c_double() if f else None for f in (lj, r, k)]
+        args = [ctypes.byref(v) if v is not None else None for v in vals]
+        self._check(self.lib.cav_energy(self.h, *args))
+        return tuple(v.value if v is not None else None for v in vals)
+
+    def point_field(self, which: str, pts):
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        f = np.empty_like(pts)
+        p = np.empty(len(pts))
+        self._check(self.lib.cav_point_field(self.h, {"r": 0, "k": 1}[which], len(pts), _dp(pts),
+                                             _dp(f), _dp(p)))
+        return f, p
+
+    def list_stats(self):
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        self._check(self.lib.cav_list_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def kernel_time(self, name: str):
+        ms = ctypes.c_double()
+        nl = ctypes.c_int()
+        self._check(self.lib.cav_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(nl)))
+        return ms.value, nl.value
+
+    def enable_timing(self, on: bool = True):
+        self._check(self.lib.cav_enable_timing(self.h, int(on)))

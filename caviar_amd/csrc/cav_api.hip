@@ -1,0 +1,969 @@
+// C ABI of caviar_amd: context lifetime, settings, upload/download, and the
+// velocity-Verlet step driver (all device work on the context's one stream).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "cav_internal.hpp"
+
+namespace cav {
+
+int fail(cav_ctx *c, int code, const std::string &msg)
+{
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_check(cav_ctx *c, hipError_t e, const char *what)
+{
+  return fail(c, CAV_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void timer_begin(cav_ctx *c, Timer &t)
+{
+  if (!c->timing) return;
+  if (!t.a) {
+    (void)hipEventCreate(&t.a);
+    (void)hipEventCreate(&t.b);
+  }
+  (void)hipEventRecord(t.a, c->stream);
+}
+
+void timer_end(cav_ctx *c, Timer &t)
+{
+  if (!c->timing) return;
+  (void)hipEventRecord(t.b, c->stream);
+  t.pending = true;
+}
+
+static void timer_harvest(Timer &t)
+{
+  if (!t.pending) return;
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+    t.total_ms += ms;
+    t.launches += 1;
+  }
+  t.pending = false;
+}
+
+// ------------------------------------------------------- integration kernels --
+
+// integrate_velocity_verlet drift (md_simulator.cpp:584-594) with the PBC wrap
+// of exchange_owned fused in (domain.cpp:418-468).  Every operation is
+// rounded as the reference's expression is: r += (v*dt + ((0.5*a)*dt)*dt).
+__global__ void k_drift(int n, double dt, const double *lo, const double *hi, const double *L,
+                        int bx, int by, int bz, double *x, double *y, double *z, const double *vx,
+                        const double *vy, const double *vz, const double *ax, const double *ay,
+                        const double *az, double *aox, double *aoy, double *aoz, int *flags)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double a[3] = {ax[i], ay[i], az[i]};
+  if (isnan(a[0]) || isnan(a[1]) || isnan(a[2])) flags[F_NAN] = 1;
+  const double v[3] = {vx[i], vy[i], vz[i]};
+  double *p[3] = {x + i, y + i, z + i};
+  const int b[3] = {bx, by, bz};
+  bool wrapped = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double h = __dmul_rn(__dmul_rn(__dmul_rn(0.5, a[d]), dt), dt);
+    double r = __dadd_rn(*p[d], __dadd_rn(__dmul_rn(v[d], dt), h));
+    if (b[d] == 1) {
+      if (r < lo[d]) {
+        r = __dadd_rn(r, L[d]);
+        wrapped = true;
+      } else if (r > hi[d]) {
+        r = __dsub_rn(r, L[d]);
+        wrapped = true;
+      }
+    }
+    *p[d] = r;
+  }
+  aox[i] = a[0];
+  aoy[i] = a[1];
+  aoz[i] = a[2];
+  if (wrapped) {
+    flags[F_WRAPPED] = 1;
+    flags[F_REBUILD] = 1;
+  }
+}
+
+// exchange_owned_single_md_domain alone (setup and recalculate).
+__global__ void k_wrap(int n, const double *lo, const double *hi, const double *L, int bx, int by,
+                       int bz, double *x, double *y, double *z, int *flags)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double *p[3] = {x + i, y + i, z + i};
+  const int b[3] = {bx, by, bz};
+  bool wrapped = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (b[d] != 1) continue;
+    double r = *p[d];
+    if (r < lo[d]) {
+      r = __dadd_rn(r, L[d]);
+      wrapped = true;
+    } else if (r > hi[d]) {
+      r = __dsub_rn(r, L[d]);
+      wrapped = true;
+    }
+    *p[d] = r;
+  }
+  if (wrapped) {
+    flags[F_WRAPPED] = 1;
+    flags[F_REBUILD] = 1;
+  }
+}
+
+// v += (0.5*dt) * (a + a_old)  (md_simulator.cpp:610-617)
+__global__ void k_kick(int n, double hdt, double *vx, double *vy, double *vz, const double *ax,
+                       const double *ay, const double *az, const double *aox, const double *aoy,
+                       const double *aoz)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  vx[i] = __dadd_rn(vx[i], __dmul_rn(hdt, __dadd_rn(ax[i], aox[i])));
+  vy[i] = __dadd_rn(vy[i], __dmul_rn(hdt, __dadd_rn(ay[i], aoy[i])));
+  vz[i] = __dadd_rn(vz[i], __dmul_rn(hdt, __dadd_rn(az[i], aoz[i])));
+}
+
+__global__ void k_begin_step(int *flags)
+{
+  flags[F_WRAPPED] = 0;
+  flags[F_REBUILD] = flags[F_FORCE];
+  flags[F_FORCE] = 0;
+  flags[F_NAN] = 0;
+  flags[F_OVERFLOW] = 0;
+  flags[F_MAXNB] = 0;
+}
+
+__global__ void k_set_flag(int *flags, int idx, int v) { flags[idx] = v; }
+
+}  // namespace cav
+
+using namespace cav;
+
+// --------------------------------------------------------------- helpers --
+
+template <typename X>
+static int dmalloc(cav_ctx *c, X **p, size_t count)
+{
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  CAV_HIP(hipMalloc((void **)p, sizeof(X) * (count ? count : 1)));
+  return CAV_OK;
+}
+
+static void free_all(cav_ctx *c)
+{
+  void *ptrs[] = {c->x,       c->y,      c->z,         c->xo,       c->yo,       c->zo,
+                  c->vx,      c->vy,     c->vz,        c->ax,       c->ay,       c->az,
+                  c->aox,     c->aoy,    c->aoz,       c->snap,     c->type,     c->id,
+                  c->mol,     c->q,      c->gsrc,      c->flags,    c->blk,      c->bin_of,
+                  c->bin_cnt, c->bin_start, c->bin_items, c->d_st_off, c->d_st,  c->scan_tmp,
+                  c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
+                  c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
+                  c->d_W,     c->kacc};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->h_flags) (void)hipHostFree(c->h_flags);
+}
+
+// box arrays on the device (lo, hi, L) live in the scalar buffer tail
+static const int kBoxSlot = 16;
+
+static int upload_params(cav_ctx *c)
+{
+  const int nt = c->nt, nt2 = nt * nt;
+  CAV_TRY(dmalloc(c, &c->d_minv, nt));
+  CAV_TRY(dmalloc(c, &c->d_charge, nt));
+  CAV_HIP(hipMemcpyAsync(c->d_minv, c->minv.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->stream));
+  CAV_HIP(hipMemcpyAsync(c->d_charge, c->charge.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->stream));
+  // LJ table blocks: [c12 | c6 | cshift | u12 | u6 | uc], see the pair kernels
+  std::vector<double> lj(6 * (size_t)std::max(nt2, 1), 0.0);
+  if (c->lj_on) {
+    const double rc = c->lj_rc, rcsi = 1 / (rc * rc);
+    for (int p = 0; p < nt2; ++p) {
+      const double e = c->eps[p], s = c->sig[p];
+      const double s2 = s * s, s6 = s2 * s2 * s2, s12 = s6 * s6;
+      // lj_cell_list.cpp:145-155: 4e(-12 rho^12/r^2 + 6 rho^6/r^2 + 12 rho_c^12/rc^2 - 6 rho_c^6/rc^2)
+      const double rhoc2 = s * s * rcsi, rhoc6 = rhoc2 * rhoc2 * rhoc2, rhoc12 = rhoc6 * rhoc6;
+      lj[p] = 48 * e * s12;
+      lj[nt2 + p] = 24 * e * s6;
+      lj[2 * nt2 + p] = 4 * e * (12 * rhoc12 * rcsi - 6 * rhoc6 * rcsi);
+      lj[3 * nt2 + p] = 4 * e * s12;
+      lj[4 * nt2 + p] = 4 * e * s6;
+      const double rc2 = rc * rc, ri6 = 1 / (rc2 * rc2 * rc2);
+      lj[5 * nt2 + p] = 4 * e * s12 * ri6 * ri6 - 4 * e * s6 * ri6 + 0.5 * lj[2 * nt2 + p] * rc2;
+    }
+  }
+  CAV_TRY(dmalloc(c, &c->d_eps, lj.size()));
+  CAV_HIP(hipMemcpyAsync(c->d_eps, lj.data(), sizeof(double) * lj.size(), hipMemcpyHostToDevice, c->stream));
+  std::vector<double> lam(std::max(nt2, 1), 1.0);
+  if (c->lambda_set) lam = c->lambda;
+  CAV_TRY(dmalloc(c, &c->d_lambda, lam.size()));
+  CAV_HIP(hipMemcpyAsync(c->d_lambda, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice, c->stream));
+  return CAV_OK;
+}
+
+// Neighborlist::init + make_neigh_bin (neighborlist.cpp:313-406,581-697).
+static int neighbor_init(cav_ctx *c)
+{
+  if (!c->nl_on) return CAV_OK;
+  if (c->nl_cutoff <= 0.0) return fail(c, CAV_ERR_ARG, "neighborlist cutoff has to be positive");
+  if (c->nl_coef < 1.0) return fail(c, CAV_ERR_ARG, "cutoff_extra_coef must be >= 1");
+  c->rce = c->nl_cutoff * c->nl_coef;
+  c->rce2 = c->rce * c->rce;
+  c->thr_sq = 0.25 * (c->rce - c->nl_cutoff) * (c->rce - c->nl_cutoff);
+  c->cinv = 1.0 / c->rce;
+  long nbins = 1;
+  for (int d = 0; d < 3; ++d) {
+    c->nb[d] = (int)std::ceil(c->L[d] * c->cinv);
+    if (c->nb[d] < 1) c->nb[d] = 1;
+    nbins *= c->nb[d];
+  }
+  if (nbins > (1L << 26)) return fail(c, CAV_ERR_ARG, "cell grid too large");
+  c->nbins = (int)nbins;
+  const int nbx = c->nb[0], nby = c->nb[1], nbz = c->nb[2];
+  std::vector<int> cnt(c->nbins, 0), cand(27 * (size_t)c->nbins);
+  for (int x = 0; x < nbx; ++x)
+    for (int y = 0; y < nby; ++y)
+      for (int z = 0; z < nbz; ++z) {
+        int lo[3] = {x - 1, y - 1, z - 1}, hi[3] = {x + 1, y + 1, z + 1};
+        for (int d = 0; d < 3; ++d)
+          if (c->bc[d] == 0) {
+            if (lo[d] == -1) lo[d] = 0;
+            if (hi[d] == c->nb[d]) hi[d] = c->nb[d] - 1;
+          }
+        const int k = x + nbx * y + nbx * nby * z;
+        int *cc = cand.data() + 27 * (size_t)k;
+        for (int a = lo[0]; a <= hi[0]; ++a)
+          for (int b = lo[1]; b <= hi[1]; ++b)
+            for (int e = lo[2]; e <= hi[2]; ++e) {
+              int v[3] = {a, b, e};
+              for (int d = 0; d < 3; ++d) {
+                if (v[d] == c->nb[d]) v[d] = 0;
+                if (v[d] == -1) v[d] = c->nb[d] - 1;
+              }
+              const int lin = v[0] + nbx * v[1] + nbx * nby * v[2];
+              bool dup = false;
+              for (int q = 0; q < cnt[k]; ++q) dup |= cc[q] == lin;
+              if (!dup) cc[cnt[k]++] = lin;
+            }
+      }
+  c->st_off.assign(c->nbins + 1, 0);
+  c->st.clear();
+  for (int k = 0; k < c->nbins; ++k) {
+    c->st_off[k] = (int)c->st.size();
+    c->st.insert(c->st.end(), cand.begin() + 27 * (size_t)k, cand.begin() + 27 * (size_t)k + cnt[k]);
+  }
+  c->st_off[c->nbins] = (int)c->st.size();
+  CAV_TRY(dmalloc(c, &c->d_st_off, c->st_off.size()));
+  CAV_TRY(dmalloc(c, &c->d_st, c->st.size()));
+  CAV_HIP(hipMemcpyAsync(c->d_st_off, c->st_off.data(), sizeof(int) * c->st_off.size(), hipMemcpyHostToDevice, c->stream));
+  CAV_HIP(hipMemcpyAsync(c->d_st, c->st.data(), sizeof(int) * c->st.size(), hipMemcpyHostToDevice, c->stream));
+  CAV_TRY(dmalloc(c, &c->bin_cnt, c->nbins));
+  CAV_TRY(dmalloc(c, &c->bin_start, c->nbins + 1));
+  CAV_TRY(dmalloc(c, &c->scan_tmp, grid_for(c->nbins) + 1));
+  return CAV_OK;
+}
+
+static int alloc_list(cav_ctx *c, int cap)
+{
+  c->nl_cap = cap;
+  CAV_TRY(dmalloc(c, &c->nl, (size_t)cap * c->n));
+  return CAV_OK;
+}
+
+static int sync_flags(cav_ctx *c)
+{
+  CAV_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int) * F_COUNT, hipMemcpyDeviceToHost, c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  return CAV_OK;
+}
+
+static int box_dev(cav_ctx *c, const double **lo, const double **hi, const double **L)
+{
+  *lo = c->scal + kBoxSlot;
+  *hi = c->scal + kBoxSlot + 3;
+  *L = c->scal + kBoxSlot + 6;
+  return CAV_OK;
+}
+
+// The force phase: reset acceleration + every enabled field (md_simulator.cpp:370-373).
+static int forces(cav_ctx *c)
+{
+  const bool pair = (c->lj_on && !c->lj_cellwalk) || c->er_on;
+  if (pair)
+    CAV_TRY(launch_pairs(c));
+  else
+    CAV_TRY(launch_zero_acc(c));
+  if (c->lj_on && c->lj_cellwalk) CAV_TRY(launch_lj_cellwalk(c));
+  if (c->ek_on) CAV_TRY(launch_ewald_k(c));
+  return CAV_OK;
+}
+
+// Neighbour phase for one step: ghosts already made; rebuild decision, cells,
+// list, old positions.  `forced` requests an unconditional rebuild.
+static int neighbor_phase(cav_ctx *c, bool forced)
+{
+  if (!c->nl_on) return CAV_OK;
+  if (forced) {
+    k_set_flag<<<1, 1, 0, c->stream>>>(c->flags, F_REBUILD, 1);
+  } else {
+    CAV_TRY(launch_rebuild_check(c));
+  }
+  timer_begin(c, c->t_nb);
+  // cells are needed on rebuild steps, and every step by the LJ cell walk
+  CAV_TRY(launch_bins(c, !c->lj_cellwalk));
+  CAV_TRY(launch_list(c, true));
+  CAV_TRY(launch_save_old(c, true));
+  timer_end(c, c->t_nb);
+  return CAV_OK;
+}
+
+static int check_capacity(cav_ctx *c, bool *redo)
+{
+  *redo = false;
+  if (c->h_flags[F_OVERFLOW]) {
+    const int need = c->h_flags[F_MAXNB];
+    int cap = need + need / 4 + 16;
+    cap = (cap + 3) / 4 * 4;
+    CAV_TRY(alloc_list(c, cap));
+    *redo = true;
+  }
+  return CAV_OK;
+}
+
+static int snapshot(cav_ctx *c, bool save)
+{
+  const size_t b = sizeof(double) * c->n;
+  double *arr[9] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az};
+  for (int k = 0; k < 9; ++k) {
+    if (save)
+      CAV_HIP(hipMemcpyAsync(c->snap + (size_t)k * c->n, arr[k], b, hipMemcpyDeviceToDevice, c->stream));
+    else
+      CAV_HIP(hipMemcpyAsync(arr[k], c->snap + (size_t)k * c->n, b, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return CAV_OK;
+}
+
+static void harvest_timers(cav_ctx *c)
+{
+  if (!c->timing) return;
+  timer_harvest(c->t_pair);
+  timer_harvest(c->t_sk);
+  timer_harvest(c->t_force);
+  timer_harvest(c->t_tab);
+  timer_harvest(c->t_nb);
+  timer_harvest(c->t_int);
+}
+
+// ------------------------------------------------------------------- C ABI --
+
+extern "C" {
+
+int cav_create(int device, cav_ctx **out)
+{
+  if (!out) return CAV_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CAV_ERR_HIP;
+  if (device < 0 || device >= ndev) return CAV_ERR_ARG;
+  cav_ctx *c = new cav_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return CAV_ERR_HIP;
+  }
+  if (hipMalloc(&c->flags, sizeof(int) * F_COUNT) != hipSuccess ||
+      hipHostMalloc(&c->h_flags, sizeof(int) * F_COUNT) != hipSuccess ||
+      hipMalloc(&c->scal, sizeof(double) * 32) != hipSuccess) {
+    delete c;
+    return CAV_ERR_HIP;
+  }
+  (void)hipMemset(c->flags, 0, sizeof(int) * F_COUNT);
+  (void)hipMemset(c->scal, 0, sizeof(double) * 32);
+  *out = c;
+  return CAV_OK;
+}
+
+int cav_destroy(cav_ctx *c)
+{
+  if (!c) return CAV_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CAV_OK;
+}
+
+const char *cav_last_error(const cav_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int cav_set_precision(cav_ctx *c, int p)
+{
+  if (!c || (p != CAV_FP64 && p != CAV_MIXED)) return fail(c, CAV_ERR_ARG, "precision must be CAV_FP64 or CAV_MIXED");
+  c->precision = p;
+  c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_set_box(cav_ctx *c, const double lower[3], const double upper[3], const int bc[3])
+{
+  if (!c || !lower || !upper || !bc) return fail(c, CAV_ERR_ARG, "null argument");
+  for (int d = 0; d < 3; ++d) {
+    if (!(upper[d] > lower[d])) return fail(c, CAV_ERR_ARG, "box upper must exceed lower");
+    if (bc[d] != 0 && bc[d] != 1) return fail(c, CAV_ERR_ARG, "boundary_condition must be 0 or 1");
+    c->lo[d] = lower[d];
+    c->hi[d] = upper[d];
+    c->L[d] = upper[d] - lower[d];  // domain.cpp:470-477
+    c->bc[d] = bc[d];
+  }
+  c->box_set = true;
+  c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_set_ghost_cutoff(cav_ctx *c, double gc)
+{
+  if (!c || gc < 0) return fail(c, CAV_ERR_ARG, "ghost_cutoff have to non-negative.");
+  c->ghost_cutoff = gc;
+  c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_set_types(cav_ctx *c, int nt, const double *mass, const double *charge)
+{
+  if (!c || nt <= 0 || !mass || !charge) return fail(c, CAV_ERR_ARG, "types need mass and charge");
+  c->nt = nt;
+  c->mass.assign(mass, mass + nt);
+  c->charge.assign(charge, charge + nt);
+  c->minv.resize(nt);
+  for (int t = 0; t < nt; ++t) c->minv[t] = mass[t] == 0.0 ? 0.0 : 1.0 / mass[t];  // atom_data.cpp:236-239
+  c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_set_neighbor(cav_ctx *c, double cutoff, double coef, int from_cells, int build_cells,
+                     int all_atom_test, int rebuild_test)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (cutoff <= 0.0) return fail(c, CAV_ERR_ARG, "cutoff have to non-negative.");
+  if (coef < 1.0) return fail(c, CAV_ERR_ARG, "cutoff_extra_coef must be larger than 1.0");
+  c->nl_on = true;
+  c->nl_cutoff = cutoff;
+  c->nl_coef = coef;
+  c->from_cells = from_cells;
+  c->build_cells = build_cells || from_cells;  // neighborlist.cpp:216-220
+  c->all_atom_test = all_atom_test;
+  c->rebuild_test = rebuild_test;
+  c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_set_lj(cav_ctx *c, double rc, const double *eps, const double *sigma)
+{
+  if (!c || !eps || !sigma || c->nt <= 0) return fail(c, CAV_ERR_ARG, "set types before lj");
+  if (rc < 0) return fail(c, CAV_ERR_ARG, "Force field cutoff have to non-negative.");
+  const int nt2 = c->nt * c->nt;
+  for (int p = 0; p < nt2; ++p)
+    if (eps[p] < 0 || sigma[p] < 0) return fail(c, CAV_ERR_ARG, "Epsilon and sigma have to be non-negative.");
+  c->lj_on = true;
+  c->lj_rc = rc;
+  c->eps.assign(eps, eps + nt2);
+  c->sig.assign(sigma, sigma + nt2);
+  c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_set_ewald_r(cav_ctx *c, double alpha, double ke, const double *lambda)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (alpha < 0) return fail(c, CAV_ERR_ARG, "alpha have to non-negative.");
+  if (ke < 0) return fail(c, CAV_ERR_ARG, "k_electrostatic has to be non-negative.");
+  c->er_on = true;
+  c->er_alpha = alpha;
+  c->er_ke = ke;
+  c->lambda_set = lambda != nullptr;
+  if (lambda) {
+    if (c->nt <= 0) return fail(c, CAV_ERR_ARG, "set types before lambda");
+    c->lambda.assign(lambda, lambda + c->nt * c->nt);
+  }
+  c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_set_ewald_k(cav_ctx *c, double alpha, double ke, int kx, int ky, int kz, int dipole,
+                    double eps_dipole, int slab_axis, double slab_lz)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (alpha <= 0) return fail(c, CAV_ERR_ARG, "Electrostatic_ewald_k needs a positive alpha");
+  if (kx < 0 || ky < 0 || kz < 0) return fail(c, CAV_ERR_ARG, "'k_max' have to non-negative.");
+  if (ke < 0) return fail(c, CAV_ERR_ARG, "'k_electrostatic' has to be non-negative.");
+  if (slab_axis < -1 || slab_axis > 2) return fail(c, CAV_ERR_ARG, "slab_normal_axis must be x, y or z");
+  c->ek_on = true;
+  c->ek_alpha = alpha;
+  c->ek_ke = ke;
+  c->kx = kx;
+  c->ky = ky;
+  c->kz = kz;
+  c->dipole = dipole;
+  c->eps_dipole = eps_dipole;
+  c->slab_axis = slab_axis;
+  c->slab_lz = slab_lz;
+  c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_upload_atoms(cav_ctx *c, int n, const int *id, const int *type, const int *mol,
+                     const double *pos, const double *vel)
+{
+  if (!c || n < 0 || (n && (!id || !type || !pos || !vel))) return fail(c, CAV_ERR_ARG, "null atom arrays");
+  if (c->nt <= 0) return fail(c, CAV_ERR_ARG, "set types before atoms");
+  for (int i = 0; i < n; ++i)
+    if (type[i] < 0 || type[i] >= c->nt) return fail(c, CAV_ERR_ARG, "atom type outside the declared types");
+  (void)hipSetDevice(c->device);
+  c->n = n;
+  c->gcap = 7 * n;  // at most seven images per owned atom
+  c->h_id.assign(id, id + n);
+  c->h_type.assign(type, type + n);
+  c->h_mol.assign(n, -1);
+  if (mol) c->h_mol.assign(mol, mol + n);
+  const size_t tot = (size_t)n + c->gcap;
+  double **pa[] = {&c->x, &c->y, &c->z, &c->xo, &c->yo, &c->zo, &c->q};
+  for (double **p : pa) CAV_TRY(dmalloc(c, p, tot));
+  double **oa[] = {&c->vx, &c->vy, &c->vz, &c->ax, &c->ay, &c->az, &c->aox, &c->aoy, &c->aoz};
+  for (double **p : oa) CAV_TRY(dmalloc(c, p, n));
+  CAV_TRY(dmalloc(c, &c->snap, 9 * (size_t)n));
+  CAV_TRY(dmalloc(c, &c->type, tot));
+  CAV_TRY(dmalloc(c, &c->id, tot));
+  CAV_TRY(dmalloc(c, &c->mol, tot));
+  CAV_TRY(dmalloc(c, &c->gsrc, c->gcap));
+  CAV_TRY(dmalloc(c, &c->bin_of, tot));
+  CAV_TRY(dmalloc(c, &c->bin_items, tot));
+  CAV_TRY(dmalloc(c, &c->nn, n));
+  c->nblk = grid_for(n) + 1;
+  CAV_TRY(dmalloc(c, &c->blk, c->nblk));
+  std::vector<double> soa(3 * (size_t)n), qv(n);
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) soa[(size_t)k * n + i] = pos[3 * i + k];
+    qv[i] = c->charge[type[i]];
+  }
+  const size_t b = sizeof(double) * n;
+  CAV_HIP(hipMemcpy(c->x, soa.data(), b, hipMemcpyHostToDevice));
+  CAV_HIP(hipMemcpy(c->y, soa.data() + n, b, hipMemcpyHostToDevice));
+  CAV_HIP(hipMemcpy(c->z, soa.data() + 2 * (size_t)n, b, hipMemcpyHostToDevice));
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) soa[(size_t)k * n + i] = vel[3 * i + k];
+  CAV_HIP(hipMemcpy(c->vx, soa.data(), b, hipMemcpyHostToDevice));
+  CAV_HIP(hipMemcpy(c->vy, soa.data() + n, b, hipMemcpyHostToDevice));
+  CAV_HIP(hipMemcpy(c->vz, soa.data() + 2 * (size_t)n, b, hipMemcpyHostToDevice));
+  CAV_HIP(hipMemset(c->ax, 0, b));
+  CAV_HIP(hipMemset(c->ay, 0, b));
+  CAV_HIP(hipMemset(c->az, 0, b));
+  CAV_HIP(hipMemset(c->aox, 0, b));
+  CAV_HIP(hipMemset(c->aoy, 0, b));
+  CAV_HIP(hipMemset(c->aoz, 0, b));
+  CAV_HIP(hipMemcpy(c->q, qv.data(), b, hipMemcpyHostToDevice));
+  CAV_HIP(hipMemcpy(c->type, type, sizeof(int) * n, hipMemcpyHostToDevice));
+  CAV_HIP(hipMemcpy(c->id, id, sizeof(int) * n, hipMemcpyHostToDevice));
+  CAV_HIP(hipMemcpy(c->mol, c->h_mol.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_num_atoms(const cav_ctx *c, int *n)
+{
+  if (!c || !n) return CAV_ERR_ARG;
+  *n = c->n;
+  return CAV_OK;
+}
+
+static double *soa_of(cav_ctx *c, int which, int k)
+{
+  double *a[4][3] = {{c->x, c->y, c->z}, {c->vx, c->vy, c->vz}, {c->ax, c->ay, c->az}, {c->aox, c->aoy, c->aoz}};
+  return a[which][k];
+}
+
+int cav_download(cav_ctx *c, int which, double *out)
+{
+  if (!c || !out || which < 0 || which > 3) return fail(c, CAV_ERR_ARG, "bad download request");
+  (void)hipSetDevice(c->device);
+  std::vector<double> soa(3 * (size_t)c->n);
+  for (int k = 0; k < 3; ++k)
+    CAV_HIP(hipMemcpyAsync(soa.data() + (size_t)k * c->n, soa_of(c, which, k), sizeof(double) * c->n,
+                           hipMemcpyDeviceToHost, c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->n; ++i)
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = soa[(size_t)k * c->n + i];
+  return CAV_OK;
+}
+
+int cav_upload(cav_ctx *c, int which, const double *in)
+{
+  if (!c || !in || which < 0 || which > 3) return fail(c, CAV_ERR_ARG, "bad upload request");
+  (void)hipSetDevice(c->device);
+  std::vector<double> soa(3 * (size_t)c->n);
+  for (int i = 0; i < c->n; ++i)
+    for (int k = 0; k < 3; ++k) soa[(size_t)k * c->n + i] = in[3 * i + k];
+  for (int k = 0; k < 3; ++k)
+    CAV_HIP(hipMemcpyAsync(soa_of(c, which, k), soa.data() + (size_t)k * c->n, sizeof(double) * c->n,
+                           hipMemcpyHostToDevice, c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  return CAV_OK;
+}
+
+static int wrap_and_ghosts(cav_ctx *c)
+{
+  const double *lo, *hi, *L;
+  box_dev(c, &lo, &hi, &L);
+  k_wrap<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, lo, hi, L, c->bc[0], c->bc[1], c->bc[2],
+                                                   c->x, c->y, c->z, c->flags);
+  CAV_HIP(hipGetLastError());
+  return launch_ghosts(c);
+}
+
+int cav_setup(cav_ctx *c)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->box_set) return fail(c, CAV_ERR_STATE, "set the box before setup");
+  if (c->nt <= 0) return fail(c, CAV_ERR_STATE, "set types before setup");
+  if ((c->lj_on || c->er_on) && !c->nl_on) return fail(c, CAV_ERR_STATE, "lj_cell_list / ewald_r need a neighborlist");
+  if (c->lj_on && !c->build_cells) return fail(c, CAV_ERR_STATE, "lj_cell_list needs build_cell_list");
+  (void)hipSetDevice(c->device);
+  // box on the device (lo, hi, L)
+  double box[9];
+  for (int d = 0; d < 3; ++d) {
+    box[d] = c->lo[d];
+    box[3 + d] = c->hi[d];
+    box[6 + d] = c->L[d];
+  }
+  CAV_HIP(hipMemcpy(c->scal + kBoxSlot, box, sizeof(box), hipMemcpyHostToDevice));
+  CAV_TRY(upload_params(c));
+  CAV_TRY(neighbor_init(c));
+  // LJ over the list needs rc_lj <= the list's own cutoff (the skin guarantee of
+  // the rebuild test); otherwise it walks the cells every step like the reference.
+  c->lj_cellwalk = c->lj_on && (c->lj_rc > c->nl_cutoff || c->all_atom_test);
+  const int npart = 3 * grid_for(std::max<long>((long)c->n, 1L << 16)) + 64;
+  CAV_TRY(dmalloc(c, &c->part, npart));
+  c->npart = npart;
+  if (c->ek_on) {
+    CAV_TRY(ewald_k_plan(c));
+    const int need = 3 * grid_for((long)c->ek.Rpad * c->ek.nZ) + 64;
+    if (need > c->npart) {
+      CAV_TRY(dmalloc(c, &c->part, need));
+      c->npart = need;
+    }
+    CAV_TRY(ewald_k_alloc(c));
+  }
+  // initial list capacity from the mean density, refined by the first build
+  if (c->nl_on) {
+    const double vol = c->L[0] * c->L[1] * c->L[2];
+    const double expect = 4.18879020478639 * c->rce * c->rce * c->rce * (c->n / vol);
+    int cap = (int)(2.0 * expect) + 32;
+    if (c->all_atom_test) cap = 8 * c->n + 8;
+    cap = (cap + 3) / 4 * 4;
+    CAV_TRY(alloc_list(c, cap));
+  }
+  k_set_flag<<<1, 1, 0, c->stream>>>(c->flags, F_NGHOST_OLD, -1);
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    k_begin_step<<<1, 1, 0, c->stream>>>(c->flags);
+    CAV_TRY(wrap_and_ghosts(c));
+    CAV_TRY(neighbor_phase(c, true));
+    CAV_TRY(forces(c));
+    CAV_TRY(sync_flags(c));
+    bool redo = false;
+    CAV_TRY(check_capacity(c, &redo));
+    if (!redo) break;
+  }
+  if (c->h_flags[F_OVERFLOW]) return fail(c, CAV_ERR_CAPACITY, "Verlet list capacity could not be met");
+  harvest_timers(c);
+  c->setup_done = true;
+  c->list_valid = c->nl_on;
+  return CAV_OK;
+}
+
+int cav_step(cav_ctx *c, double dt, int nsteps)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run before cav_step");
+  (void)hipSetDevice(c->device);
+  const double *lo, *hi, *L;
+  box_dev(c, &lo, &hi, &L);
+  for (int s = 0; s < nsteps; ++s) {
+    bool redo = true;
+    for (int attempt = 0; redo && attempt < 4; ++attempt) {
+      CAV_TRY(snapshot(c, true));
+      k_begin_step<<<1, 1, 0, c->stream>>>(c->flags);
+      timer_begin(c, c->t_int);
+      k_drift<<<grid_for(c->n), kBlock, 0, c->stream>>>(
+          c->n, dt, lo, hi, L, c->bc[0], c->bc[1], c->bc[2], c->x, c->y, c->z, c->vx, c->vy, c->vz,
+          c->ax, c->ay, c->az, c->aox, c->aoy, c->aoz, c->flags);
+      timer_end(c, c->t_int);
+      CAV_HIP(hipGetLastError());
+      CAV_TRY(launch_ghosts(c));
+      CAV_TRY(neighbor_phase(c, false));
+      CAV_TRY(forces(c));
+      k_kick<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, 0.5 * dt, c->vx, c->vy, c->vz, c->ax,
+                                                       c->ay, c->az, c->aox, c->aoy, c->aoz);
+      CAV_HIP(hipGetLastError());
+      CAV_TRY(sync_flags(c));
+      harvest_timers(c);
+      if (c->h_flags[F_NAN]) {
+        CAV_TRY(snapshot(c, false));
+        CAV_HIP(hipStreamSynchronize(c->stream));
+        return fail(c, CAV_ERR_NAN, "NaN acceleration before the drift");
+      }
+      CAV_TRY(check_capacity(c, &redo));
+      if (redo) {
+        CAV_TRY(snapshot(c, false));
+        k_set_flag<<<1, 1, 0, c->stream>>>(c->flags, F_FORCE, 1);
+      }
+    }
+    if (redo) return fail(c, CAV_ERR_CAPACITY, "Verlet list capacity could not be met");
+  }
+  return CAV_OK;
+}
+
+int cav_recalculate(cav_ctx *c, int force_rebuild)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  (void)hipSetDevice(c->device);
+  k_begin_step<<<1, 1, 0, c->stream>>>(c->flags);
+  CAV_TRY(wrap_and_ghosts(c));
+  CAV_TRY(neighbor_phase(c, force_rebuild != 0));
+  CAV_TRY(forces(c));
+  CAV_TRY(sync_flags(c));
+  bool redo = false;
+  CAV_TRY(check_capacity(c, &redo));
+  if (redo) {
+    CAV_TRY(neighbor_phase(c, true));
+    CAV_TRY(forces(c));
+    CAV_TRY(sync_flags(c));
+  }
+  harvest_timers(c);
+  return CAV_OK;
+}
+
+int cav_neighbor_build(cav_ctx *c, int update, int *rebuilt)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  k_set_flag<<<1, 1, 0, c->stream>>>(c->flags, F_REBUILD, 0);
+  CAV_TRY(neighbor_phase(c, update != 0));
+  CAV_TRY(sync_flags(c));
+  if (rebuilt) *rebuilt = c->h_flags[F_REBUILD];
+  return CAV_OK;
+}
+
+int cav_num_ghosts(cav_ctx *c, int *g)
+{
+  if (!c || !g) return CAV_ERR_ARG;
+  CAV_TRY(sync_flags(c));
+  *g = c->h_flags[F_NGHOST];
+  return CAV_OK;
+}
+
+int cav_get_ghosts(cav_ctx *c, double *gpos, int *gsrc)
+{
+  if (!c) return CAV_ERR_ARG;
+  CAV_TRY(sync_flags(c));
+  const int G = c->h_flags[F_NGHOST];
+  std::vector<double> t(3 * (size_t)G);
+  double *arr[3] = {c->x, c->y, c->z};
+  for (int k = 0; k < 3; ++k)
+    CAV_HIP(hipMemcpyAsync(t.data() + (size_t)k * G, arr[k] + c->n, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  if (gsrc) CAV_HIP(hipMemcpyAsync(gsrc, c->gsrc, sizeof(int) * G, hipMemcpyDeviceToHost, c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  if (gpos)
+    for (int g = 0; g < G; ++g)
+      for (int k = 0; k < 3; ++k) gpos[3 * g + k] = t[(size_t)k * G + g];
+  return CAV_OK;
+}
+
+static int list_host(cav_ctx *c, std::vector<int> &nl, std::vector<int> &nn)
+{
+  nl.resize((size_t)c->nl_cap * c->n);
+  nn.resize(c->n);
+  CAV_HIP(hipMemcpyAsync(nl.data(), c->nl, sizeof(int) * nl.size(), hipMemcpyDeviceToHost, c->stream));
+  CAV_HIP(hipMemcpyAsync(nn.data(), c->nn, sizeof(int) * nn.size(), hipMemcpyDeviceToHost, c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  return CAV_OK;
+}
+
+int cav_num_pairs(cav_ctx *c, long *np)
+{
+  if (!c || !np) return CAV_ERR_ARG;
+  if (!c->nl_on || !c->setup_done) return fail(c, CAV_ERR_STATE, "no neighbour list");
+  std::vector<int> nl, nn;
+  CAV_TRY(list_host(c, nl, nn));
+  long cnt = 0;
+  for (int i = 0; i < c->n; ++i)
+    for (int k = 0; k < nn[i]; ++k) {
+      const int j = nl[(size_t)k * c->n + i];
+      cnt += (j >= c->n || j > i);
+    }
+  *np = cnt;
+  return CAV_OK;
+}
+
+int cav_get_pairs(cav_ctx *c, int *ij, long capacity, long *np)
+{
+  if (!c || !ij) return CAV_ERR_ARG;
+  if (!c->nl_on || !c->setup_done) return fail(c, CAV_ERR_STATE, "no neighbour list");
+  std::vector<int> nl, nn;
+  CAV_TRY(list_host(c, nl, nn));
+  long w = 0;
+  for (int i = 0; i < c->n; ++i)
+    for (int k = 0; k < nn[i]; ++k) {
+      const int j = nl[(size_t)k * c->n + i];
+      if (!(j >= c->n || j > i)) continue;
+      if (w < capacity) {
+        ij[2 * w] = i;
+        ij[2 * w + 1] = j;
+      }
+      ++w;
+    }
+  if (np) *np = w;
+  if (w > capacity) return fail(c, CAV_ERR_CAPACITY, "pair buffer too small");
+  return CAV_OK;
+}
+
+int cav_list_stats(cav_ctx *c, double *ppa, double *lj_ppa)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->nl_on || !c->setup_done) return fail(c, CAV_ERR_STATE, "no neighbour list");
+  std::vector<int> nl, nn;
+  CAV_TRY(list_host(c, nl, nn));
+  std::vector<double> x(c->n + (size_t)c->gcap), y(x.size()), z(x.size());
+  CAV_TRY(sync_flags(c));
+  const size_t tot = (size_t)c->n + c->h_flags[F_NGHOST];
+  CAV_HIP(hipMemcpy(x.data(), c->x, sizeof(double) * tot, hipMemcpyDeviceToHost));
+  CAV_HIP(hipMemcpy(y.data(), c->y, sizeof(double) * tot, hipMemcpyDeviceToHost));
+  CAV_HIP(hipMemcpy(z.data(), c->z, sizeof(double) * tot, hipMemcpyDeviceToHost));
+  long half = 0, lj = 0;
+  const double rc2 = c->lj_rc * c->lj_rc;
+  for (int i = 0; i < c->n; ++i)
+    for (int k = 0; k < nn[i]; ++k) {
+      const int j = nl[(size_t)k * c->n + i];
+      if (!(j >= c->n || j > i)) continue;
+      ++half;
+      const double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+      if (c->lj_on && dx * dx + dy * dy + dz * dz <= rc2) ++lj;
+    }
+  if (ppa) *ppa = c->n ? (double)half / c->n : 0;
+  if (lj_ppa) *lj_ppa = c->n ? (double)lj / c->n : 0;
+  return CAV_OK;
+}
+
+int cav_virial(cav_ctx *c, double *v)
+{
+  if (!c || !v) return CAV_ERR_ARG;
+  CAV_HIP(hipMemcpyAsync(v, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  return CAV_OK;
+}
+
+int cav_num_kvectors(cav_ctx *c, int *nk)
+{
+  if (!c || !nk) return CAV_ERR_ARG;
+  if (!c->ek_on || !c->setup_done) return fail(c, CAV_ERR_STATE, "no Ewald_k");
+  *nk = (int)c->ek.ref_r.size();
+  return CAV_OK;
+}
+
+int cav_get_sk(cav_ctx *c, double *out)
+{
+  if (!c || !out) return CAV_ERR_ARG;
+  if (!c->ek_on || !c->setup_done) return fail(c, CAV_ERR_STATE, "no Ewald_k");
+  const EwaldKPlan &p = c->ek;
+  std::vector<double> sk((size_t)p.Rpad * 2 * p.nZ);
+  CAV_HIP(hipMemcpyAsync(sk.data(), c->sk, sizeof(double) * sk.size(), hipMemcpyDeviceToHost, c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < p.ref_r.size(); ++k) {
+    const double *s = sk.data() + (size_t)p.ref_r[k] * 2 * p.nZ + 2 * p.ref_c[k];
+    out[2 * k] = s[0];
+    out[2 * k + 1] = p.ref_conj[k] ? -s[1] : s[1];
+  }
+  return CAV_OK;
+}
+
+int cav_energy(cav_ctx *c, double *e_lj, double *e_r, double *e_k)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  (void)hipSetDevice(c->device);
+  if (e_k) {
+    if (!c->ek_on) return fail(c, CAV_ERR_STATE, "no Ewald_k");
+    CAV_TRY(launch_energy_k(c, e_k));
+  }
+  if (e_r || e_lj) {
+    if (!c->nl_on) return fail(c, CAV_ERR_STATE, "no neighbour list");
+    // the cell walk needs cells of the current positions and ghosts
+    CAV_TRY(launch_bins(c, false));
+  }
+  if (e_r) {
+    if (!c->er_on) return fail(c, CAV_ERR_STATE, "no Ewald_r");
+    if (!c->build_cells) return fail(c, CAV_ERR_STATE, "Ewald_r energy walks the cell list: needs build_cell_list");
+    CAV_TRY(launch_energy_r(c, e_r));
+  }
+  if (e_lj) {
+    if (!c->lj_on) return fail(c, CAV_ERR_STATE, "no lj_cell_list");
+    CAV_TRY(launch_energy_lj(c, e_lj));
+  }
+  return CAV_OK;
+}
+
+int cav_point_field(cav_ctx *c, int which, int np, const double *pts, double *field, double *pot)
+{
+  if (!c || np < 0 || (np && !pts)) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  (void)hipSetDevice(c->device);
+  if (which == 0) {
+    if (!c->er_on || !c->build_cells) return fail(c, CAV_ERR_STATE, "Ewald_r point queries need build_cell_list");
+    CAV_TRY(launch_bins(c, false));
+    return point_field_r(c, np, pts, field, pot);
+  }
+  if (which == 1) {
+    if (!c->ek_on) return fail(c, CAV_ERR_STATE, "no Ewald_k");
+    return point_field_k(c, np, pts, field, pot);
+  }
+  return fail(c, CAV_ERR_ARG, "which must be 0 (ewald_r) or 1 (ewald_k)");
+}
+
+int cav_enable_timing(cav_ctx *c, int on)
+{
+  if (!c) return CAV_ERR_ARG;
+  c->timing = on != 0;
+  Timer *ts[] = {&c->t_pair, &c->t_sk, &c->t_force, &c->t_tab, &c->t_nb, &c->t_int};
+  for (Timer *t : ts) {
+    t->total_ms = 0;
+    t->launches = 0;
+    t->pending = false;
+  }
+  return CAV_OK;
+}
+
+int cav_kernel_time(cav_ctx *c, const char *name, double *ms, int *launches)
+{
+  if (!c || !name) return CAV_ERR_ARG;
+  Timer *t = nullptr;
+  if (!strcmp(name, "pair")) t = &c->t_pair;
+  else if (!strcmp(name, "ewald_k_sk")) t = &c->t_sk;
+  else if (!strcmp(name, "ewald_k_force")) t = &c->t_force;
+  else if (!strcmp(name, "ewald_k_tables")) t = &c->t_tab;
+  else if (!strcmp(name, "neighbor")) t = &c->t_nb;
+  else if (!strcmp(name, "integrate")) t = &c->t_int;
+  if (!t) return fail(c, CAV_ERR_ARG, "unknown kernel family");
+  if (ms) *ms = t->launches ? t->total_ms / t->launches : 0.0;
+  if (launches) *launches = t->launches;
+  return CAV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,787 @@
+// Electrostatic_ewald_k on gfx950 as two complex GEMMs on MFMA tiles.
+//
+// The k lattice is separable: exp(i k.r) = X(nx) Y(ny) Z(nz) with
+// X(n) = exp(2 pi i n x / lx) etc., and S(-k) = conj S(k) for real charges, so
+// only the half lattice nx >= 0 is evaluated and each k carries a multiplicity
+// m(k) = [k in K] + [-k in K] over the reference's truncated set K (the first
+// (2kx)(2ky)(2kz) lexicographic vectors, electrostatic_ewald_k/initialize.cpp:
+// 191,201-221).  With rows r = (nx, ny) and columns c = nz:
+//
+//   S pass:     S[r][c]  = sum_j (q_j X_j(nx) Y_j(ny)) Z_j(c)     (reduce over atoms)
+//   force pass: U_i[r]   = sum_c g[r][c] Z_i(c),  g = m f conj(S)  (reduce over nz)
+//               Uz_i[r]  = sum_c nz g[r][c] Z_i(c)
+//               F_i      = C q_i sum_r (2pi nx/lx, 2pi ny/ly, 2pi/lz) Im(P_i(r) (U, U, Uz))
+//               phi_i    = C sum_r Re(P_i(r) U_i[r]),  P_i(r) = X_i(nx) Y_i(ny)
+//
+// Complex products are real-embedded (re/im interleaved along the reduction
+// axis) and run on v_mfma_f64_16x16x4f64 (plain form) or
+// v_mfma_f32_16x16x4f32 (mixed form).  The S pass splits the atom axis into
+// chunks; chunk partials are summed in FP64 in a fixed order, so the plain
+// form is reproducible run to run.
+#include <cmath>
+
+#include "cav_internal.hpp"
+
+namespace cav {
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4 V;
+  static __device__ __forceinline__ V op(double a, double b, V c)
+  {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  typedef f4 V;
+  static __device__ __forceinline__ V op(float a, float b, V c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T> struct Cx { T re, im; };
+
+// ------------------------------------------------------------- phase tables --
+
+// Per atom a record of TS complex values [X(0..kx) | Y(-ky..ky) | Z(-kz..kz)].
+// The phase fraction n*x/l is reduced to [-1/2, 1/2] in FP64 before sincospi,
+// so the FP32 tables of the mixed form carry ~1e-7 absolute phase error even
+// where n k.r reaches hundreds of radians.
+template <typename T>
+__global__ void k_tables(long npts, int TS, int kx, int ky, int kz, double ilx, double ily,
+                         double ilz, const double *x, const double *y, const double *z,
+                         Cx<T> *tab)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= npts * TS) return;
+  const long a = t / TS;
+  const int e = (int)(t - a * TS);
+  const int nX = kx + 1, nY = 2 * ky + 1;
+  double u;
+  int m;
+  if (e < nX) {
+    m = e;
+    u = x[a] * ilx;
+  } else if (e < nX + nY) {
+    m = e - nX - ky;
+    u = y[a] * ily;
+  } else {
+    m = e - nX - nY - kz;
+    u = z[a] * ilz;
+  }
+  double f = (double)m * u;
+  f -= rint(f);
+  Cx<T> v;
+  if constexpr (sizeof(T) == 8) {
+    double s, c;
+    sincospi(2.0 * f, &s, &c);
+    v.re = c;
+    v.im = s;
+  } else {
+    float s, c;
+    sincospif((float)(2.0 * f), &s, &c);
+    v.re = c;
+    v.im = s;
+  }
+  tab[t] = v;
+}
+
+// ------------------------------------------------------------------- S pass --
+
+constexpr int kStage = 32;   // atoms staged in LDS per iteration (16 K-steps)
+constexpr int kWavesS = 8;   // waves per S-pass workgroup
+
+struct SArgs {
+  int n, TS, nX, nY, nZ, ky, kz, Rtiles, chunk;
+  const double *q;
+  const void *tab;
+  double *part;  // [chunk][Rpad][SC]
+  int Rpad, SC;
+};
+
+// One workgroup: an atom chunk x a group of row tiles; wave w owns RT row
+// tiles and all SCT column tiles.  A (rows x (atom,re/im)) is generated from
+// the staged X, Y tables and charges, B ((atom,re/im) x (nz,re/im)) from Z.
+template <typename T, int RT, int SCT>
+__global__ __launch_bounds__(64 * kWavesS) void k_sk(SArgs a)
+{
+  typedef typename Mfma<T>::V V;
+  extern __shared__ unsigned char smem_raw[];
+  Cx<T> *st = reinterpret_cast<Cx<T> *>(smem_raw);      // [kStage][TS]
+  T *sq = reinterpret_cast<T *>(st + kStage * a.TS);    // [kStage]
+  const Cx<T> *tab = reinterpret_cast<const Cx<T> *>(a.tab);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int chunk = blockIdx.x;
+  const int rt0 = (blockIdx.y * kWavesS + wave) * RT;
+  const long a0 = (long)chunk * a.chunk;
+  long a1 = a0 + a.chunk;
+  if (a1 > a.n) a1 = a.n;
+
+  V acc[RT][SCT];
+  double acc64[RT][SCT][4];
+#pragma unroll
+  for (int r = 0; r < RT; ++r)
+#pragma unroll
+    for (int s = 0; s < SCT; ++s) {
+      acc[r][s] = V{0, 0, 0, 0};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc64[r][s][e] = 0.0;
+    }
+
+  // lane roles inside a 16x16x4 tile: A[row=l%16][k=l/16], B[k=l/16][col=l%16]
+  const int lr = lane & 15, lk = lane >> 4;
+  const int jj = lk >> 1, part = lk & 1;  // atom within the K-step, re/im half
+  int rnx[RT], rny[RT];
+  bool rvalid[RT];
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    const int row = (rt0 + r) * 16 + lr;
+    rvalid[r] = (rt0 + r) < a.Rtiles && row < a.nX * a.nY;
+    rnx[r] = rvalid[r] ? row / a.nY : 0;
+    rny[r] = rvalid[r] ? row % a.nY : 0;  // index into Y (ny + ky)
+  }
+  int zc[SCT], zt[SCT];
+  bool zvalid[SCT];
+#pragma unroll
+  for (int s = 0; s < SCT; ++s) {
+    const int col = s * 16 + lr;
+    zc[s] = col >> 1;
+    zt[s] = col & 1;
+    zvalid[s] = zc[s] < a.nZ;
+  }
+  const int offY = a.nX, offZ = a.nX + a.nY;
+
+  for (long base = a0; base < a1; base += kStage) {
+    __syncthreads();
+    // stage kStage atom records (contiguous in the table) and their charges
+    const int cnt = (int)((a1 - base) < kStage ? (a1 - base) : kStage);
+    for (int t = threadIdx.x; t < kStage * a.TS; t += 64 * kWavesS) {
+      const int j = t / a.TS;
+      Cx<T> v{0, 0};
+      if (j < cnt) v = tab[base * a.TS + t];
+      st[t] = v;
+    }
+    if (threadIdx.x < kStage)
+      sq[threadIdx.x] = threadIdx.x < cnt ? (T)a.q[base + threadIdx.x] : (T)0;
+    __syncthreads();
+    if (rt0 >= a.Rtiles) continue;
+#pragma unroll 2
+    for (int ks = 0; ks < kStage / 2; ++ks) {
+      const int j = 2 * ks + jj;
+      const Cx<T> *rec = st + j * a.TS;
+      const T qj = sq[j];
+      T av[RT];
+#pragma unroll
+      for (int r = 0; r < RT; ++r) {
+        const Cx<T> X = rec[rnx[r]], Y = rec[offY + rny[r]];
+        const T pr = X.re * Y.re - X.im * Y.im;
+        const T pi = X.re * Y.im + X.im * Y.re;
+        av[r] = rvalid[r] ? qj * (part ? pi : pr) : (T)0;
+      }
+#pragma unroll
+      for (int s = 0; s < SCT; ++s) {
+        const Cx<T> Z = rec[offZ + (zvalid[s] ? zc[s] : 0)];
+        // re column: (Zr, -Zi); im column: (Zi, Zr) along (re, im) of A
+        T bv = zt[s] == 0 ? (part ? -Z.im : Z.re) : (part ? Z.re : Z.im);
+        bv = zvalid[s] ? bv : (T)0;
+#pragma unroll
+        for (int r = 0; r < RT; ++r) acc[r][s] = Mfma<T>::op(av[r], bv, acc[r][s]);
+      }
+    }
+    if constexpr (sizeof(T) == 4) {
+      // mixed form: FP32 products within a stage, FP64 accumulation across stages
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int s = 0; s < SCT; ++s) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc64[r][s][e] += (double)acc[r][s][e];
+          acc[r][s] = V{0, 0, 0, 0};
+        }
+    }
+  }
+  if (rt0 >= a.Rtiles) return;
+  // D[row = 4(l/16)+e][col = l%16]
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    if (rt0 + r >= a.Rtiles) break;
+#pragma unroll
+    for (int s = 0; s < SCT; ++s)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = (rt0 + r) * 16 + 4 * lk + e;
+        const int col = s * 16 + lr;
+        const double v = sizeof(T) == 8 ? (double)acc[r][s][e] : acc64[r][s][e];
+        a.part[((long)chunk * a.Rpad + row) * a.SC + col] = v;
+      }
+  }
+}
+
+// --------------------------------------------------- reduction and operand --
+
+// S = sum over chunks (fixed order, FP64); g = W conj(S); write the force-pass
+// operand B[kappa][col] (kappa = 2c+s, col = rb*64 + t*16 + rr) and the
+// per-thread k-space energy term W |S|^2.
+template <typename T>
+__global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz, int KZ,
+                            const double *part, const double *W, double *sk, T *bm,
+                            double *epart)
+{
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double e = 0;
+  if (t < Rpad * nZ) {
+    const int r = t / nZ, c = t - r * nZ;
+    double sr = 0, si = 0;
+    for (int k = 0; k < nchunks; ++k) {
+      const double *p = part + ((long)k * Rpad + r) * SC + 2 * c;
+      sr += p[0];
+      si += p[1];
+    }
+    sk[(long)r * 2 * nZ + 2 * c] = sr;
+    sk[(long)r * 2 * nZ + 2 * c + 1] = si;
+    const double w = r < R ? W[(long)r * nZ + c] : 0.0;
+    e = w * (sr * sr + si * si);
+    const double gr = w * sr, gi = -w * si, nz = (double)(c - kz);
+    const int rb = r >> 4, rr = r & 15;
+    const long cols = 4L * Rpad;
+    const long k0 = 2L * c, k1 = 2L * c + 1;
+    const long cb = (long)rb * 64 + rr;
+    bm[k0 * cols + cb + 0] = (T)gr;
+    bm[k1 * cols + cb + 0] = (T)(-gi);
+    bm[k0 * cols + cb + 16] = (T)gi;
+    bm[k1 * cols + cb + 16] = (T)gr;
+    bm[k0 * cols + cb + 32] = (T)(nz * gr);
+    bm[k1 * cols + cb + 32] = (T)(-nz * gi);
+    bm[k0 * cols + cb + 48] = (T)(nz * gi);
+    bm[k1 * cols + cb + 48] = (T)(nz * gr);
+  }
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) epart[blockIdx.x] = red[0];
+}
+
+// zero the padding rows of the operand (kappa >= 2 nZ)
+template <typename T>
+__global__ void k_zero_t(long n, T *p)
+{
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = (T)0;
+}
+
+// --------------------------------------------------------------- force pass --
+
+constexpr int kWavesF = 4;  // waves per force-pass workgroup, 32 atoms each
+
+struct FArgs {
+  int n, TS, nX, nY, nZ, ky, R, Rpad, KZ;
+  const void *tab;
+  const void *bm;
+  const double *q, *minv;
+  const int *type;
+  double c0, kxs, kys, kzs;  // C = 4 pi ke / V; 2 pi / l per axis
+  double *ax, *ay, *az;
+  double *out4;  // optional per-atom (fx, fy, fz, phi) without charge/mass
+};
+
+template <typename T, int KS>
+__global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
+{
+  typedef typename Mfma<T>::V V;
+  const Cx<T> *tab = reinterpret_cast<const Cx<T> *>(a.tab);
+  const T *bm = reinterpret_cast<const T *>(a.bm);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const long base = ((long)blockIdx.x * kWavesF + wave) * 32;
+  const int offY = a.nX, offZ = a.nX + a.nY;
+
+  // A fragments: A[atom = l%16][kappa = 4s + l/16] = (Zr, Zi)(c = kappa/2)
+  T am[2][KS];
+#pragma unroll
+  for (int tau = 0; tau < 2; ++tau) {
+    const long atom = base + tau * 16 + lr;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int kap = 4 * s + lk, c = kap >> 1;
+      T v = 0;
+      if (atom < a.n && c < a.nZ && s * 4 < a.KZ) {
+        const Cx<T> Z = tab[atom * a.TS + offZ + c];
+        v = (kap & 1) ? Z.im : Z.re;
+      }
+      am[tau][s] = v;
+    }
+  }
+  T fx[2][4], fy[2][4], fz[2][4], ph[2][4];
+#pragma unroll
+  for (int tau = 0; tau < 2; ++tau)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) fx[tau][e] = fy[tau][e] = fz[tau][e] = ph[tau][e] = 0;
+
+  const long cols = 4L * a.Rpad;
+  const int nrb = a.Rpad >> 4;
+  for (int rb = 0; rb < nrb; ++rb) {
+    V acc[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = V{0, 0, 0, 0};
+    const T *bcol = bm + (long)rb * 64 + lr;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      if (4 * s >= a.KZ) break;
+      const T *brow = bcol + (long)(4 * s + lk) * cols;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const T b = brow[t * 16];
+#pragma unroll
+        for (int tau = 0; tau < 2; ++tau) acc[t][tau] = Mfma<T>::op(am[tau][s], b, acc[t][tau]);
+      }
+    }
+    // epilogue: lane holds rows r = rb*16 + l%16 for atoms 4(l/16)+e of each tile
+    const int r = rb * 16 + lr;
+    if (r < a.R) {
+      const int nxi = r / a.nY, nyi = r - (r / a.nY) * a.nY;
+      const T wx = (T)nxi, wy = (T)(nyi - a.ky);
+#pragma unroll
+      for (int tau = 0; tau < 2; ++tau)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const long atom = base + tau * 16 + 4 * lk + e;
+          if (atom >= a.n) continue;
+          const Cx<T> X = tab[atom * a.TS + nxi], Y = tab[atom * a.TS + offY + nyi];
+          const T pr = X.re * Y.re - X.im * Y.im;
+          const T pi = X.re * Y.im + X.im * Y.re;
+          const T ur = acc[0][tau][e], ui = acc[1][tau][e];
+          const T uzr = acc[2][tau][e], uzi = acc[3][tau][e];
+          const T h = pr * ui + pi * ur;
+          fx[tau][e] += wx * h;
+          fy[tau][e] += wy * h;
+          fz[tau][e] += pr * uzi + pi * uzr;
+          ph[tau][e] += pr * ur - pi * ui;
+        }
+    }
+  }
+  // reduce over the 16 lanes that share l/16 (they hold different rows)
+#pragma unroll
+  for (int tau = 0; tau < 2; ++tau)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double vx = fx[tau][e], vy = fy[tau][e], vz = fz[tau][e], vp = ph[tau][e];
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        vx += __shfl_xor(vx, o, 64);
+        vy += __shfl_xor(vy, o, 64);
+        vz += __shfl_xor(vz, o, 64);
+        vp += __shfl_xor(vp, o, 64);
+      }
+      const long atom = base + tau * 16 + 4 * lk + e;
+      if (lr == 0 && atom < a.n) {
+        const double gx = a.c0 * a.kxs * vx, gy = a.c0 * a.kys * vy, gz = a.c0 * a.kzs * vz;
+        if (a.out4) {
+          a.out4[4 * atom + 0] = gx;
+          a.out4[4 * atom + 1] = gy;
+          a.out4[4 * atom + 2] = gz;
+          a.out4[4 * atom + 3] = a.c0 * vp;
+        } else {
+          const double qm = a.q[atom] * a.minv[a.type[atom]];
+          a.ax[atom] += gx * qm;
+          a.ay[atom] += gy * qm;
+          a.az[atom] += gz * qm;
+        }
+      }
+    }
+}
+
+// ------------------------------------------------------------ dipole term --
+
+// calculate_dipole_sum (initialize.cpp:94-115): per-block FP64 partials of q r.
+__global__ void k_dipole_part(int n, const double *q, const double *x, const double *y,
+                              const double *z, double *part)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ double red[3][kBlock];
+  double v[3] = {0, 0, 0};
+  if (i < n) {
+    v[0] = q[i] * x[i];
+    v[1] = q[i] * y[i];
+    v[2] = q[i] * z[i];
+  }
+  for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = v[d];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; ++d) part[3 * blockIdx.x + d] = red[d][0];
+}
+
+// The reference adds q_i * dipole_field inside its k loop, i.e. once per used
+// k-vector (electrostatic_ewald_k.cpp:197,216-219); the same total is added here.
+__global__ void k_dipole_apply(int n, int nblk, const double *part, double dip_coef, double nk,
+                               const double *q, const double *minv, const int *type, double *ax,
+                               double *ay, double *az, double *dsum_out)
+{
+  __shared__ double f[3];
+  if (threadIdx.x < 3) {
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += part[3 * b + threadIdx.x];
+    f[threadIdx.x] = -s * dip_coef;
+    if (blockIdx.x == 0) dsum_out[threadIdx.x] = s;
+  }
+  __syncthreads();
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double qm = q[i] * minv[type[i]] * nk;
+  ax[i] += f[0] * qm;
+  ay[i] += f[1] * qm;
+  az[i] += f[2] * qm;
+}
+
+// ------------------------------------------------------------------- host --
+
+static int roundup(int v, int m) { return (v + m - 1) / m * m; }
+
+int ewald_k_plan(cav_ctx *c)
+{
+  EwaldKPlan &p = c->ek;
+  p.kx = c->kx;
+  p.ky = c->ky;
+  p.kz = c->kz;
+  if (p.kx < 0 || p.ky < 0 || p.kz < 0) return fail(c, CAV_ERR_ARG, "negative k_max");
+  p.nX = p.kx + 1;
+  p.nY = 2 * p.ky + 1;
+  p.nZ = 2 * p.kz + 1;
+  p.R = p.nX * p.nY;
+  p.Rpad = roundup(p.R, 16);
+  p.KZ = roundup(2 * p.nZ, 4);
+  if (p.KZ / 4 > 32) return fail(c, CAV_ERR_ARG, "kz_max above 31 is not supported");
+  if (roundup(2 * p.nZ, 16) / 16 > 8) return fail(c, CAV_ERR_ARG, "kz_max above 31 is not supported");
+  p.nk_ref = (2 * p.kx) * (2 * p.ky) * (2 * p.kz);
+  // initialize.cpp:42-77: lengths from size_global, slab override of one axis
+  p.lx = c->L[0];
+  p.ly = c->L[1];
+  p.lz = c->L[2];
+  if (c->slab_axis == 0) p.lx = c->slab_lz;
+  if (c->slab_axis == 1) p.ly = c->slab_lz;
+  if (c->slab_axis == 2) p.lz = c->slab_lz;
+  const double lxi = 1.0 / p.lx, lyi = 1.0 / p.ly, lzi = 1.0 / p.lz;
+  p.lxyz_inv = 1.0 / (p.lx * p.ly * p.lz);
+  if (c->dipole) p.dip_coef = c->ek_ke * CAV_4PI * p.lxyz_inv / (1 + 2.0 * c->eps_dipole);
+  // make_k_vectors (initialize.cpp:184-222): f(k) as the reference computes it;
+  // the index of (kx,ky,kz) in the lexicographic list without 0 decides membership.
+  const double a2 = c->ek_alpha * c->ek_alpha, f4a2 = 1.0 / (4.0 * a2);
+  const long zero_idx = ((long)p.kx * p.nY + p.ky) * p.nZ + p.kz;
+  auto ref_index = [&](int nx, int ny, int nz) -> long {
+    const long g = ((long)(nx + p.kx) * p.nY + (ny + p.ky)) * p.nZ + (nz + p.kz);
+    return g > zero_idx ? g - 1 : g;
+  };
+  auto in_K = [&](int nx, int ny, int nz) -> bool {
+    if (nx == 0 && ny == 0 && nz == 0) return false;
+    return ref_index(nx, ny, nz) < p.nk_ref;
+  };
+  auto fk = [&](int nx, int ny, int nz) -> double {
+    const double k0 = CAV_2PI * nx * lxi, k1 = CAV_2PI * ny * lyi, k2 = CAV_2PI * nz * lzi;
+    const double ksq = k0 * k0 + k1 * k1 + k2 * k2;
+    return (1.0 / ksq) * std::exp(-ksq * f4a2);
+  };
+  p.W.assign((size_t)p.R * p.nZ, 0.0);
+  for (int nx = 0; nx <= p.kx; ++nx)
+    for (int ny = -p.ky; ny <= p.ky; ++ny)
+      for (int nz = -p.kz; nz <= p.kz; ++nz) {
+        const bool half = nx > 0 || (nx == 0 && ny > 0) || (nx == 0 && ny == 0 && nz > 0);
+        if (!half) continue;
+        const int m = (int)in_K(nx, ny, nz) + (int)in_K(-nx, -ny, -nz);
+        if (!m) continue;
+        const int r = nx * p.nY + (ny + p.ky);
+        p.W[(size_t)r * p.nZ + (nz + p.kz)] = m * fk(nx, ny, nz);
+      }
+  // reference order of K for cav_get_sk: each k maps to a computed (row, col),
+  // directly (nx >= 0) or through S(k) = conj S(-k)
+  p.ref_r.clear();
+  p.ref_c.clear();
+  p.ref_conj.clear();
+  for (int nx = -p.kx; nx <= p.kx; ++nx)
+    for (int ny = -p.ky; ny <= p.ky; ++ny)
+      for (int nz = -p.kz; nz <= p.kz; ++nz) {
+        if (!in_K(nx, ny, nz)) continue;
+        const bool direct = nx >= 0;
+        const int ax_ = direct ? nx : -nx, ay_ = direct ? ny : -ny, az_ = direct ? nz : -nz;
+        p.ref_r.push_back(ax_ * p.nY + (ay_ + p.ky));
+        p.ref_c.push_back(az_ + p.kz);
+        p.ref_conj.push_back(direct ? 0 : 1);
+      }
+  return CAV_OK;
+}
+
+static int s_rt(int prec) { return prec == CAV_FP64 ? 3 : 3; }
+
+static int s_groups(const cav_ctx *c)
+{
+  const int tiles = c->ek.Rpad / 16;
+  const int per = kWavesS * s_rt(c->precision);
+  return (tiles + per - 1) / per;
+}
+
+static int s_chunk(const cav_ctx *c, int ntarget)
+{
+  // aim for >= ntarget workgroups over the whole grid; chunk multiple of kStage
+  const int groups = s_groups(c);
+  long per = ((long)c->n * groups + ntarget - 1) / ntarget;
+  per = (per + kStage - 1) / kStage * kStage;
+  if (per < kStage) per = kStage;
+  return (int)per;
+}
+
+int ewald_k_alloc(cav_ctx *c)
+{
+  const EwaldKPlan &p = c->ek;
+  const size_t tsz = c->precision == CAV_FP64 ? sizeof(double) : sizeof(float);
+  const int TS = p.nX + p.nY + p.nZ;
+  const size_t tab_bytes = (size_t)c->n * TS * 2 * tsz;
+  if (c->tab) (void)hipFree(c->tab);
+  CAV_HIP(hipMalloc(&c->tab, tab_bytes ? tab_bytes : 16));
+  c->tab_bytes = tab_bytes;
+  c->tab_cap = c->n;
+  const int chunk = s_chunk(c, 512);
+  c->sk_chunks = (c->n + chunk - 1) / chunk;
+  if (c->sk_chunks < 1) c->sk_chunks = 1;
+  const int SC = roundup(2 * p.nZ, 16);
+  if (c->sk_part) (void)hipFree(c->sk_part);
+  CAV_HIP(hipMalloc(&c->sk_part, sizeof(double) * (size_t)c->sk_chunks * p.Rpad * SC));
+  CAV_HIP(hipMemsetAsync(c->sk_part, 0, sizeof(double) * (size_t)c->sk_chunks * p.Rpad * SC,
+                         c->stream));
+  if (c->sk) (void)hipFree(c->sk);
+  CAV_HIP(hipMalloc(&c->sk, sizeof(double) * (size_t)p.Rpad * 2 * p.nZ));
+  if (c->bmat) (void)hipFree(c->bmat);
+  const size_t bm_elems = (size_t)p.KZ * 4 * p.Rpad;
+  CAV_HIP(hipMalloc(&c->bmat, bm_elems * tsz));
+  if (c->precision == CAV_FP64)
+    k_zero_t<double><<<grid_for(bm_elems), kBlock, 0, c->stream>>>(bm_elems, (double *)c->bmat);
+  else
+    k_zero_t<float><<<grid_for(bm_elems), kBlock, 0, c->stream>>>(bm_elems, (float *)c->bmat);
+  if (c->d_W) (void)hipFree(c->d_W);
+  CAV_HIP(hipMalloc(&c->d_W, sizeof(double) * (size_t)p.R * p.nZ));
+  CAV_HIP(hipMemcpyAsync(c->d_W, p.W.data(), sizeof(double) * p.W.size(), hipMemcpyHostToDevice,
+                         c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  return CAV_OK;
+}
+
+template <typename K>
+static void allow_smem(K kern, size_t smem)
+{
+  if (smem > 48 * 1024)
+    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)smem);
+}
+
+template <typename T, int RT>
+static void launch_sk_rt(cav_ctx *c, const SArgs &a, dim3 grid, size_t smem, int SCT)
+{
+  allow_smem(k_sk<T, RT, 1>, smem);
+  allow_smem(k_sk<T, RT, 2>, smem);
+  allow_smem(k_sk<T, RT, 3>, smem);
+  allow_smem(k_sk<T, RT, 4>, smem);
+  allow_smem(k_sk<T, RT, 5>, smem);
+  allow_smem(k_sk<T, RT, 6>, smem);
+  allow_smem(k_sk<T, RT, 7>, smem);
+  allow_smem(k_sk<T, RT, 8>, smem);
+  switch (SCT) {
+  case 1: k_sk<T, RT, 1><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
+  case 2: k_sk<T, RT, 2><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
+  case 3: k_sk<T, RT, 3><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
+  case 4: k_sk<T, RT, 4><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
+  case 5: k_sk<T, RT, 5><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
+  case 6: k_sk<T, RT, 6><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
+  case 7: k_sk<T, RT, 7><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
+  default: k_sk<T, RT, 8><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
+  }
+}
+
+template <typename T>
+static void launch_kforce_t(cav_ctx *c, const FArgs &a, int KS, int npts)
+{
+  const int g = (npts + 32 * kWavesF - 1) / (32 * kWavesF);
+  if (KS <= 8)
+    k_kforce<T, 8><<<g, 64 * kWavesF, 0, c->stream>>>(a);
+  else if (KS <= 12)
+    k_kforce<T, 12><<<g, 64 * kWavesF, 0, c->stream>>>(a);
+  else if (KS <= 17)
+    k_kforce<T, 17><<<g, 64 * kWavesF, 0, c->stream>>>(a);
+  else if (KS <= 24)
+    k_kforce<T, 24><<<g, 64 * kWavesF, 0, c->stream>>>(a);
+  else
+    k_kforce<T, 32><<<g, 64 * kWavesF, 0, c->stream>>>(a);
+}
+
+// Phase tables + S pass + reduction for npts points (the owned atoms, or query
+// points when tab/positions are the caller's).
+template <typename T>
+static int sk_pipeline(cav_ctx *c)
+{
+  const EwaldKPlan &p = c->ek;
+  const int TS = p.nX + p.nY + p.nZ;
+  timer_begin(c, c->t_tab);
+  k_tables<T><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
+      c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
+      (Cx<T> *)c->tab);
+  timer_end(c, c->t_tab);
+  SArgs a;
+  a.n = c->n;
+  a.TS = TS;
+  a.nX = p.nX;
+  a.nY = p.nY;
+  a.nZ = p.nZ;
+  a.ky = p.ky;
+  a.kz = p.kz;
+  a.Rtiles = p.Rpad / 16;
+  a.chunk = (c->n + c->sk_chunks - 1) / c->sk_chunks;
+  a.chunk = (a.chunk + kStage - 1) / kStage * kStage;
+  a.q = c->q;
+  a.tab = c->tab;
+  a.part = c->sk_part;
+  a.Rpad = p.Rpad;
+  a.SC = roundup(2 * p.nZ, 16);
+  const int nch = (c->n + a.chunk - 1) / a.chunk;
+  dim3 grid(nch, s_groups(c));
+  const size_t smem = kStage * TS * sizeof(Cx<T>) + kStage * sizeof(T);
+  timer_begin(c, c->t_sk);
+  launch_sk_rt<T, 3>(c, a, grid, smem, a.SC / 16);
+  timer_end(c, c->t_sk);
+  const int nthr = p.Rpad * p.nZ;
+  k_sk_reduce<T><<<grid_for(nthr), kBlock, 0, c->stream>>>(
+      nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KZ, c->sk_part, c->d_W, c->sk, (T *)c->bmat, c->part);
+  CAV_HIP(hipGetLastError());
+  return reduce_partials(c, grid_for(nthr), 3);
+}
+
+template <typename T>
+static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
+{
+  const EwaldKPlan &p = c->ek;
+  FArgs f;
+  f.n = npts;
+  f.TS = p.nX + p.nY + p.nZ;
+  f.nX = p.nX;
+  f.nY = p.nY;
+  f.nZ = p.nZ;
+  f.ky = p.ky;
+  f.R = p.R;
+  f.Rpad = p.Rpad;
+  f.KZ = p.KZ;
+  f.tab = tab;
+  f.bm = c->bmat;
+  f.q = c->q;
+  f.minv = c->d_minv;
+  f.type = c->type;
+  f.c0 = CAV_4PI * c->ek_ke * p.lxyz_inv;
+  f.kxs = CAV_2PI / p.lx;
+  f.kys = CAV_2PI / p.ly;
+  f.kzs = CAV_2PI / p.lz;
+  f.ax = c->ax;
+  f.ay = c->ay;
+  f.az = c->az;
+  f.out4 = out4;
+  timer_begin(c, c->t_force);
+  launch_kforce_t<T>(c, f, p.KZ / 4, npts);
+  timer_end(c, c->t_force);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+int launch_ewald_k(cav_ctx *c)
+{
+  if (c->n == 0) return CAV_OK;
+  if (c->precision == CAV_FP64) {
+    CAV_TRY(sk_pipeline<double>(c));
+    CAV_TRY(kforce<double>(c, c->n, c->tab, nullptr));
+  } else {
+    CAV_TRY(sk_pipeline<float>(c));
+    CAV_TRY(kforce<float>(c, c->n, c->tab, nullptr));
+  }
+  if (c->dipole) {
+    const int nb = grid_for(c->n);
+    k_dipole_part<<<nb, kBlock, 0, c->stream>>>(c->n, c->q, c->x, c->y, c->z, c->part);
+    k_dipole_apply<<<nb, kBlock, 0, c->stream>>>(c->n, nb, c->part, c->ek.dip_coef,
+                                                 (double)c->ek.nk_ref, c->q, c->d_minv, c->type,
+                                                 c->ax, c->ay, c->az, c->scal + 8);
+    CAV_HIP(hipGetLastError());
+  }
+  return CAV_OK;
+}
+
+// E_k = 1/2 C sum W |S|^2 + self (+ dipole), valid after a force evaluation at
+// the same positions (the reference's S(k) coupling, energy.cpp:61-116).
+int launch_energy_k(cav_ctx *c, double *e)
+{
+  double v[12];
+  CAV_HIP(hipMemcpyAsync(v, c->scal, sizeof(double) * 12, hipMemcpyDeviceToHost, c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  double q2 = 0;
+  for (int i = 0; i < c->n; ++i) {
+    const double qq = c->charge[c->h_type[i]];
+    q2 += qq * qq;
+  }
+  const double self = -CAV_PIS_INV * c->ek_alpha * q2 * c->ek_ke;
+  double ek = 0.5 * CAV_4PI * c->ek_ke * c->ek.lxyz_inv * v[3] + self;
+  if (c->dipole) {
+    const double d2 = v[8] * v[8] + v[9] * v[9] + v[10] * v[10];
+    ek += 0.5 * c->ek.dip_coef * d2;
+  }
+  *e = ek;
+  return CAV_OK;
+}
+
+// Point queries: tables for the points, then the force pass with q = 1 into a
+// per-point (field, potential) buffer; the dipole field is added on the host.
+int point_field_k(cav_ctx *c, int np, const double *pts, double *field, double *pot)
+{
+  const EwaldKPlan &p = c->ek;
+  const int TS = p.nX + p.nY + p.nZ;
+  const size_t tsz = c->precision == CAV_FP64 ? sizeof(double) : sizeof(float);
+  double *d;
+  void *tab;
+  CAV_HIP(hipMalloc(&d, sizeof(double) * 7 * (size_t)(np > 0 ? np : 1)));
+  CAV_HIP(hipMalloc(&tab, (size_t)(np > 0 ? np : 1) * TS * 2 * tsz));
+  std::vector<double> soa(3 * (size_t)np);
+  for (int i = 0; i < np; ++i)
+    for (int k = 0; k < 3; ++k) soa[(size_t)k * np + i] = pts[3 * i + k];
+  CAV_HIP(hipMemcpyAsync(d, soa.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream));
+  if (c->precision == CAV_FP64) {
+    k_tables<double><<<grid_for((long)np * TS), kBlock, 0, c->stream>>>(
+        np, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, d, d + np, d + 2 * np,
+        (Cx<double> *)tab);
+    CAV_TRY(kforce<double>(c, np, tab, d + 3 * np));
+  } else {
+    k_tables<float><<<grid_for((long)np * TS), kBlock, 0, c->stream>>>(
+        np, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, d, d + np, d + 2 * np,
+        (Cx<float> *)tab);
+    CAV_TRY(kforce<float>(c, np, tab, d + 3 * np));
+  }
+  std::vector<double> o4(4 * (size_t)np);
+  CAV_HIP(hipMemcpyAsync(o4.data(), d + 3 * np, sizeof(double) * 4 * np, hipMemcpyDeviceToHost,
+                         c->stream));
+  double dsum[3] = {0, 0, 0};
+  if (c->dipole)
+    CAV_HIP(hipMemcpyAsync(dsum, c->scal + 8, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+  CAV_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < np; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      double f = o4[4 * (size_t)i + k];
+      if (c->dipole) f += -dsum[k] * p.dip_coef;
+      if (field) field[3 * i + k] = f;
+    }
+    if (pot) pot[i] = o4[4 * (size_t)i + 3];
+  }
+  CAV_HIP(hipFree(d));
+  CAV_HIP(hipFree(tab));
+  return CAV_OK;
+}
+
+}  // namespace cav

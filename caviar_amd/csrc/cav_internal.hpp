@@ -1,0 +1,195 @@
+// Internal state and helpers of the caviar_amd HIP library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/caviar_amd.h"
+
+#define CAV_PI 3.14159265358979323846
+#define CAV_2PI 6.28318530717958647692     // macro_constants.h:22
+#define CAV_4PI 12.56637061435917295384    // macro_constants.h:24
+#define CAV_PIS_INV 0.56418958354775628695 // macro_constants.h:28
+
+namespace cav {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Device status words, read back by the host after every step.
+enum Flag : int {
+  F_WRAPPED = 0,   // an owned atom was shifted by one box length this step
+  F_REBUILD = 1,   // the Verlet list is rebuilt this step
+  F_NAN = 2,       // a NaN acceleration was seen before the drift
+  F_OVERFLOW = 3,  // a Verlet row exceeded the list capacity
+  F_NGHOST = 4,    // ghost count of the current step
+  F_NGHOST_OLD = 5,// ghost count at the last list build
+  F_MAXNB = 6,     // longest Verlet row seen at the last build
+  F_FORCE = 7,     // host-requested rebuild
+  F_COUNT = 16
+};
+
+constexpr int kBlock = 256;
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  double total_ms = 0;
+  int launches = 0;
+  bool pending = false;
+};
+
+struct EwaldKPlan {
+  // k lattice: nx in [0,kx] (half space), ny in [-ky,ky], nz in [-kz,kz]
+  int kx = 0, ky = 0, kz = 0, nX = 0, nY = 0, nZ = 0;
+  int R = 0;      // rows (nx,ny)
+  int Rpad = 0;   // rows rounded up to 16
+  int KZ = 0;     // 2*nZ rounded up to a multiple of 4 (real-embedded nz axis)
+  int nk_ref = 0; // (2kx)(2ky)(2kz), the reference's used count
+  double lx = 1, ly = 1, lz = 1, lxyz_inv = 0, dip_coef = 0;
+  std::vector<double> W;        // R*nZ: multiplicity(k) * f(k) over the half space
+  std::vector<int> ref_r, ref_c, ref_conj;  // reference k order -> (row, col, conjugate?)
+};
+
+}  // namespace cav
+
+struct cav_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int precision = CAV_FP64;
+
+  // domain
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, L[3] = {0, 0, 0};
+  int bc[3] = {0, 0, 0};
+  bool box_set = false;
+  double ghost_cutoff = 0;
+
+  // types
+  int nt = 0;
+  std::vector<double> mass, minv, charge;
+
+  // neighbour list
+  bool nl_on = false;
+  double nl_cutoff = 0, nl_coef = 1.12347, rce = 0, rce2 = 0, thr_sq = 0, cinv = 0;
+  int from_cells = 0, build_cells = 0, all_atom_test = 0, rebuild_test = 0;
+  int nb[3] = {0, 0, 0}, nbins = 0;
+  std::vector<int> st_off, st;
+
+  // force fields
+  bool lj_on = false;
+  double lj_rc = 0;
+  std::vector<double> eps, sig;
+  bool lj_cellwalk = false;  // LJ walks the cells every step (cutoff beyond the list guarantee)
+  bool er_on = false;
+  double er_alpha = 1, er_ke = 1;
+  bool lambda_set = false;
+  std::vector<double> lambda;
+  bool ek_on = false;
+  double ek_alpha = 0, ek_ke = 1, eps_dipole = 1, slab_lz = 1;
+  int kx = 1, ky = 1, kz = 1, dipole = 0, slab_axis = -1;
+  cav::EwaldKPlan ek;
+
+  // atoms
+  int n = 0, gcap = 0;
+  std::vector<int> h_id, h_type, h_mol;
+
+  // device arrays: positions/type/id/mol/charge over owned + ghost capacity
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  double *xo = nullptr, *yo = nullptr, *zo = nullptr;  // positions at the last list build
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  double *ax = nullptr, *ay = nullptr, *az = nullptr;
+  double *aox = nullptr, *aoy = nullptr, *aoz = nullptr;
+  double *snap = nullptr;  // 9n doubles: pos/vel/acc at the start of a step
+  int *type = nullptr, *id = nullptr, *mol = nullptr;
+  double *q = nullptr;
+  int *gsrc = nullptr;
+  int *flags = nullptr;
+  int *h_flags = nullptr;  // pinned
+  int *blk = nullptr;      // scan block sums
+  int nblk = 0;
+
+  // cells
+  int *bin_of = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_items = nullptr;
+  int *d_st_off = nullptr, *d_st = nullptr;
+  int *scan_tmp = nullptr;
+
+  // list (column-major: nl[k*n + i])
+  int *nl = nullptr, *nn = nullptr;
+  int nl_cap = 0;
+
+  // per-type tables on the device
+  double *d_minv = nullptr, *d_charge = nullptr, *d_eps = nullptr, *d_sig = nullptr,
+         *d_lambda = nullptr;
+
+  // reductions: per-block partials and results
+  double *part = nullptr;  // per-block partial sums (virial, energies)
+  int npart = 0;
+  double *scal = nullptr;  // [0] virial, [1] e_lj, [2] e_r, [3] e_k, [4..] scratch
+  double *h_scal = nullptr;
+
+  // Ewald_k device buffers
+  void *tab = nullptr;       // phase tables (T): X[nX][n], Y[nY][n], Z[nZ][n] complex
+  size_t tab_bytes = 0;
+  int tab_cap = 0;           // atoms the tables hold
+  double *sk_part = nullptr; // [chunks][Rpad][2nZ]
+  int sk_chunks = 0;
+  double *sk = nullptr;      // [Rpad][2nZ] FP64
+  void *bmat = nullptr;      // [KZ][4*Rpad] (T) force-pass operand
+  double *d_W = nullptr;     // [Rpad][nZ]
+  double *kacc = nullptr;    // per-atom k-space (fx, fy, fz, phi) scratch for point queries
+
+  bool setup_done = false;
+  bool list_valid = false;
+  bool cells_current = false;
+
+  bool timing = false;
+  cav::Timer t_pair, t_sk, t_force, t_tab, t_nb, t_int;
+};
+
+// helpers shared by the translation units
+namespace cav {
+int fail(cav_ctx *c, int code, const std::string &msg);
+int hip_check(cav_ctx *c, hipError_t e, const char *what);
+void timer_begin(cav_ctx *c, Timer &t);
+void timer_end(cav_ctx *c, Timer &t);
+
+// launchers (defined in the .hip files)
+int launch_ghosts(cav_ctx *c);
+int launch_rebuild_check(cav_ctx *c);
+int launch_bins(cav_ctx *c, bool conditional);
+int launch_list(cav_ctx *c, bool conditional);
+int launch_save_old(cav_ctx *c, bool conditional);
+int launch_pairs(cav_ctx *c);
+int launch_lj_cellwalk(cav_ctx *c);
+int launch_zero_acc(cav_ctx *c);
+int launch_ewald_k(cav_ctx *c);
+int launch_drift(cav_ctx *c, double dt);
+int launch_kick(cav_ctx *c, double dt);
+int launch_begin_step(cav_ctx *c);
+int ewald_k_plan(cav_ctx *c);
+int ewald_k_alloc(cav_ctx *c);
+int launch_energy_r(cav_ctx *c, double *e);
+int launch_energy_lj(cav_ctx *c, double *e);
+int launch_energy_k(cav_ctx *c, double *e);
+int point_field_r(cav_ctx *c, int np, const double *pts, double *field, double *pot);
+int point_field_k(cav_ctx *c, int np, const double *pts, double *field, double *pot);
+int reduce_partials(cav_ctx *c, int count, int slot);
+
+inline int grid_for(long n, int block = kBlock) { return (int)((n + block - 1) / block); }
+
+}  // namespace cav
+
+#define CAV_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) return cav::hip_check(c, e_, #call);               \
+  } while (0)
+
+#define CAV_TRY(call)                                                        \
+  do {                                                                       \
+    int r_ = (call);                                                         \
+    if (r_ != CAV_OK) return r_;                                             \
+  } while (0)

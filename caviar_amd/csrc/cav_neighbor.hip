@@ -1,0 +1,399 @@
+// Ghost images, cell binning, rebuild test and Verlet-list construction on
+// gfx950.  Every kernel that depends on the per-step rebuild decision reads the
+// device flag and returns early, so a step never waits for the host.
+#include "cav_internal.hpp"
+
+namespace cav {
+
+// ------------------------------------------------------------------ scans --
+
+// Exclusive scan of one value per thread over a 256-thread block: wave64
+// prefix sums with DPP-backed shuffles, then the 4 wave totals through LDS.
+__device__ __forceinline__ int block_exclusive_scan(int v, int *total)
+{
+  __shared__ int wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  int base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wave) base += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return base + inc - v;
+}
+
+// Per-block sums of `count(i)`; the functor recomputes the per-item count.
+template <class F>
+__global__ void k_block_sums(long nitems, F count, int *blk, const int *cond)
+{
+  if (cond && !*cond) return;
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  int v = i < nitems ? count(i) : 0;
+  int tot;
+  block_exclusive_scan(v, &tot);
+  if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the block sums in one block; writes the grand total.
+__global__ void k_scan_blocks(int nblk, int *blk, int *total, const int *cond)
+{
+  if (cond && !*cond) return;
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nblk; base += kBlock) {
+    const int i = base + threadIdx.x;
+    int v = i < nblk ? blk[i] : 0;
+    int tot;
+    int ex = block_exclusive_scan(v, &tot);
+    if (i < nblk) blk[i] = ex + carry;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// ----------------------------------------------------------------- ghosts --
+
+struct GhostRule {
+  const double *x, *y, *z;
+  double xl, xu, yl, yu, zl, zu;
+  int bx, by, bz;
+  __device__ __forceinline__ void sides(long i, int &sx, int &sy, int &sz) const
+  {
+    // exchange_ghost_single_md_domain.cpp:70-101: strict tests, times the bc flag
+    sx = (x[i] < xl ? -1 : (x[i] > xu ? 1 : 0)) * bx;
+    sy = (y[i] < yl ? -1 : (y[i] > yu ? 1 : 0)) * by;
+    sz = (z[i] < zl ? -1 : (z[i] > zu ? 1 : 0)) * bz;
+  }
+  __device__ __forceinline__ int operator()(long i) const
+  {
+    int sx, sy, sz;
+    sides(i, sx, sy, sz);
+    const int a = sx != 0, b = sy != 0, c = sz != 0;
+    return a + b + c + (a & b) + (a & c) + (b & c) + (a & b & c);
+  }
+};
+
+// Writes the images of owned atom i at ghost slots [off, off+count): order x,
+// y, z, xy, xz, yz, xyz (exchange_ghost_single_md_domain.cpp:167-215); ghost g
+// lives at index n+g of every per-atom array.
+__global__ void k_ghost_scatter(int n, GhostRule rule, double Lx, double Ly, double Lz,
+                                const int *blk, double *x, double *y, double *z, int *type,
+                                int *id, int *mol, double *q, int *gsrc)
+{
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int cnt = i < n ? rule(i) : 0;
+  const int off = block_exclusive_scan(cnt, nullptr) + blk[blockIdx.x];
+  if (i >= n || cnt == 0) return;
+  int sx, sy, sz;
+  rule.sides(i, sx, sy, sz);
+  const double px = x[i], py = y[i], pz = z[i];
+  const double gx = px - sx * Lx, gy = py - sy * Ly, gz = pz - sz * Lz;
+  const int t = type[i], d = id[i], m = mol[i];
+  const double qq = q[i];
+  int g = off;
+  auto put = [&](double a, double b, double c) {
+    const long s = (long)n + g;
+    x[s] = a;
+    y[s] = b;
+    z[s] = c;
+    type[s] = t;
+    id[s] = d;
+    mol[s] = m;
+    q[s] = qq;
+    gsrc[g] = (int)i;
+    ++g;
+  };
+  if (sx) put(gx, py, pz);
+  if (sy) put(px, gy, pz);
+  if (sz) put(px, py, gz);
+  if (sx && sy) put(gx, gy, pz);
+  if (sx && sz) put(gx, py, gz);
+  if (sy && sz) put(px, gy, gz);
+  if (sx && sy && sz) put(gx, gy, gz);
+}
+
+int launch_ghosts(cav_ctx *c)
+{
+  GhostRule r;
+  r.x = c->x;
+  r.y = c->y;
+  r.z = c->z;
+  r.xl = c->lo[0] + c->ghost_cutoff;
+  r.xu = c->hi[0] - c->ghost_cutoff;
+  r.yl = c->lo[1] + c->ghost_cutoff;
+  r.yu = c->hi[1] - c->ghost_cutoff;
+  r.zl = c->lo[2] + c->ghost_cutoff;
+  r.zu = c->hi[2] - c->ghost_cutoff;
+  r.bx = c->bc[0];
+  r.by = c->bc[1];
+  r.bz = c->bc[2];
+  const int nb = grid_for(c->n);
+  k_block_sums<<<nb, kBlock, 0, c->stream>>>(c->n, r, c->blk, nullptr);
+  k_scan_blocks<<<1, kBlock, 0, c->stream>>>(nb, c->blk, c->flags + F_NGHOST, nullptr);
+  k_ghost_scatter<<<nb, kBlock, 0, c->stream>>>(c->n, r, c->L[0], c->L[1], c->L[2], c->blk, c->x,
+                                                c->y, c->z, c->type, c->id, c->mol, c->q,
+                                                c->gsrc);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+// ------------------------------------------------------------ rebuild test --
+
+__device__ __forceinline__ double dot_rn(double a, double b, double c)
+{
+  // Vector::operator* as the reference evaluates it: (a*a + b*b) + c*c, no FMA.
+  return __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
+}
+
+// Neighborlist::update_is_needed (neighborlist.cpp:91-139): size change, or any
+// owned/ghost displacement^2 (ghosts compared by index) above the threshold.
+__global__ void k_rebuild_check(int n, long cap_total, const double *x, const double *y,
+                                const double *z, const double *xo, const double *yo,
+                                const double *zo, double thr_sq, int rebuild_test, int *flags)
+{
+  if (flags[F_REBUILD]) return;
+  const int G = flags[F_NGHOST], Gold = flags[F_NGHOST_OLD];
+  if (rebuild_test || G != Gold) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) flags[F_REBUILD] = 1;
+    return;
+  }
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= (long)n + G || a >= cap_total) return;
+  const double dx = __dsub_rn(x[a], xo[a]), dy = __dsub_rn(y[a], yo[a]), dz = __dsub_rn(z[a], zo[a]);
+  if (dot_rn(dx, dy, dz) > thr_sq) flags[F_REBUILD] = 1;
+}
+
+int launch_rebuild_check(cav_ctx *c)
+{
+  const long tot = (long)c->n + c->gcap;
+  k_rebuild_check<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->x, c->y, c->z, c->xo,
+                                                          c->yo, c->zo, c->thr_sq,
+                                                          c->rebuild_test, c->flags);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+// ------------------------------------------------------------------- cells --
+
+struct BinRule {
+  double lox, loy, loz, cinv;
+  int nbx, nby, nbz;
+  // binlist_index + neigh_bin_index (neighborlist.cpp:408-436): floor, clamp,
+  // x-fastest linear index.
+  __device__ __forceinline__ int operator()(double px, double py, double pz) const
+  {
+    int ix = (int)floor(__dmul_rn(__dsub_rn(px, lox), cinv));
+    int iy = (int)floor(__dmul_rn(__dsub_rn(py, loy), cinv));
+    int iz = (int)floor(__dmul_rn(__dsub_rn(pz, loz), cinv));
+    ix = ix < 0 ? 0 : (ix >= nbx ? nbx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy >= nby ? nby - 1 : iy);
+    iz = iz < 0 ? 0 : (iz >= nbz ? nbz - 1 : iz);
+    return ix + nbx * iy + nbx * nby * iz;
+  }
+};
+
+static BinRule bin_rule(const cav_ctx *c)
+{
+  BinRule b;
+  b.lox = c->lo[0];
+  b.loy = c->lo[1];
+  b.loz = c->lo[2];
+  b.cinv = c->cinv;
+  b.nbx = c->nb[0];
+  b.nby = c->nb[1];
+  b.nbz = c->nb[2];
+  return b;
+}
+
+__global__ void k_zero_int(int *p, long n, const int *cond)
+{
+  if (cond && !*cond) return;
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_bin_count(int n, long cap_total, BinRule rule, const double *x, const double *y,
+                            const double *z, int *bin_of, int *bin_cnt, const int *flags,
+                            const int *cond)
+{
+  if (cond && !*cond) return;
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= (long)n + flags[F_NGHOST] || a >= cap_total) return;
+  const int b = rule(x[a], y[a], z[a]);
+  bin_of[a] = b;
+  atomicAdd(bin_cnt + b, 1);
+}
+
+struct CountOf {
+  const int *cnt;
+  __device__ __forceinline__ int operator()(long i) const { return cnt[i]; }
+};
+
+__global__ void k_bin_start(int nbins, const int *cnt, const int *blk, int *start, const int *cond)
+{
+  if (cond && !*cond) return;
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int v = i < nbins ? cnt[i] : 0;
+  const int off = block_exclusive_scan(v, nullptr) + blk[blockIdx.x];
+  if (i < nbins) start[i] = off;
+  if (i == nbins - 1) start[nbins] = off + v;
+}
+
+__global__ void k_bin_fill(int n, long cap_total, const int *bin_of, const int *start, int *cur,
+                           int *items, const int *flags, const int *cond)
+{
+  if (cond && !*cond) return;
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= (long)n + flags[F_NGHOST] || a >= cap_total) return;
+  const int b = bin_of[a];
+  items[start[b] + atomicAdd(cur + b, 1)] = (int)a;
+}
+
+// Insertion order of update_cell_list (owned ascending, then ghosts n+g
+// ascending, neighborlist.cpp:457-471) is ascending index order: sort each bin.
+__global__ void k_bin_sort(int nbins, const int *start, int *items, const int *cond)
+{
+  if (cond && !*cond) return;
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= nbins) return;
+  const int s = start[b], e = start[b + 1];
+  for (int i = s + 1; i < e; ++i) {
+    const int v = items[i];
+    int j = i - 1;
+    while (j >= s && items[j] > v) {
+      items[j + 1] = items[j];
+      --j;
+    }
+    items[j + 1] = v;
+  }
+}
+
+int launch_bins(cav_ctx *c, bool conditional)
+{
+  const int *cond = conditional ? c->flags + F_REBUILD : nullptr;
+  const long tot = (long)c->n + c->gcap;
+  const BinRule rule = bin_rule(c);
+  k_zero_int<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->bin_cnt, c->nbins, cond);
+  k_bin_count<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, rule, c->x, c->y, c->z,
+                                                      c->bin_of, c->bin_cnt, c->flags, cond);
+  const int nb = grid_for(c->nbins);
+  CountOf cnt{c->bin_cnt};
+  k_block_sums<<<nb, kBlock, 0, c->stream>>>(c->nbins, cnt, c->scan_tmp, cond);
+  k_scan_blocks<<<1, kBlock, 0, c->stream>>>(nb, c->scan_tmp, c->flags + F_COUNT - 1, cond);
+  k_bin_start<<<nb, kBlock, 0, c->stream>>>(c->nbins, c->bin_cnt, c->scan_tmp, c->bin_start, cond);
+  k_zero_int<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->bin_cnt, c->nbins, cond);
+  k_bin_fill<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->bin_of, c->bin_start,
+                                                     c->bin_cnt, c->bin_items, c->flags, cond);
+  k_bin_sort<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->nbins, c->bin_start, c->bin_items,
+                                                          cond);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+// ------------------------------------------------------------- Verlet list --
+
+// Full Verlet rows: for owned i every owned j != i and every ghost within
+// cutoff_extra (strict, dr = pos_i - pos_j evaluated as the reference does,
+// neighborlist.cpp:531-566).  The reference's half list is the subset with
+// owned j > i plus every ghost; both directions are kept so that forces are
+// gathered per atom without atomics.  Rows are column-major: nl[k*n + i].
+__global__ void k_list_cells(int n, int cap, double rce2, const double *x, const double *y,
+                             const double *z, const int *bin_of, const int *st_off, const int *st,
+                             const int *start, const int *items, int *nl, int *nn, int *flags,
+                             const int *cond)
+{
+  if (cond && !*cond) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double px = x[i], py = y[i], pz = z[i];
+  const int b = bin_of[i];
+  int cnt = 0;
+  for (int s = st_off[b]; s < st_off[b + 1]; ++s) {
+    const int cell = st[s];
+    for (int t = start[cell]; t < start[cell + 1]; ++t) {
+      const int j = items[t];
+      if (j == i) continue;
+      const double dx = __dsub_rn(px, x[j]), dy = __dsub_rn(py, y[j]), dz = __dsub_rn(pz, z[j]);
+      if (dot_rn(dx, dy, dz) < rce2) {
+        if (cnt < cap) nl[(long)cnt * n + i] = j;
+        ++cnt;
+      }
+    }
+  }
+  nn[i] = cnt < cap ? cnt : cap;
+  if (cnt > cap) flags[F_OVERFLOW] = 1;
+  atomicMax(flags + F_MAXNB, cnt);
+}
+
+// all_atom_test_function (neighborlist.cpp:161-191): every owned j != i and
+// every ghost, no distance test.
+__global__ void k_list_all(int n, int cap, int *nl, int *nn, int *flags, const int *cond)
+{
+  if (cond && !*cond) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int tot = n + flags[F_NGHOST];
+  int cnt = 0;
+  for (int j = 0; j < tot; ++j) {
+    if (j == i) continue;
+    if (cnt < cap) nl[(long)cnt * n + i] = j;
+    ++cnt;
+  }
+  nn[i] = cnt < cap ? cnt : cap;
+  if (cnt > cap) flags[F_OVERFLOW] = 1;
+  atomicMax(flags + F_MAXNB, cnt);
+}
+
+int launch_list(cav_ctx *c, bool conditional)
+{
+  const int *cond = conditional ? c->flags + F_REBUILD : nullptr;
+  if (c->all_atom_test)
+    k_list_all<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->nl_cap, c->nl, c->nn, c->flags,
+                                                        cond);
+  else
+    k_list_cells<<<grid_for(c->n), kBlock, 0, c->stream>>>(
+        c->n, c->nl_cap, c->rce2, c->x, c->y, c->z, c->bin_of, c->d_st_off, c->d_st, c->bin_start,
+        c->bin_items, c->nl, c->nn, c->flags, cond);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+// pos_old = pos, ghost_pos_old = ghost pos (neighborlist.cpp:302-303,568-569).
+__global__ void k_save_old(int n, long cap_total, const double *x, const double *y, const double *z,
+                           double *xo, double *yo, double *zo, int *flags, const int *cond)
+{
+  if (cond && !*cond) return;
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int G = flags[F_NGHOST];
+  if (a == 0) flags[F_NGHOST_OLD] = G;
+  if (a >= (long)n + G || a >= cap_total) return;
+  xo[a] = x[a];
+  yo[a] = y[a];
+  zo[a] = z[a];
+}
+
+int launch_save_old(cav_ctx *c, bool conditional)
+{
+  if (c->all_atom_test) return CAV_OK;  // the reference returns before saving
+  const int *cond = conditional ? c->flags + F_REBUILD : nullptr;
+  const long tot = (long)c->n + c->gcap;
+  k_save_old<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->x, c->y, c->z, c->xo, c->yo,
+                                                     c->zo, c->flags, cond);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+}  // namespace cav

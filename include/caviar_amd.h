@@ -1,0 +1,140 @@
+/* caviar_amd: MI355X (gfx950) short-range + Ewald force path of CAVIAR,
+ * exposed as a plain C ABI (no torch types, plain pointers and sizes).
+ *
+ * One context = one single-domain system resident on one GPU.  Host buffers
+ * passed in and out use the reference's layout: AoS xyz doubles, owned atoms
+ * 0..n-1, ghost atoms n+g.  Every function returns CAV_OK or an error code
+ * and never exits; cav_last_error() gives the message (the reference prints
+ * and calls exit(1) instead, interpreter/error.cpp:27-30,175-202).
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference tree, src/caviar/objects/ unless given in full).
+ */
+#ifndef CAVIAR_AMD_H
+#define CAVIAR_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cav_ctx cav_ctx;
+
+enum cav_status {
+  CAV_OK = 0,
+  CAV_ERR_ARG = 1,      /* invalid argument or setting */
+  CAV_ERR_HIP = 2,      /* HIP runtime failure */
+  CAV_ERR_STATE = 3,    /* call out of order (e.g. step before setup) */
+  CAV_ERR_NAN = 4,      /* NaN acceleration before a drift (md_simulator.cpp:584-590) */
+  CAV_ERR_CAPACITY = 5  /* a device buffer could not be grown */
+};
+
+enum cav_precision {
+  CAV_FP64 = 0,  /* plain form: every term in FP64 */
+  CAV_MIXED = 1  /* FP32 pair/erfc/LJ math and FP32 k-space products; FP64 S(k)
+                    accumulation across atom blocks, energy/virial, integration */
+};
+
+enum cav_array { CAV_POS = 0, CAV_VEL = 1, CAV_ACC = 2, CAV_ACC_OLD = 3 };
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Replaces the objects the interpreter creates (object_creator/objects_types.cpp:25-51). */
+int cav_create(int device, cav_ctx **out);
+int cav_destroy(cav_ctx *ctx);
+const char *cav_last_error(const cav_ctx *ctx);
+int cav_set_precision(cav_ctx *ctx, int precision);
+
+/* ---- settings (the CASL object commands) ------------------------------- */
+/* domain box: xmin..zmax, boundary_condition (domain.cpp:206-266). */
+int cav_set_box(cav_ctx *ctx, const double lower[3], const double upper[3], const int bc[3]);
+/* atom_data basic: ghost_cutoff (atom_data.cpp:87-91). */
+int cav_set_ghost_cutoff(cav_ctx *ctx, double ghost_cutoff);
+/* atom_data basic: add_type_mass / add_type_charge (atom_data.cpp:206-245). */
+int cav_set_types(cav_ctx *ctx, int ntypes, const double *mass, const double *charge);
+/* neighborlist basic: cutoff, cutoff_extra_coef, make_verlet_list_from_cell_list,
+ * build_cell_list, all_atom_test, rebuild_test (neighborlist.cpp:193-251). */
+int cav_set_neighbor(cav_ctx *ctx, double cutoff, double cutoff_extra_coef, int from_cells,
+                     int build_cells, int all_atom_test, int rebuild_test);
+/* force_field lj_cell_list: cutoff, epsilon i j, sigma i j (lj_cell_list.cpp:33-76);
+ * eps/sigma are ntypes*ntypes row-major [ti][tj]. */
+int cav_set_lj(cav_ctx *ctx, double cutoff, const double *epsilon, const double *sigma);
+/* force_field electrostatic_ewald_r: alpha, k_electrostatic, lambda i j
+ * (electrostatic_ewald_r.cpp:31-110); lambda may be NULL. */
+int cav_set_ewald_r(cav_ctx *ctx, double alpha, double k_electrostatic, const double *lambda);
+/* force_field electrostatic_ewald_k: alpha, k_electrostatic, kx/ky/kz_max,
+ * epsilon_dipole (dipole != 0), slab_normal_axis (-1 = none), slab_lz
+ * (electrostatic_ewald_k.cpp:31-139). */
+int cav_set_ewald_k(cav_ctx *ctx, double alpha, double k_electrostatic, int kx_max, int ky_max,
+                    int kz_max, int dipole, double epsilon_dipole, int slab_normal_axis,
+                    double slab_lz);
+
+/* ---- atoms --------------------------------------------------------------- */
+/* Atom_data::add_atom for n atoms at once (atom_data.cpp:487-521); mol may be
+ * NULL (molecule index -1, as add_atom sets).  Host AoS input. */
+int cav_upload_atoms(cav_ctx *ctx, int n, const int *id, const int *type, const int *mol,
+                     const double *pos, const double *vel);
+/* Read / overwrite one owned array (atom_struct_owned.position, velocity,
+ * acceleration, acceleration_old), host AoS 3n doubles. */
+int cav_download(cav_ctx *ctx, int which, double *out);
+int cav_upload(cav_ctx *ctx, int which, const double *in);
+/* Number of owned atoms. */
+int cav_num_atoms(const cav_ctx *ctx, int *n);
+
+/* ---- the MD path ----------------------------------------------------------- */
+/* Md_simulator::setup (md_simulator.cpp:504-549): neighbour init, wrap, ghosts,
+ * forced list build, reset acceleration, all force fields. */
+int cav_setup(cav_ctx *ctx);
+/* nsteps velocity-Verlet steps, Md_simulator::step -> integrate_velocity_verlet
+ * (md_simulator.cpp:382-617).  On CAV_ERR_NAN the state is that before the
+ * offending step. */
+int cav_step(cav_ctx *ctx, double dt, int nsteps);
+/* re_calculate_acc without the drift (md_simulator.cpp:361-374): wrap, ghosts,
+ * Neighborlist::build(update), reset acceleration, every force field. */
+int cav_recalculate(cav_ctx *ctx, int force_rebuild);
+/* Neighborlist::build(update) alone (neighborlist.cpp:66-88) on the current
+ * ghosts; *rebuilt (may be NULL) receives whether the Verlet list was rebuilt. */
+int cav_neighbor_build(cav_ctx *ctx, int update, int *rebuilt);
+
+/* ---- inspection (parity) ------------------------------------------------- */
+/* atom_struct_ghost after exchange_ghost (exchange_ghost_single_md_domain.cpp):
+ * positions (3g doubles) and the owned index each ghost images (g ints). */
+int cav_num_ghosts(cav_ctx *ctx, int *g);
+int cav_get_ghosts(cav_ctx *ctx, double *gpos, int *gsrc);
+/* Neighborlist::neighlist as (i,j) pairs with the reference's membership
+ * (owned j>i, every ghost j as n+g; neighborlist.cpp:257-307,515-573).
+ * Order within an atom is not the reference's; compare sorted. */
+int cav_num_pairs(cav_ctx *ctx, long *npairs);
+int cav_get_pairs(cav_ctx *ctx, int *ij, long capacity, long *npairs);
+/* Atom_data::virialForce accumulated by the last force evaluation (atom_data.h:250). */
+int cav_virial(cav_ctx *ctx, double *virial);
+/* Electrostatic_ewald_k::potential_k_coef_cmplx, i.e. S(k) in the reference's
+ * k order (electrostatic_ewald_k/initialize.cpp:117-146); 2*nk doubles. */
+int cav_num_kvectors(cav_ctx *ctx, int *nk);
+int cav_get_sk(cav_ctx *ctx, double *sk);
+
+/* ---- energies and point queries (off the MD path) ------------------------ */
+/* Electrostatic_ewald_r::energy (electrostatic_ewald_r/energy.cpp:30-51),
+ * Electrostatic_ewald_k::energy (electrostatic_ewald_k/energy.cpp:109-116) and
+ * the shifted-force LJ energy matching Lj_cell_list's force (no reference
+ * counterpart; the reference's Lj_cell_list has no energy). NULL outputs skipped. */
+int cav_energy(cav_ctx *ctx, double *e_lj, double *e_r, double *e_k);
+/* Batched Force_field::field(const Vector&) / potential(const Vector&):
+ * which 0 = Electrostatic_ewald_r (electrostatic_ewald_r/field.cpp:31-96,
+ * potential.cpp:30-86), 1 = Electrostatic_ewald_k (electrostatic_ewald_k/
+ * field.cpp:80-132, potential.cpp:31-129).  np points (AoS 3np doubles). */
+int cav_point_field(cav_ctx *ctx, int which, int np, const double *pts, double *field,
+                    double *potential);
+
+/* ---- measurement ----------------------------------------------------------- */
+/* Statistics of the current Verlet list: list entries per atom with the
+ * reference's half-list membership, and those inside the LJ cutoff. */
+int cav_list_stats(cav_ctx *ctx, double *pairs_per_atom, double *lj_pairs_per_atom);
+/* Average device time (ms) per launch of each kernel family over the last
+ * cav_step call, measured with HIP events on the context's stream.
+ * names: "pair", "ewald_k_sk", "ewald_k_force", "ewald_k_tables", "neighbor", "integrate". */
+int cav_kernel_time(cav_ctx *ctx, const char *name, double *ms_per_launch, int *launches);
+int cav_enable_timing(cav_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,188 @@
+"""Parity of the HIP path (through the C ABI) with the reference's results.
+
+The golden fixtures hold the reference library's own outputs on each input
+(tests/golden/make_golden.py); the oracle, itself pinned bit-for-bit to those
+fixtures, supplies the same quantities for inputs generated here.
+
+Bars (written in each assertion):
+* ghost images: bit-exact positions and source ids;
+* Verlet pair sets: bit-exact as sorted (i, j) lists;
+* FP64 accelerations, virial, energies, S(k), point fields: 1e-12 relative
+  (norm-wise), trajectories after tens of steps: 1e-9;
+* mixed form: force RMS relative error <= 1e-5 against FP64.
+"""
+import numpy as np
+import pytest
+
+import fixtures as F
+from caviar_amd import Engine, NaNAccelerationError, configs
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+SETUP = [n for n in F.names() if n.endswith("_setup")]
+RUNS = [n for n in F.names() if "_run" in n]
+
+
+def engine_for(name, precision="fp64"):
+    s, ref, meta = F.load(name)
+    print(f"[case] {name} n={s.n} precision={precision}", flush=True)
+    e = Engine(s, precision=precision)
+    e.setup()
+    if meta["run_steps"]:
+        e.step(meta["dt"], meta["run_steps"])
+    return e, s, ref, meta
+
+
+@pytest.mark.parametrize("name", SETUP)
+def test_setup_parity_fp64(gpu, name):
+    e, s, ref, meta = engine_for(name)
+    gp, gs = e.ghosts()
+    assert np.array_equal(gp, ref["ghost_pos"])                       # bit-exact
+    assert np.array_equal(s.ids[gs], ref["ghost_id"])
+    assert np.array_equal(F.sorted_pairs(e.pairs()), F.sorted_pairs(ref["pairs"]))  # bit-exact set
+    assert F.rel_err(e.get("acc"), ref["acc"]) <= 1e-12
+    v = float(ref["virial"])
+    assert abs(e.virial() - v) <= 1e-12 * max(abs(v), 1e-300) or v == e.virial()
+    if "energy_r" in ref or "energy_k" in ref:
+        elj, er, ek = e.energy(lj=False, r="energy_r" in ref, k="energy_k" in ref)
+        if "energy_r" in ref:
+            assert abs(er - ref["energy_r"]) <= 1e-12 * abs(ref["energy_r"])
+        if "energy_k" in ref:
+            assert abs(ek - ref["energy_k"]) <= 1e-12 * abs(ref["energy_k"])
+    if "sk" in ref:
+        sk = e.sk()
+        assert F.rel_err(np.c_[sk.real, sk.imag], ref["sk"]) <= 1e-12
+    shifted = e.get("pos") + np.array([0.1, -0.2, 0.3])
+    if "field_k" in ref:
+        f, p = e.point_field("k", shifted)
+        assert F.rel_err(f, ref["field_k"][:, :3]) <= 1e-12
+        assert F.rel_err(p, ref["field_k"][:, 3]) <= 1e-12
+    if "field_r" in ref:
+        f, p = e.point_field("r", shifted)
+        assert F.rel_err(f, ref["field_r"][:, :3]) <= 1e-12
+        assert F.rel_err(p, ref["field_r"][:, 3]) <= 1e-12
+
+
+@pytest.mark.parametrize("name", RUNS)
+def test_trajectory_parity_fp64(gpu, name):
+    e, s, ref, meta = engine_for(name)
+    assert F.rel_err(e.get("pos"), ref["pos"]) <= 1e-9
+    assert F.rel_err(e.get("vel"), ref["vel"]) <= 1e-9
+    assert F.rel_err(e.get("acc"), ref["acc"]) <= 1e-9
+    assert np.array_equal(F.sorted_pairs(e.pairs()), F.sorted_pairs(ref["pairs"]))
+
+
+def test_b2_published_known_answer(gpu):
+    pub = F.published_b2()
+    s = configs.b2_ewald(pub["pos"], pub["q"], kmax=25)
+    e = Engine(s)
+    e.setup()
+    _, er, ek = e.energy(lj=False)
+    assert abs(er + ek - float(pub["energy"])) <= 2e-12 * abs(float(pub["energy"]))
+    assert F.rel_err(e.get("acc"), pub["forces"]) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["ions600_setup", "ions2000_setup", "aniso400_setup",
+                                  "lambda_dipole300_setup", "small200_setup", "lj4000_setup"])
+def test_mixed_precision_force_error(gpu, name):
+    e64, s, ref, meta = engine_for(name)
+    e32, _, _, _ = engine_for(name, "mixed")
+    assert np.array_equal(F.sorted_pairs(e32.pairs()), F.sorted_pairs(ref["pairs"]))
+    a64, a32 = e64.get("acc"), e32.get("acc")
+    rms = np.sqrt(((a32 - a64) ** 2).sum(axis=1).mean() / (a64 ** 2).sum(axis=1).mean())
+    assert rms <= 1e-5                     # force RMS relative error bar of the mixed form
+    v64, v32 = e64.virial(), e32.virial()
+    assert abs(v32 - v64) <= 1e-5 * max(abs(v64), 1.0)
+
+
+def test_fp64_is_bitwise_reproducible(gpu):
+    s = configs.ions(n=3000, kmax=6, seed=9)
+    runs = []
+    for _ in range(2):
+        e = Engine(s)
+        e.setup()
+        e.step(0.002, 5)
+        runs.append((e.get("pos"), e.get("acc"), e.virial()))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2]
+
+
+def test_against_oracle_generated_inputs(gpu):
+    """Inputs made here (not fixtures): unequal kmax per axis with a slab
+    length, and a box with the lower corner away from the origin."""
+    s = configs.ions(n=700, kmax=4, seed=77)
+    shift = np.array([-3.0, 5.5, 1.25])
+    s.lower = s.lower + shift
+    s.upper = s.upper + shift
+    s.pos = s.pos + shift
+    s.ewald_k.kx_max, s.ewald_k.ky_max, s.ewald_k.kz_max = 3, 5, 4
+    o = Oracle(s)
+    o.setup()
+    e = Engine(s)
+    e.setup()
+    assert np.array_equal(F.sorted_pairs(e.pairs()), F.sorted_pairs(o.pairs()))
+    assert F.rel_err(e.get("acc"), o.get("acc")) <= 1e-12
+    sk = e.sk()
+    osk = o.sk()
+    assert F.rel_err(np.c_[sk.real, sk.imag], np.c_[osk.real, osk.imag]) <= 1e-12
+    _, er, ek = e.energy(lj=False)
+    assert abs(ek - o.energy_k()) <= 1e-12 * abs(o.energy_k())
+    assert abs(er - o.energy_r()) <= 1e-12 * abs(o.energy_r())
+    # run both for 25 steps (list rebuilds included) and compare
+    assert o.run(0.002, 25) == 0
+    e.step(0.002, 25)
+    assert F.rel_err(e.get("pos"), o.get("pos")) <= 1e-9
+    assert F.rel_err(e.get("acc"), o.get("acc")) <= 1e-9
+
+
+def test_lj_energy_matches_oracle(gpu):
+    s = configs.lj_fcc(ncell=4, seed=5)
+    o = Oracle(s)
+    o.setup()
+    e = Engine(s)
+    e.setup()
+    elj, _, _ = e.energy(lj=True, r=False, k=False)
+    assert abs(elj - o.energy_lj()) <= 1e-12 * abs(o.energy_lj())
+
+
+def test_lj_cell_walk_path(gpu):
+    """LJ cutoff above the list cutoff: the cell-walk kernel (every step) is used."""
+    s = configs.ions(n=500, kmax=3, seed=3, nl_cutoff=2.0, rc_lj=2.5)
+    o = Oracle(s)
+    o.setup()
+    e = Engine(s)
+    e.setup()
+    assert F.rel_err(e.get("acc"), o.get("acc")) <= 1e-12
+    assert o.run(0.002, 10) == 0
+    e.step(0.002, 10)
+    assert F.rel_err(e.get("acc"), o.get("acc")) <= 1e-9
+
+
+def test_nan_acceleration_is_reported_and_state_kept(gpu):
+    s, ref, meta = F.load("lj256_setup")
+    e = Engine(s)
+    e.setup()
+    acc = e.get("acc")
+    acc[3, 1] = np.nan
+    e.set("acc", acc)
+    before = e.get("pos")
+    with pytest.raises(NaNAccelerationError):
+        e.step(0.005, 1)
+    assert np.array_equal(e.get("pos"), before)
+
+
+def test_list_capacity_growth_is_transparent(gpu):
+    """A compressed configuration overflows the initial row capacity; the step
+    is redone with a larger list and matches the oracle."""
+    s = configs.ions(n=800, kmax=3, seed=13)
+    # squeeze half the atoms into one corner (still >= 0.8 apart: scaled lattice)
+    g = np.stack(np.meshgrid(*[np.arange(8) * 0.82] * 3, indexing="ij"), -1).reshape(-1, 3)[:400]
+    s.pos[:400] = g + 0.5
+    o = Oracle(s)
+    o.setup()
+    e = Engine(s)
+    e.setup()
+    assert np.array_equal(F.sorted_pairs(e.pairs()), F.sorted_pairs(o.pairs()))
+    assert F.rel_err(e.get("acc"), o.get("acc")) <= 1e-12

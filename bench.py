@@ -50,7 +50,7 @@ def run_engine(system, precision, warmup, steps, dt):
     e.step(dt, steps)          # every step ends with a stream synchronisation
     t = time.perf_counter() - t0
     timings = {k: e.kernel_time(k) for k in ("pair", "ewald_k_sk", "ewald_k_force",
-                                              "ewald_k_tables", "neighbor", "integrate")}
+                                              "ewald_k_tables", "neighbor", "list", "integrate")}
     e.enable_timing(False)
     log(f"{precision}: {steps} steps in {t:.3f}s -> {system.n * steps / t:.4e} atom-steps/s; "
         + ", ".join(f"{k} {v[0]:.3f}ms" for k, v in timings.items()))

@@ -67,11 +67,15 @@ def random_nonoverlap(n: int, box: float, dmin: float, seed: int, lower: float =
 
 
 def lj_fcc(ncell: int = 10, rho: float = 0.8442, temperature: float = 0.72, seed: int = 1,
-           rc: float = 2.5, skin: float = 0.3) -> System:
+           rc: float = 2.5, skin: float = 0.3, jitter: float = 0.0) -> System:
+    """fcc LJ fluid; ``jitter`` > 0 displaces every site by N(0, jitter^2) per
+    component (a perfect lattice has zero net force on every atom)."""
     n = 4 * ncell ** 3
     box = (n / rho) ** (1.0 / 3.0)
     rng = np.random.default_rng(seed)
     pos = fcc_lattice(ncell, box) + 0.5 * box / ncell * 0.25  # off the box faces
+    if jitter > 0:
+        pos = pos + np.random.default_rng(seed + 500).normal(scale=jitter, size=pos.shape)
     pos = np.mod(pos, box)
     types = np.zeros(n, dtype=np.int32)
     mass = np.array([1.0])

@@ -167,7 +167,7 @@ static void free_all(cav_ctx *c)
                   c->bin_cnt, c->bin_start, c->bin_items, c->d_st_off, c->d_st,  c->scan_tmp,
                   c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
-                  c->d_W,     c->kacc};
+                  c->d_W,     c->kacc,      c->order,    c->ord_blk};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -320,9 +320,11 @@ static int neighbor_phase(cav_ctx *c, bool forced)
   timer_begin(c, c->t_nb);
   // cells are needed on rebuild steps, and every step by the LJ cell walk
   CAV_TRY(launch_bins(c, !c->lj_cellwalk));
+  timer_end(c, c->t_nb);
+  timer_begin(c, c->t_list);
   CAV_TRY(launch_list(c, true));
   CAV_TRY(launch_save_old(c, true));
-  timer_end(c, c->t_nb);
+  timer_end(c, c->t_list);
   return CAV_OK;
 }
 
@@ -361,6 +363,7 @@ static void harvest_timers(cav_ctx *c)
   timer_harvest(c->t_tab);
   timer_harvest(c->t_nb);
   timer_harvest(c->t_int);
+  timer_harvest(c->t_list);
 }
 
 // ------------------------------------------------------------------- C ABI --
@@ -547,6 +550,8 @@ int cav_upload_atoms(cav_ctx *c, int n, const int *id, const int *type, const in
   CAV_TRY(dmalloc(c, &c->bin_of, tot));
   CAV_TRY(dmalloc(c, &c->bin_items, tot));
   CAV_TRY(dmalloc(c, &c->nn, n));
+  CAV_TRY(dmalloc(c, &c->order, n));
+  CAV_TRY(dmalloc(c, &c->ord_blk, grid_for((long)tot) + 1));
   c->nblk = grid_for(n) + 1;
   CAV_TRY(dmalloc(c, &c->blk, c->nblk));
   std::vector<double> soa(3 * (size_t)n), qv(n);
@@ -941,7 +946,7 @@ int cav_enable_timing(cav_ctx *c, int on)
 {
   if (!c) return CAV_ERR_ARG;
   c->timing = on != 0;
-  Timer *ts[] = {&c->t_pair, &c->t_sk, &c->t_force, &c->t_tab, &c->t_nb, &c->t_int};
+  Timer *ts[] = {&c->t_pair, &c->t_sk, &c->t_force, &c->t_tab, &c->t_nb, &c->t_int, &c->t_list};
   for (Timer *t : ts) {
     t->total_ms = 0;
     t->launches = 0;
@@ -960,6 +965,7 @@ int cav_kernel_time(cav_ctx *c, const char *name, double *ms, int *launches)
   else if (!strcmp(name, "ewald_k_tables")) t = &c->t_tab;
   else if (!strcmp(name, "neighbor")) t = &c->t_nb;
   else if (!strcmp(name, "integrate")) t = &c->t_int;
+  else if (!strcmp(name, "list")) t = &c->t_list;
   if (!t) return fail(c, CAV_ERR_ARG, "unknown kernel family");
   if (ms) *ms = t->launches ? t->total_ms / t->launches : 0.0;
   if (launches) *launches = t->launches;

@@ -42,6 +42,15 @@ template <> struct Mfma<float> {
 
 template <typename T> struct Cx { T re, im; };
 
+// Output row held in element e of lane group lk = lane/16 of a 16x16x4 MFMA:
+// the f32 form keeps four consecutive rows per lane group, the f64 form
+// interleaves them (row = lk + 4e).  Columns are lane%16 in both.
+template <typename T>
+__device__ __forceinline__ int mfma_row(int lk, int e)
+{
+  return sizeof(T) == 8 ? lk + 4 * e : 4 * lk + e;
+}
+
 // ------------------------------------------------------------- phase tables --
 
 // Per atom a record of TS complex values [X(0..kx) | Y(-ky..ky) | Z(-kz..kz)].
@@ -203,7 +212,7 @@ __global__ __launch_bounds__(64 * kWavesS) void k_sk(SArgs a)
     }
   }
   if (rt0 >= a.Rtiles) return;
-  // D[row = 4(l/16)+e][col = l%16]
+  // D[row = mfma_row(l/16, e)][col = l%16]
 #pragma unroll
   for (int r = 0; r < RT; ++r) {
     if (rt0 + r >= a.Rtiles) break;
@@ -211,7 +220,7 @@ __global__ __launch_bounds__(64 * kWavesS) void k_sk(SArgs a)
     for (int s = 0; s < SCT; ++s)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = (rt0 + r) * 16 + 4 * lk + e;
+        const int row = (rt0 + r) * 16 + mfma_row<T>(lk, e);
         const int col = s * 16 + lr;
         const double v = sizeof(T) == 8 ? (double)acc[r][s][e] : acc64[r][s][e];
         a.part[((long)chunk * a.Rpad + row) * a.SC + col] = v;
@@ -341,7 +350,7 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
         for (int tau = 0; tau < 2; ++tau) acc[t][tau] = Mfma<T>::op(am[tau][s], b, acc[t][tau]);
       }
     }
-    // epilogue: lane holds rows r = rb*16 + l%16 for atoms 4(l/16)+e of each tile
+    // epilogue: lane holds rows r = rb*16 + l%16 for atoms mfma_row(l/16, e) of each tile
     const int r = rb * 16 + lr;
     if (r < a.R) {
       const int nxi = r / a.nY, nyi = r - (r / a.nY) * a.nY;
@@ -350,7 +359,7 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
       for (int tau = 0; tau < 2; ++tau)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const long atom = base + tau * 16 + 4 * lk + e;
+          const long atom = base + tau * 16 + mfma_row<T>(lk, e);
           if (atom >= a.n) continue;
           const Cx<T> X = tab[atom * a.TS + nxi], Y = tab[atom * a.TS + offY + nyi];
           const T pr = X.re * Y.re - X.im * Y.im;
@@ -378,7 +387,7 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
         vz += __shfl_xor(vz, o, 64);
         vp += __shfl_xor(vp, o, 64);
       }
-      const long atom = base + tau * 16 + 4 * lk + e;
+      const long atom = base + tau * 16 + mfma_row<T>(lk, e);
       if (lr == 0 && atom < a.n) {
         const double gx = a.c0 * a.kxs * vx, gy = a.c0 * a.kys * vy, gz = a.c0 * a.kzs * vz;
         if (a.out4) {

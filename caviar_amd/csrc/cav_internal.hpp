@@ -115,6 +115,8 @@ struct cav_ctx {
   int *bin_of = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_items = nullptr;
   int *d_st_off = nullptr, *d_st = nullptr;
   int *scan_tmp = nullptr;
+  int *order = nullptr;    // owned atoms in cell order (compaction of bin_items)
+  int *ord_blk = nullptr;  // scan block sums for the compaction
 
   // list (column-major: nl[k*n + i])
   int *nl = nullptr, *nn = nullptr;
@@ -146,7 +148,7 @@ struct cav_ctx {
   bool cells_current = false;
 
   bool timing = false;
-  cav::Timer t_pair, t_sk, t_force, t_tab, t_nb, t_int;
+  cav::Timer t_pair, t_sk, t_force, t_tab, t_nb, t_int, t_list;
 };
 
 // helpers shared by the translation units

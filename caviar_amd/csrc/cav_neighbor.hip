@@ -281,6 +281,30 @@ __global__ void k_bin_sort(int nbins, const int *start, int *items, const int *c
   }
 }
 
+// Owned atoms in cell order: stable compaction of bin_items (ghost entries
+// dropped).  Consecutive lanes then walk overlapping stencils.
+__global__ void k_order_scatter(long cap_total, int n, const int *items, const int *blk, int *order,
+                                const int *flags, const int *cond)
+{
+  if (cond && !*cond) return;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long tot = (long)n + flags[F_NGHOST];
+  const int a = (t < tot && t < cap_total) ? items[t] : n;
+  const int f = a < n ? 1 : 0;
+  const int off = block_exclusive_scan(f, nullptr) + blk[blockIdx.x];
+  if (f) order[off] = a;
+}
+
+struct OwnedFlagBounded {
+  const int *items;
+  const int *flags;
+  int n;
+  __device__ __forceinline__ int operator()(long t) const
+  {
+    return (t < (long)n + flags[F_NGHOST] && items[t] < n) ? 1 : 0;
+  }
+};
+
 int launch_bins(cav_ctx *c, bool conditional)
 {
   const int *cond = conditional ? c->flags + F_REBUILD : nullptr;
@@ -299,6 +323,12 @@ int launch_bins(cav_ctx *c, bool conditional)
                                                      c->bin_cnt, c->bin_items, c->flags, cond);
   k_bin_sort<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->nbins, c->bin_start, c->bin_items,
                                                           cond);
+  const int nbo = grid_for(tot);
+  OwnedFlagBounded of{c->bin_items, c->flags, c->n};
+  k_block_sums<<<nbo, kBlock, 0, c->stream>>>(tot, of, c->ord_blk, cond);
+  k_scan_blocks<<<1, kBlock, 0, c->stream>>>(nbo, c->ord_blk, c->flags + F_COUNT - 2, cond);
+  k_order_scatter<<<nbo, kBlock, 0, c->stream>>>(tot, c->n, c->bin_items, c->ord_blk, c->order,
+                                                 c->flags, cond);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }
@@ -312,12 +342,13 @@ int launch_bins(cav_ctx *c, bool conditional)
 // gathered per atom without atomics.  Rows are column-major: nl[k*n + i].
 __global__ void k_list_cells(int n, int cap, double rce2, const double *x, const double *y,
                              const double *z, const int *bin_of, const int *st_off, const int *st,
-                             const int *start, const int *items, int *nl, int *nn, int *flags,
-                             const int *cond)
+                             const int *start, const int *items, const int *order, int *nl,
+                             int *nn, int *flags, const int *cond)
 {
   if (cond && !*cond) return;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const int i = order[t];
   const double px = x[i], py = y[i], pz = z[i];
   const int b = bin_of[i];
   int cnt = 0;
@@ -366,7 +397,7 @@ int launch_list(cav_ctx *c, bool conditional)
   else
     k_list_cells<<<grid_for(c->n), kBlock, 0, c->stream>>>(
         c->n, c->nl_cap, c->rce2, c->x, c->y, c->z, c->bin_of, c->d_st_off, c->d_st, c->bin_start,
-        c->bin_items, c->nl, c->nn, c->flags, cond);
+        c->bin_items, c->order, c->nl, c->nn, c->flags, cond);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }

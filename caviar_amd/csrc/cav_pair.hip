@@ -36,6 +36,7 @@ struct PairArgs {
   int n, nt, cap;
   const double *x, *y, *z, *q, *minv;
   const int *type, *id, *mol, *nl, *nn;
+  const int *order;  // owned atoms in cell order (lane t handles atom order[t])
   // LJ per type pair: coef(r) = -c12 r^-14 + c6 r^-8 + cshift
   const double *c12, *c6, *cshift;
   double lj_rc2;
@@ -51,9 +52,10 @@ struct PairArgs {
 template <typename T, bool LJ, bool ER, bool LAM>
 __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
 {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
   double vir = 0.0;
-  if (i < a.n) {
+  if (t < a.n) {
+    const int i = a.order[t];
     const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
     const int ti = a.type[i], idi = a.id[i];
     const int moli = LAM ? a.mol[i] : 0;
@@ -169,6 +171,7 @@ static PairArgs pair_args(cav_ctx *c)
   a.mol = c->mol;
   a.nl = c->nl;
   a.nn = c->nn;
+  a.order = c->order;
   const int nt2 = c->nt * c->nt;
   a.c12 = c->d_eps;           // laid out as [c12 | c6 | cshift] blocks, see upload
   a.c6 = c->d_eps + nt2;

@@ -240,11 +240,11 @@ def main():
     run_case("b2_setup", b2, "setup+er_energy+ek_energy+field")
 
     # 2. Plain LJ fcc (the LJ workload's recipe at 4x4x4 cells), setup and 30 steps.
-    lj = configs.lj_fcc(ncell=4, seed=3)
+    lj = configs.lj_fcc(ncell=4, seed=3, jitter=0.05)
     run_case("lj256_setup", lj, "setup")
     run_case("lj256_run30", lj, "run", run_steps=30, dt=0.005)
     # 3. The full LJ workload (N=4000), setup only: pair list, ghosts, forces.
-    run_case("lj4000_setup", configs.lj_fcc(ncell=10, seed=1), "setup")
+    run_case("lj4000_setup", configs.lj_fcc(ncell=10, seed=1, jitter=0.05), "setup")
 
     # 4. Charged LJ + Ewald at reduced N, cell list and brute-force list.
     ion = configs.ions(n=600, kmax=5, seed=11)

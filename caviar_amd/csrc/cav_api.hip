@@ -167,7 +167,8 @@ static void free_all(cav_ctx *c)
                   c->bin_cnt, c->bin_start, c->bin_items, c->d_st_off, c->d_st,  c->scan_tmp,
                   c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
-                  c->d_W,     c->kacc,      c->order,    c->ord_blk};
+                  c->d_W,     c->kacc,      c->order,    c->ord_blk,  c->xq,
+                  c->tid};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -551,6 +552,8 @@ int cav_upload_atoms(cav_ctx *c, int n, const int *id, const int *type, const in
   CAV_TRY(dmalloc(c, &c->bin_items, tot));
   CAV_TRY(dmalloc(c, &c->nn, n));
   CAV_TRY(dmalloc(c, &c->order, n));
+  CAV_TRY(dmalloc(c, &c->xq, tot));
+  CAV_TRY(dmalloc(c, &c->tid, tot));
   CAV_TRY(dmalloc(c, &c->ord_blk, grid_for((long)tot) + 1));
   c->nblk = grid_for(n) + 1;
   CAV_TRY(dmalloc(c, &c->blk, c->nblk));
@@ -654,7 +657,8 @@ int cav_setup(cav_ctx *c)
   // LJ over the list needs rc_lj <= the list's own cutoff (the skin guarantee of
   // the rebuild test); otherwise it walks the cells every step like the reference.
   c->lj_cellwalk = c->lj_on && (c->lj_rc > c->nl_cutoff || c->all_atom_test);
-  const int npart = 3 * grid_for(std::max<long>((long)c->n, 1L << 16)) + 64;
+  const int npart = (int)std::max<long>(3L * grid_for(std::max<long>((long)c->n, 1L << 16)),
+                                        (long)grid_for(16L * c->n)) + 64;
   CAV_TRY(dmalloc(c, &c->part, npart));
   c->npart = npart;
   if (c->ek_on) {
@@ -811,7 +815,7 @@ int cav_num_pairs(cav_ctx *c, long *np)
   long cnt = 0;
   for (int i = 0; i < c->n; ++i)
     for (int k = 0; k < nn[i]; ++k) {
-      const int j = nl[(size_t)k * c->n + i];
+      const int j = nl[(size_t)i * c->nl_cap + k];
       cnt += (j >= c->n || j > i);
     }
   *np = cnt;
@@ -827,7 +831,7 @@ int cav_get_pairs(cav_ctx *c, int *ij, long capacity, long *np)
   long w = 0;
   for (int i = 0; i < c->n; ++i)
     for (int k = 0; k < nn[i]; ++k) {
-      const int j = nl[(size_t)k * c->n + i];
+      const int j = nl[(size_t)i * c->nl_cap + k];
       if (!(j >= c->n || j > i)) continue;
       if (w < capacity) {
         ij[2 * w] = i;
@@ -856,7 +860,7 @@ int cav_list_stats(cav_ctx *c, double *ppa, double *lj_ppa)
   const double rc2 = c->lj_rc * c->lj_rc;
   for (int i = 0; i < c->n; ++i)
     for (int k = 0; k < nn[i]; ++k) {
-      const int j = nl[(size_t)k * c->n + i];
+      const int j = nl[(size_t)i * c->nl_cap + k];
       if (!(j >= c->n || j > i)) continue;
       ++half;
       const double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];

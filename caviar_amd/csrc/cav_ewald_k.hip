@@ -98,46 +98,62 @@ __global__ void k_tables(long npts, int TS, int kx, int ky, int kz, double ilx, 
 
 // ------------------------------------------------------------------- S pass --
 
-constexpr int kStage = 32;   // atoms staged in LDS per iteration (16 K-steps)
-constexpr int kWavesS = 8;   // waves per S-pass workgroup
+constexpr int kWavesMax = 8;  // waves per S-pass workgroup (upper bound)
 
 struct SArgs {
-  int n, TS, nX, nY, nZ, ky, kz, Rtiles, chunk;
+  int n, TS, nX, nY, nZ, ky, kz, Rtiles, chunk, stage, tiles_per_group;
   const double *q;
   const void *tab;
   double *part;  // [chunk][Rpad][SC]
   int Rpad, SC;
 };
 
+// Issue the asynchronous copy of `bytes` contiguous bytes (a multiple of 16,
+// source padded) from global memory straight into LDS, 16 bytes per lane.
+__device__ __forceinline__ void copy_to_lds_async(const unsigned char *src, unsigned char *dst,
+                                                  int bytes, int wave, int nwaves, int lane)
+{
+  for (int off = wave * 1024; off < bytes; off += nwaves * 1024) {
+    if (off + lane * 16 < bytes)
+      __builtin_amdgcn_global_load_lds((const void *)(src + off + lane * 16),
+                                       (__attribute__((address_space(3))) void *)(dst + off), 16,
+                                       0, 0);
+  }
+}
+
 // One workgroup: an atom chunk x a group of row tiles; wave w owns RT row
 // tiles and all SCT column tiles.  A (rows x (atom,re/im)) is generated from
 // the staged X, Y tables and charges, B ((atom,re/im) x (nz,re/im)) from Z.
+// Stages of atom records are double-buffered in LDS: the next stage streams
+// in from HBM (direct-to-LDS loads) while the current one feeds the MFMAs.
+// In the mixed form the MFMAs accumulate in FP32 over one chunk (<= 1024
+// atoms) and the chunk partials are summed in FP64.
 template <typename T, int RT, int SCT>
-__global__ __launch_bounds__(64 * kWavesS) void k_sk(SArgs a)
+__global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
 {
   typedef typename Mfma<T>::V V;
   extern __shared__ unsigned char smem_raw[];
-  Cx<T> *st = reinterpret_cast<Cx<T> *>(smem_raw);      // [kStage][TS]
-  T *sq = reinterpret_cast<T *>(st + kStage * a.TS);    // [kStage]
-  const Cx<T> *tab = reinterpret_cast<const Cx<T> *>(a.tab);
+  const int stage = a.stage;
+  const size_t stage_bytes = (size_t)stage * a.TS * sizeof(Cx<T>);
+  Cx<T> *buf[2] = {reinterpret_cast<Cx<T> *>(smem_raw),
+                   reinterpret_cast<Cx<T> *>(smem_raw + stage_bytes)};
+  T *sqb[2] = {reinterpret_cast<T *>(smem_raw + 2 * stage_bytes),
+               reinterpret_cast<T *>(smem_raw + 2 * stage_bytes) + stage};
+  const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const int chunk = blockIdx.x;
-  const int rt0 = (blockIdx.y * kWavesS + wave) * RT;
+  const int rt0 = blockIdx.y * a.tiles_per_group + wave * RT;
+  const int rt_end = min(a.Rtiles, (int)(blockIdx.y + 1) * a.tiles_per_group);
   const long a0 = (long)chunk * a.chunk;
   long a1 = a0 + a.chunk;
   if (a1 > a.n) a1 = a.n;
 
   V acc[RT][SCT];
-  double acc64[RT][SCT][4];
 #pragma unroll
   for (int r = 0; r < RT; ++r)
 #pragma unroll
-    for (int s = 0; s < SCT; ++s) {
-      acc[r][s] = V{0, 0, 0, 0};
-#pragma unroll
-      for (int e = 0; e < 4; ++e) acc64[r][s][e] = 0.0;
-    }
+    for (int s = 0; s < SCT; ++s) acc[r][s] = V{0, 0, 0, 0};
 
   // lane roles inside a 16x16x4 tile: A[row=l%16][k=l/16], B[k=l/16][col=l%16]
   const int lr = lane & 15, lk = lane >> 4;
@@ -147,7 +163,7 @@ __global__ __launch_bounds__(64 * kWavesS) void k_sk(SArgs a)
 #pragma unroll
   for (int r = 0; r < RT; ++r) {
     const int row = (rt0 + r) * 16 + lr;
-    rvalid[r] = (rt0 + r) < a.Rtiles && row < a.nX * a.nY;
+    rvalid[r] = (rt0 + r) < rt_end && row < a.nX * a.nY;
     rnx[r] = rvalid[r] ? row / a.nY : 0;
     rny[r] = rvalid[r] ? row % a.nY : 0;  // index into Y (ny + ky)
   }
@@ -161,68 +177,68 @@ __global__ __launch_bounds__(64 * kWavesS) void k_sk(SArgs a)
     zvalid[s] = zc[s] < a.nZ;
   }
   const int offY = a.nX, offZ = a.nX + a.nY;
+  const bool active = rt0 < rt_end;
 
-  for (long base = a0; base < a1; base += kStage) {
-    __syncthreads();
-    // stage kStage atom records (contiguous in the table) and their charges
-    const int cnt = (int)((a1 - base) < kStage ? (a1 - base) : kStage);
-    for (int t = threadIdx.x; t < kStage * a.TS; t += 64 * kWavesS) {
-      const int j = t / a.TS;
-      Cx<T> v{0, 0};
-      if (j < cnt) v = tab[base * a.TS + t];
-      st[t] = v;
-    }
-    if (threadIdx.x < kStage)
-      sq[threadIdx.x] = threadIdx.x < cnt ? (T)a.q[base + threadIdx.x] : (T)0;
-    __syncthreads();
-    if (rt0 >= a.Rtiles) continue;
+  auto issue = [&](long base, int b) {
+    const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
+    const int bytes = (int)(((size_t)cnt * a.TS * sizeof(Cx<T>) + 15) / 16 * 16);
+    copy_to_lds_async(tab + (size_t)base * a.TS * sizeof(Cx<T>),
+                      reinterpret_cast<unsigned char *>(buf[b]), bytes, wave, nwaves, lane);
+    if (threadIdx.x < stage)
+      sqb[b][threadIdx.x] = threadIdx.x < cnt ? (T)a.q[base + threadIdx.x] : (T)0;
+  };
+
+  int b = 0;
+  if (a0 < a1) issue(a0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (long base = a0; base < a1; base += stage) {
+    if (base + stage < a1) issue(base + stage, b ^ 1);  // streams in during the MFMAs below
+    const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
+    const Cx<T> *st = buf[b];
+    const T *sq = sqb[b];
+    if (active) {
 #pragma unroll 2
-    for (int ks = 0; ks < kStage / 2; ++ks) {
-      const int j = 2 * ks + jj;
-      const Cx<T> *rec = st + j * a.TS;
-      const T qj = sq[j];
-      T av[RT];
+      for (int ks = 0; ks < stage / 2; ++ks) {
+        const int j = 2 * ks + jj;
+        const bool jv = j < cnt;
+        const Cx<T> *rec = st + (jv ? j : 0) * a.TS;
+        const T qj = jv ? sq[j] : (T)0;
+        T av[RT];
 #pragma unroll
-      for (int r = 0; r < RT; ++r) {
-        const Cx<T> X = rec[rnx[r]], Y = rec[offY + rny[r]];
-        const T pr = X.re * Y.re - X.im * Y.im;
-        const T pi = X.re * Y.im + X.im * Y.re;
-        av[r] = rvalid[r] ? qj * (part ? pi : pr) : (T)0;
-      }
-#pragma unroll
-      for (int s = 0; s < SCT; ++s) {
-        const Cx<T> Z = rec[offZ + (zvalid[s] ? zc[s] : 0)];
-        // re column: (Zr, -Zi); im column: (Zi, Zr) along (re, im) of A
-        T bv = zt[s] == 0 ? (part ? -Z.im : Z.re) : (part ? Z.re : Z.im);
-        bv = zvalid[s] ? bv : (T)0;
-#pragma unroll
-        for (int r = 0; r < RT; ++r) acc[r][s] = Mfma<T>::op(av[r], bv, acc[r][s]);
-      }
-    }
-    if constexpr (sizeof(T) == 4) {
-      // mixed form: FP32 products within a stage, FP64 accumulation across stages
-#pragma unroll
-      for (int r = 0; r < RT; ++r)
+        for (int r = 0; r < RT; ++r) {
+          const Cx<T> X = rec[rnx[r]], Y = rec[offY + rny[r]];
+          const T pr = X.re * Y.re - X.im * Y.im;
+          const T pi = X.re * Y.im + X.im * Y.re;
+          av[r] = rvalid[r] ? qj * (part ? pi : pr) : (T)0;
+        }
 #pragma unroll
         for (int s = 0; s < SCT; ++s) {
+          const Cx<T> Z = rec[offZ + (zvalid[s] ? zc[s] : 0)];
+          // re column: (Zr, -Zi); im column: (Zi, Zr) along (re, im) of A
+          T bv = zt[s] == 0 ? (part ? -Z.im : Z.re) : (part ? Z.re : Z.im);
+          bv = (zvalid[s] && jv) ? bv : (T)0;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) acc64[r][s][e] += (double)acc[r][s][e];
-          acc[r][s] = V{0, 0, 0, 0};
+          for (int r = 0; r < RT; ++r) acc[r][s] = Mfma<T>::op(av[r], bv, acc[r][s]);
         }
+      }
     }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    b ^= 1;
   }
-  if (rt0 >= a.Rtiles) return;
+  if (!active) return;
   // D[row = mfma_row(l/16, e)][col = l%16]
 #pragma unroll
   for (int r = 0; r < RT; ++r) {
-    if (rt0 + r >= a.Rtiles) break;
+    if (rt0 + r >= rt_end) break;
 #pragma unroll
     for (int s = 0; s < SCT; ++s)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = (rt0 + r) * 16 + mfma_row<T>(lk, e);
         const int col = s * 16 + lr;
-        const double v = sizeof(T) == 8 ? (double)acc[r][s][e] : acc64[r][s][e];
+        const double v = (double)acc[r][s][e];
         a.part[((long)chunk * a.Rpad + row) * a.SC + col] = v;
       }
   }
@@ -231,10 +247,12 @@ __global__ __launch_bounds__(64 * kWavesS) void k_sk(SArgs a)
 // --------------------------------------------------- reduction and operand --
 
 // S = sum over chunks (fixed order, FP64); g = W conj(S); write the force-pass
-// operand B[kappa][col] (kappa = 2c+s, col = rb*64 + t*16 + rr) and the
-// per-thread k-space energy term W |S|^2.
+// operand B[kappa][col] (kappa = 2c+s, col = rb*64 + t*16 + rr) in fragment-major
+// order: for row block rb and component t, lane (kappa%4)*16 + rr holds its
+// KS values (kappa/4 = 0..) contiguously, so the force pass loads them as
+// 16-byte vectors.  Also the per-thread k-space energy term W |S|^2.
 template <typename T>
-__global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz, int KZ,
+__global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz, int KSP,
                             const double *part, const double *W, double *sk, T *bm,
                             double *epart)
 {
@@ -254,17 +272,16 @@ __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz
     e = w * (sr * sr + si * si);
     const double gr = w * sr, gi = -w * si, nz = (double)(c - kz);
     const int rb = r >> 4, rr = r & 15;
-    const long cols = 4L * Rpad;
-    const long k0 = 2L * c, k1 = 2L * c + 1;
-    const long cb = (long)rb * 64 + rr;
-    bm[k0 * cols + cb + 0] = (T)gr;
-    bm[k1 * cols + cb + 0] = (T)(-gi);
-    bm[k0 * cols + cb + 16] = (T)gi;
-    bm[k1 * cols + cb + 16] = (T)gr;
-    bm[k0 * cols + cb + 32] = (T)(nz * gr);
-    bm[k1 * cols + cb + 32] = (T)(-nz * gi);
-    bm[k0 * cols + cb + 48] = (T)(nz * gi);
-    bm[k1 * cols + cb + 48] = (T)(nz * gr);
+    // values for kappa = 2c (s=0) and 2c+1 (s=1) of the four components
+    const double v[4][2] = {{gr, -gi}, {gi, gr}, {nz * gr, -nz * gi}, {nz * gi, nz * gr}};
+#pragma unroll
+    for (int comp = 0; comp < 4; ++comp)
+#pragma unroll
+      for (int sidx = 0; sidx < 2; ++sidx) {
+        const int kap = 2 * c + sidx;
+        const int lanef = (kap & 3) * 16 + rr;
+        bm[(((long)rb * 4 + comp) * 64 + lanef) * KSP + (kap >> 2)] = (T)v[comp][sidx];
+      }
   }
   __shared__ double red[kBlock];
   red[threadIdx.x] = e;
@@ -286,10 +303,10 @@ __global__ void k_zero_t(long n, T *p)
 
 // --------------------------------------------------------------- force pass --
 
-constexpr int kWavesF = 4;  // waves per force-pass workgroup, 32 atoms each
+constexpr int kWavesF = 8;  // waves per force-pass workgroup (share each operand block)
 
 struct FArgs {
-  int n, TS, nX, nY, nZ, ky, R, Rpad, KZ;
+  int n, TS, nX, nY, nZ, ky, R, Rpad, KZ, KSP;
   const void *tab;
   const void *bm;
   const double *q, *minv;
@@ -299,21 +316,25 @@ struct FArgs {
   double *out4;  // optional per-atom (fx, fy, fz, phi) without charge/mass
 };
 
-template <typename T, int KS>
+template <typename T, int KS, int NT>
 __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
 {
   typedef typename Mfma<T>::V V;
   const Cx<T> *tab = reinterpret_cast<const Cx<T> *>(a.tab);
-  const T *bm = reinterpret_cast<const T *>(a.bm);
+  const unsigned char *bm = reinterpret_cast<const unsigned char *>(a.bm);
+  extern __shared__ unsigned char smem_raw[];
+  const int blk_elems = 4 * 64 * a.KSP;
+  T *lb[2] = {reinterpret_cast<T *>(smem_raw), reinterpret_cast<T *>(smem_raw) + blk_elems};
+  const int nwaves = blockDim.x >> 6;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const long base = ((long)blockIdx.x * kWavesF + wave) * 32;
+  const long base = ((long)blockIdx.x * nwaves + wave) * (16 * NT);
   const int offY = a.nX, offZ = a.nX + a.nY;
 
   // A fragments: A[atom = l%16][kappa = 4s + l/16] = (Zr, Zi)(c = kappa/2)
-  T am[2][KS];
+  T am[NT][KS];
 #pragma unroll
-  for (int tau = 0; tau < 2; ++tau) {
+  for (int tau = 0; tau < NT; ++tau) {
     const long atom = base + tau * 16 + lr;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
@@ -326,28 +347,50 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
       am[tau][s] = v;
     }
   }
-  T fx[2][4], fy[2][4], fz[2][4], ph[2][4];
+  T fx[NT][4], fy[NT][4], fz[NT][4], ph[NT][4];
 #pragma unroll
-  for (int tau = 0; tau < 2; ++tau)
+  for (int tau = 0; tau < NT; ++tau)
 #pragma unroll
     for (int e = 0; e < 4; ++e) fx[tau][e] = fy[tau][e] = fz[tau][e] = ph[tau][e] = 0;
 
-  const long cols = 4L * a.Rpad;
+  constexpr int VW = 16 / (int)sizeof(T);  // values per 16-byte vector
+  typedef T Tvw __attribute__((ext_vector_type(VW)));
   const int nrb = a.Rpad >> 4;
+  const int nks = a.KZ >> 2;  // K-steps in use (<= KS)
+  const int blk_bytes = blk_elems * (int)sizeof(T);
+  // the operand of row block rb streams into LDS while rb-1 is multiplied
+  copy_to_lds_async(bm, reinterpret_cast<unsigned char *>(lb[0]), blk_bytes, wave, nwaves, lane);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
   for (int rb = 0; rb < nrb; ++rb) {
-    V acc[4][2];
+    const int cur = rb & 1;
+    if (rb + 1 < nrb)
+      copy_to_lds_async(bm + (size_t)(rb + 1) * blk_bytes,
+                        reinterpret_cast<unsigned char *>(lb[cur ^ 1]), blk_bytes, wave, nwaves,
+                        lane);
+    V acc[4][NT];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = V{0, 0, 0, 0};
-    const T *bcol = bm + (long)rb * 64 + lr;
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      if (4 * s >= a.KZ) break;
-      const T *brow = bcol + (long)(4 * s + lk) * cols;
+      for (int tau = 0; tau < NT; ++tau) acc[t][tau] = V{0, 0, 0, 0};
+    // operand fragments of this row block: component t, this lane, KSP values
+    const T *bfr = lb[cur] + lane * a.KSP;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const T b = brow[t * 16];
+    for (int q = 0; q < (KS + VW - 1) / VW; ++q) {
+      if (VW * q >= nks) break;
+      Tvw bq[4];
 #pragma unroll
-        for (int tau = 0; tau < 2; ++tau) acc[t][tau] = Mfma<T>::op(am[tau][s], b, acc[t][tau]);
+      for (int t = 0; t < 4; ++t)
+        bq[t] = *reinterpret_cast<const Tvw *>(bfr + t * 64 * a.KSP + VW * q);
+#pragma unroll
+      for (int u = 0; u < VW; ++u) {
+        const int s = VW * q + u;
+        if (s >= nks || s >= KS) break;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int tau = 0; tau < NT; ++tau)
+            acc[t][tau] = Mfma<T>::op(am[tau][s], bq[t][u], acc[t][tau]);
       }
     }
     // epilogue: lane holds rows r = rb*16 + l%16 for atoms mfma_row(l/16, e) of each tile
@@ -356,7 +399,7 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
       const int nxi = r / a.nY, nyi = r - (r / a.nY) * a.nY;
       const T wx = (T)nxi, wy = (T)(nyi - a.ky);
 #pragma unroll
-      for (int tau = 0; tau < 2; ++tau)
+      for (int tau = 0; tau < NT; ++tau)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const long atom = base + tau * 16 + mfma_row<T>(lk, e);
@@ -373,10 +416,12 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
           ph[tau][e] += pr * ur - pi * ui;
         }
     }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
   }
   // reduce over the 16 lanes that share l/16 (they hold different rows)
 #pragma unroll
-  for (int tau = 0; tau < 2; ++tau)
+  for (int tau = 0; tau < NT; ++tau)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       double vx = fx[tau][e], vy = fy[tau][e], vz = fz[tau][e], vp = ph[tau][e];
@@ -469,6 +514,14 @@ int ewald_k_plan(cav_ctx *c)
   p.R = p.nX * p.nY;
   p.Rpad = roundup(p.R, 16);
   p.KZ = roundup(2 * p.nZ, 4);
+  // fragment stride: whole 16-byte vectors, an odd number of them per lane so
+  // that the lanes' LDS reads of the force-pass operand hit distinct banks
+  {
+    const int vw = c->precision == CAV_FP64 ? 2 : 4;
+    int ksp = roundup(p.KZ / 4, vw);
+    if ((ksp / vw) % 2 == 0) ksp += vw;
+    p.KSP = ksp;
+  }
   if (p.KZ / 4 > 32) return fail(c, CAV_ERR_ARG, "kz_max above 31 is not supported");
   if (roundup(2 * p.nZ, 16) / 16 > 8) return fail(c, CAV_ERR_ARG, "kz_max above 31 is not supported");
   p.nk_ref = (2 * p.kx) * (2 * p.ky) * (2 * p.kz);
@@ -528,22 +581,36 @@ int ewald_k_plan(cav_ctx *c)
   return CAV_OK;
 }
 
-static int s_rt(int prec) { return prec == CAV_FP64 ? 3 : 3; }
+constexpr int kRT = 3;  // row tiles per S-pass wave
 
-static int s_groups(const cav_ctx *c)
+// Row-tile groups of the S pass: as few groups as the 8-wave limit allows,
+// split evenly so that no wave of a group idles.
+static void s_layout(const cav_ctx *c, int *groups, int *tiles_per_group, int *waves)
 {
   const int tiles = c->ek.Rpad / 16;
-  const int per = kWavesS * s_rt(c->precision);
-  return (tiles + per - 1) / per;
+  const int per_max = kWavesMax * kRT;
+  const int g = (tiles + per_max - 1) / per_max;
+  int tpg = (tiles + g - 1) / g;
+  tpg = (tpg + kRT - 1) / kRT * kRT;
+  *groups = g;
+  *tiles_per_group = tpg;
+  *waves = tpg / kRT;
+}
+
+static int s_stage(const cav_ctx *c, size_t tsz)
+{
+  const int TS = c->ek.nX + c->ek.nY + c->ek.nZ;
+  return 2 * 32 * (size_t)TS * 2 * tsz + 2 * 32 * tsz <= 150 * 1024 ? 32 : 16;
 }
 
 static int s_chunk(const cav_ctx *c, int ntarget)
 {
-  // aim for >= ntarget workgroups over the whole grid; chunk multiple of kStage
-  const int groups = s_groups(c);
+  // aim for >= ntarget workgroups over the whole grid; chunk multiple of 32
+  int groups, tpg, waves;
+  s_layout(c, &groups, &tpg, &waves);
   long per = ((long)c->n * groups + ntarget - 1) / ntarget;
-  per = (per + kStage - 1) / kStage * kStage;
-  if (per < kStage) per = kStage;
+  per = (per + 31) / 32 * 32;
+  if (per < 32) per = 32;
   return (int)per;
 }
 
@@ -552,9 +619,10 @@ int ewald_k_alloc(cav_ctx *c)
   const EwaldKPlan &p = c->ek;
   const size_t tsz = c->precision == CAV_FP64 ? sizeof(double) : sizeof(float);
   const int TS = p.nX + p.nY + p.nZ;
-  const size_t tab_bytes = (size_t)c->n * TS * 2 * tsz;
+  // padded: the S pass streams whole 16-byte chunks past the last record
+  const size_t tab_bytes = (size_t)c->n * TS * 2 * tsz + 4096;
   if (c->tab) (void)hipFree(c->tab);
-  CAV_HIP(hipMalloc(&c->tab, tab_bytes ? tab_bytes : 16));
+  CAV_HIP(hipMalloc(&c->tab, tab_bytes));
   c->tab_bytes = tab_bytes;
   c->tab_cap = c->n;
   const int chunk = s_chunk(c, 512);
@@ -568,7 +636,7 @@ int ewald_k_alloc(cav_ctx *c)
   if (c->sk) (void)hipFree(c->sk);
   CAV_HIP(hipMalloc(&c->sk, sizeof(double) * (size_t)p.Rpad * 2 * p.nZ));
   if (c->bmat) (void)hipFree(c->bmat);
-  const size_t bm_elems = (size_t)p.KZ * 4 * p.Rpad;
+  const size_t bm_elems = (size_t)(p.Rpad / 16) * 4 * 64 * p.KSP;
   CAV_HIP(hipMalloc(&c->bmat, bm_elems * tsz));
   if (c->precision == CAV_FP64)
     k_zero_t<double><<<grid_for(bm_elems), kBlock, 0, c->stream>>>(bm_elems, (double *)c->bmat);
@@ -591,8 +659,9 @@ static void allow_smem(K kern, size_t smem)
 }
 
 template <typename T, int RT>
-static void launch_sk_rt(cav_ctx *c, const SArgs &a, dim3 grid, size_t smem, int SCT)
+static void launch_sk_rt(cav_ctx *c, const SArgs &a, dim3 grid, size_t smem, int SCT, int waves)
 {
+  const int block = 64 * waves;
   allow_smem(k_sk<T, RT, 1>, smem);
   allow_smem(k_sk<T, RT, 2>, smem);
   allow_smem(k_sk<T, RT, 3>, smem);
@@ -602,31 +671,43 @@ static void launch_sk_rt(cav_ctx *c, const SArgs &a, dim3 grid, size_t smem, int
   allow_smem(k_sk<T, RT, 7>, smem);
   allow_smem(k_sk<T, RT, 8>, smem);
   switch (SCT) {
-  case 1: k_sk<T, RT, 1><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
-  case 2: k_sk<T, RT, 2><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
-  case 3: k_sk<T, RT, 3><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
-  case 4: k_sk<T, RT, 4><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
-  case 5: k_sk<T, RT, 5><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
-  case 6: k_sk<T, RT, 6><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
-  case 7: k_sk<T, RT, 7><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
-  default: k_sk<T, RT, 8><<<grid, 64 * kWavesS, smem, c->stream>>>(a); break;
+  case 1: k_sk<T, RT, 1><<<grid, block, smem, c->stream>>>(a); break;
+  case 2: k_sk<T, RT, 2><<<grid, block, smem, c->stream>>>(a); break;
+  case 3: k_sk<T, RT, 3><<<grid, block, smem, c->stream>>>(a); break;
+  case 4: k_sk<T, RT, 4><<<grid, block, smem, c->stream>>>(a); break;
+  case 5: k_sk<T, RT, 5><<<grid, block, smem, c->stream>>>(a); break;
+  case 6: k_sk<T, RT, 6><<<grid, block, smem, c->stream>>>(a); break;
+  case 7: k_sk<T, RT, 7><<<grid, block, smem, c->stream>>>(a); break;
+  default: k_sk<T, RT, 8><<<grid, block, smem, c->stream>>>(a); break;
   }
 }
+
+// Atom tiles per force-pass wave: two in FP32 (operand reuse), one in FP64
+// (keeps the kernel at 2+ waves per SIMD).
+template <typename T> constexpr int kforce_nt() { return sizeof(T) == 8 ? 1 : 2; }
 
 template <typename T>
 static void launch_kforce_t(cav_ctx *c, const FArgs &a, int KS, int npts)
 {
-  const int g = (npts + 32 * kWavesF - 1) / (32 * kWavesF);
-  if (KS <= 8)
-    k_kforce<T, 8><<<g, 64 * kWavesF, 0, c->stream>>>(a);
-  else if (KS <= 12)
-    k_kforce<T, 12><<<g, 64 * kWavesF, 0, c->stream>>>(a);
-  else if (KS <= 17)
-    k_kforce<T, 17><<<g, 64 * kWavesF, 0, c->stream>>>(a);
-  else if (KS <= 24)
-    k_kforce<T, 24><<<g, 64 * kWavesF, 0, c->stream>>>(a);
-  else
-    k_kforce<T, 32><<<g, 64 * kWavesF, 0, c->stream>>>(a);
+  constexpr int NT = kforce_nt<T>();
+  const int g = (npts + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF);
+  const size_t smem = 2 * (size_t)4 * 64 * a.KSP * sizeof(T);
+  if (KS <= 8) {
+    allow_smem(k_kforce<T, 8, NT>, smem);
+    k_kforce<T, 8, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+  } else if (KS <= 12) {
+    allow_smem(k_kforce<T, 12, NT>, smem);
+    k_kforce<T, 12, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+  } else if (KS <= 17) {
+    allow_smem(k_kforce<T, 17, NT>, smem);
+    k_kforce<T, 17, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+  } else if (KS <= 24) {
+    allow_smem(k_kforce<T, 24, NT>, smem);
+    k_kforce<T, 24, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+  } else {
+    allow_smem(k_kforce<T, 32, NT>, smem);
+    k_kforce<T, 32, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+  }
 }
 
 // Phase tables + S pass + reduction for npts points (the owned atoms, or query
@@ -651,21 +732,24 @@ static int sk_pipeline(cav_ctx *c)
   a.kz = p.kz;
   a.Rtiles = p.Rpad / 16;
   a.chunk = (c->n + c->sk_chunks - 1) / c->sk_chunks;
-  a.chunk = (a.chunk + kStage - 1) / kStage * kStage;
+  a.chunk = (a.chunk + 31) / 32 * 32;
+  a.stage = s_stage(c, sizeof(T));
   a.q = c->q;
   a.tab = c->tab;
   a.part = c->sk_part;
   a.Rpad = p.Rpad;
   a.SC = roundup(2 * p.nZ, 16);
+  int groups, waves;
+  s_layout(c, &groups, &a.tiles_per_group, &waves);
   const int nch = (c->n + a.chunk - 1) / a.chunk;
-  dim3 grid(nch, s_groups(c));
-  const size_t smem = kStage * TS * sizeof(Cx<T>) + kStage * sizeof(T);
+  dim3 grid(nch, groups);
+  const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
   timer_begin(c, c->t_sk);
-  launch_sk_rt<T, 3>(c, a, grid, smem, a.SC / 16);
+  launch_sk_rt<T, kRT>(c, a, grid, smem, a.SC / 16, waves);
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
   k_sk_reduce<T><<<grid_for(nthr), kBlock, 0, c->stream>>>(
-      nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KZ, c->sk_part, c->d_W, c->sk, (T *)c->bmat, c->part);
+      nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_part, c->d_W, c->sk, (T *)c->bmat, c->part);
   CAV_HIP(hipGetLastError());
   return reduce_partials(c, grid_for(nthr), 3);
 }
@@ -684,6 +768,7 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
   f.R = p.R;
   f.Rpad = p.Rpad;
   f.KZ = p.KZ;
+  f.KSP = p.KSP;
   f.tab = tab;
   f.bm = c->bmat;
   f.q = c->q;

@@ -47,6 +47,7 @@ struct EwaldKPlan {
   int R = 0;      // rows (nx,ny)
   int Rpad = 0;   // rows rounded up to 16
   int KZ = 0;     // 2*nZ rounded up to a multiple of 4 (real-embedded nz axis)
+  int KSP = 0;    // K-steps (KZ/4) rounded up to a multiple of 4: fragment stride
   int nk_ref = 0; // (2kx)(2ky)(2kz), the reference's used count
   double lx = 1, ly = 1, lz = 1, lxyz_inv = 0, dip_coef = 0;
   std::vector<double> W;        // R*nZ: multiplicity(k) * f(k) over the half space
@@ -105,6 +106,8 @@ struct cav_ctx {
   double *snap = nullptr;  // 9n doubles: pos/vel/acc at the start of a step
   int *type = nullptr, *id = nullptr, *mol = nullptr;
   double *q = nullptr;
+  double4 *xq = nullptr;   // packed (x, y, z, q) for the pair gathers
+  int2 *tid = nullptr;     // packed (type, id)
   int *gsrc = nullptr;
   int *flags = nullptr;
   int *h_flags = nullptr;  // pinned

@@ -263,21 +263,53 @@ __global__ void k_bin_fill(int n, long cap_total, const int *bin_of, const int *
 }
 
 // Insertion order of update_cell_list (owned ascending, then ghosts n+g
-// ascending, neighborlist.cpp:457-471) is ascending index order: sort each bin.
-__global__ void k_bin_sort(int nbins, const int *start, int *items, const int *cond)
+// ascending, neighborlist.cpp:457-471) is ascending index order.  Each bin is
+// sorted by one 256-thread block with a bitonic network in LDS (bins hold up
+// to 256 atoms; ghost-heavy corner bins of a periodic box reach ~150 at
+// rho=0.3); a rare larger bin falls back to an insertion sort.
+constexpr int kSortMax = 256;
+
+__global__ __launch_bounds__(kSortMax) void k_bin_sort(int nbins, const int *start, int *items,
+                                                       const int *cond)
 {
   if (cond && !*cond) return;
-  const int b = blockIdx.x * kBlock + threadIdx.x;
+  const int b = blockIdx.x;
   if (b >= nbins) return;
-  const int s = start[b], e = start[b + 1];
+  const int tid = threadIdx.x;
+  const int s = start[b], e = start[b + 1], m = e - s;
+  if (m <= 1) return;
+  if (m <= kSortMax) {
+    __shared__ int v[kSortMax];
+    int p2 = 2;
+    while (p2 < m) p2 <<= 1;
+    if (tid < p2) v[tid] = tid < m ? items[s + tid] : 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= p2; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int o = tid ^ j;
+        if (tid < p2 && o > tid) {
+          const int a = v[tid], c = v[o];
+          const bool up = (tid & k) == 0;
+          if ((a > c) == up) {
+            v[tid] = c;
+            v[o] = a;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (tid < m) items[s + tid] = v[tid];
+    return;
+  }
+  if (tid != 0) return;
   for (int i = s + 1; i < e; ++i) {
-    const int v = items[i];
+    const int val = items[i];
     int j = i - 1;
-    while (j >= s && items[j] > v) {
+    while (j >= s && items[j] > val) {
       items[j + 1] = items[j];
       --j;
     }
-    items[j + 1] = v;
+    items[j + 1] = val;
   }
 }
 
@@ -321,8 +353,7 @@ int launch_bins(cav_ctx *c, bool conditional)
   k_zero_int<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->bin_cnt, c->nbins, cond);
   k_bin_fill<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->bin_of, c->bin_start,
                                                      c->bin_cnt, c->bin_items, c->flags, cond);
-  k_bin_sort<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->nbins, c->bin_start, c->bin_items,
-                                                          cond);
+  k_bin_sort<<<c->nbins, kSortMax, 0, c->stream>>>(c->nbins, c->bin_start, c->bin_items, cond);
   const int nbo = grid_for(tot);
   OwnedFlagBounded of{c->bin_items, c->flags, c->n};
   k_block_sums<<<nbo, kBlock, 0, c->stream>>>(tot, of, c->ord_blk, cond);
@@ -335,38 +366,101 @@ int launch_bins(cav_ctx *c, bool conditional)
 
 // ------------------------------------------------------------- Verlet list --
 
+constexpr int kCand = 512;  // stencil candidates staged in LDS per pass
+
 // Full Verlet rows: for owned i every owned j != i and every ghost within
-// cutoff_extra (strict, dr = pos_i - pos_j evaluated as the reference does,
+// cutoff_extra (strict, dr = pos_i - pos_j rounded as the reference rounds it,
 // neighborlist.cpp:531-566).  The reference's half list is the subset with
 // owned j > i plus every ghost; both directions are kept so that forces are
-// gathered per atom without atomics.  Rows are column-major: nl[k*n + i].
-__global__ void k_list_cells(int n, int cap, double rce2, const double *x, const double *y,
-                             const double *z, const int *bin_of, const int *st_off, const int *st,
-                             const int *start, const int *items, const int *order, int *nl,
-                             int *nn, int *flags, const int *cond)
+// gathered per atom without atomics.  Rows are row-major: nl[i*cap + k].
+//
+// One wave per cell: the candidates of the cell's de-duplicated stencil are
+// staged in LDS (in stencil order, each cell in bin order), then each owned
+// atom of the cell tests 64 candidates per iteration; passing lanes append in
+// lane order through a ballot and a prefix popcount, so rows are written
+// coalesced and in a deterministic order.
+__global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, double rce2,
+                                                  const double *x, const double *y, const double *z,
+                                                  const int *st_off, const int *st,
+                                                  const int *start, const int *items, int *nl,
+                                                  int *nn, int *flags, const int *cond)
 {
   if (cond && !*cond) return;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n) return;
-  const int i = order[t];
-  const double px = x[i], py = y[i], pz = z[i];
-  const int b = bin_of[i];
-  int cnt = 0;
-  for (int s = st_off[b]; s < st_off[b + 1]; ++s) {
-    const int cell = st[s];
-    for (int t = start[cell]; t < start[cell + 1]; ++t) {
-      const int j = items[t];
-      if (j == i) continue;
-      const double dx = __dsub_rn(px, x[j]), dy = __dsub_rn(py, y[j]), dz = __dsub_rn(pz, z[j]);
-      if (dot_rn(dx, dy, dz) < rce2) {
-        if (cnt < cap) nl[(long)cnt * n + i] = j;
-        ++cnt;
-      }
-    }
+  const int b = blockIdx.x;
+  if (b >= nbins) return;
+  __shared__ double sx[kCand], sy[kCand], sz[kCand];
+  __shared__ int sj[kCand];
+  __shared__ int scnt[64];
+  const int lane = threadIdx.x;
+  const int s0 = start[b], e0 = start[b + 1];
+  // owned atoms come first in a sorted bin
+  int no = 0;
+  for (int t = s0; t < e0; t += 64) {
+    const int v = (t + lane < e0) ? items[t + lane] : n;
+    no += __popcll(__ballot(v < n));
   }
-  nn[i] = cnt < cap ? cnt : cap;
-  if (cnt > cap) flags[F_OVERFLOW] = 1;
-  atomicMax(flags + F_MAXNB, cnt);
+  if (no == 0) return;
+  const int sb = st_off[b], se = st_off[b + 1];
+  for (int og = 0; og < no; og += 64) {
+    const int gsize = (no - og) < 64 ? (no - og) : 64;
+    scnt[lane] = 0;
+    // candidate cursor: stencil cell index and position inside that cell
+    int cs = sb, ci = 0;
+    while (cs < se) {
+      // stage up to kCand candidates
+      int m = 0;
+      while (cs < se && m < kCand) {
+        const int cell = st[cs];
+        const int cb = start[cell], cn = start[cell + 1] - cb;
+        const int take = (cn - ci) < (kCand - m) ? (cn - ci) : (kCand - m);
+        for (int t = lane; t < take; t += 64) {
+          const int j = items[cb + ci + t];
+          sj[m + t] = j;
+          sx[m + t] = x[j];
+          sy[m + t] = y[j];
+          sz[m + t] = z[j];
+        }
+        m += take;
+        ci += take;
+        if (ci == cn) {
+          ++cs;
+          ci = 0;
+        }
+      }
+      __syncthreads();
+      for (int o = 0; o < gsize; ++o) {
+        const int i = items[s0 + og + o];
+        const double px = x[i], py = y[i], pz = z[i];
+        int c0 = scnt[o];
+        for (int cc = 0; cc < m; cc += 64) {
+          const int idx = cc + lane;
+          bool pass = false;
+          int j = 0;
+          if (idx < m) {
+            j = sj[idx];
+            const double dx = __dsub_rn(px, sx[idx]), dy = __dsub_rn(py, sy[idx]),
+                         dz = __dsub_rn(pz, sz[idx]);
+            pass = (j != i) && dot_rn(dx, dy, dz) < rce2;
+          }
+          const unsigned long long mask = __ballot(pass);
+          const int pre = __popcll(mask & ((1ull << lane) - 1ull));
+          if (pass && c0 + pre < cap) nl[(long)i * cap + c0 + pre] = j;
+          c0 += __popcll(mask);
+        }
+        if (lane == 0) scnt[o] = c0;
+        __syncthreads();
+      }
+      __syncthreads();
+    }
+    if (lane < gsize) {
+      const int i = items[s0 + og + lane];
+      const int cnt = scnt[lane];
+      nn[i] = cnt < cap ? cnt : cap;
+      if (cnt > cap) flags[F_OVERFLOW] = 1;
+      atomicMax(flags + F_MAXNB, cnt);
+    }
+    __syncthreads();
+  }
 }
 
 // all_atom_test_function (neighborlist.cpp:161-191): every owned j != i and
@@ -380,7 +474,7 @@ __global__ void k_list_all(int n, int cap, int *nl, int *nn, int *flags, const i
   int cnt = 0;
   for (int j = 0; j < tot; ++j) {
     if (j == i) continue;
-    if (cnt < cap) nl[(long)cnt * n + i] = j;
+    if (cnt < cap) nl[(long)i * cap + cnt] = j;
     ++cnt;
   }
   nn[i] = cnt < cap ? cnt : cap;
@@ -395,9 +489,9 @@ int launch_list(cav_ctx *c, bool conditional)
     k_list_all<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->nl_cap, c->nl, c->nn, c->flags,
                                                         cond);
   else
-    k_list_cells<<<grid_for(c->n), kBlock, 0, c->stream>>>(
-        c->n, c->nl_cap, c->rce2, c->x, c->y, c->z, c->bin_of, c->d_st_off, c->d_st, c->bin_start,
-        c->bin_items, c->order, c->nl, c->nn, c->flags, cond);
+    k_list_wave<<<c->nbins, 64, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->x, c->y,
+                                                c->z, c->d_st_off, c->d_st, c->bin_start,
+                                                c->bin_items, c->nl, c->nn, c->flags, cond);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }

@@ -36,7 +36,9 @@ struct PairArgs {
   int n, nt, cap;
   const double *x, *y, *z, *q, *minv;
   const int *type, *id, *mol, *nl, *nn;
-  const int *order;  // owned atoms in cell order (lane t handles atom order[t])
+  const int *order;   // owned atoms in cell order
+  const double4 *xq;  // packed (x, y, z, q) of owned + ghost atoms
+  const int2 *tid;    // packed (type, id)
   // LJ per type pair: coef(r) = -c12 r^-14 + c6 r^-8 + cshift
   const double *c12, *c6, *cshift;
   double lj_rc2;
@@ -47,25 +49,35 @@ struct PairArgs {
   double *part;
 };
 
+constexpr int kLPA = 16;  // lanes per atom in the pair kernel
+
 // Per-pair force coefficient (force on i = coef * (r_j - r_i)) for T = double
-// (reference arithmetic) or float (mixed form).
+// (reference arithmetic) or float (mixed form).  kLPA lanes share one atom's
+// row (coalesced row reads, balanced work); their partial forces are summed
+// with a fixed shuffle tree, so FP64 results stay reproducible.
 template <typename T, bool LJ, bool ER, bool LAM>
 __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
 {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int t = (int)(gid / kLPA), sub = (int)(gid % kLPA);
   double vir = 0.0;
+  T fx = 0, fy = 0, fz = 0;
+  int i = 0, ti = 0;
   if (t < a.n) {
-    const int i = a.order[t];
-    const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
-    const int ti = a.type[i], idi = a.id[i];
+    i = a.order[t];
+    const double4 pi4 = a.xq[i];
+    const int2 tii = a.tid[i];
+    ti = tii.x;
+    const int idi = tii.y;
     const int moli = LAM ? a.mol[i] : 0;
-    const T qi = (T)a.q[i];
+    const T qi = (T)pi4.w;
     const T keqi = (T)(a.ke) * qi;
-    T fx = 0, fy = 0, fz = 0;
     const int cnt = a.nn[i];
-    for (int k = 0; k < cnt; ++k) {
-      const int j = a.nl[(long)k * a.n + i];
-      const double dxd = a.x[j] - xi, dyd = a.y[j] - yi, dzd = a.z[j] - zi;
+    const int *row = a.nl + (long)i * a.cap;
+    for (int k = sub; k < cnt; k += kLPA) {
+      const int j = row[k];
+      const double4 pj = a.xq[j];
+      const double dxd = pj.x - pi4.x, dyd = pj.y - pi4.y, dzd = pj.z - pi4.z;
       const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd;
       T r2;
       if constexpr (sizeof(T) == 8)
@@ -73,9 +85,11 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
       else
         r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
       T coef = 0;
+      int2 tj = make_int2(0, 0);
+      if constexpr (LJ || ER) tj = a.tid[j];
       if constexpr (LJ) {
         if (!(r2 > (T)a.lj_rc2)) {  // lj_cell_list.cpp:141-142: skip dr^2 > rc^2
-          const int p = ti * a.nt + a.type[j];
+          const int p = ti * a.nt + tj.x;
           const T ri2 = (T)1 / r2;
           const T ri4 = ri2 * ri2, ri8 = ri4 * ri4;
           coef += -(T)a.c12[p] * (ri8 * ri4 * ri2) + (T)a.c6[p] * ri8 + (T)a.cshift[p];
@@ -88,13 +102,13 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
           bool skip = false;
           if constexpr (LAM) {
             if (moli == a.mol[j]) {
-              const T l = (T)a.lambda[ti * a.nt + a.type[j]];
+              const T l = (T)a.lambda[ti * a.nt + tj.x];
               if (l == 0) skip = true;
               lam = l;
             }
           }
           if (!skip) {
-            const T qj = (T)a.q[j];
+            const T qj = (T)pj.w;
             T ce;
             if constexpr (sizeof(T) == 8) {
               const double rn = sqrt(r2);
@@ -110,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
             }
             coef += ce;
             // virial once per reference list entry (owned j > i or ghost), id_i < id_j
-            if ((j >= a.n || j > i) && idi < a.id[j]) vir += -(double)ce * (double)r2;
+            if ((j >= a.n || j > i) && idi < tj.y) vir += -(double)ce * (double)r2;
           }
         }
       }
@@ -118,6 +132,14 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
       fy = fma(coef, dy, fy);
       fz = fma(coef, dz, fz);
     }
+  }
+#pragma unroll
+  for (int o = 1; o < kLPA; o <<= 1) {
+    fx += __shfl_xor(fx, o, 64);
+    fy += __shfl_xor(fy, o, 64);
+    fz += __shfl_xor(fz, o, 64);
+  }
+  if (t < a.n && sub == 0) {
     const double mi = a.minv[ti];
     a.ax[i] = (double)fx * mi;
     a.ay[i] = (double)fy * mi;
@@ -132,6 +154,17 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
     __syncthreads();
   }
   if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
+}
+
+// Packed copies for the pair gathers: (x, y, z, q) and (type, id).
+__global__ void k_pack(int n, long cap_total, const double *x, const double *y, const double *z,
+                       const double *q, const int *type, const int *id, const int *flags,
+                       double4 *xq, int2 *tid)
+{
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= (long)n + flags[F_NGHOST] || a >= cap_total) return;
+  xq[a] = make_double4(x[a], y[a], z[a], q[a]);
+  tid[a] = make_int2(type[a], id[a]);
 }
 
 __global__ void k_reduce_part(int count, const double *part, double *out)
@@ -172,6 +205,8 @@ static PairArgs pair_args(cav_ctx *c)
   a.nl = c->nl;
   a.nn = c->nn;
   a.order = c->order;
+  a.xq = c->xq;
+  a.tid = c->tid;
   const int nt2 = c->nt * c->nt;
   a.c12 = c->d_eps;           // laid out as [c12 | c6 | cshift] blocks, see upload
   a.c6 = c->d_eps + nt2;
@@ -192,7 +227,7 @@ static PairArgs pair_args(cav_ctx *c)
 template <typename T>
 static void launch_pairs_t(cav_ctx *c, const PairArgs &a, bool lj, bool er, bool lam)
 {
-  const int g = grid_for(c->n);
+  const int g = grid_for((long)c->n * kLPA);
   if (lj && er && lam)
     k_pairs<T, true, true, true><<<g, kBlock, 0, c->stream>>>(a);
   else if (lj && er)
@@ -210,6 +245,9 @@ int launch_pairs(cav_ctx *c)
   const bool lj = c->lj_on && !c->lj_cellwalk, er = c->er_on;
   if (!lj && !er) return CAV_OK;
   PairArgs a = pair_args(c);
+  const long tot = (long)c->n + c->gcap;
+  k_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->x, c->y, c->z, c->q, c->type,
+                                                 c->id, c->flags, c->xq, c->tid);
   timer_begin(c, c->t_pair);
   if (c->precision == CAV_FP64)
     launch_pairs_t<double>(c, a, lj, er, c->lambda_set);
@@ -217,7 +255,7 @@ int launch_pairs(cav_ctx *c)
     launch_pairs_t<float>(c, a, lj, er, c->lambda_set);
   timer_end(c, c->t_pair);
   CAV_HIP(hipGetLastError());
-  return reduce_partials(c, grid_for(c->n), 0);
+  return reduce_partials(c, grid_for((long)c->n * kLPA), 0);
 }
 
 __global__ void k_zero_acc(int n, double *ax, double *ay, double *az)
@@ -431,7 +469,7 @@ __global__ void k_energy_lj(PairArgs a, const double *u12, const double *u6, con
     const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
     const int ti = a.type[i];
     for (int k = 0; k < a.nn[i]; ++k) {
-      const int j = a.nl[(long)k * a.n + i];
+      const int j = a.nl[(long)i * a.cap + k];
       const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
       const double r2 = dot_rn_p(dx, dy, dz);
       if (r2 > a.lj_rc2) continue;

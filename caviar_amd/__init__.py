@@ -87,6 +87,7 @@ def load_library():
         "cav_list_stats": [P, dp, dp],
         "cav_kernel_time": [P, ctypes.c_char_p, dp, ip],
         "cav_enable_timing": [P, I],
+        "cav_adjoint": [P, dp, dp, dp, dp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -252,6 +253,37 @@ class Engine:
         self._check(self.lib.cav_point_field(self.h, {"r": 0, "k": 1}[which], len(pts), _dp(pts),
                                              _dp(f), _dp(p)))
         return f, p
+
+    def set_parameters(self, epsilon=None, sigma=None, charge=None, mass=None):
+        """Change force-field parameter values after setup (same structure);
+        they take effect at the next recalculate()/step()."""
+        s = self.system
+        nt = s.ntypes
+        if charge is not None or mass is not None:
+            if charge is not None:
+                s.charge = np.asarray(charge, dtype=np.float64).reshape(nt)
+            if mass is not None:
+                s.mass = np.asarray(mass, dtype=np.float64).reshape(nt)
+            self._check(self.lib.cav_set_types(self.h, nt, _dp(s.mass.copy()), _dp(s.charge.copy())))
+        if epsilon is not None or sigma is not None:
+            if epsilon is not None:
+                s.lj.epsilon = np.asarray(epsilon, dtype=np.float64).reshape(nt, nt)
+            if sigma is not None:
+                s.lj.sigma = np.asarray(sigma, dtype=np.float64).reshape(nt, nt)
+            eps = np.ascontiguousarray(s.lj.epsilon, dtype=np.float64).reshape(nt * nt)
+            sig = np.ascontiguousarray(s.lj.sigma, dtype=np.float64).reshape(nt * nt)
+            self._check(self.lib.cav_set_lj(self.h, s.lj.cutoff, _dp(eps), _dp(sig)))
+
+    def adjoint(self, weights):
+        """g = sum_i w_i . dF_i/dtheta for the LJ epsilon/sigma matrices and the
+        type charges at the current positions and parameters."""
+        nt = self.system.ntypes
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(self.n, 3)
+        ge = np.zeros(nt * nt)
+        gs = np.zeros(nt * nt)
+        gq = np.zeros(nt)
+        self._check(self.lib.cav_adjoint(self.h, _dp(w), _dp(ge), _dp(gs), _dp(gq)))
+        return ge.reshape(nt, nt), gs.reshape(nt, nt), gq
 
     def list_stats(self):
         a = ctypes.c_double()

@@ -1,0 +1,291 @@
+// Force-matching adjoint: the vector-Jacobian product of the forces with
+// respect to the force-field parameters,
+//
+//   g_theta = sum_i w_i . dF_i/dtheta,   w_i = dL/dF_i (given),
+//
+// i.e. the mixed second derivatives d2U/(dr dtheta) contracted with the
+// residual weights.  Parameters: lj_cell_list epsilon[a][b] and sigma[a][b]
+// (including the sigma dependence of the force shift at rc), and the type
+// charges through Electrostatic_ewald_r and Electrostatic_ewald_k.  The
+// reference has no differentiation; this is the gradient of its force formulas
+// (lj_cell_list.cpp:143-155, electrostatic_ewald_r.cpp:183-211,
+// electrostatic_ewald_k.cpp:197-226).
+#include <cmath>
+
+#include "cav_internal.hpp"
+
+namespace cav {
+
+// Ewald_k pieces exposed by cav_ewald_k.hip
+int ewald_k_weighted_sk(cav_ctx *c, const double *weights, double *sk_out);
+int ewald_k_field(cav_ctx *c, double *out4);
+
+constexpr int kLPAa = 16;
+
+struct AdjArgs {
+  int n, nt, cap;
+  const double4 *xq;
+  const int2 *tid;
+  const int *mol, *nl, *nn;
+  const double *w;        // 3n residual weights (AoS)
+  const double *eps, *sig;
+  double lj_rc, lj_rc2;
+  int lj_on, er_on, lambda_set;
+  double alpha, a2, ke, two_a_rpi;
+  const double *lambda, *charge;
+  double *part;  // [block][nvals]: g_eps[nt2] | g_sig[nt2] | g_q[nt]
+};
+
+// One list entry: force on i is coef * (r_j - r_i); its derivative along the
+// residual weight w_i is (w_i . r_ij) dcoef/dtheta.
+__global__ __launch_bounds__(kBlock) void k_adjoint_pairs(AdjArgs a)
+{
+  extern __shared__ double sh[];  // [kBlock/kLPAa][nvals] per-atom-group partials
+  const int nt2 = a.nt * a.nt, nv = 2 * nt2 + a.nt;
+  const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int t = (int)(gid / kLPAa), sub = (int)(gid % kLPAa);
+  double gl[2 * 16 + 4];  // per-lane accumulators (nt <= 4)
+  for (int v = 0; v < nv; ++v) gl[v] = 0;
+  if (t < a.n) {
+    const int i = t;
+    const double4 pi4 = a.xq[i];
+    const int2 tii = a.tid[i];
+    const int ti = tii.x;
+    const int moli = a.lambda_set ? a.mol[i] : 0;
+    const double wx = a.w[3 * i], wy = a.w[3 * i + 1], wz = a.w[3 * i + 2];
+    const int cnt = a.nn[i];
+    const int *row = a.nl + (long)i * a.cap;
+    for (int k = sub; k < cnt; k += kLPAa) {
+      const int j = row[k];
+      const double4 pj = a.xq[j];
+      const double dx = pj.x - pi4.x, dy = pj.y - pi4.y, dz = pj.z - pi4.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double wr = wx * dx + wy * dy + wz * dz;  // w_i . r_ij
+      const int tj = a.tid[j].x;
+      // type-pair entry as the reference indexes it: [lower owned index][partner]
+      const int p = (j < a.n && j < i) ? tj * a.nt + ti : ti * a.nt + tj;
+      if (a.lj_on && !(r2 > a.lj_rc2)) {
+        const double e = a.eps[p], s = a.sig[p];
+        const double ri2 = 1.0 / r2, rci2 = 1.0 / a.lj_rc2;
+        const double s2 = s * s, s5 = s2 * s2 * s, s6 = s5 * s, s11 = s6 * s5, s12 = s6 * s6;
+        const double r6 = ri2 * ri2 * ri2, r8 = r6 * ri2, r14 = r6 * r6 * ri2;
+        const double c6 = rci2 * rci2 * rci2, c8 = c6 * rci2, c14 = c6 * c6 * rci2;
+        // coef = 4e(-12 s^12 r^-14 + 6 s^6 r^-8 + 12 s^12 rc^-14 - 6 s^6 rc^-8)
+        const double dce = 4.0 * (-12 * s12 * r14 + 6 * s6 * r8 + 12 * s12 * c14 - 6 * s6 * c8);
+        const double dcs = 4.0 * e * (-144 * s11 * r14 + 36 * s5 * r8 + 144 * s11 * c14 - 36 * s5 * c8);
+        gl[p] += wr * dce;
+        gl[nt2 + p] += wr * dcs;
+      }
+      if (a.er_on && !(dx == 0 && dy == 0 && dz == 0)) {
+        double lam = 1.0;
+        bool skip = false;
+        if (a.lambda_set && moli == a.mol[j]) {
+          lam = a.lambda[p];
+          if (lam == 0.0) skip = true;
+        }
+        if (!skip) {
+          const double rn = sqrt(r2);
+          const double sr = (a.two_a_rpi * exp(-a.a2 * r2) + erfc(a.alpha * rn) / rn) / r2;
+          // coef = -lam ke q_a q_b sr: d/dq_a = -lam ke q_b sr, d/dq_b = -lam ke q_a sr
+          const double base = -lam * a.ke * sr * wr;
+          gl[2 * nt2 + ti] += base * a.charge[tj];
+          gl[2 * nt2 + tj] += base * a.charge[ti];
+        }
+      }
+    }
+  }
+  // fixed-order reduction: 16 lanes of an atom, then the block's atoms
+  for (int v = 0; v < nv; ++v) {
+    double x = gl[v];
+#pragma unroll
+    for (int o = 1; o < kLPAa; o <<= 1) x += __shfl_xor(x, o, 64);
+    gl[v] = x;
+  }
+  const int grp = threadIdx.x / kLPAa;
+  if (sub == 0)
+    for (int v = 0; v < nv; ++v) sh[grp * nv + v] = gl[v];
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double x = 0;
+    for (int g = 0; g < kBlock / kLPAa; ++g) x += sh[g * nv + threadIdx.x];
+    a.part[(long)blockIdx.x * nv + threadIdx.x] = x;
+  }
+}
+
+__global__ void k_reduce_cols(int rows, int cols, const double *part, double *out)
+{
+  const int v = threadIdx.x;
+  if (v >= cols) return;
+  double x = 0;
+  for (int r = 0; r < rows; ++r) x += part[(long)r * cols + v];
+  out[v] = x;
+}
+
+// k-space term of dL/dq_s through S: C sum_H W Im(conj(S^(s)) T), with
+// T = sum_d (2 pi n_d / l_d) S^(q w_d).  Per-thread partial over (row, col).
+__global__ void k_adjoint_kspace(int R, int Rpad, int nY, int nZ, int ky, int kz, int nt,
+                                 double kxs, double kys, double kzs, const double *W,
+                                 const double *ss, const double *sw, double *part)
+{
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ double red[kBlock];
+  const long plane = (long)Rpad * 2 * nZ;  // doubles per weighted S
+  for (int s = 0; s < nt; ++s) {
+    double v = 0;
+    if (t < R * nZ) {
+      const int r = t / nZ, c = t - r * nZ;
+      const double w = W[(long)r * nZ + c];
+      if (w != 0) {
+        const int nx = r / nY, ny = r % nY - ky, nz = c - kz;
+        const long o = (long)r * 2 * nZ + 2 * c;
+        const double tr = kxs * nx * sw[o] + kys * ny * sw[plane + o] + kzs * nz * sw[2 * plane + o];
+        const double ti = kxs * nx * sw[o + 1] + kys * ny * sw[plane + o + 1] +
+                          kzs * nz * sw[2 * plane + o + 1];
+        const double sr = ss[s * plane + o], si = ss[s * plane + o + 1];
+        v = w * (sr * ti - si * tr);  // Im(conj(S) T)
+      }
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int q = kBlock / 2; q > 0; q >>= 1) {
+      if (threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long)blockIdx.x * nt + s] = red[0];
+    __syncthreads();
+  }
+}
+
+// Per-atom term: sum over atoms of species s of w_i . G_i (G = k-space field).
+__global__ void k_adjoint_field(int n, int nt, const double *w, const double *f4, const int *type,
+                                double *part)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ double red[kBlock];
+  for (int s = 0; s < nt; ++s) {
+    double v = 0;
+    if (i < n && type[i] == s)
+      v = w[3 * i] * f4[4 * i] + w[3 * i + 1] * f4[4 * i + 1] + w[3 * i + 2] * f4[4 * i + 2];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int q = kBlock / 2; q > 0; q >>= 1) {
+      if (threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long)blockIdx.x * nt + s] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill_weights(int n, int mode, int s, const int *type, const double *q,
+                               const double *w, double *u)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  u[i] = mode < 0 ? (type[i] == s ? 1.0 : 0.0) : q[i] * w[3 * i + mode];
+}
+
+}  // namespace cav
+
+using namespace cav;
+
+extern "C" int cav_adjoint(cav_ctx *c, const double *w_host, double *g_eps, double *g_sigma,
+                           double *g_charge)
+{
+  if (!c || !w_host) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run before cav_adjoint");
+  if (c->lj_on && c->lj_cellwalk)
+    return fail(c, CAV_ERR_STATE, "the adjoint needs the LJ cutoff within the neighbour-list cutoff");
+  if (c->dipole) return fail(c, CAV_ERR_STATE, "the adjoint does not cover the epsilon_dipole term");
+  if (c->nt > 4) return fail(c, CAV_ERR_ARG, "the adjoint supports up to 4 atom types");
+  (void)hipSetDevice(c->device);
+  const int n = c->n, nt = c->nt, nt2 = nt * nt, nv = 2 * nt2 + nt;
+  double *dw = nullptr, *dpart = nullptr, *dout = nullptr;
+  const int nblk = grid_for((long)n * kLPAa);
+  CAV_HIP(hipMalloc(&dw, sizeof(double) * (3 * (size_t)n + 1)));
+  CAV_HIP(hipMalloc(&dpart, sizeof(double) * ((size_t)nblk * nv + 4 * (size_t)grid_for(n) * nt +
+                                              (size_t)grid_for((long)c->ek.R * c->ek.nZ + 1) * nt + 64)));
+  CAV_HIP(hipMalloc(&dout, sizeof(double) * (nv + 2 * nt)));
+  CAV_HIP(hipMemcpyAsync(dw, w_host, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  std::vector<double> hout(nv + 2 * (size_t)nt, 0.0);
+  if (c->lj_on || c->er_on) {
+    AdjArgs a;
+    a.n = n;
+    a.nt = nt;
+    a.cap = c->nl_cap;
+    a.xq = c->xq;
+    a.tid = c->tid;
+    a.mol = c->mol;
+    a.nl = c->nl;
+    a.nn = c->nn;
+    a.w = dw;
+    a.eps = c->d_raw_eps;
+    a.sig = c->d_raw_sig;
+    a.lj_rc = c->lj_rc;
+    a.lj_rc2 = c->lj_rc * c->lj_rc;
+    a.lj_on = c->lj_on;
+    a.er_on = c->er_on;
+    a.lambda_set = c->lambda_set;
+    a.alpha = c->er_alpha;
+    a.a2 = c->er_alpha * c->er_alpha;
+    a.ke = c->er_ke;
+    a.two_a_rpi = 2 * c->er_alpha * CAV_PIS_INV;
+    a.lambda = c->d_lambda;
+    a.charge = c->d_charge;
+    a.part = dpart;
+    const size_t sh = sizeof(double) * (kBlock / kLPAa) * nv;
+    k_adjoint_pairs<<<nblk, kBlock, sh, c->stream>>>(a);
+    k_reduce_cols<<<1, 64, 0, c->stream>>>(nblk, nv, dpart, dout);
+    CAV_HIP(hipGetLastError());
+    CAV_HIP(hipMemcpyAsync(hout.data(), dout, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
+    CAV_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (c->ek_on) {
+    const EwaldKPlan &p = c->ek;
+    const size_t plane = (size_t)p.Rpad * 2 * p.nZ;
+    double *u = nullptr, *ss = nullptr, *sw = nullptr, *f4 = nullptr;
+    CAV_HIP(hipMalloc(&u, sizeof(double) * (size_t)(n + 1)));
+    CAV_HIP(hipMalloc(&ss, sizeof(double) * plane * nt));
+    CAV_HIP(hipMalloc(&sw, sizeof(double) * plane * 3));
+    CAV_HIP(hipMalloc(&f4, sizeof(double) * 4 * (size_t)(n + 1)));
+    // per-atom k-space field G_i (uses the operand of the last force evaluation)
+    CAV_TRY(ewald_k_field(c, f4));
+    for (int s = 0; s < nt; ++s) {
+      k_fill_weights<<<grid_for(n), kBlock, 0, c->stream>>>(n, -1, s, c->type, c->q, dw, u);
+      CAV_TRY(ewald_k_weighted_sk(c, u, ss + s * plane));
+    }
+    for (int d = 0; d < 3; ++d) {
+      k_fill_weights<<<grid_for(n), kBlock, 0, c->stream>>>(n, d, 0, c->type, c->q, dw, u);
+      CAV_TRY(ewald_k_weighted_sk(c, u, sw + d * plane));
+    }
+    const int nb = grid_for((long)p.R * p.nZ);
+    k_adjoint_kspace<<<nb, kBlock, 0, c->stream>>>(p.R, p.Rpad, p.nY, p.nZ, p.ky, p.kz, nt,
+                                                   CAV_2PI / p.lx, CAV_2PI / p.ly, CAV_2PI / p.lz,
+                                                   c->d_W, ss, sw, dpart);
+    k_reduce_cols<<<1, 64, 0, c->stream>>>(nb, nt, dpart, dout + nv);
+    const int nb2 = grid_for(n);
+    double *dpart2 = dpart + (size_t)nb * nt + 16;
+    k_adjoint_field<<<nb2, kBlock, 0, c->stream>>>(n, nt, dw, f4, c->type, dpart2);
+    k_reduce_cols<<<1, 64, 0, c->stream>>>(nb2, nt, dpart2, dout + nv + nt);
+    CAV_HIP(hipGetLastError());
+    CAV_HIP(hipMemcpyAsync(hout.data() + nv, dout + nv, sizeof(double) * 2 * nt,
+                           hipMemcpyDeviceToHost, c->stream));
+    CAV_HIP(hipStreamSynchronize(c->stream));
+    const double C = CAV_4PI * c->ek_ke * p.lxyz_inv;
+    for (int s = 0; s < nt; ++s) hout[2 * nt2 + s] += C * hout[nv + s] + hout[nv + nt + s];
+    (void)hipFree(u);
+    (void)hipFree(ss);
+    (void)hipFree(sw);
+    (void)hipFree(f4);
+  }
+  (void)hipFree(dw);
+  (void)hipFree(dpart);
+  (void)hipFree(dout);
+  if (g_eps)
+    for (int p = 0; p < nt2; ++p) g_eps[p] = hout[p];
+  if (g_sigma)
+    for (int p = 0; p < nt2; ++p) g_sigma[p] = hout[nt2 + p];
+  if (g_charge)
+    for (int s = 0; s < nt; ++s) g_charge[s] = hout[2 * nt2 + s];
+  return CAV_OK;
+}

@@ -141,6 +141,12 @@ __global__ void k_begin_step(int *flags)
 
 __global__ void k_set_flag(int *flags, int idx, int v) { flags[idx] = v; }
 
+__global__ void k_charge_of_type(int n, const int *type, const double *charge, double *q)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) q[i] = charge[type[i]];
+}
+
 }  // namespace cav
 
 using namespace cav;
@@ -168,7 +174,7 @@ static void free_all(cav_ctx *c)
                   c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
                   c->d_W,     c->kacc,      c->order,    c->ord_blk,  c->xq,
-                  c->tid};
+                  c->tid,     c->d_raw_eps, c->d_raw_sig};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -202,6 +208,15 @@ static int upload_params(cav_ctx *c)
       lj[5 * nt2 + p] = 4 * e * s12 * ri6 * ri6 - 4 * e * s6 * ri6 + 0.5 * lj[2 * nt2 + p] * rc2;
     }
   }
+  std::vector<double> re(std::max(nt2, 1), 0.0), rs(std::max(nt2, 1), 0.0);
+  if (c->lj_on) {
+    re = c->eps;
+    rs = c->sig;
+  }
+  CAV_TRY(dmalloc(c, &c->d_raw_eps, re.size()));
+  CAV_TRY(dmalloc(c, &c->d_raw_sig, rs.size()));
+  CAV_HIP(hipMemcpyAsync(c->d_raw_eps, re.data(), sizeof(double) * re.size(), hipMemcpyHostToDevice, c->stream));
+  CAV_HIP(hipMemcpyAsync(c->d_raw_sig, rs.data(), sizeof(double) * rs.size(), hipMemcpyHostToDevice, c->stream));
   CAV_TRY(dmalloc(c, &c->d_eps, lj.size()));
   CAV_HIP(hipMemcpyAsync(c->d_eps, lj.data(), sizeof(double) * lj.size(), hipMemcpyHostToDevice, c->stream));
   std::vector<double> lam(std::max(nt2, 1), 1.0);
@@ -292,6 +307,24 @@ static int box_dev(cav_ctx *c, const double **lo, const double **hi, const doubl
   *lo = c->scal + kBoxSlot;
   *hi = c->scal + kBoxSlot + 3;
   *L = c->scal + kBoxSlot + 6;
+  return CAV_OK;
+}
+
+// New parameter values (same structure) set after setup: upload them, refresh
+// the per-atom charges and the k-space weights.  Ghost charges follow from the
+// next ghost exchange.
+static int refresh_params(cav_ctx *c)
+{
+  if (!c->params_dirty) return CAV_OK;
+  CAV_TRY(upload_params(c));
+  k_charge_of_type<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->type, c->d_charge, c->q);
+  CAV_HIP(hipGetLastError());
+  if (c->ek_on) {
+    CAV_TRY(ewald_k_plan(c));
+    CAV_HIP(hipMemcpyAsync(c->d_W, c->ek.W.data(), sizeof(double) * c->ek.W.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  }
+  c->params_dirty = false;
   return CAV_OK;
 }
 
@@ -444,12 +477,16 @@ int cav_set_ghost_cutoff(cav_ctx *c, double gc)
 int cav_set_types(cav_ctx *c, int nt, const double *mass, const double *charge)
 {
   if (!c || nt <= 0 || !mass || !charge) return fail(c, CAV_ERR_ARG, "types need mass and charge");
+  const bool keep = c->setup_done && nt == c->nt;
   c->nt = nt;
   c->mass.assign(mass, mass + nt);
   c->charge.assign(charge, charge + nt);
   c->minv.resize(nt);
   for (int t = 0; t < nt; ++t) c->minv[t] = mass[t] == 0.0 ? 0.0 : 1.0 / mass[t];  // atom_data.cpp:236-239
-  c->setup_done = false;
+  if (keep)
+    c->params_dirty = true;  // values only: refreshed before the next force evaluation
+  else
+    c->setup_done = false;
   return CAV_OK;
 }
 
@@ -477,11 +514,15 @@ int cav_set_lj(cav_ctx *c, double rc, const double *eps, const double *sigma)
   const int nt2 = c->nt * c->nt;
   for (int p = 0; p < nt2; ++p)
     if (eps[p] < 0 || sigma[p] < 0) return fail(c, CAV_ERR_ARG, "Epsilon and sigma have to be non-negative.");
+  const bool keep = c->setup_done && c->lj_on && rc == c->lj_rc;
   c->lj_on = true;
   c->lj_rc = rc;
   c->eps.assign(eps, eps + nt2);
   c->sig.assign(sigma, sigma + nt2);
-  c->setup_done = false;
+  if (keep)
+    c->params_dirty = true;
+  else
+    c->setup_done = false;
   return CAV_OK;
 }
 
@@ -490,6 +531,7 @@ int cav_set_ewald_r(cav_ctx *c, double alpha, double ke, const double *lambda)
   if (!c) return CAV_ERR_ARG;
   if (alpha < 0) return fail(c, CAV_ERR_ARG, "alpha have to non-negative.");
   if (ke < 0) return fail(c, CAV_ERR_ARG, "k_electrostatic has to be non-negative.");
+  const bool keep = c->setup_done && c->er_on && (lambda != nullptr) == c->lambda_set;
   c->er_on = true;
   c->er_alpha = alpha;
   c->er_ke = ke;
@@ -498,7 +540,10 @@ int cav_set_ewald_r(cav_ctx *c, double alpha, double ke, const double *lambda)
     if (c->nt <= 0) return fail(c, CAV_ERR_ARG, "set types before lambda");
     c->lambda.assign(lambda, lambda + c->nt * c->nt);
   }
-  c->setup_done = false;
+  if (keep)
+    c->params_dirty = true;
+  else
+    c->setup_done = false;
   return CAV_OK;
 }
 
@@ -510,6 +555,8 @@ int cav_set_ewald_k(cav_ctx *c, double alpha, double ke, int kx, int ky, int kz,
   if (kx < 0 || ky < 0 || kz < 0) return fail(c, CAV_ERR_ARG, "'k_max' have to non-negative.");
   if (ke < 0) return fail(c, CAV_ERR_ARG, "'k_electrostatic' has to be non-negative.");
   if (slab_axis < -1 || slab_axis > 2) return fail(c, CAV_ERR_ARG, "slab_normal_axis must be x, y or z");
+  const bool keep = c->setup_done && c->ek_on && kx == c->kx && ky == c->ky && kz == c->kz &&
+                    slab_axis == c->slab_axis && (dipole != 0) == (c->dipole != 0);
   c->ek_on = true;
   c->ek_alpha = alpha;
   c->ek_ke = ke;
@@ -520,7 +567,10 @@ int cav_set_ewald_k(cav_ctx *c, double alpha, double ke, int kx, int ky, int kz,
   c->eps_dipole = eps_dipole;
   c->slab_axis = slab_axis;
   c->slab_lz = slab_lz;
-  c->setup_done = false;
+  if (keep)
+    c->params_dirty = true;
+  else
+    c->setup_done = false;
   return CAV_OK;
 }
 
@@ -702,6 +752,7 @@ int cav_step(cav_ctx *c, double dt, int nsteps)
   if (!c) return CAV_ERR_ARG;
   if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run before cav_step");
   (void)hipSetDevice(c->device);
+  CAV_TRY(refresh_params(c));
   const double *lo, *hi, *L;
   box_dev(c, &lo, &hi, &L);
   for (int s = 0; s < nsteps; ++s) {
@@ -744,6 +795,7 @@ int cav_recalculate(cav_ctx *c, int force_rebuild)
   if (!c) return CAV_ERR_ARG;
   if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
   (void)hipSetDevice(c->device);
+  CAV_TRY(refresh_params(c));
   k_begin_step<<<1, 1, 0, c->stream>>>(c->flags);
   CAV_TRY(wrap_and_ghosts(c));
   CAV_TRY(neighbor_phase(c, force_rebuild != 0));

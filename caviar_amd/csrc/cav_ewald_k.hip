@@ -810,6 +810,74 @@ int launch_ewald_k(cav_ctx *c)
   return CAV_OK;
 }
 
+// S over the half lattice with arbitrary real per-atom weights u_j in place
+// of the charges (the adjoint's species and residual-weighted structure
+// factors); written as [Rpad][nZ] complex FP64.
+__global__ void k_sk_sum(int nchunks, int Rpad, int SC, int nZ, const double *part, double *out)
+{
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= Rpad * nZ) return;
+  const int r = t / nZ, c = t - r * nZ;
+  double sr = 0, si = 0;
+  for (int k = 0; k < nchunks; ++k) {
+    const double *p = part + ((long)k * Rpad + r) * SC + 2 * c;
+    sr += p[0];
+    si += p[1];
+  }
+  out[(long)r * 2 * nZ + 2 * c] = sr;
+  out[(long)r * 2 * nZ + 2 * c + 1] = si;
+}
+
+template <typename T>
+static int weighted_sk_t(cav_ctx *c, const double *weights, double *out)
+{
+  const EwaldKPlan &p = c->ek;
+  const int TS = p.nX + p.nY + p.nZ;
+  k_tables<T><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
+      c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
+      (Cx<T> *)c->tab);
+  SArgs a;
+  a.n = c->n;
+  a.TS = TS;
+  a.nX = p.nX;
+  a.nY = p.nY;
+  a.nZ = p.nZ;
+  a.ky = p.ky;
+  a.kz = p.kz;
+  a.Rtiles = p.Rpad / 16;
+  a.chunk = (c->n + c->sk_chunks - 1) / c->sk_chunks;
+  a.chunk = (a.chunk + 31) / 32 * 32;
+  a.stage = s_stage(c, sizeof(T));
+  a.q = weights;
+  a.tab = c->tab;
+  a.part = c->sk_part;
+  a.Rpad = p.Rpad;
+  a.SC = roundup(2 * p.nZ, 16);
+  int groups, waves;
+  s_layout(c, &groups, &a.tiles_per_group, &waves);
+  const int nch = (c->n + a.chunk - 1) / a.chunk;
+  const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
+  launch_sk_rt<T, kRT>(c, a, dim3(nch, groups), smem, a.SC / 16, waves);
+  k_sk_sum<<<grid_for((long)p.Rpad * p.nZ), kBlock, 0, c->stream>>>(nch, p.Rpad, a.SC, p.nZ,
+                                                                    c->sk_part, out);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+int ewald_k_weighted_sk(cav_ctx *c, const double *weights, double *out)
+{
+  if (c->precision == CAV_FP64) return weighted_sk_t<double>(c, weights, out);
+  return weighted_sk_t<float>(c, weights, out);
+}
+
+// Per-atom k-space field (C sum k Im(g e^{ik.r}), and the potential) with the
+// operand of the last force evaluation.
+int ewald_k_field(cav_ctx *c, double *out4)
+{
+  if (c->precision == CAV_FP64) return kforce<double>(c, c->n, c->tab, out4);
+  return kforce<float>(c, c->n, c->tab, out4);
+}
+
 // E_k = 1/2 C sum W |S|^2 + self (+ dipole), valid after a force evaluation at
 // the same positions (the reference's S(k) coupling, energy.cpp:61-116).
 int launch_energy_k(cav_ctx *c, double *e)

@@ -128,6 +128,8 @@ struct cav_ctx {
   // per-type tables on the device
   double *d_minv = nullptr, *d_charge = nullptr, *d_eps = nullptr, *d_sig = nullptr,
          *d_lambda = nullptr;
+  double *d_raw_eps = nullptr, *d_raw_sig = nullptr;  // epsilon/sigma matrices as given
+  bool params_dirty = false;  // parameter values changed after setup
 
   // reductions: per-block partials and results
   double *part = nullptr;  // per-block partial sums (virial, energies)

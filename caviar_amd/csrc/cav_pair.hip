@@ -87,9 +87,10 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
       T coef = 0;
       int2 tj = make_int2(0, 0);
       if constexpr (LJ || ER) tj = a.tid[j];
+      // type-pair entry as the reference indexes it: [lower owned index][partner]
+      const int p = (j < a.n && j < i) ? tj.x * a.nt + ti : ti * a.nt + tj.x;
       if constexpr (LJ) {
         if (!(r2 > (T)a.lj_rc2)) {  // lj_cell_list.cpp:141-142: skip dr^2 > rc^2
-          const int p = ti * a.nt + tj.x;
           const T ri2 = (T)1 / r2;
           const T ri4 = ri2 * ri2, ri8 = ri4 * ri4;
           coef += -(T)a.c12[p] * (ri8 * ri4 * ri2) + (T)a.c6[p] * ri8 + (T)a.cshift[p];
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
           bool skip = false;
           if constexpr (LAM) {
             if (moli == a.mol[j]) {
-              const T l = (T)a.lambda[ti * a.nt + tj.x];
+              const T l = (T)a.lambda[p];
               if (l == 0) skip = true;
               lam = l;
             }
@@ -340,7 +341,8 @@ __global__ void k_lj_cellwalk(CellArgs c, int nt, const double *c12, const doubl
       const double dx = __dsub_rn(c.x[j], xi), dy = __dsub_rn(c.y[j], yi), dz = __dsub_rn(c.z[j], zi);
       const double r2 = dot_rn_p(dx, dy, dz);
       if (r2 > rc2) continue;
-      const int p = ti * nt + c.type[j];
+      const int tj = c.type[j];
+      const int p = (j < c.n && j < i) ? tj * nt + ti : ti * nt + tj;
       const double ri2 = 1.0 / r2, ri4 = ri2 * ri2, ri8 = ri4 * ri4;
       const double coef = -c12[p] * (ri8 * ri4 * ri2) + c6[p] * ri8 + cshift[p];
       fx += coef * dx;
@@ -473,7 +475,8 @@ __global__ void k_energy_lj(PairArgs a, const double *u12, const double *u6, con
       const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
       const double r2 = dot_rn_p(dx, dy, dz);
       if (r2 > a.lj_rc2) continue;
-      const int p = ti * a.nt + a.type[j];
+      const int tj = a.type[j];
+      const int p = (j < a.n && j < i) ? tj * a.nt + ti : ti * a.nt + tj;
       const double ri2 = 1.0 / r2, ri6 = ri2 * ri2 * ri2;
       e += 0.5 * (u12[p] * ri6 * ri6 - u6[p] * ri6 + 0.5 * a.cshift[p] * r2 - uc[p]);
     }

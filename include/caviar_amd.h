@@ -124,6 +124,19 @@ int cav_energy(cav_ctx *ctx, double *e_lj, double *e_r, double *e_k);
 int cav_point_field(cav_ctx *ctx, int which, int np, const double *pts, double *field,
                     double *potential);
 
+/* ---- differentiation (no counterpart in the reference) ------------------ */
+/* Force-matching adjoint: given residual weights w (3n doubles, w_i = dL/dF_i
+ * with F_i = mass_i * acceleration_i at the current positions and parameters),
+ * returns g = sum_i w_i . dF_i/dtheta for theta = lj_cell_list epsilon[a][b]
+ * and sigma[a][b] (nt*nt each, the force shift at rc included) and the type
+ * charges (nt) through Electrostatic_ewald_r and Electrostatic_ewald_k.  The
+ * gradient of the reference's force formulas (lj_cell_list.cpp:143-155,
+ * electrostatic_ewald_r.cpp:183-211, electrostatic_ewald_k.cpp:197-226).
+ * Parameter values may be changed with the cav_set_* calls after setup; they
+ * take effect at the next cav_recalculate / cav_step.  NULL outputs skipped. */
+int cav_adjoint(cav_ctx *ctx, const double *weights, double *g_epsilon, double *g_sigma,
+                double *g_charge);
+
 /* ---- measurement ----------------------------------------------------------- */
 /* Statistics of the current Verlet list: list entries per atom with the
  * reference's half-list membership, and those inside the LJ cutoff. */

@@ -78,6 +78,7 @@ def _load():
         "orc_energy_lj": (_d, [_p]),
         "orc_point_r": (None, [_p, _i, _dp, _dp, _dp]),
         "orc_point_k": (None, [_p, _i, _dp, _dp, _dp]),
+        "orc_fm_adjoint": (None, [_p, _dp, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -253,6 +254,15 @@ class Oracle:
         p = np.empty(len(pts))
         self.lib.orc_point_k(self.h, len(pts), _dptr(pts), _dptr(f), _dptr(p))
         return f, p
+
+    def adjoint(self, weights):
+        nt = len(self.cfg.mass)
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1, 3)
+        ge = np.zeros(nt * nt)
+        gs = np.zeros(nt * nt)
+        gq = np.zeros(nt)
+        self.lib.orc_fm_adjoint(self.h, _dptr(w), _dptr(ge), _dptr(gs), _dptr(gq))
+        return ge.reshape(nt, nt), gs.reshape(nt, nt), gq
 
     def ewald_k_sample(self, k0, k1, i0, i1):
         ts = ctypes.c_double()

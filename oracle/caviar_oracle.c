@@ -1041,3 +1041,147 @@ void orc_get_kvectors(const orc_sys *s, double *kvec, double *fk)
 }
 
 void orc_get_sk(const orc_sys *s, double *sk) { memcpy(sk, s->sk, sizeof(double) * 2 * s->nk); }
+
+/* ------------------------------------------------------ force-matching adjoint --
+ * FP64 reference of g = sum_i w_i . dF_i/dtheta (no counterpart in the reference;
+ * the derivative of its force formulas).  Written independently of the GPU
+ * kernels: pairs once per reference list entry with Newton's third law, so each
+ * pair term is contracted with (w_i - w_j) (w_j = 0 for a ghost partner), and
+ * k-space over the reference's full truncated set K without symmetry.  F = mass *
+ * acceleration.  Outputs: g_eps[nt*nt], g_sig[nt*nt], g_q[nt]. */
+void orc_fm_adjoint(orc_sys *s, const double *w, double *g_eps, double *g_sig, double *g_q)
+{
+  const int nt = s->nt, nt2 = nt * nt;
+  for (int p = 0; p < nt2; ++p)
+    g_eps[p] = g_sig[p] = 0;
+  for (int t = 0; t < nt; ++t)
+    g_q[t] = 0;
+  /* LJ: the cell walk of Lj_cell_list (owned j > i and every ghost) */
+  if (s->lj_on && s->build_cells) {
+    update_cell_list(s);
+    const double rc = s->lj_rc, rc2 = rc * rc;
+    const double rci2 = 1.0 / rc2, c6 = rci2 * rci2 * rci2, c8 = c6 * rci2, c14 = c6 * c6 * rci2;
+    for (int i = 0; i < s->n; ++i) {
+      const double *pi = s->pos + 3 * i;
+      const int ti = s->type[i];
+      const int b = bin_of(s, pi);
+      for (int q = s->st_off[b]; q < s->st_off[b + 1]; ++q) {
+        const int c = s->st[q];
+        for (int t = s->bin_off[c]; t < s->bin_off[c + 1]; ++t) {
+          const int j = s->bin_items[t];
+          const int ghost = j >= s->n;
+          if (!ghost && !(i < j))
+            continue;
+          const double *pj = apos(s, j);
+          const int tj = atype(s, j);
+          double dr[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+          const double r2 = dot3(dr);
+          if (r2 > rc2)
+            continue;
+          double dw[3] = {w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+          if (!ghost)
+            for (int d = 0; d < 3; ++d)
+              dw[d] -= w[3 * j + d];
+          const double wr = dw[0] * dr[0] + dw[1] * dr[1] + dw[2] * dr[2];
+          const int p = ti * nt + tj;
+          const double e = s->eps[p], sg = s->sig[p];
+          const double ri2 = 1.0 / r2, r6 = ri2 * ri2 * ri2, r8 = r6 * ri2, r14 = r6 * r6 * ri2;
+          const double s2 = sg * sg, s5 = s2 * s2 * sg, s6 = s5 * sg, s11 = s6 * s5, s12 = s6 * s6;
+          g_eps[p] += wr * 4.0 * (-12 * s12 * r14 + 6 * s6 * r8 + 12 * s12 * c14 - 6 * s6 * c8);
+          g_sig[p] += wr * 4.0 * e * (-144 * s11 * r14 + 36 * s5 * r8 + 144 * s11 * c14 - 36 * s5 * c8);
+        }
+      }
+    }
+  }
+  /* Ewald_r: every reference Verlet-list entry */
+  if (s->er_on && s->nl_off) {
+    const double alpha = s->er_alpha, a2 = alpha * alpha;
+    for (int i = 0; i < s->n; ++i) {
+      const double *pi = s->pos + 3 * i;
+      const int ti = s->type[i];
+      for (long t = s->nl_off[i]; t < s->nl_off[i + 1]; ++t) {
+        const int j = s->nl_items[t];
+        const int ghost = j >= s->n;
+        const double *pj = apos(s, j);
+        const int tj = atype(s, j);
+        double lam = 1.0;
+        if (s->lambda_set && s->mol[i] == amol(s, j)) {
+          if (s->lambda[ti * nt + tj] == 0.0)
+            continue;
+          lam = s->lambda[ti * nt + tj];
+        }
+        double r[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+        if (r[0] == 0 && r[1] == 0 && r[2] == 0)
+          continue;
+        double dw[3] = {w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+        if (!ghost)
+          for (int d = 0; d < 3; ++d)
+            dw[d] -= w[3 * j + d];
+        const double wr = dw[0] * r[0] + dw[1] * r[1] + dw[2] * r[2];
+        const double r2 = dot3(r), rn = sqrt(r2);
+        const double sr = (2 * alpha * ORC_PIS_INV * exp(-a2 * r2) + erfc(alpha * rn) / rn) / r2;
+        const double base = -lam * s->er_ke * sr * wr;
+        g_q[ti] += base * s->charge[tj];
+        g_q[tj] += base * s->charge[ti];
+      }
+    }
+  }
+  /* Ewald_k over the full truncated set K */
+  if (s->ek_on) {
+    if (!s->ek_init)
+      ek_init(s);
+    const double C = ORC_4PI * s->ek_ke * s->lxyz_inv;
+    double *G = (double *)calloc(3 * (size_t)(s->n ? s->n : 1), sizeof(double));
+    double *gk = (double *)calloc((size_t)nt, sizeof(double));
+#pragma omp parallel num_threads(g_threads)
+    {
+      double *gk_loc = (double *)calloc((size_t)nt, sizeof(double));
+      double complex *ss = (double complex *)malloc(sizeof(double complex) * (nt ? nt : 1));
+#pragma omp for schedule(static)
+      for (int k = 0; k < s->nk; ++k) {
+        const double *kv = s->kvec + 3 * k;
+        double complex S = 0, T = 0;
+        for (int t = 0; t < nt; ++t)
+          ss[t] = 0;
+        for (int j = 0; j < s->n; ++j) {
+          const double *p = s->pos + 3 * j;
+          const double th = kv[0] * p[0] + kv[1] * p[1] + kv[2] * p[2];
+          const double complex e = cos(th) + I * sin(th);
+          const double q = s->charge[s->type[j]];
+          S += q * e;
+          ss[s->type[j]] += e;
+          T += q * (w[3 * j] * kv[0] + w[3 * j + 1] * kv[1] + w[3 * j + 2] * kv[2]) * e;
+        }
+        for (int t = 0; t < nt; ++t)
+          gk_loc[t] += s->fk[k] * cimag(conj(ss[t]) * T);
+      }
+#pragma omp critical
+      for (int t = 0; t < nt; ++t)
+        gk[t] += gk_loc[t];
+      free(gk_loc);
+      free(ss);
+    }
+    /* per-atom field G_i = C sum_k f k Im(conj(S_k) e^{ik.r_i}) with S_k from ek_structure_factor */
+    ek_structure_factor(s, 0, s->nk);
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int i = 0; i < s->n; ++i) {
+      const double *p = s->pos + 3 * i;
+      double g[3] = {0, 0, 0};
+      for (int k = 0; k < s->nk; ++k) {
+        const double *kv = s->kvec + 3 * k;
+        const double th = kv[0] * p[0] + kv[1] * p[1] + kv[2] * p[2];
+        const double im = s->sk[2 * k] * sin(th) - s->sk[2 * k + 1] * cos(th);
+        for (int d = 0; d < 3; ++d)
+          g[d] += s->fk[k] * kv[d] * im;
+      }
+      for (int d = 0; d < 3; ++d)
+        G[3 * i + d] = C * g[d];
+    }
+    for (int i = 0; i < s->n; ++i)
+      g_q[s->type[i]] += w[3 * i] * G[3 * i] + w[3 * i + 1] * G[3 * i + 1] + w[3 * i + 2] * G[3 * i + 2];
+    for (int t = 0; t < nt; ++t)
+      g_q[t] += C * gk[t];
+    free(G);
+    free(gk);
+  }
+}

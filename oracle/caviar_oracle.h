@@ -89,6 +89,11 @@ double orc_energy_lj(orc_sys *s);     /* shifted-force LJ energy (no reference c
 void orc_point_r(orc_sys *s, int np, const double *pts, double *field, double *pot);
 void orc_point_k(orc_sys *s, int np, const double *pts, double *field, double *pot);
 
+/* Force-matching adjoint reference: g = sum_i w_i . dF_i/dtheta for the LJ
+ * epsilon/sigma matrices (nt*nt) and the type charges (nt), F = mass * acc.
+ * No reference counterpart. */
+void orc_fm_adjoint(orc_sys *s, const double *w, double *g_eps, double *g_sig, double *g_q);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,9 +5,11 @@ reached through its C ABI (``include/caviar_amd.h``) with ctypes.  There is no
 CPU fallback: importing works without a GPU, but creating an :class:`Engine`
 fails loudly when the library or a device is missing.
 
-The Python classes mirror the reference's objects for this path (see
-``caviar_amd.objects`` for the CASL-facing names); :class:`Engine` is one
-device-resident single-domain system.
+:class:`Engine` is one device-resident single-domain system.  The C++ classes
+with the reference's own names and CASL commands (Domain, Atom_data,
+Neighborlist, force_field::Lj_cell_list / Electrostatic_ewald_r /
+Electrostatic_ewald_k, Md_simulator) and the ``caviar_amd_run`` script driver
+are in ``caviar_amd/host`` on the same C ABI.
 """
 from __future__ import annotations
 
@@ -24,6 +26,7 @@ LIB_PATH = os.path.join(_HERE, "libcaviar_amd.so")
 
 CAV_FP64 = 0
 CAV_MIXED = 1
+FORCE_FIELDS = {"lj": 1, "ewald_r": 2, "ewald_k": 4}
 _ARRAYS = {"pos": 0, "vel": 1, "acc": 2, "acc_old": 3}
 _ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "call out of order",
            4: "NaN acceleration", 5: "capacity"}
@@ -88,6 +91,10 @@ def load_library():
         "cav_kernel_time": [P, ctypes.c_char_p, dp, ip],
         "cav_enable_timing": [P, I],
         "cav_adjoint": [P, dp, dp, dp, dp],
+        "cav_exchange": [P, ip],
+        "cav_reset_acceleration": [P],
+        "cav_reset_virial": [P],
+        "cav_forces": [P, I],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -200,6 +207,27 @@ class Engine:
         r = ctypes.c_int()
         self._check(self.lib.cav_neighbor_build(self.h, int(update), ctypes.byref(r)))
         return bool(r.value)
+
+    # ---- the pieces of a step, one object at a time
+    def exchange(self) -> bool:
+        """Wrap owned atoms and rebuild the ghosts; True if any atom wrapped."""
+        w = ctypes.c_int()
+        self._check(self.lib.cav_exchange(self.h, ctypes.byref(w)))
+        return bool(w.value)
+
+    def reset_acceleration(self) -> None:
+        self._check(self.lib.cav_reset_acceleration(self.h))
+
+    def reset_virial(self) -> None:
+        self._check(self.lib.cav_reset_virial(self.h))
+
+    def forces(self, *fields: str) -> None:
+        """Add the accelerations (and virial) of the named force fields
+        ("lj", "ewald_r", "ewald_k"), as each Force_field::calculate_acceleration does."""
+        mask = 0
+        for f in fields:
+            mask |= FORCE_FIELDS[f]
+        self._check(self.lib.cav_forces(self.h, mask))
 
     # ---- state
     def get(self, what: str) -> np.ndarray:

@@ -1,4 +1,5 @@
-"""Build the in-tree HIP library (gfx950): ``python -m caviar_amd.build``."""
+"""Build the in-tree HIP library (gfx950) and the C++ host layer on top of it:
+``python -m caviar_amd.build``."""
 from __future__ import annotations
 
 import os
@@ -9,8 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def build(jobs: int = 8) -> str:
-    csrc = os.path.join(HERE, "csrc")
-    subprocess.run(["make", "-s", "-C", csrc, f"-j{jobs}"], check=True)
+    for sub in ("csrc", "host"):
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, sub), f"-j{jobs}"], check=True)
     return os.path.join(HERE, "libcaviar_amd.so")
 
 

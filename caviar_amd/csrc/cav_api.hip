@@ -141,6 +141,14 @@ __global__ void k_begin_step(int *flags)
 
 __global__ void k_set_flag(int *flags, int idx, int v) { flags[idx] = v; }
 
+__global__ void k_set_scalar(double *p, double v) { *p = v; }
+
+__global__ void k_zero3(int n, double *a, double *b, double *c)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) a[i] = b[i] = c[i] = 0.0;
+}
+
 __global__ void k_charge_of_type(int n, const int *type, const double *charge, double *q)
 {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -333,7 +341,7 @@ static int forces(cav_ctx *c)
 {
   const bool pair = (c->lj_on && !c->lj_cellwalk) || c->er_on;
   if (pair)
-    CAV_TRY(launch_pairs(c));
+    CAV_TRY(launch_pairs(c, true, true, false));
   else
     CAV_TRY(launch_zero_acc(c));
   if (c->lj_on && c->lj_cellwalk) CAV_TRY(launch_lj_cellwalk(c));
@@ -820,6 +828,52 @@ int cav_neighbor_build(cav_ctx *c, int update, int *rebuilt)
   CAV_TRY(neighbor_phase(c, update != 0));
   CAV_TRY(sync_flags(c));
   if (rebuilt) *rebuilt = c->h_flags[F_REBUILD];
+  return CAV_OK;
+}
+
+int cav_exchange(cav_ctx *c, int *wrapped)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  (void)hipSetDevice(c->device);
+  k_set_flag<<<1, 1, 0, c->stream>>>(c->flags, F_WRAPPED, 0);
+  CAV_TRY(wrap_and_ghosts(c));
+  CAV_TRY(sync_flags(c));
+  if (wrapped) *wrapped = c->h_flags[F_WRAPPED] != 0;
+  return CAV_OK;
+}
+
+int cav_reset_acceleration(cav_ctx *c)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  k_zero3<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->ax, c->ay, c->az);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+int cav_reset_virial(cav_ctx *c)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  k_set_scalar<<<1, 1, 0, c->stream>>>(c->scal, 0.0);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+int cav_forces(cav_ctx *c, int mask)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  if (mask & ~(CAV_FF_LJ | CAV_FF_EWALD_R | CAV_FF_EWALD_K))
+    return fail(c, CAV_ERR_ARG, "unknown force-field bit in mask");
+  (void)hipSetDevice(c->device);
+  CAV_TRY(refresh_params(c));
+  const bool lj = (mask & CAV_FF_LJ) && c->lj_on, er = (mask & CAV_FF_EWALD_R) && c->er_on;
+  if ((lj && !c->lj_cellwalk) || er) CAV_TRY(launch_pairs(c, lj, er, true));
+  if (lj && c->lj_cellwalk) CAV_TRY(launch_lj_cellwalk(c));
+  if ((mask & CAV_FF_EWALD_K) && c->ek_on) CAV_TRY(launch_ewald_k(c));
+  CAV_HIP(hipStreamSynchronize(c->stream));
   return CAV_OK;
 }
 

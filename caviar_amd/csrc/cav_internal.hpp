@@ -169,7 +169,7 @@ int launch_rebuild_check(cav_ctx *c);
 int launch_bins(cav_ctx *c, bool conditional);
 int launch_list(cav_ctx *c, bool conditional);
 int launch_save_old(cav_ctx *c, bool conditional);
-int launch_pairs(cav_ctx *c);
+int launch_pairs(cav_ctx *c, bool lj, bool er, bool accumulate);
 int launch_lj_cellwalk(cav_ctx *c);
 int launch_zero_acc(cav_ctx *c);
 int launch_ewald_k(cav_ctx *c);
@@ -183,7 +183,7 @@ int launch_energy_lj(cav_ctx *c, double *e);
 int launch_energy_k(cav_ctx *c, double *e);
 int point_field_r(cav_ctx *c, int np, const double *pts, double *field, double *pot);
 int point_field_k(cav_ctx *c, int np, const double *pts, double *field, double *pot);
-int reduce_partials(cav_ctx *c, int count, int slot);
+int reduce_partials(cav_ctx *c, int count, int slot, bool add = false);
 
 inline int grid_for(long n, int block = kBlock) { return (int)((n + block - 1) / block); }
 

@@ -47,6 +47,7 @@ struct PairArgs {
   const double *lambda;
   double *ax, *ay, *az;
   double *part;
+  int accumulate;  // add into the acceleration instead of overwriting it
 };
 
 constexpr int kLPA = 16;  // lanes per atom in the pair kernel
@@ -142,9 +143,15 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
   }
   if (t < a.n && sub == 0) {
     const double mi = a.minv[ti];
-    a.ax[i] = (double)fx * mi;
-    a.ay[i] = (double)fy * mi;
-    a.az[i] = (double)fz * mi;
+    if (a.accumulate) {
+      a.ax[i] += (double)fx * mi;
+      a.ay[i] += (double)fy * mi;
+      a.az[i] += (double)fz * mi;
+    } else {
+      a.ax[i] = (double)fx * mi;
+      a.ay[i] = (double)fy * mi;
+      a.az[i] = (double)fz * mi;
+    }
   }
   // deterministic per-block virial partial
   __shared__ double red[kBlock];
@@ -168,7 +175,7 @@ __global__ void k_pack(int n, long cap_total, const double *x, const double *y, 
   tid[a] = make_int2(type[a], id[a]);
 }
 
-__global__ void k_reduce_part(int count, const double *part, double *out)
+__global__ void k_reduce_part(int count, const double *part, double *out, int add)
 {
   __shared__ double red[kBlock];
   double s = 0;
@@ -179,12 +186,12 @@ __global__ void k_reduce_part(int count, const double *part, double *out)
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) *out = add ? *out + red[0] : red[0];
 }
 
-int reduce_partials(cav_ctx *c, int count, int slot)
+int reduce_partials(cav_ctx *c, int count, int slot, bool add)
 {
-  k_reduce_part<<<1, kBlock, 0, c->stream>>>(count, c->part, c->scal + slot);
+  k_reduce_part<<<1, kBlock, 0, c->stream>>>(count, c->part, c->scal + slot, add ? 1 : 0);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }
@@ -241,11 +248,12 @@ static void launch_pairs_t(cav_ctx *c, const PairArgs &a, bool lj, bool er, bool
     k_pairs<T, false, true, false><<<g, kBlock, 0, c->stream>>>(a);
 }
 
-int launch_pairs(cav_ctx *c)
+int launch_pairs(cav_ctx *c, bool want_lj, bool want_er, bool accumulate)
 {
-  const bool lj = c->lj_on && !c->lj_cellwalk, er = c->er_on;
+  const bool lj = want_lj && c->lj_on && !c->lj_cellwalk, er = want_er && c->er_on;
   if (!lj && !er) return CAV_OK;
   PairArgs a = pair_args(c);
+  a.accumulate = accumulate ? 1 : 0;
   const long tot = (long)c->n + c->gcap;
   k_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->x, c->y, c->z, c->q, c->type,
                                                  c->id, c->flags, c->xq, c->tid);
@@ -256,7 +264,7 @@ int launch_pairs(cav_ctx *c)
     launch_pairs_t<float>(c, a, lj, er, c->lambda_set);
   timer_end(c, c->t_pair);
   CAV_HIP(hipGetLastError());
-  return reduce_partials(c, grid_for((long)c->n * kLPA), 0);
+  return reduce_partials(c, grid_for((long)c->n * kLPA), 0, accumulate);
 }
 
 __global__ void k_zero_acc(int n, double *ax, double *ay, double *az)

@@ -1,0 +1,250 @@
+// Host-side objects of the LJ + Ewald path, with the reference's class names,
+// CASL commands and virtual interface, running on the caviar_amd C ABI.
+//
+// Reference interfaces mirrored here (paths under the reference tree):
+//   Domain                      include/caviar/objects/domain.h, objects/domain.cpp:206-266
+//   Atom_data (basic)           include/caviar/objects/atom_data.h, objects/atom_data.cpp:78-306
+//   Neighborlist (basic)        include/caviar/objects/neighborlist.h:35-186,
+//                               objects/neighborlist.cpp:66-88,193-251
+//   Force_field                 include/caviar/objects/force_field.h:33-61
+//   force_field::Lj_cell_list   objects/force_field/lj_cell_list.cpp:33-83
+//   force_field::Electrostatic_ewald_r  objects/force_field/electrostatic_ewald_r.cpp:31-110
+//   force_field::Electrostatic_ewald_k  objects/force_field/electrostatic_ewald_k.cpp:31-159
+//   Md_simulator (basic)        objects/md_simulator.cpp:58-233,382-617
+//
+// The particle arrays belong to Atom_data, as in the reference; here they live
+// on the GPU inside one cav_ctx that Atom_data owns, and the host vectors below
+// are mirrors refreshed by Atom_data::sync_from_device().  Force fields and the
+// neighbour list of one Atom_data share its context.  Md_simulator steps the
+// whole set in one fused call (cav_step); the per-object calls
+// (Neighborlist::build, Force_field::calculate_acceleration, ...) serve callers
+// that drive the objects themselves.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "caviar_amd.h"
+#include "casl.hpp"
+
+namespace caviar_amd {
+
+struct Vector {
+  double x = 0.0, y = 0.0, z = 0.0;
+};
+
+// Throws caviar_amd::Error with cav_last_error() when a C-ABI call fails.
+void check(cav_ctx *ctx, int status, const char *what);
+
+class Objects;
+
+// What every object can reach, as the reference's Pointers base gives each
+// object its object_container (include/caviar/utility/pointers.h).
+struct Pointers {
+  Objects *objects = nullptr;
+};
+
+class Domain : public Pointers {
+ public:
+  bool read(casl::Parser *parser);
+  void generate();
+  Vector lower_global, upper_global;
+  int boundary_condition[3] = {0, 0, 0};
+  bool generated = false;
+};
+
+class Atom_data : public Pointers {
+ public:
+  Atom_data();
+  ~Atom_data();
+  Atom_data(const Atom_data &) = delete;
+  Atom_data &operator=(const Atom_data &) = delete;
+
+  bool read(casl::Parser *parser);
+  void verify_settings();
+  bool add_atom(unsigned id, unsigned type, const Vector &pos, const Vector &vel);
+  void add_xyz_data_file(casl::Parser *parser);
+  void reset_owned_acceleration();
+  void reset_virial();
+  double virial();
+  // Number of atom types: one past the largest type in use or with a mass.
+  int num_types() const;
+
+  // The device context holding this atom data (created on first use).
+  cav_ctx *device();
+  // Push box, types and atoms to the device (before setup).
+  void upload();
+  // Refresh the host mirrors of the owned arrays from the device.
+  void sync_from_device();
+  // Ghost positions and the owned index each ghost images.
+  void ghosts(std::vector<Vector> &pos, std::vector<int> &src);
+
+  struct Owned {
+    std::vector<unsigned> id, type;
+    std::vector<int> molecule_index;
+    std::vector<Vector> position, velocity, acceleration;
+  } atom_struct_owned;
+  struct TypeParams {
+    std::vector<double> mass, mass_inv, charge, radius;
+  } atom_type_params;
+  double ghost_cutoff = 0.0;
+  double k_b = 1.0;
+  Domain *domain = nullptr;
+  int device_index = 0;
+  int precision = CAV_FP64;
+
+ private:
+  cav_ctx *ctx_ = nullptr;
+};
+
+class Neighborlist : public Pointers {
+ public:
+  bool read(casl::Parser *parser);
+  void verify_settings();
+  // Neighborlist::init: push the settings into the context (before setup).
+  void init();
+  // Neighborlist::build(update): rebuild test + list; returns whether it rebuilt.
+  bool build(bool update);
+  // The reference's neighlist: for each owned i, the owned j > i and every
+  // ghost (as n + g) within cutoff_extra.  Order within a row is the build's.
+  std::vector<std::vector<unsigned>> neighlist();
+
+  double cutoff = 0.0;
+  double cutoff_extra_coef = 1.12347;  // neighborlist.h:134
+  double dt = 0.0;                     // accepted, unused (as in the reference)
+  bool make_verlet_list_from_cell_list = false;
+  bool build_cell_list = false;
+  bool all_atom_test = false;
+  bool rebuild_test = false;
+  Atom_data *atom_data = nullptr;
+  Domain *domain = nullptr;
+};
+
+class Force_field : public Pointers {
+ public:
+  virtual ~Force_field() = default;
+  virtual bool read(casl::Parser *parser) = 0;
+  virtual void verify_settings() = 0;
+  // Push this field's settings into the context (before setup).
+  virtual void init() = 0;
+  // Adds this field's accelerations (and virial) into Atom_data.
+  virtual void calculate_acceleration();
+  virtual double energy();
+  virtual double potential(const Vector &r);
+  virtual Vector field(const Vector &r);
+  // Which cav_force_field bit this object is.
+  virtual int mask() const = 0;
+  virtual const char *type_name() const = 0;
+
+  double cutoff = 0.0;
+  Atom_data *atom_data = nullptr;
+  Domain *domain = nullptr;
+  Neighborlist *neighborlist = nullptr;
+};
+
+namespace force_field {
+
+class Lj_cell_list : public Force_field {
+ public:
+  bool read(casl::Parser *parser) override;
+  void verify_settings() override;
+  void init() override;
+  double energy() override;
+  int mask() const override { return CAV_FF_LJ; }
+  const char *type_name() const override { return "lj_cell_list"; }
+  // Row-ragged like the reference's std::vector<std::vector<double>>; entries
+  // never set read as 0.
+  std::vector<std::vector<double>> epsilon, sigma;
+};
+
+class Electrostatic_ewald_r : public Force_field {
+ public:
+  Electrostatic_ewald_r() { cutoff = 0.0; }
+  bool read(casl::Parser *parser) override;
+  void verify_settings() override;
+  void init() override;
+  double energy() override;
+  double potential(const Vector &r) override;
+  Vector field(const Vector &r) override;
+  int mask() const override { return CAV_FF_EWALD_R; }
+  const char *type_name() const override { return "electrostatic_ewald_r"; }
+  double k_electrostatic = 1.0;
+  double alpha = 1.0;
+  bool lambda_is_set = false;
+  std::vector<std::vector<double>> lambda;  // missing entries read as 1
+};
+
+class Electrostatic_ewald_k : public Force_field {
+ public:
+  bool read(casl::Parser *parser) override;
+  void verify_settings() override;
+  void init() override;
+  double energy() override;
+  double self_energy();
+  double potential(const Vector &r) override;
+  Vector field(const Vector &r) override;
+  int mask() const override { return CAV_FF_EWALD_K; }
+  const char *type_name() const override { return "electrostatic_ewald_k"; }
+  // S(k) in the reference's k order (potential_k_coef_cmplx), re/im interleaved.
+  std::vector<double> potential_k_coef();
+
+  double k_electrostatic = 1.0;
+  double alpha = -1.0;  // no default in the reference: must be set
+  int kx_max = 1, ky_max = 1, kz_max = 1;
+  bool dipole = false;
+  double epsilon_dipole = 1.0;
+  // set by slab_normal_axis or slab_lz; axis z and thickness 1 by default
+  // (electrostatic_ewald_k.cpp:31-45)
+  bool slab_geometry = false;
+  int slab_normal_axis = 2;
+  double slab_lz = 1.0;
+};
+
+}  // namespace force_field
+
+class Md_simulator : public Pointers {
+ public:
+  bool read(casl::Parser *parser);
+  void verify_settings();
+  // Md_simulator::initialize + setup (md_simulator.cpp:303,504-549).
+  void setup();
+  // One velocity-Verlet step (md_simulator.cpp:382-617).
+  void step();
+  // Md_simulator::run: steps from initial_step to final_step.
+  void run();
+
+  Atom_data *atom_data = nullptr;
+  std::vector<Force_field *> force_field;
+  std::vector<Neighborlist *> neighborlist;
+  double dt = 0.001;
+  long initial_step = 0, final_step = 0, current_step = 0;
+  bool initialized = false;
+  // Read scripts without the GPU: run / step only count their steps.
+  static bool dry_run;
+  long steps_requested = 0;
+
+ private:
+  void advance(long nsteps);
+};
+
+// Named objects of a script (the reference's Object_container,
+// interpreter/object_container.cpp); owns them.
+class Objects {
+ public:
+  std::map<std::string, std::unique_ptr<Domain>> domain;
+  std::map<std::string, std::unique_ptr<Atom_data>> atom_data;
+  std::map<std::string, std::unique_ptr<Neighborlist>> neighborlist;
+  std::map<std::string, std::unique_ptr<Force_field>> force_field;
+  std::map<std::string, std::unique_ptr<Md_simulator>> md_simulator;
+  std::vector<std::string> order;  // creation order of every name
+
+  bool has(const std::string &name) const;
+  Domain *find_domain(const std::string &name, casl::Parser *p);
+  Atom_data *find_atom_data(const std::string &name, casl::Parser *p);
+  Neighborlist *find_neighborlist(const std::string &name, casl::Parser *p);
+  Force_field *find_force_field(const std::string &name, casl::Parser *p);
+};
+
+}  // namespace caviar_amd

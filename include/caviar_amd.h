@@ -36,6 +36,9 @@ enum cav_precision {
 
 enum cav_array { CAV_POS = 0, CAV_VEL = 1, CAV_ACC = 2, CAV_ACC_OLD = 3 };
 
+/* Force fields of one context, as bits of the cav_forces mask. */
+enum cav_force_field { CAV_FF_LJ = 1, CAV_FF_EWALD_R = 2, CAV_FF_EWALD_K = 4 };
+
 /* ---- lifetime ---------------------------------------------------------- */
 /* Replaces the objects the interpreter creates (object_creator/objects_types.cpp:25-51). */
 int cav_create(int device, cav_ctx **out);
@@ -93,6 +96,22 @@ int cav_recalculate(cav_ctx *ctx, int force_rebuild);
 /* Neighborlist::build(update) alone (neighborlist.cpp:66-88) on the current
  * ghosts; *rebuilt (may be NULL) receives whether the Verlet list was rebuilt. */
 int cav_neighbor_build(cav_ctx *ctx, int update, int *rebuilt);
+
+/* ---- the pieces of a step, for callers that drive the objects one by one -- */
+/* Atom_data::exchange_owned + exchange_ghost for one domain
+ * (exchange_owned_single_md_domain.cpp:31-59, exchange_ghost_single_md_domain.cpp:
+ * 26-218): wrap owned positions, rebuild the ghosts; *wrapped (may be NULL)
+ * receives whether any atom was wrapped (which forces the next list rebuild). */
+int cav_exchange(cav_ctx *ctx, int *wrapped);
+/* Atom_data::reset_owned_acceleration (atom_data.cpp:308-316). */
+int cav_reset_acceleration(cav_ctx *ctx);
+/* Atom_data::reset_virial (atom_data.h:250; called per step, md_simulator.cpp:408). */
+int cav_reset_virial(cav_ctx *ctx);
+/* Force_field::calculate_acceleration of the fields in mask (lj_cell_list.cpp:
+ * 85-175, electrostatic_ewald_r.cpp:115-317, electrostatic_ewald_k.cpp:161-294):
+ * ADDS their accelerations and virial on the current ghosts and Verlet list,
+ * as each reference force field adds into atom_struct_owned.acceleration. */
+int cav_forces(cav_ctx *ctx, int mask);
 
 /* ---- inspection (parity) ------------------------------------------------- */
 /* atom_struct_ghost after exchange_ghost (exchange_ghost_single_md_domain.cpp):

@@ -186,3 +186,37 @@ def test_list_capacity_growth_is_transparent(gpu):
     e.setup()
     assert np.array_equal(F.sorted_pairs(e.pairs()), F.sorted_pairs(o.pairs()))
     assert F.rel_err(e.get("acc"), o.get("acc")) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["ions600_setup", "lj256_setup", "lambda_dipole300_setup"])
+def test_per_field_calculate_acceleration(gpu, name):
+    """Force_field::calculate_acceleration one field at a time (cav_forces):
+    each field's acceleration against the oracle's same field, and the three
+    added together against the fused force phase (1e-12 relative)."""
+    s, ref, meta = F.load(name)
+    print(f"[case] per-field {name}", flush=True)
+    e = Engine(s)
+    e.setup()
+    fused = e.get("acc")
+    o = Oracle(s)
+    o.setup()
+    fields = [f for f, on in (("lj", s.lj), ("ewald_r", s.ewald_r), ("ewald_k", s.ewald_k)) if on is not None]
+    orc = {"lj": o.lib.orc_lj_acc, "ewald_r": o.lib.orc_ewald_r_acc, "ewald_k": o.lib.orc_ewald_k_acc}
+    total = np.zeros_like(fused)
+    for f in fields:
+        e.reset_acceleration()
+        e.forces(f)
+        mine = e.get("acc")
+        o.lib.orc_reset_acc(o.h)
+        orc[f](o.h)
+        want = o.get("acc")
+        assert F.rel_err(mine, want) <= 1e-12, f
+        total += mine
+    e.reset_acceleration()
+    e.forces(*fields)
+    assert F.rel_err(e.get("acc"), fused) <= 1e-12
+    assert F.rel_err(total, fused) <= 1e-12
+    # the exchange + build pieces reproduce the fused path's ghosts and list
+    e.exchange()
+    e.neighbor_build(False)
+    assert np.array_equal(F.sorted_pairs(e.pairs()), F.sorted_pairs(ref["pairs"]))

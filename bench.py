@@ -90,21 +90,6 @@ def cpu_baseline(system, threads):
                        f"{na} of {n} atoms over all k ({threads} threads), scaled to one full step")}
 
 
-def copy_probe(mib):
-    """A device copy and an elementwise negation of ``mib`` MiB: kernels whose HBM
-    traffic is known (read mib, write mib), to calibrate byte counters."""
-    if mib <= 0:
-        return
-    import torch
-    a = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.neg(a, out=b)
-    torch.cuda.synchronize()
-    log(f"copy probe: {mib} MiB copied and negated")
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,9 +102,6 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
                     help="run one form only (for profiling); the JSON line needs both")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="after the run, copy MIB MiB device-to-device (and negate it) so that "
-                         "hardware byte counters can be scaled by a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -133,7 +115,6 @@ def main():
 
     if args.precision != "both":
         run_engine(system, args.precision, args.warmup, args.steps, args.dt)
-        copy_probe(args.copy_probe)
         return
     e64, t64, tm64 = run_engine(system, "fp64", args.warmup, args.steps, args.dt)
     e32, t32, tm32 = run_engine(system, "mixed", args.warmup, args.steps, args.dt)

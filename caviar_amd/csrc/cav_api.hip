@@ -182,7 +182,7 @@ static void free_all(cav_ctx *c)
                   c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
                   c->d_W,     c->kacc,      c->order,    c->ord_blk,  c->xq,
-                  c->tid,     c->d_raw_eps, c->d_raw_sig};
+                  c->tid,     c->d_raw_eps, c->d_raw_sig, c->ztab, c->bmat_h};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_flags) (void)hipHostFree(c->h_flags);

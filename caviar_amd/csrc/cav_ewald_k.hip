@@ -57,10 +57,30 @@ __device__ __forceinline__ int mfma_row(int lk, int e)
 // The phase fraction n*x/l is reduced to [-1/2, 1/2] in FP64 before sincospi,
 // so the FP32 tables of the mixed form carry ~1e-7 absolute phase error even
 // where n k.r reaches hundreds of radians.
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned pack_h2(_Float16 a, _Float16 b)
+{
+  const h2 v = {a, b};
+  return __builtin_bit_cast(unsigned, v);
+}
+
+// hi = fp16(v), lo = fp16(v - hi) for a pair of values, packed (first value low)
+__device__ __forceinline__ void split_pair(float a, float b, unsigned &hi, unsigned &lo)
+{
+  const _Float16 ah = (_Float16)a, bh = (_Float16)b;
+  hi = pack_h2(ah, bh);
+  lo = pack_h2((_Float16)(a - (float)ah), (_Float16)(b - (float)bh));
+}
+
+// ztab (optional, mixed form): for every Z entry a 16-byte record of the two
+// real-embedding columns split into halves, (Zr, -Zi) hi, lo, (Zi, Zr) hi, lo.
 template <typename T>
 __global__ void k_tables(long npts, int TS, int kx, int ky, int kz, double ilx, double ily,
                          double ilz, const double *x, const double *y, const double *z,
-                         Cx<T> *tab)
+                         Cx<T> *tab, u4v *ztab)
 {
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= npts * TS) return;
@@ -92,6 +112,12 @@ __global__ void k_tables(long npts, int TS, int kx, int ky, int kz, double ilx, 
     sincospif((float)(2.0 * f), &s, &c);
     v.re = c;
     v.im = s;
+    if (ztab && e >= nX + nY) {
+      unsigned h0, l0, h1, l1;
+      split_pair(c, -s, h0, l0);
+      split_pair(s, c, h1, l1);
+      ztab[a * (2 * kz + 1) + (e - nX - nY)] = u4v{h0, l0, h1, l1};
+    }
   }
   tab[t] = v;
 }
@@ -254,7 +280,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
 template <typename T>
 __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz, int KSP,
                             const double *part, const double *W, double *sk, T *bm,
-                            double *epart)
+                            double *epart, double inv_sa, _Float16 *bmh, int nkh, double sb)
 {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   double e = 0;
@@ -266,6 +292,8 @@ __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz
       sr += p[0];
       si += p[1];
     }
+    sr *= inv_sa;
+    si *= inv_sa;
     sk[(long)r * 2 * nZ + 2 * c] = sr;
     sk[(long)r * 2 * nZ + 2 * c + 1] = si;
     const double w = r < R ? W[(long)r * nZ + c] : 0.0;
@@ -281,6 +309,16 @@ __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz
         const int kap = 2 * c + sidx;
         const int lanef = (kap & 3) * 16 + rr;
         bm[(((long)rb * 4 + comp) * 64 + lanef) * KSP + (kap >> 2)] = (T)v[comp][sidx];
+        if (bmh) {
+          // split form: K-step kap/32, lane (kap%32/8)*16 + rr, element kap%8;
+          // 64 lanes x 8 hi halves, then 64 lanes x 8 lo halves
+          const long unit = (((long)rb * 4 + comp) * nkh + (kap >> 5)) * 2;
+          const int ln = ((kap & 31) >> 3) * 16 + rr, el = kap & 7;
+          const float vf = (float)(v[comp][sidx] * sb);
+          const _Float16 h = (_Float16)vf;
+          bmh[(unit * 64 + ln) * 8 + el] = h;
+          bmh[((unit + 1) * 64 + ln) * 8 + el] = (_Float16)(vf - (float)h);
+        }
       }
   }
   __shared__ double red[kBlock];
@@ -450,6 +488,307 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
     }
 }
 
+
+// ------------------------------------------------- split-FP16 mixed form --
+//
+// The mixed form runs both GEMMs on v_mfma_f32_16x16x32_f16 (16 atoms or 16
+// kappa values per K-step instead of 2 / 4 on the FP32 tiles).  Each FP32
+// operand v is split into hi = fp16(v) and lo = fp16(v - hi) and a product is
+// taken as hi*hi + hi*lo + lo*hi, which keeps ~22 significant bits (the
+// dropped lo*lo term is below 2^-22 relative); accumulation is FP32 within
+// an atom chunk and FP64 across chunks, as on the FP32 tiles.  Operands are
+// pre-scaled by powers of two (sA on q X Y, sB on the force-pass operand,
+// sB chosen from the bound |g| <= max W * sum |q|) so that no half overflows
+// and the scales are removed exactly afterwards.
+
+__device__ __forceinline__ f4 mfma_h(u4v a, u4v b, f4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
+                                                __builtin_bit_cast(h8, b), c, 0, 0, 0);
+}
+
+struct SArgsH {
+  int n, TS, nX, nY, nZ, Rtiles, chunk, stage, tiles_per_group, Rpad, SC;
+  float sa;
+  const double *q;
+  const void *tab;   // FP32 records [X | Y | Z] per atom
+  const void *ztab;  // split Z records, nZ x 16 B per atom
+  double *part;      // [chunk][Rpad][SC]
+};
+
+// Same work split as k_sk: a workgroup is an atom chunk x a group of row
+// tiles, wave w owns RT row tiles and every column tile; a K-step is 16 atoms.
+// Lane l feeds rows / columns l%16 with atoms 4(l/16)..4(l/16)+3 of the step,
+// (re, im) of each atom adjacent along the reduction axis.
+template <int RT, int SCT>
+__global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
+{
+  extern __shared__ unsigned char smem_raw[];
+  const int stage = a.stage;
+  const size_t rec_bytes = (size_t)stage * a.TS * sizeof(Cx<float>);
+  const size_t z_bytes = (size_t)stage * a.nZ * 16;
+  const size_t buf_bytes = rec_bytes + z_bytes + (size_t)stage * sizeof(float);
+  const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
+  const unsigned char *ztab = reinterpret_cast<const unsigned char *>(a.ztab);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int chunk = blockIdx.x;
+  const int rt0 = blockIdx.y * a.tiles_per_group + wave * RT;
+  const int rt_end = min(a.Rtiles, (int)(blockIdx.y + 1) * a.tiles_per_group);
+  const long a0 = (long)chunk * a.chunk;
+  long a1 = a0 + a.chunk;
+  if (a1 > a.n) a1 = a.n;
+
+  f4 acc[RT][SCT];
+#pragma unroll
+  for (int r = 0; r < RT; ++r)
+#pragma unroll
+    for (int s = 0; s < SCT; ++s) acc[r][s] = f4{0, 0, 0, 0};
+
+  const int lr = lane & 15, lk = lane >> 4;
+  int rnx[RT], rny[RT];
+  bool rvalid[RT];
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    const int row = (rt0 + r) * 16 + lr;
+    rvalid[r] = (rt0 + r) < rt_end && row < a.nX * a.nY;
+    rnx[r] = rvalid[r] ? row / a.nY : 0;
+    rny[r] = rvalid[r] ? row % a.nY : 0;
+  }
+  int zoff[SCT];  // byte offset of this lane's column inside an atom's split-Z block
+  bool zvalid[SCT];
+#pragma unroll
+  for (int s = 0; s < SCT; ++s) {
+    const int col = s * 16 + lr, zc = col >> 1;
+    zvalid[s] = zc < a.nZ;
+    zoff[s] = (zvalid[s] ? zc : 0) * 16 + (col & 1) * 8;
+  }
+  const int offY = a.nX;
+  const bool active = rt0 < rt_end;
+
+  auto issue = [&](long base, int b) {
+    unsigned char *dst = smem_raw + b * buf_bytes;
+    const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
+    const int rb = (int)(((size_t)cnt * a.TS * sizeof(Cx<float>) + 15) / 16 * 16);
+    copy_to_lds_async(tab + (size_t)base * a.TS * sizeof(Cx<float>), dst, rb, wave, nwaves, lane);
+    copy_to_lds_async(ztab + (size_t)base * a.nZ * 16, dst + rec_bytes, cnt * a.nZ * 16, wave,
+                      nwaves, lane);
+    float *sq = reinterpret_cast<float *>(dst + rec_bytes + z_bytes);
+    if (threadIdx.x < stage) sq[threadIdx.x] = threadIdx.x < cnt ? (float)a.q[base + threadIdx.x] * a.sa : 0.f;
+  };
+
+  int b = 0;
+  if (a0 < a1) issue(a0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (long base = a0; base < a1; base += stage) {
+    if (base + stage < a1) issue(base + stage, b ^ 1);
+    const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
+    const unsigned char *cur = smem_raw + b * buf_bytes;
+    const Cx<float> *recs = reinterpret_cast<const Cx<float> *>(cur);
+    const unsigned char *zs = cur + rec_bytes;
+    const float *sq = reinterpret_cast<const float *>(cur + rec_bytes + z_bytes);
+    if (active) {
+      for (int ks = 0; ks < stage / 16; ++ks) {
+        const int j0 = 16 * ks + 4 * lk;
+        u4v ahi[RT], alo[RT];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u;
+            const bool ok = j < cnt && rvalid[r];
+            const Cx<float> *rec = recs + (ok ? j : 0) * a.TS;
+            const Cx<float> X = rec[rnx[r]], Y = rec[offY + rny[r]];
+            const float qj = ok ? sq[j] : 0.f;
+            const float pr = X.re * Y.re - X.im * Y.im;
+            const float pi = X.re * Y.im + X.im * Y.re;
+            unsigned h, l;
+            split_pair(qj * pr, qj * pi, h, l);
+            ahi[r][u] = h;
+            alo[r][u] = l;
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < SCT; ++s) {
+          u4v bhi, blo;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u;
+            const bool ok = j < cnt && zvalid[s];
+            const uint2 w = *reinterpret_cast<const uint2 *>(zs + (size_t)(ok ? j : 0) * a.nZ * 16 + zoff[s]);
+            bhi[u] = ok ? w.x : 0u;
+            blo[u] = ok ? w.y : 0u;
+          }
+#pragma unroll
+          for (int r = 0; r < RT; ++r) {
+            acc[r][s] = mfma_h(ahi[r], bhi, acc[r][s]);
+            acc[r][s] = mfma_h(ahi[r], blo, acc[r][s]);
+            acc[r][s] = mfma_h(alo[r], bhi, acc[r][s]);
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    b ^= 1;
+  }
+  if (!active) return;
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    if (rt0 + r >= rt_end) break;
+#pragma unroll
+    for (int s = 0; s < SCT; ++s)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = (rt0 + r) * 16 + 4 * lk + e;
+        const int col = s * 16 + lr;
+        a.part[((long)chunk * a.Rpad + row) * a.SC + col] = (double)acc[r][s][e];
+      }
+  }
+}
+
+struct FArgsH {
+  int n, TS, nX, nY, nZ, ky, R, Rpad, nkh;
+  const void *tab, *ztab, *bmh;
+  const double *q, *minv;
+  const int *type;
+  double c0, kxs, kys, kzs;  // c0 includes 1/sB
+  double *ax, *ay, *az;
+  double *out4;
+};
+
+// Force pass of the split form: A = the atoms' Z (kappa = 2 nz + re/im, from
+// the split records), B = the operand blocks staged in LDS per row block and
+// shared by the workgroup's waves; the epilogue is the FP32 one.
+template <int NK, int NT>
+__global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
+{
+  const Cx<float> *tab = reinterpret_cast<const Cx<float> *>(a.tab);
+  const u4v *ztab = reinterpret_cast<const u4v *>(a.ztab);
+  const unsigned char *bm = reinterpret_cast<const unsigned char *>(a.bmh);
+  extern __shared__ unsigned char smem_raw[];
+  const int blk_bytes = 4 * a.nkh * 2 * 64 * 16;
+  const unsigned char *lb[2] = {smem_raw, smem_raw + blk_bytes};
+  const int nwaves = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const long base = ((long)blockIdx.x * nwaves + wave) * (16 * NT);
+  const int offY = a.nX;
+
+  // A fragments: atom l%16 of tile tau, kappa = 32 s + 8 (l/16) + i, i.e.
+  // nz = 16 s + 4 (l/16) + i/2, plain (Zr, Zi) = the (Zr, -Zi) record with the
+  // sign of its upper half flipped
+  u4v ahi[NT][NK], alo[NT][NK];
+#pragma unroll
+  for (int tau = 0; tau < NT; ++tau) {
+    const long atom = base + tau * 16 + lr;
+#pragma unroll
+    for (int s = 0; s < NK; ++s)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = 16 * s + 4 * lk + u;
+        unsigned h = 0, l = 0;
+        if (atom < a.n && c < a.nZ && s < a.nkh) {
+          const u4v rec = ztab[atom * a.nZ + c];
+          h = rec[0] ^ 0x80000000u;
+          l = rec[1] ^ 0x80000000u;
+        }
+        ahi[tau][s][u] = h;
+        alo[tau][s][u] = l;
+      }
+  }
+  float fx[NT][4], fy[NT][4], fz[NT][4], ph[NT][4];
+#pragma unroll
+  for (int tau = 0; tau < NT; ++tau)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) fx[tau][e] = fy[tau][e] = fz[tau][e] = ph[tau][e] = 0;
+
+  const int nrb = a.Rpad >> 4;
+  copy_to_lds_async(bm, const_cast<unsigned char *>(lb[0]), blk_bytes, wave, nwaves, lane);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int rb = 0; rb < nrb; ++rb) {
+    const int cur = rb & 1;
+    if (rb + 1 < nrb)
+      copy_to_lds_async(bm + (size_t)(rb + 1) * blk_bytes, const_cast<unsigned char *>(lb[cur ^ 1]),
+                        blk_bytes, wave, nwaves, lane);
+    f4 acc[4][NT];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int tau = 0; tau < NT; ++tau) acc[t][tau] = f4{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < NK; ++s) {
+      if (s >= a.nkh) break;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const unsigned char *u = lb[cur] + (size_t)((t * a.nkh + s) * 2) * 1024 + lane * 16;
+        const u4v bh = *reinterpret_cast<const u4v *>(u);
+        const u4v bl = *reinterpret_cast<const u4v *>(u + 1024);
+#pragma unroll
+        for (int tau = 0; tau < NT; ++tau) {
+          acc[t][tau] = mfma_h(ahi[tau][s], bh, acc[t][tau]);
+          acc[t][tau] = mfma_h(ahi[tau][s], bl, acc[t][tau]);
+          acc[t][tau] = mfma_h(alo[tau][s], bh, acc[t][tau]);
+        }
+      }
+    }
+    const int r = rb * 16 + lr;
+    if (r < a.R) {
+      const int nxi = r / a.nY, nyi = r - (r / a.nY) * a.nY;
+      const float wx = (float)nxi, wy = (float)(nyi - a.ky);
+#pragma unroll
+      for (int tau = 0; tau < NT; ++tau)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const long atom = base + tau * 16 + 4 * lk + e;
+          if (atom >= a.n) continue;
+          const Cx<float> X = tab[atom * a.TS + nxi], Y = tab[atom * a.TS + offY + nyi];
+          const float pr = X.re * Y.re - X.im * Y.im;
+          const float pi = X.re * Y.im + X.im * Y.re;
+          const float ur = acc[0][tau][e], ui = acc[1][tau][e];
+          const float uzr = acc[2][tau][e], uzi = acc[3][tau][e];
+          const float h = pr * ui + pi * ur;
+          fx[tau][e] += wx * h;
+          fy[tau][e] += wy * h;
+          fz[tau][e] += pr * uzi + pi * uzr;
+          ph[tau][e] += pr * ur - pi * ui;
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int tau = 0; tau < NT; ++tau)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double vx = fx[tau][e], vy = fy[tau][e], vz = fz[tau][e], vp = ph[tau][e];
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        vx += __shfl_xor(vx, o, 64);
+        vy += __shfl_xor(vy, o, 64);
+        vz += __shfl_xor(vz, o, 64);
+        vp += __shfl_xor(vp, o, 64);
+      }
+      const long atom = base + tau * 16 + 4 * lk + e;
+      if (lr == 0 && atom < a.n) {
+        const double gx = a.c0 * a.kxs * vx, gy = a.c0 * a.kys * vy, gz = a.c0 * a.kzs * vz;
+        if (a.out4) {
+          a.out4[4 * atom + 0] = gx;
+          a.out4[4 * atom + 1] = gy;
+          a.out4[4 * atom + 2] = gz;
+          a.out4[4 * atom + 3] = a.c0 * vp;
+        } else {
+          const double qm = a.q[atom] * a.minv[a.type[atom]];
+          a.ax[atom] += gx * qm;
+          a.ay[atom] += gy * qm;
+          a.az[atom] += gz * qm;
+        }
+      }
+    }
+}
+
 // ------------------------------------------------------------ dipole term --
 
 // calculate_dipole_sum (initialize.cpp:94-115): per-block FP64 partials of q r.
@@ -563,6 +902,21 @@ int ewald_k_plan(cav_ctx *c)
         const int r = nx * p.nY + (ny + p.ky);
         p.W[(size_t)r * p.nZ + (nz + p.kz)] = m * fk(nx, ny, nz);
       }
+  // split form: K-steps of 32 along kappa; operand scales from the bounds
+  // |q X Y| <= max|q| and |g| <= max W * sum_j |q_j|
+  c->nkh = (2 * p.nZ + 31) / 32;
+  {
+    double qmax = 0, qsum = 0, wmax = 0;
+    for (int i = 0; i < c->n; ++i) {
+      const double qa = std::fabs(c->charge[c->h_type[i]]);
+      qmax = std::max(qmax, qa);
+      qsum += qa;
+    }
+    for (double w : p.W) wmax = std::max(wmax, std::fabs(w));
+    c->sA = qmax > 0 ? std::ldexp(1.0, -(int)std::ceil(std::log2(qmax))) : 1.0;
+    const double gb = wmax * qsum;
+    c->sB = gb > 0 ? std::ldexp(1.0, 14 - (int)std::ceil(std::log2(gb))) : 1.0;
+  }
   // reference order of K for cav_get_sk: each k maps to a computed (row, col),
   // directly (nx >= 0) or through S(k) = conj S(-k)
   p.ref_r.clear();
@@ -642,6 +996,15 @@ int ewald_k_alloc(cav_ctx *c)
     k_zero_t<double><<<grid_for(bm_elems), kBlock, 0, c->stream>>>(bm_elems, (double *)c->bmat);
   else
     k_zero_t<float><<<grid_for(bm_elems), kBlock, 0, c->stream>>>(bm_elems, (float *)c->bmat);
+  if (c->ztab) (void)hipFree(c->ztab);
+  if (c->bmat_h) (void)hipFree(c->bmat_h);
+  c->ztab = c->bmat_h = nullptr;
+  if (c->precision == CAV_MIXED) {
+    CAV_HIP(hipMalloc(&c->ztab, (size_t)c->n * p.nZ * 16 + 4096));
+    const size_t bh_elems = (size_t)(p.Rpad / 16) * 4 * c->nkh * 2 * 64 * 8;
+    CAV_HIP(hipMalloc(&c->bmat_h, bh_elems * sizeof(_Float16)));
+    k_zero_t<_Float16><<<grid_for(bh_elems), kBlock, 0, c->stream>>>(bh_elems, (_Float16 *)c->bmat_h);
+  }
   if (c->d_W) (void)hipFree(c->d_W);
   CAV_HIP(hipMalloc(&c->d_W, sizeof(double) * (size_t)p.R * p.nZ));
   CAV_HIP(hipMemcpyAsync(c->d_W, p.W.data(), sizeof(double) * p.W.size(), hipMemcpyHostToDevice,
@@ -720,7 +1083,7 @@ static int sk_pipeline(cav_ctx *c)
   timer_begin(c, c->t_tab);
   k_tables<T><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
       c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
-      (Cx<T> *)c->tab);
+      (Cx<T> *)c->tab, nullptr);
   timer_end(c, c->t_tab);
   SArgs a;
   a.n = c->n;
@@ -749,7 +1112,8 @@ static int sk_pipeline(cav_ctx *c)
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
   k_sk_reduce<T><<<grid_for(nthr), kBlock, 0, c->stream>>>(
-      nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_part, c->d_W, c->sk, (T *)c->bmat, c->part);
+      nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_part, c->d_W, c->sk, (T *)c->bmat, c->part,
+      1.0, nullptr, 0, 1.0);
   CAV_HIP(hipGetLastError());
   return reduce_partials(c, grid_for(nthr), 3);
 }
@@ -789,6 +1153,110 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
   return CAV_OK;
 }
 
+template <int RT>
+static void launch_sk_h(cav_ctx *c, const SArgsH &a, dim3 grid, size_t smem, int SCT, int waves)
+{
+  const int block = 64 * waves;
+#define CAV_SKH(S)                                                      \
+  case S:                                                               \
+    allow_smem(k_sk_h<RT, S>, smem);                                    \
+    k_sk_h<RT, S><<<grid, block, smem, c->stream>>>(a);                 \
+    break;
+  switch (SCT) {
+    CAV_SKH(1) CAV_SKH(2) CAV_SKH(3) CAV_SKH(4) CAV_SKH(5) CAV_SKH(6) CAV_SKH(7)
+    default:
+      allow_smem(k_sk_h<RT, 8>, smem);
+      k_sk_h<RT, 8><<<grid, block, smem, c->stream>>>(a);
+  }
+#undef CAV_SKH
+}
+
+// Mixed form: FP32 tables + split Z records, split-FP16 S pass, FP64 chunk
+// reduction writing S, the FP32 operand (point queries, adjoint) and the split one.
+static int sk_pipeline_h(cav_ctx *c)
+{
+  const EwaldKPlan &p = c->ek;
+  const int TS = p.nX + p.nY + p.nZ;
+  timer_begin(c, c->t_tab);
+  k_tables<float><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
+      c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
+      (Cx<float> *)c->tab, (u4v *)c->ztab);
+  timer_end(c, c->t_tab);
+  SArgsH a;
+  a.n = c->n;
+  a.TS = TS;
+  a.nX = p.nX;
+  a.nY = p.nY;
+  a.nZ = p.nZ;
+  a.Rtiles = p.Rpad / 16;
+  a.chunk = (c->n + c->sk_chunks - 1) / c->sk_chunks;
+  a.chunk = (a.chunk + 31) / 32 * 32;
+  const size_t per_atom = (size_t)TS * sizeof(Cx<float>) + (size_t)p.nZ * 16 + sizeof(float);
+  a.stage = 2 * 32 * per_atom <= 150 * 1024 ? 32 : 16;
+  a.sa = (float)c->sA;
+  a.q = c->q;
+  a.tab = c->tab;
+  a.ztab = c->ztab;
+  a.part = c->sk_part;
+  a.Rpad = p.Rpad;
+  a.SC = roundup(2 * p.nZ, 16);
+  int groups, waves;
+  s_layout(c, &groups, &a.tiles_per_group, &waves);
+  const int nch = (c->n + a.chunk - 1) / a.chunk;
+  const size_t smem = 2 * (size_t)a.stage * per_atom;
+  timer_begin(c, c->t_sk);
+  launch_sk_h<kRT>(c, a, dim3(nch, groups), smem, a.SC / 16, waves);
+  timer_end(c, c->t_sk);
+  const int nthr = p.Rpad * p.nZ;
+  k_sk_reduce<float><<<grid_for(nthr), kBlock, 0, c->stream>>>(
+      nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_part, c->d_W, c->sk, (float *)c->bmat,
+      c->part, 1.0 / c->sA, (_Float16 *)c->bmat_h, c->nkh, c->sB);
+  CAV_HIP(hipGetLastError());
+  return reduce_partials(c, grid_for(nthr), 3);
+}
+
+static int kforce_h(cav_ctx *c)
+{
+  const EwaldKPlan &p = c->ek;
+  FArgsH f;
+  f.n = c->n;
+  f.TS = p.nX + p.nY + p.nZ;
+  f.nX = p.nX;
+  f.nY = p.nY;
+  f.nZ = p.nZ;
+  f.ky = p.ky;
+  f.R = p.R;
+  f.Rpad = p.Rpad;
+  f.nkh = c->nkh;
+  f.tab = c->tab;
+  f.ztab = c->ztab;
+  f.bmh = c->bmat_h;
+  f.q = c->q;
+  f.minv = c->d_minv;
+  f.type = c->type;
+  f.c0 = CAV_4PI * c->ek_ke * p.lxyz_inv / c->sB;
+  f.kxs = CAV_2PI / p.lx;
+  f.kys = CAV_2PI / p.ly;
+  f.kzs = CAV_2PI / p.lz;
+  f.ax = c->ax;
+  f.ay = c->ay;
+  f.az = c->az;
+  f.out4 = nullptr;
+  constexpr int NT = 2;
+  const int g = (c->n + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF);
+  const size_t smem = 2 * (size_t)4 * c->nkh * 2 * 64 * 16;
+  timer_begin(c, c->t_force);
+  switch (c->nkh) {
+  case 1: allow_smem(k_kforce_h<1, NT>, smem); k_kforce_h<1, NT><<<g, 64 * kWavesF, smem, c->stream>>>(f); break;
+  case 2: allow_smem(k_kforce_h<2, NT>, smem); k_kforce_h<2, NT><<<g, 64 * kWavesF, smem, c->stream>>>(f); break;
+  case 3: allow_smem(k_kforce_h<3, NT>, smem); k_kforce_h<3, NT><<<g, 64 * kWavesF, smem, c->stream>>>(f); break;
+  default: allow_smem(k_kforce_h<4, NT>, smem); k_kforce_h<4, NT><<<g, 64 * kWavesF, smem, c->stream>>>(f); break;
+  }
+  timer_end(c, c->t_force);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
 int launch_ewald_k(cav_ctx *c)
 {
   if (c->n == 0) return CAV_OK;
@@ -796,8 +1264,8 @@ int launch_ewald_k(cav_ctx *c)
     CAV_TRY(sk_pipeline<double>(c));
     CAV_TRY(kforce<double>(c, c->n, c->tab, nullptr));
   } else {
-    CAV_TRY(sk_pipeline<float>(c));
-    CAV_TRY(kforce<float>(c, c->n, c->tab, nullptr));
+    CAV_TRY(sk_pipeline_h(c));
+    CAV_TRY(kforce_h(c));
   }
   if (c->dipole) {
     const int nb = grid_for(c->n);
@@ -835,7 +1303,7 @@ static int weighted_sk_t(cav_ctx *c, const double *weights, double *out)
   const int TS = p.nX + p.nY + p.nZ;
   k_tables<T><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
       c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
-      (Cx<T> *)c->tab);
+      (Cx<T> *)c->tab, nullptr);
   SArgs a;
   a.n = c->n;
   a.TS = TS;
@@ -918,12 +1386,12 @@ int point_field_k(cav_ctx *c, int np, const double *pts, double *field, double *
   if (c->precision == CAV_FP64) {
     k_tables<double><<<grid_for((long)np * TS), kBlock, 0, c->stream>>>(
         np, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, d, d + np, d + 2 * np,
-        (Cx<double> *)tab);
+        (Cx<double> *)tab, nullptr);
     CAV_TRY(kforce<double>(c, np, tab, d + 3 * np));
   } else {
     k_tables<float><<<grid_for((long)np * TS), kBlock, 0, c->stream>>>(
         np, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, d, d + np, d + 2 * np,
-        (Cx<float> *)tab);
+        (Cx<float> *)tab, nullptr);
     CAV_TRY(kforce<float>(c, np, tab, d + 3 * np));
   }
   std::vector<double> o4(4 * (size_t)np);

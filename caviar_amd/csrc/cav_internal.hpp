@@ -121,7 +121,7 @@ struct cav_ctx {
   int *order = nullptr;    // owned atoms in cell order (compaction of bin_items)
   int *ord_blk = nullptr;  // scan block sums for the compaction
 
-  // list (column-major: nl[k*n + i])
+  // list (row-major: nl[i*nl_cap + k], nn[i] entries in row i)
   int *nl = nullptr, *nn = nullptr;
   int nl_cap = 0;
 
@@ -145,6 +145,11 @@ struct cav_ctx {
   int sk_chunks = 0;
   double *sk = nullptr;      // [Rpad][2nZ] FP64
   void *bmat = nullptr;      // [KZ][4*Rpad] (T) force-pass operand
+  // mixed form: split-FP16 Z records and force-pass operand (hi + lo halves)
+  void *ztab = nullptr;      // [n][nZ] x 16 B: (Zr,-Zi) hi, (Zr,-Zi) lo, (Zi,Zr) hi, (Zi,Zr) lo
+  void *bmat_h = nullptr;    // [Rpad/16][4][NKH][64] x 32 B: 8 hi halves, 8 lo halves
+  int nkh = 0;               // 32-deep K-steps along the force-pass reduction axis
+  double sA = 1, sB = 1;     // power-of-two operand scales of the split form
   double *d_W = nullptr;     // [Rpad][nZ]
   double *kacc = nullptr;    // per-atom k-space (fx, fy, fz, phi) scratch for point queries
 

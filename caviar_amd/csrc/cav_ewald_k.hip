@@ -501,6 +501,43 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
 // sB chosen from the bound |g| <= max W * sum |q|) so that no half overflows
 // and the scales are removed exactly afterwards.
 
+// Tables of the split form: per atom an FP32 record [X(0..kx) | Y(-ky..ky)]
+// (TXY entries) and the split Z records (nZ x 16 B).
+__global__ void k_tables_h(long npts, int kx, int ky, int kz, double ilx, double ily, double ilz,
+                           const double *x, const double *y, const double *z, Cx<float> *xy,
+                           u4v *ztab)
+{
+  const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= npts * TS) return;
+  const long a = t / TS;
+  const int e = (int)(t - a * TS);
+  double u;
+  int m;
+  if (e < nX) {
+    m = e;
+    u = x[a] * ilx;
+  } else if (e < nX + nY) {
+    m = e - nX - ky;
+    u = y[a] * ily;
+  } else {
+    m = e - nX - nY - kz;
+    u = z[a] * ilz;
+  }
+  double f = (double)m * u;
+  f -= rint(f);
+  float sn, cs;
+  sincospif((float)(2.0 * f), &sn, &cs);
+  if (e < nX + nY) {
+    xy[a * (nX + nY) + e] = Cx<float>{cs, sn};
+  } else {
+    unsigned h0, l0, h1, l1;
+    split_pair(cs, -sn, h0, l0);
+    split_pair(sn, cs, h1, l1);
+    ztab[a * nZ + (e - nX - nY)] = u4v{h0, l0, h1, l1};
+  }
+}
+
 __device__ __forceinline__ f4 mfma_h(u4v a, u4v b, f4 c)
 {
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
@@ -511,7 +548,7 @@ struct SArgsH {
   int n, TS, nX, nY, nZ, Rtiles, chunk, stage, tiles_per_group, Rpad, SC;
   float sa;
   const double *q;
-  const void *tab;   // FP32 records [X | Y | Z] per atom
+  const void *tab;   // FP32 records [X | Y] per atom (TS = nX + nY entries)
   const void *ztab;  // split Z records, nZ x 16 B per atom
   double *part;      // [chunk][Rpad][SC]
 };
@@ -649,7 +686,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
 }
 
 struct FArgsH {
-  int n, TS, nX, nY, nZ, ky, R, Rpad, nkh;
+  int n, TS, nX, nY, nZ, ky, R, Rpad, nkh;  // TS = nX + nY: FP32 [X | Y] record length
   const void *tab, *ztab, *bmh;
   const double *q, *minv;
   const int *type;
@@ -660,21 +697,31 @@ struct FArgsH {
 
 // Force pass of the split form: A = the atoms' Z (kappa = 2 nz + re/im, from
 // the split records), B = the operand blocks staged in LDS per row block and
-// shared by the workgroup's waves; the epilogue is the FP32 one.
+// shared by the workgroup's waves.  The workgroup's [X | Y] records are staged
+// in LDS once, so the epilogue's per-(atom, row) phase factors are LDS reads.
 template <int NK, int NT>
 __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
 {
-  const Cx<float> *tab = reinterpret_cast<const Cx<float> *>(a.tab);
+  const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
   const u4v *ztab = reinterpret_cast<const u4v *>(a.ztab);
   const unsigned char *bm = reinterpret_cast<const unsigned char *>(a.bmh);
   extern __shared__ unsigned char smem_raw[];
-  const int blk_bytes = 4 * a.nkh * 2 * 64 * 16;
-  const unsigned char *lb[2] = {smem_raw, smem_raw + blk_bytes};
   const int nwaves = blockDim.x >> 6;
+  const int blk_bytes = 4 * a.nkh * 2 * 64 * 16;
+  const int wg_atoms = 16 * NT * nwaves;
+  const int xy_bytes = wg_atoms * a.TS * (int)sizeof(Cx<float>);
+  const unsigned char *lb[2] = {smem_raw + xy_bytes, smem_raw + xy_bytes + blk_bytes};
+  const Cx<float> *lxy = reinterpret_cast<const Cx<float> *>(smem_raw);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const long base = ((long)blockIdx.x * nwaves + wave) * (16 * NT);
+  const long wg_base = (long)blockIdx.x * wg_atoms;
+  const long base = wg_base + (long)wave * (16 * NT);
   const int offY = a.nX;
+  {
+    const long cnt = min((long)wg_atoms, a.n - wg_base);
+    copy_to_lds_async(tab + (size_t)wg_base * a.TS * sizeof(Cx<float>), smem_raw,
+                      (int)(cnt * a.TS * sizeof(Cx<float>) + 15) / 16 * 16, wave, nwaves, lane);
+  }
 
   // A fragments: atom l%16 of tile tau, kappa = 32 s + 8 (l/16) + i, i.e.
   // nz = 16 s + 4 (l/16) + i/2, plain (Zr, Zi) = the (Zr, -Zi) record with the
@@ -744,7 +791,8 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
         for (int e = 0; e < 4; ++e) {
           const long atom = base + tau * 16 + 4 * lk + e;
           if (atom >= a.n) continue;
-          const Cx<float> X = tab[atom * a.TS + nxi], Y = tab[atom * a.TS + offY + nyi];
+          const Cx<float> *rec = lxy + (atom - wg_base) * a.TS;
+          const Cx<float> X = rec[nxi], Y = rec[offY + nyi];
           const float pr = X.re * Y.re - X.im * Y.im;
           const float pi = X.re * Y.im + X.im * Y.re;
           const float ur = acc[0][tau][e], ui = acc[1][tau][e];
@@ -1178,20 +1226,20 @@ static int sk_pipeline_h(cav_ctx *c)
   const EwaldKPlan &p = c->ek;
   const int TS = p.nX + p.nY + p.nZ;
   timer_begin(c, c->t_tab);
-  k_tables<float><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
-      c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
+  k_tables_h<<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
+      c->n, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
       (Cx<float> *)c->tab, (u4v *)c->ztab);
   timer_end(c, c->t_tab);
   SArgsH a;
   a.n = c->n;
-  a.TS = TS;
+  a.TS = p.nX + p.nY;
   a.nX = p.nX;
   a.nY = p.nY;
   a.nZ = p.nZ;
   a.Rtiles = p.Rpad / 16;
   a.chunk = (c->n + c->sk_chunks - 1) / c->sk_chunks;
   a.chunk = (a.chunk + 31) / 32 * 32;
-  const size_t per_atom = (size_t)TS * sizeof(Cx<float>) + (size_t)p.nZ * 16 + sizeof(float);
+  const size_t per_atom = (size_t)a.TS * sizeof(Cx<float>) + (size_t)p.nZ * 16 + sizeof(float);
   a.stage = 2 * 32 * per_atom <= 150 * 1024 ? 32 : 16;
   a.sa = (float)c->sA;
   a.q = c->q;
@@ -1215,12 +1263,19 @@ static int sk_pipeline_h(cav_ctx *c)
   return reduce_partials(c, grid_for(nthr), 3);
 }
 
+// LDS of the split force pass: the workgroup's [X | Y] records + two operand blocks
+static size_t split_force_lds(const cav_ctx *c)
+{
+  return 2 * (size_t)4 * c->nkh * 2 * 64 * 16 +
+         (size_t)16 * 2 * kWavesF * (c->ek.nX + c->ek.nY) * sizeof(Cx<float>);
+}
+
 static int kforce_h(cav_ctx *c)
 {
   const EwaldKPlan &p = c->ek;
   FArgsH f;
   f.n = c->n;
-  f.TS = p.nX + p.nY + p.nZ;
+  f.TS = p.nX + p.nY;
   f.nX = p.nX;
   f.nY = p.nY;
   f.nZ = p.nZ;
@@ -1244,7 +1299,7 @@ static int kforce_h(cav_ctx *c)
   f.out4 = nullptr;
   constexpr int NT = 2;
   const int g = (c->n + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF);
-  const size_t smem = 2 * (size_t)4 * c->nkh * 2 * 64 * 16;
+  const size_t smem = split_force_lds(c);
   timer_begin(c, c->t_force);
   switch (c->nkh) {
   case 1: allow_smem(k_kforce_h<1, NT>, smem); k_kforce_h<1, NT><<<g, 64 * kWavesF, smem, c->stream>>>(f); break;
@@ -1263,9 +1318,12 @@ int launch_ewald_k(cav_ctx *c)
   if (c->precision == CAV_FP64) {
     CAV_TRY(sk_pipeline<double>(c));
     CAV_TRY(kforce<double>(c, c->n, c->tab, nullptr));
-  } else {
+  } else if (split_force_lds(c) <= 160 * 1024) {
     CAV_TRY(sk_pipeline_h(c));
     CAV_TRY(kforce_h(c));
+  } else {  // k lattices too wide for the split form's LDS tiles: FP32 tiles
+    CAV_TRY(sk_pipeline<float>(c));
+    CAV_TRY(kforce<float>(c, c->n, c->tab, nullptr));
   }
   if (c->dipole) {
     const int nb = grid_for(c->n);
@@ -1343,6 +1401,13 @@ int ewald_k_weighted_sk(cav_ctx *c, const double *weights, double *out)
 int ewald_k_field(cav_ctx *c, double *out4)
 {
   if (c->precision == CAV_FP64) return kforce<double>(c, c->n, c->tab, out4);
+  // the split form leaves [X | Y] records in tab: rebuild the full FP32 ones
+  const EwaldKPlan &p = c->ek;
+  const int TS = p.nX + p.nY + p.nZ;
+  k_tables<float><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
+      c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
+      (Cx<float> *)c->tab, nullptr);
+  CAV_HIP(hipGetLastError());
   return kforce<float>(c, c->n, c->tab, out4);
 }
 

@@ -375,10 +375,13 @@ constexpr int kCand = 512;  // stencil candidates staged in LDS per pass
 // gathered per atom without atomics.  Rows are row-major: nl[i*cap + k].
 //
 // One wave per cell: the candidates of the cell's de-duplicated stencil are
-// staged in LDS (in stencil order, each cell in bin order), then each owned
-// atom of the cell tests 64 candidates per iteration; passing lanes append in
-// lane order through a ballot and a prefix popcount, so rows are written
-// coalesced and in a deterministic order.
+// staged in LDS (in stencil order, each cell in bin order), then the cell's
+// owned atoms are tested kAPI at a time against 64 candidates per iteration
+// (each staged candidate is read once for kAPI independent tests); passing
+// lanes append in lane order through a ballot and a prefix popcount, so rows
+// are written coalesced and in a deterministic order.
+constexpr int kAPI = 4;
+
 __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, double rce2,
                                                   const double *x, const double *y, const double *z,
                                                   const int *st_off, const int *st,
@@ -401,6 +404,7 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
   }
   if (no == 0) return;
   const int sb = st_off[b], se = st_off[b + 1];
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (int og = 0; og < no; og += 64) {
     const int gsize = (no - og) < 64 ? (no - og) : 64;
     scnt[lane] = 0;
@@ -428,27 +432,40 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
         }
       }
       __syncthreads();
-      for (int o = 0; o < gsize; ++o) {
-        const int i = items[s0 + og + o];
-        const double px = x[i], py = y[i], pz = z[i];
-        int c0 = scnt[o];
+      for (int o0 = 0; o0 < gsize; o0 += kAPI) {
+        int ia[kAPI], c0[kAPI];
+        double px[kAPI], py[kAPI], pz[kAPI];
+#pragma unroll
+        for (int u = 0; u < kAPI; ++u) {
+          const bool ok = o0 + u < gsize;
+          ia[u] = ok ? items[s0 + og + o0 + u] : -1;
+          const int ii = ok ? ia[u] : items[s0 + og];
+          px[u] = x[ii];
+          py[u] = y[ii];
+          pz[u] = z[ii];
+          c0[u] = ok ? scnt[o0 + u] : 0;
+        }
         for (int cc = 0; cc < m; cc += 64) {
           const int idx = cc + lane;
-          bool pass = false;
-          int j = 0;
-          if (idx < m) {
-            j = sj[idx];
-            const double dx = __dsub_rn(px, sx[idx]), dy = __dsub_rn(py, sy[idx]),
-                         dz = __dsub_rn(pz, sz[idx]);
-            pass = (j != i) && dot_rn(dx, dy, dz) < rce2;
+          const bool in = idx < m;
+          const int j = in ? sj[idx] : -1;
+          const double qx = in ? sx[idx] : 0.0, qy = in ? sy[idx] : 0.0, qz = in ? sz[idx] : 0.0;
+#pragma unroll
+          for (int u = 0; u < kAPI; ++u) {
+            const double dx = __dsub_rn(px[u], qx), dy = __dsub_rn(py[u], qy),
+                         dz = __dsub_rn(pz[u], qz);
+            const bool pass = in && ia[u] >= 0 && j != ia[u] && dot_rn(dx, dy, dz) < rce2;
+            const unsigned long long mask = __ballot(pass);
+            const int pre = __popcll(mask & below);
+            if (pass && c0[u] + pre < cap) nl[(long)ia[u] * cap + c0[u] + pre] = j;
+            c0[u] += __popcll(mask);
           }
-          const unsigned long long mask = __ballot(pass);
-          const int pre = __popcll(mask & ((1ull << lane) - 1ull));
-          if (pass && c0 + pre < cap) nl[(long)i * cap + c0 + pre] = j;
-          c0 += __popcll(mask);
         }
-        if (lane == 0) scnt[o] = c0;
-        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+          for (int u = 0; u < kAPI; ++u)
+            if (o0 + u < gsize) scnt[o0 + u] = c0[u];
+        }
       }
       __syncthreads();
     }

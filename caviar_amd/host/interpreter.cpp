@@ -113,9 +113,12 @@ void Interpreter::create(casl::Parser &p, const std::string &base)
     o->precision = precision_;
     objects.atom_data[nm] = std::move(o);
   } else if (base == "neighborlist") {
-    if (ty != "basic") p.fail(undefined);
+    // basic (neighborlist/macro/basic.h); cell_list and verlet_list name its two
+    // modes the way older scripts do: cells + Verlet list from cells, or brute force
+    if (ty != "basic" && ty != "cell_list" && ty != "verlet_list") p.fail(undefined);
     auto o = std::make_unique<Neighborlist>();
     o->objects = &objects;
+    if (ty == "cell_list") o->build_cell_list = o->make_verlet_list_from_cell_list = true;
     objects.neighborlist[nm] = std::move(o);
   } else if (base == "force_field") {
     std::unique_ptr<Force_field> o;

@@ -4,7 +4,8 @@
 // names compared case-insensitively) and the `int` / `real` variables
 // (object_creator/basic_types.cpp:209-260).
 //
-// Accepted creations: domain box, atom_data basic, neighborlist basic,
+// Accepted creations: domain box, atom_data basic, neighborlist basic (also as
+// cell_list = cells + Verlet list from cells, verlet_list = brute force),
 // force_field lj_cell_list | electrostatic_ewald_r | electrostatic_ewald_k,
 // md_simulator basic.  Anything else is reported as an error naming the
 // command, the way the reference reports an unknown command.

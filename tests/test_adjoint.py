@@ -137,3 +137,24 @@ def test_torch_autograd_wrapper(gpu):
         lo.backward()
         opt.step()
     assert float(fitting.ForceMatchingLoss.apply(th, fm)) < l0
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_mixed_form_close_to_fp64(gpu):
+    """The adjoint in the mixed form (FP32 pair and k-space products, FP64
+    sums) against the FP64 adjoint on the same frame: 1e-5 relative."""
+    from caviar_amd import Engine
+    base, frames = small_frames(n=400, nframes=1, kmax=5, seed=12)
+    theta = fitting.neutral_start(THETA_T, np.array([200, 200]), seed=9)
+    s = base.copy()
+    s.lj.epsilon, s.lj.sigma = fitting.mix(theta[0:2], theta[2:4])
+    s.charge = theta[4:6]
+    s.pos = frames[0]
+    w = np.random.default_rng(3).normal(size=(s.n, 3))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        e = Engine(s.copy(), precision=prec)
+        e.setup()
+        out[prec] = e.adjoint(w)
+    for g64, g32 in zip(out["fp64"], out["mixed"]):
+        assert np.linalg.norm(g32 - g64) <= 1e-5 * np.linalg.norm(g64)

@@ -182,6 +182,19 @@ def test_reader_reports_errors_like_the_reference(tmp_path, script, message):
     assert message in r.stderr, r.stderr
 
 
+@pytest.mark.parametrize("kind, cells", [("basic", False), ("cell_list", True), ("verlet_list", False)])
+def test_neighborlist_mode_aliases(tmp_path, kind, cells):
+    needs_driver()
+    s, ref, meta, script = write_case(tmp_path, "ions600_brute_setup")
+    text = open(script).read().replace("neighborlist basic neigh", f"neighborlist {kind} neigh")
+    (tmp_path / "alias.casl").write_text(text)
+    r = run_driver("--parse-only", str(tmp_path / "alias.casl"))
+    assert r.returncode == 0, r.stderr
+    nl = json.loads(r.stdout)["neighbor"]
+    assert nl["build_cells"] == (cells or s.neighbor.build_cells)
+    assert nl["from_cells"] == (cells or s.neighbor.from_cells)
+
+
 def test_missing_settings_fail_at_setup(tmp_path):
     """Ewald_k without alpha (no default in the reference) and a force field
     without its neighbour list are refused when the run sets up."""

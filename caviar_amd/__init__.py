@@ -92,6 +92,23 @@ def load_library():
         "cav_enable_timing": [P, I],
         "cav_adjoint": [P, dp, dp, dp, dp],
         "cav_exchange": [P, ip],
+        "cav_batch_create": [I, I, ctypes.POINTER(P)],
+        "cav_batch_destroy": [P],
+        "cav_batch_set_types": [P, I, dp, dp],
+        "cav_batch_set_neighbor": [P, D, D, D],
+        "cav_batch_set_lj": [P, D, dp, dp],
+        "cav_batch_set_ewald_r": [P, D, D],
+        "cav_batch_set_ewald_k": [P, D, D, I, I, I],
+        "cav_batch_add_replica": [P, dp, dp, I, ip, dp, dp, ip],
+        "cav_batch_num_replicas": [P, ip],
+        "cav_batch_num_atoms": [P, I, ip],
+        "cav_batch_setup": [P],
+        "cav_batch_step": [P, D, I, I],
+        "cav_batch_download": [P, I, I, dp],
+        "cav_batch_virial": [P, I, dp],
+        "cav_batch_get_pairs": [P, I, ip, L, ctypes.POINTER(L)],
+        "cav_batch_autotune": [P],
+        "cav_batch_tiles": [P, ip, ip, ip, ip],
         "cav_reset_acceleration": [P],
         "cav_reset_virial": [P],
         "cav_forces": [P, I],
@@ -102,6 +119,8 @@ def load_library():
         fn.restype = I
     lib.cav_last_error.argtypes = [P]
     lib.cav_last_error.restype = ctypes.c_char_p
+    lib.cav_batch_last_error.argtypes = [P]
+    lib.cav_batch_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
@@ -327,3 +346,111 @@ class Engine:
 
     def enable_timing(self, on: bool = True):
         self._check(self.lib.cav_enable_timing(self.h, int(on)))
+
+
+class Batch:
+    """Many independent systems of one force field stepped together on one GPU
+    (``cav_batch_*``): each replica has its own atoms and periodic box; every
+    kernel covers all replicas and one step of all of them is one graph launch.
+
+    The systems must share types, cutoffs and force-field settings (Verlet list
+    from cells, Lj_cell_list + Ewald_r + Ewald_k without lambda / dipole / slab).
+    """
+
+    def __init__(self, systems, precision: str = "fp64", device: int = 0):
+        self.lib = load_library()
+        if not systems:
+            raise ValueError("no systems")
+        s0 = systems[0]
+        for s in systems[1:]:
+            if (s.ntypes != s0.ntypes or not np.array_equal(s.mass, s0.mass)
+                    or not np.array_equal(s.charge, s0.charge) or s.ghost_cutoff != s0.ghost_cutoff
+                    or s.neighbor != s0.neighbor):
+                raise ValueError("batched systems must share types and neighbour settings")
+        h = ctypes.c_void_p()
+        rc = self.lib.cav_batch_create(int(device), CAV_FP64 if precision == "fp64" else CAV_MIXED,
+                                       ctypes.byref(h))
+        if rc != 0:
+            raise CaviarError(rc, "cav_batch_create failed (no usable HIP device?)")
+        self.h = h
+        self.systems = systems
+        self.precision = precision
+        self._check(self.lib.cav_batch_set_types(h, s0.ntypes, _dp(s0.mass.copy()), _dp(s0.charge.copy())))
+        nl = s0.neighbor
+        if nl is None or not (nl.from_cells and nl.build_cells) or nl.all_atom_test or nl.rebuild_test:
+            raise ValueError("batched replicas use the Verlet list built from cells")
+        self._check(self.lib.cav_batch_set_neighbor(h, float(s0.ghost_cutoff), float(nl.cutoff),
+                                                    float(nl.cutoff_extra_coef)))
+        nt = s0.ntypes
+        if s0.lj is not None:
+            e = np.ascontiguousarray(s0.lj.epsilon, dtype=np.float64).reshape(nt * nt)
+            g = np.ascontiguousarray(s0.lj.sigma, dtype=np.float64).reshape(nt * nt)
+            self._check(self.lib.cav_batch_set_lj(h, float(s0.lj.cutoff), _dp(e), _dp(g)))
+        if s0.ewald_r is not None:
+            if s0.ewald_r.lambda_ is not None:
+                raise ValueError("lambda is not supported for batched replicas")
+            self._check(self.lib.cav_batch_set_ewald_r(h, s0.ewald_r.alpha, s0.ewald_r.k_electrostatic))
+        if s0.ewald_k is not None:
+            k = s0.ewald_k
+            if k.epsilon_dipole is not None or k.slab_normal_axis is not None:
+                raise ValueError("dipole / slab Ewald_k is not supported for batched replicas")
+            self._check(self.lib.cav_batch_set_ewald_k(h, k.alpha, k.k_electrostatic, k.kx_max,
+                                                       k.ky_max, k.kz_max))
+        for s in systems:
+            if list(s.bc) != [1, 1, 1]:
+                raise ValueError("batched replicas are periodic in every direction")
+            typ = np.ascontiguousarray(s.types, dtype=np.int32)
+            pos = np.ascontiguousarray(s.pos, dtype=np.float64)
+            vel = np.ascontiguousarray(s.vel, dtype=np.float64)
+            self._check(self.lib.cav_batch_add_replica(h, _dp(s.lower.copy()), _dp(s.upper.copy()),
+                                                       s.n, _ip(typ), _dp(pos), _dp(vel), None))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.cav_batch_last_error(self.h).decode() if getattr(self, "h", None) else ""
+            if rc == 4:
+                raise NaNAccelerationError(rc, msg)
+            raise CaviarError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cav_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setup(self):
+        self._check(self.lib.cav_batch_setup(self.h))
+
+    def step(self, dt: float, nsteps: int = 1, graph: bool = True):
+        self._check(self.lib.cav_batch_step(self.h, float(dt), int(nsteps), int(bool(graph))))
+
+    def get(self, replica: int, what: str) -> np.ndarray:
+        out = np.empty((self.systems[replica].n, 3), dtype=np.float64)
+        self._check(self.lib.cav_batch_download(self.h, int(replica), _ARRAYS[what], _dp(out)))
+        return out
+
+    def virial(self, replica: int) -> float:
+        v = ctypes.c_double()
+        self._check(self.lib.cav_batch_virial(self.h, int(replica), ctypes.byref(v)))
+        return v.value
+
+    def pairs(self, replica: int) -> np.ndarray:
+        n = ctypes.c_long()
+        self._check(self.lib.cav_batch_get_pairs(self.h, int(replica), None, 0, ctypes.byref(n)))
+        out = np.empty((n.value, 2), dtype=np.int32)
+        self._check(self.lib.cav_batch_get_pairs(self.h, int(replica), _ip(out), n.value, ctypes.byref(n)))
+        return out
+
+    def autotune(self) -> dict:
+        self._check(self.lib.cav_batch_autotune(self.h))
+        return self.tiles()
+
+    def tiles(self) -> dict:
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(self.lib.cav_batch_tiles(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("s_rows", "s_stage", "s_chunk", "f_tiles"), (x.value for x in v)))

@@ -125,3 +125,42 @@ def b2_ewald(pos: np.ndarray, charge_sign: np.ndarray, kmax: int = 25) -> System
                   types=types, neighbor=nl, ewald_r=EwaldRSettings(alpha=alpha),
                   ewald_k=EwaldKSettings.cube(alpha, kmax),
                   meta={"workload": "b2_ewald", "n": n})
+
+
+def half_charged(n: int, rho: float, kmax: int = 6, seed: int = 1, alpha: float = 0.9,
+                 nl_cutoff: float = 3.5, rc_lj: float = 2.5, ke: float = 2.0, dmin: float = 0.8,
+                 temperature: float = 1.0, coef: float = 1.12347) -> System:
+    """One replica of the batched workload: half the atoms charged (+1 / -1 in
+    equal numbers, types 0 and 1), half neutral (type 2), LJ eps = sigma = 1
+    between every type pair, uniform random positions with minimum separation
+    ``dmin`` in a periodic cube at number density ``rho``."""
+    box = (n / rho) ** (1.0 / 3.0)
+    pos = random_nonoverlap(n, box, dmin, seed)
+    rng = np.random.default_rng(seed + 1000)
+    types = np.full(n, 2, dtype=np.int32)
+    nq = (n // 2) // 2
+    types[:nq] = 0
+    types[nq:2 * nq] = 1
+    types = types[rng.permutation(n)]
+    mass = np.array([1.0, 1.0, 1.0])
+    vel = maxwell_boltzmann(n, temperature, mass[types], rng)
+    rce = nl_cutoff * coef
+    return System(lower=[0, 0, 0], upper=[box] * 3, bc=[1, 1, 1], ghost_cutoff=rce + 0.1,
+                  mass=mass, charge=[1.0, -1.0, 0.0], pos=pos, vel=vel, types=types,
+                  neighbor=NeighborSettings(cutoff=nl_cutoff, cutoff_extra_coef=coef),
+                  lj=LJSettings(cutoff=rc_lj, epsilon=np.ones((3, 3)), sigma=np.ones((3, 3))),
+                  ewald_r=EwaldRSettings(alpha=alpha, k_electrostatic=ke),
+                  ewald_k=EwaldKSettings.cube(alpha, kmax, ke),
+                  meta={"workload": "half_charged", "n": n, "rho": rho, "kmax": kmax, "seed": seed,
+                        "dt": 0.002})
+
+
+def replica_set(count: int = 512, n_range=(200, 4000), rho_range=(0.2, 0.8), kmax: int = 6,
+                seed: int = 1) -> list:
+    """The batched-replica workload: ``count`` half-charged systems with N and
+    density drawn uniformly from the ranges (seeded)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
+    rhos = rng.uniform(rho_range[0], rho_range[1], size=count)
+    return [half_charged(int(n), float(r), kmax=kmax, seed=seed * 100003 + i)
+            for i, (n, r) in enumerate(zip(ns, rhos))]

@@ -18,6 +18,7 @@
 // v_mfma_f32_16x16x4f32 (mixed form).  The S pass splits the atom axis into
 // chunks; chunk partials are summed in FP64 in a fixed order, so the plain
 // form is reproducible run to run.
+#include <algorithm>
 #include <cmath>
 
 #include "cav_internal.hpp"
@@ -132,6 +133,7 @@ struct SArgs {
   const void *tab;
   double *part;  // [chunk][Rpad][SC]
   int Rpad, SC;
+  const int4 *items;  // batched: per workgroup column (first atom, end atom, chunk slot, replica)
 };
 
 // Issue the asynchronous copy of `bytes` contiguous bytes (a multiple of 16,
@@ -168,12 +170,18 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  const int chunk = blockIdx.x;
+  int chunk = blockIdx.x;
   const int rt0 = blockIdx.y * a.tiles_per_group + wave * RT;
   const int rt_end = min(a.Rtiles, (int)(blockIdx.y + 1) * a.tiles_per_group);
-  const long a0 = (long)chunk * a.chunk;
+  long a0 = (long)chunk * a.chunk;
   long a1 = a0 + a.chunk;
   if (a1 > a.n) a1 = a.n;
+  if (a.items) {
+    const int4 it = a.items[blockIdx.x];
+    a0 = it.x;
+    a1 = it.y;
+    chunk = it.z;
+  }
 
   V acc[RT][SCT];
 #pragma unroll
@@ -280,8 +288,19 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
 template <typename T>
 __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz, int KSP,
                             const double *part, const double *W, double *sk, T *bm,
-                            double *epart, double inv_sa, _Float16 *bmh, int nkh, double sb)
+                            double *epart, double inv_sa, _Float16 *bmh, int nkh, double sb,
+                            const int2 *rchunks, long bm_stride, long bmh_stride)
 {
+  if (rchunks) {  // batched: grid row b reduces replica b's chunks into its own S and operands
+    const int b = blockIdx.y;
+    part += (long)rchunks[b].x * Rpad * SC;
+    nchunks = rchunks[b].y;
+    W += (long)b * R * nZ;
+    sk += (long)b * Rpad * 2 * nZ;
+    bm += b * bm_stride;
+    if (bmh) bmh += b * bmh_stride;
+    epart += (long)b * gridDim.x;
+  }
   const int t = blockIdx.x * kBlock + threadIdx.x;
   double e = 0;
   if (t < Rpad * nZ) {
@@ -352,6 +371,11 @@ struct FArgs {
   double c0, kxs, kys, kzs;  // C = 4 pi ke / V; 2 pi / l per axis
   double *ax, *ay, *az;
   double *out4;  // optional per-atom (fx, fy, fz, phi) without charge/mass
+  // batched: per workgroup (first atom, end atom, -, replica), the replicas'
+  // constants, and the operand stride (bytes) between replicas
+  const int4 *items;
+  const RepDev *rep;
+  long bm_stride;
 };
 
 template <typename T, int KS, int NT>
@@ -366,7 +390,18 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
   const int nwaves = blockDim.x >> 6;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const long base = ((long)blockIdx.x * nwaves + wave) * (16 * NT);
+  long base = ((long)blockIdx.x * nwaves + wave) * (16 * NT);
+  if (a.items) {
+    const int4 it = a.items[blockIdx.x];
+    const RepDev &r = a.rep[it.w];
+    base = it.x + (long)wave * (16 * NT);
+    a.n = it.y;
+    bm += it.w * a.bm_stride;
+    a.c0 = r.c0;
+    a.kxs = r.kxs;
+    a.kys = r.kys;
+    a.kzs = r.kzs;
+  }
   const int offY = a.nX, offZ = a.nX + a.nY;
 
   // A fragments: A[atom = l%16][kappa = 4s + l/16] = (Zr, Zi)(c = kappa/2)
@@ -551,6 +586,7 @@ struct SArgsH {
   const void *tab;   // FP32 records [X | Y] per atom (TS = nX + nY entries)
   const void *ztab;  // split Z records, nZ x 16 B per atom
   double *part;      // [chunk][Rpad][SC]
+  const int4 *items; // batched: as SArgs::items
 };
 
 // Same work split as k_sk: a workgroup is an atom chunk x a group of row
@@ -569,12 +605,18 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   const unsigned char *ztab = reinterpret_cast<const unsigned char *>(a.ztab);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  const int chunk = blockIdx.x;
+  int chunk = blockIdx.x;
   const int rt0 = blockIdx.y * a.tiles_per_group + wave * RT;
   const int rt_end = min(a.Rtiles, (int)(blockIdx.y + 1) * a.tiles_per_group);
-  const long a0 = (long)chunk * a.chunk;
+  long a0 = (long)chunk * a.chunk;
   long a1 = a0 + a.chunk;
   if (a1 > a.n) a1 = a.n;
+  if (a.items) {
+    const int4 it = a.items[blockIdx.x];
+    a0 = it.x;
+    a1 = it.y;
+    chunk = it.z;
+  }
 
   f4 acc[RT][SCT];
 #pragma unroll
@@ -693,6 +735,10 @@ struct FArgsH {
   double c0, kxs, kys, kzs;  // c0 includes 1/sB
   double *ax, *ay, *az;
   double *out4;
+  const int4 *items;  // batched, as FArgs
+  const RepDev *rep;
+  long bm_stride;
+  double inv_sb;      // batched: c0 of a replica is multiplied by it
 };
 
 // Force pass of the split form: A = the atoms' Z (kappa = 2 nz + re/im, from
@@ -714,7 +760,18 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
   const Cx<float> *lxy = reinterpret_cast<const Cx<float> *>(smem_raw);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const long wg_base = (long)blockIdx.x * wg_atoms;
+  long wg_base = (long)blockIdx.x * wg_atoms;
+  if (a.items) {
+    const int4 it = a.items[blockIdx.x];
+    const RepDev &r = a.rep[it.w];
+    wg_base = it.x;
+    a.n = it.y;
+    bm += it.w * a.bm_stride;
+    a.c0 = r.c0 * a.inv_sb;
+    a.kxs = r.kxs;
+    a.kys = r.kys;
+    a.kzs = r.kzs;
+  }
   const long base = wg_base + (long)wave * (16 * NT);
   const int offY = a.nX;
   {
@@ -888,13 +945,15 @@ __global__ void k_dipole_apply(int n, int nblk, const double *part, double dip_c
 
 static int roundup(int v, int m) { return (v + m - 1) / m * m; }
 
-int ewald_k_plan(cav_ctx *c)
+// The k lattice of one box: sizes, W = m(k) f(k) over the half lattice and
+// the map from the reference's k order.  Returns an error message or nullptr.
+const char *plan_k_lattice(EwaldKPlan &p, int kx, int ky, int kz, int precision, double alpha,
+                           double lx, double ly, double lz)
 {
-  EwaldKPlan &p = c->ek;
-  p.kx = c->kx;
-  p.ky = c->ky;
-  p.kz = c->kz;
-  if (p.kx < 0 || p.ky < 0 || p.kz < 0) return fail(c, CAV_ERR_ARG, "negative k_max");
+  p.kx = kx;
+  p.ky = ky;
+  p.kz = kz;
+  if (p.kx < 0 || p.ky < 0 || p.kz < 0) return "negative k_max";
   p.nX = p.kx + 1;
   p.nY = 2 * p.ky + 1;
   p.nZ = 2 * p.kz + 1;
@@ -904,27 +963,22 @@ int ewald_k_plan(cav_ctx *c)
   // fragment stride: whole 16-byte vectors, an odd number of them per lane so
   // that the lanes' LDS reads of the force-pass operand hit distinct banks
   {
-    const int vw = c->precision == CAV_FP64 ? 2 : 4;
+    const int vw = precision == CAV_FP64 ? 2 : 4;
     int ksp = roundup(p.KZ / 4, vw);
     if ((ksp / vw) % 2 == 0) ksp += vw;
     p.KSP = ksp;
   }
-  if (p.KZ / 4 > 32) return fail(c, CAV_ERR_ARG, "kz_max above 31 is not supported");
-  if (roundup(2 * p.nZ, 16) / 16 > 8) return fail(c, CAV_ERR_ARG, "kz_max above 31 is not supported");
+  if (p.KZ / 4 > 32) return "kz_max above 31 is not supported";
+  if (roundup(2 * p.nZ, 16) / 16 > 8) return "kz_max above 31 is not supported";
   p.nk_ref = (2 * p.kx) * (2 * p.ky) * (2 * p.kz);
-  // initialize.cpp:42-77: lengths from size_global, slab override of one axis
-  p.lx = c->L[0];
-  p.ly = c->L[1];
-  p.lz = c->L[2];
-  if (c->slab_axis == 0) p.lx = c->slab_lz;
-  if (c->slab_axis == 1) p.ly = c->slab_lz;
-  if (c->slab_axis == 2) p.lz = c->slab_lz;
+  p.lx = lx;
+  p.ly = ly;
+  p.lz = lz;
   const double lxi = 1.0 / p.lx, lyi = 1.0 / p.ly, lzi = 1.0 / p.lz;
   p.lxyz_inv = 1.0 / (p.lx * p.ly * p.lz);
-  if (c->dipole) p.dip_coef = c->ek_ke * CAV_4PI * p.lxyz_inv / (1 + 2.0 * c->eps_dipole);
   // make_k_vectors (initialize.cpp:184-222): f(k) as the reference computes it;
   // the index of (kx,ky,kz) in the lexicographic list without 0 decides membership.
-  const double a2 = c->ek_alpha * c->ek_alpha, f4a2 = 1.0 / (4.0 * a2);
+  const double a2 = alpha * alpha, f4a2 = 1.0 / (4.0 * a2);
   const long zero_idx = ((long)p.kx * p.nY + p.ky) * p.nZ + p.kz;
   auto ref_index = [&](int nx, int ny, int nz) -> long {
     const long g = ((long)(nx + p.kx) * p.nY + (ny + p.ky)) * p.nZ + (nz + p.kz);
@@ -950,21 +1004,6 @@ int ewald_k_plan(cav_ctx *c)
         const int r = nx * p.nY + (ny + p.ky);
         p.W[(size_t)r * p.nZ + (nz + p.kz)] = m * fk(nx, ny, nz);
       }
-  // split form: K-steps of 32 along kappa; operand scales from the bounds
-  // |q X Y| <= max|q| and |g| <= max W * sum_j |q_j|
-  c->nkh = (2 * p.nZ + 31) / 32;
-  {
-    double qmax = 0, qsum = 0, wmax = 0;
-    for (int i = 0; i < c->n; ++i) {
-      const double qa = std::fabs(c->charge[c->h_type[i]]);
-      qmax = std::max(qmax, qa);
-      qsum += qa;
-    }
-    for (double w : p.W) wmax = std::max(wmax, std::fabs(w));
-    c->sA = qmax > 0 ? std::ldexp(1.0, -(int)std::ceil(std::log2(qmax))) : 1.0;
-    const double gb = wmax * qsum;
-    c->sB = gb > 0 ? std::ldexp(1.0, 14 - (int)std::ceil(std::log2(gb))) : 1.0;
-  }
   // reference order of K for cav_get_sk: each k maps to a computed (row, col),
   // directly (nx >= 0) or through S(k) = conj S(-k)
   p.ref_r.clear();
@@ -980,6 +1019,37 @@ int ewald_k_plan(cav_ctx *c)
         p.ref_c.push_back(az_ + p.kz);
         p.ref_conj.push_back(direct ? 0 : 1);
       }
+  return nullptr;
+}
+
+// Power-of-two operand scales of the split form from the bounds
+// |q X Y| <= max|q| and |g| <= max W * sum_j |q_j|.
+void split_scales(double qmax, double qsum, double wmax, double *sA, double *sB)
+{
+  *sA = qmax > 0 ? std::ldexp(1.0, -(int)std::ceil(std::log2(qmax))) : 1.0;
+  const double gb = wmax * qsum;
+  *sB = gb > 0 ? std::ldexp(1.0, 14 - (int)std::ceil(std::log2(gb))) : 1.0;
+}
+
+int ewald_k_plan(cav_ctx *c)
+{
+  EwaldKPlan &p = c->ek;
+  // initialize.cpp:42-77: lengths from size_global, slab override of one axis
+  double l[3] = {c->L[0], c->L[1], c->L[2]};
+  if (c->slab_axis >= 0 && c->slab_axis < 3) l[c->slab_axis] = c->slab_lz;
+  if (const char *e = plan_k_lattice(p, c->kx, c->ky, c->kz, c->precision, c->ek_alpha, l[0], l[1], l[2]))
+    return fail(c, CAV_ERR_ARG, e);
+  if (c->dipole) p.dip_coef = c->ek_ke * CAV_4PI * p.lxyz_inv / (1 + 2.0 * c->eps_dipole);
+  // split form: K-steps of 32 along kappa
+  c->nkh = (2 * p.nZ + 31) / 32;
+  double qmax = 0, qsum = 0, wmax = 0;
+  for (int i = 0; i < c->n; ++i) {
+    const double qa = std::fabs(c->charge[c->h_type[i]]);
+    qmax = std::max(qmax, qa);
+    qsum += qa;
+  }
+  for (double w : p.W) wmax = std::max(wmax, std::fabs(w));
+  split_scales(qmax, qsum, wmax, &c->sA, &c->sB);
   return CAV_OK;
 }
 
@@ -1070,7 +1140,7 @@ static void allow_smem(K kern, size_t smem)
 }
 
 template <typename T, int RT>
-static void launch_sk_rt(cav_ctx *c, const SArgs &a, dim3 grid, size_t smem, int SCT, int waves)
+static void launch_sk_rt(hipStream_t st, const SArgs &a, dim3 grid, size_t smem, int SCT, int waves)
 {
   const int block = 64 * waves;
   allow_smem(k_sk<T, RT, 1>, smem);
@@ -1082,14 +1152,14 @@ static void launch_sk_rt(cav_ctx *c, const SArgs &a, dim3 grid, size_t smem, int
   allow_smem(k_sk<T, RT, 7>, smem);
   allow_smem(k_sk<T, RT, 8>, smem);
   switch (SCT) {
-  case 1: k_sk<T, RT, 1><<<grid, block, smem, c->stream>>>(a); break;
-  case 2: k_sk<T, RT, 2><<<grid, block, smem, c->stream>>>(a); break;
-  case 3: k_sk<T, RT, 3><<<grid, block, smem, c->stream>>>(a); break;
-  case 4: k_sk<T, RT, 4><<<grid, block, smem, c->stream>>>(a); break;
-  case 5: k_sk<T, RT, 5><<<grid, block, smem, c->stream>>>(a); break;
-  case 6: k_sk<T, RT, 6><<<grid, block, smem, c->stream>>>(a); break;
-  case 7: k_sk<T, RT, 7><<<grid, block, smem, c->stream>>>(a); break;
-  default: k_sk<T, RT, 8><<<grid, block, smem, c->stream>>>(a); break;
+  case 1: k_sk<T, RT, 1><<<grid, block, smem, st>>>(a); break;
+  case 2: k_sk<T, RT, 2><<<grid, block, smem, st>>>(a); break;
+  case 3: k_sk<T, RT, 3><<<grid, block, smem, st>>>(a); break;
+  case 4: k_sk<T, RT, 4><<<grid, block, smem, st>>>(a); break;
+  case 5: k_sk<T, RT, 5><<<grid, block, smem, st>>>(a); break;
+  case 6: k_sk<T, RT, 6><<<grid, block, smem, st>>>(a); break;
+  case 7: k_sk<T, RT, 7><<<grid, block, smem, st>>>(a); break;
+  default: k_sk<T, RT, 8><<<grid, block, smem, st>>>(a); break;
   }
 }
 
@@ -1098,26 +1168,25 @@ static void launch_sk_rt(cav_ctx *c, const SArgs &a, dim3 grid, size_t smem, int
 template <typename T> constexpr int kforce_nt() { return sizeof(T) == 8 ? 1 : 2; }
 
 template <typename T>
-static void launch_kforce_t(cav_ctx *c, const FArgs &a, int KS, int npts)
+static void launch_kforce_t(hipStream_t st, const FArgs &a, int KS, int g)
 {
   constexpr int NT = kforce_nt<T>();
-  const int g = (npts + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF);
   const size_t smem = 2 * (size_t)4 * 64 * a.KSP * sizeof(T);
   if (KS <= 8) {
     allow_smem(k_kforce<T, 8, NT>, smem);
-    k_kforce<T, 8, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+    k_kforce<T, 8, NT><<<g, 64 * kWavesF, smem, st>>>(a);
   } else if (KS <= 12) {
     allow_smem(k_kforce<T, 12, NT>, smem);
-    k_kforce<T, 12, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+    k_kforce<T, 12, NT><<<g, 64 * kWavesF, smem, st>>>(a);
   } else if (KS <= 17) {
     allow_smem(k_kforce<T, 17, NT>, smem);
-    k_kforce<T, 17, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+    k_kforce<T, 17, NT><<<g, 64 * kWavesF, smem, st>>>(a);
   } else if (KS <= 24) {
     allow_smem(k_kforce<T, 24, NT>, smem);
-    k_kforce<T, 24, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+    k_kforce<T, 24, NT><<<g, 64 * kWavesF, smem, st>>>(a);
   } else {
     allow_smem(k_kforce<T, 32, NT>, smem);
-    k_kforce<T, 32, NT><<<g, 64 * kWavesF, smem, c->stream>>>(a);
+    k_kforce<T, 32, NT><<<g, 64 * kWavesF, smem, st>>>(a);
   }
 }
 
@@ -1148,6 +1217,7 @@ static int sk_pipeline(cav_ctx *c)
   a.q = c->q;
   a.tab = c->tab;
   a.part = c->sk_part;
+  a.items = nullptr;
   a.Rpad = p.Rpad;
   a.SC = roundup(2 * p.nZ, 16);
   int groups, waves;
@@ -1156,12 +1226,12 @@ static int sk_pipeline(cav_ctx *c)
   dim3 grid(nch, groups);
   const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
   timer_begin(c, c->t_sk);
-  launch_sk_rt<T, kRT>(c, a, grid, smem, a.SC / 16, waves);
+  launch_sk_rt<T, kRT>(c->stream, a, grid, smem, a.SC / 16, waves);
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
   k_sk_reduce<T><<<grid_for(nthr), kBlock, 0, c->stream>>>(
       nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_part, c->d_W, c->sk, (T *)c->bmat, c->part,
-      1.0, nullptr, 0, 1.0);
+      1.0, nullptr, 0, 1.0, nullptr, 0, 0);
   CAV_HIP(hipGetLastError());
   return reduce_partials(c, grid_for(nthr), 3);
 }
@@ -1194,27 +1264,31 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
   f.ay = c->ay;
   f.az = c->az;
   f.out4 = out4;
+  f.items = nullptr;
+  f.rep = nullptr;
+  f.bm_stride = 0;
   timer_begin(c, c->t_force);
-  launch_kforce_t<T>(c, f, p.KZ / 4, npts);
+  constexpr int NT = kforce_nt<T>();
+  launch_kforce_t<T>(c->stream, f, p.KZ / 4, (npts + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF));
   timer_end(c, c->t_force);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }
 
 template <int RT>
-static void launch_sk_h(cav_ctx *c, const SArgsH &a, dim3 grid, size_t smem, int SCT, int waves)
+static void launch_sk_h(hipStream_t st, const SArgsH &a, dim3 grid, size_t smem, int SCT, int waves)
 {
   const int block = 64 * waves;
 #define CAV_SKH(S)                                                      \
   case S:                                                               \
     allow_smem(k_sk_h<RT, S>, smem);                                    \
-    k_sk_h<RT, S><<<grid, block, smem, c->stream>>>(a);                 \
+    k_sk_h<RT, S><<<grid, block, smem, st>>>(a);                 \
     break;
   switch (SCT) {
     CAV_SKH(1) CAV_SKH(2) CAV_SKH(3) CAV_SKH(4) CAV_SKH(5) CAV_SKH(6) CAV_SKH(7)
     default:
       allow_smem(k_sk_h<RT, 8>, smem);
-      k_sk_h<RT, 8><<<grid, block, smem, c->stream>>>(a);
+      k_sk_h<RT, 8><<<grid, block, smem, st>>>(a);
   }
 #undef CAV_SKH
 }
@@ -1246,6 +1320,7 @@ static int sk_pipeline_h(cav_ctx *c)
   a.tab = c->tab;
   a.ztab = c->ztab;
   a.part = c->sk_part;
+  a.items = nullptr;
   a.Rpad = p.Rpad;
   a.SC = roundup(2 * p.nZ, 16);
   int groups, waves;
@@ -1253,22 +1328,23 @@ static int sk_pipeline_h(cav_ctx *c)
   const int nch = (c->n + a.chunk - 1) / a.chunk;
   const size_t smem = 2 * (size_t)a.stage * per_atom;
   timer_begin(c, c->t_sk);
-  launch_sk_h<kRT>(c, a, dim3(nch, groups), smem, a.SC / 16, waves);
+  launch_sk_h<kRT>(c->stream, a, dim3(nch, groups), smem, a.SC / 16, waves);
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
   k_sk_reduce<float><<<grid_for(nthr), kBlock, 0, c->stream>>>(
       nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_part, c->d_W, c->sk, (float *)c->bmat,
-      c->part, 1.0 / c->sA, (_Float16 *)c->bmat_h, c->nkh, c->sB);
+      c->part, 1.0 / c->sA, (_Float16 *)c->bmat_h, c->nkh, c->sB, nullptr, 0, 0);
   CAV_HIP(hipGetLastError());
   return reduce_partials(c, grid_for(nthr), 3);
 }
 
 // LDS of the split force pass: the workgroup's [X | Y] records + two operand blocks
-static size_t split_force_lds(const cav_ctx *c)
+static size_t split_force_lds_p(const EwaldKPlan &p, int nkh)
 {
-  return 2 * (size_t)4 * c->nkh * 2 * 64 * 16 +
-         (size_t)16 * 2 * kWavesF * (c->ek.nX + c->ek.nY) * sizeof(Cx<float>);
+  return 2 * (size_t)4 * nkh * 2 * 64 * 16 + (size_t)16 * 2 * kWavesF * (p.nX + p.nY) * sizeof(Cx<float>);
 }
+
+static size_t split_force_lds(const cav_ctx *c) { return split_force_lds_p(c->ek, c->nkh); }
 
 static int kforce_h(cav_ctx *c)
 {
@@ -1297,6 +1373,10 @@ static int kforce_h(cav_ctx *c)
   f.ay = c->ay;
   f.az = c->az;
   f.out4 = nullptr;
+  f.items = nullptr;
+  f.rep = nullptr;
+  f.bm_stride = 0;
+  f.inv_sb = 1.0;
   constexpr int NT = 2;
   const int g = (c->n + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF);
   const size_t smem = split_force_lds(c);
@@ -1377,13 +1457,14 @@ static int weighted_sk_t(cav_ctx *c, const double *weights, double *out)
   a.q = weights;
   a.tab = c->tab;
   a.part = c->sk_part;
+  a.items = nullptr;
   a.Rpad = p.Rpad;
   a.SC = roundup(2 * p.nZ, 16);
   int groups, waves;
   s_layout(c, &groups, &a.tiles_per_group, &waves);
   const int nch = (c->n + a.chunk - 1) / a.chunk;
   const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
-  launch_sk_rt<T, kRT>(c, a, dim3(nch, groups), smem, a.SC / 16, waves);
+  launch_sk_rt<T, kRT>(c->stream, a, dim3(nch, groups), smem, a.SC / 16, waves);
   k_sk_sum<<<grid_for((long)p.Rpad * p.nZ), kBlock, 0, c->stream>>>(nch, p.Rpad, a.SC, p.nZ,
                                                                     c->sk_part, out);
   CAV_HIP(hipGetLastError());
@@ -1477,6 +1558,377 @@ int point_field_k(cav_ctx *c, int np, const double *pts, double *field, double *
   CAV_HIP(hipFree(d));
   CAV_HIP(hipFree(tab));
   return CAV_OK;
+}
+
+
+// ---------------------------------------------------------- batched replicas --
+//
+// All replicas share the k lattice (kx, ky, kz) but not the box: W, the phase
+// table lengths and the force constants are per replica.  The S pass runs over
+// a table of (replica, atom chunk) work items, the reduction over a grid row
+// per replica, and the force pass over (replica, atom group) items, so one
+// launch of each kernel covers every replica.
+
+// Phase tables of every replica: grid row = replica.  Full FP64/FP32 records
+// (plain form) or FP32 [X | Y] + split Z records (mixed form).
+template <typename T>
+__global__ void kb_tables(const RepDev *rep, int kx, int ky, int kz, const double *x,
+                          const double *y, const double *z, Cx<T> *tab)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long)r.n * TS) return;
+  const long a = t / TS;
+  const int e = (int)(t - a * TS);
+  const long s = r.base + a;
+  double u;
+  int m;
+  if (e < nX) {
+    m = e;
+    u = x[s] * r.il[0];
+  } else if (e < nX + nY) {
+    m = e - nX - ky;
+    u = y[s] * r.il[1];
+  } else {
+    m = e - nX - nY - kz;
+    u = z[s] * r.il[2];
+  }
+  double f = (double)m * u;
+  f -= rint(f);
+  Cx<T> v;
+  if constexpr (sizeof(T) == 8) {
+    double sn, cs;
+    sincospi(2.0 * f, &sn, &cs);
+    v.re = cs;
+    v.im = sn;
+  } else {
+    float sn, cs;
+    sincospif((float)(2.0 * f), &sn, &cs);
+    v.re = cs;
+    v.im = sn;
+  }
+  tab[(r.obase + a) * TS + e] = v;
+}
+
+__global__ void kb_tables_h(const RepDev *rep, int kx, int ky, int kz, const double *x,
+                            const double *y, const double *z, Cx<float> *xy, u4v *ztab)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long)r.n * TS) return;
+  const long a = t / TS;
+  const int e = (int)(t - a * TS);
+  const long s = r.base + a, o = r.obase + a;
+  double u;
+  int m;
+  if (e < nX) {
+    m = e;
+    u = x[s] * r.il[0];
+  } else if (e < nX + nY) {
+    m = e - nX - ky;
+    u = y[s] * r.il[1];
+  } else {
+    m = e - nX - nY - kz;
+    u = z[s] * r.il[2];
+  }
+  double f = (double)m * u;
+  f -= rint(f);
+  float sn, cs;
+  sincospif((float)(2.0 * f), &sn, &cs);
+  if (e < nX + nY) {
+    xy[o * (nX + nY) + e] = Cx<float>{cs, sn};
+  } else {
+    unsigned h0, l0, h1, l1;
+    split_pair(cs, -sn, h0, l0);
+    split_pair(sn, cs, h1, l1);
+    ztab[o * nZ + (e - nX - nY)] = u4v{h0, l0, h1, l1};
+  }
+}
+
+static void s_layout_rt(int Rpad, int RT, int *groups, int *tiles_per_group, int *waves)
+{
+  const int tiles = Rpad / 16;
+  const int per_max = kWavesMax * RT;
+  const int g = (tiles + per_max - 1) / per_max;
+  int tpg = (tiles + g - 1) / g;
+  tpg = (tpg + RT - 1) / RT * RT;
+  *groups = g;
+  *tiles_per_group = tpg;
+  *waves = tpg / RT;
+}
+
+static bool batch_split(const cav_batch *b)
+{
+  return b->precision == CAV_MIXED;
+}
+
+int batch_ewald_k_plan(cav_batch *b)
+{
+  if (!b->ek_on) return CAV_OK;
+  b->W.clear();
+  double qmax = 0, gb = 0;
+  for (int r = 0; r < b->B; ++r) {
+    RepDev &d = b->reps[r];
+    EwaldKPlan p;
+    if (const char *e = plan_k_lattice(p, b->kx, b->ky, b->kz, b->precision, b->ek_alpha, d.L[0],
+                                       d.L[1], d.L[2]))
+      return bfail(b, CAV_ERR_ARG, e);
+    if (r == 0) b->ek = p;
+    b->W.insert(b->W.end(), p.W.begin(), p.W.end());
+    d.c0 = CAV_4PI * b->ek_ke * p.lxyz_inv;
+    d.kxs = CAV_2PI / p.lx;
+    d.kys = CAV_2PI / p.ly;
+    d.kzs = CAV_2PI / p.lz;
+    double qsum = 0, wmax = 0;
+    for (int t : b->hreps[r].type) {
+      const double qa = std::fabs(b->charge[t]);
+      qmax = std::max(qmax, qa);
+      qsum += qa;
+    }
+    for (double w : p.W) wmax = std::max(wmax, std::fabs(w));
+    gb = std::max(gb, wmax * qsum);
+  }
+  b->nkh = (2 * b->ek.nZ + 31) / 32;
+  split_scales(qmax, gb > 0 ? 1.0 : 0.0, gb, &b->sA, &b->sB);
+  return CAV_OK;
+}
+
+static int batch_free_k(cav_batch *b)
+{
+  void *ptrs[] = {b->d_W, b->tab, b->ztab, b->bmat, b->bmat_h, b->sk_part, b->sk, b->epart,
+                  b->s_items, b->f_items, b->rchunks};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  b->d_W = b->sk_part = b->sk = b->epart = nullptr;
+  b->tab = b->ztab = b->bmat = b->bmat_h = nullptr;
+  b->s_items = b->f_items = nullptr;
+  b->rchunks = nullptr;
+  return CAV_OK;
+}
+
+// Work tables and buffers of the batched Ewald_k passes for the current tiles.
+int batch_ewald_k_alloc(cav_batch *b)
+{
+  batch_free_k(b);
+  if (!b->ek_on) return CAV_OK;
+  const EwaldKPlan &p = b->ek;
+  const bool split = batch_split(b);
+  const size_t tsz = b->precision == CAV_FP64 ? sizeof(double) : sizeof(float);
+  const int TS = p.nX + p.nY + p.nZ;
+  BAT_HIP(hipMalloc(&b->tab, (size_t)b->nown * TS * 2 * tsz + 4096));
+  if (split) BAT_HIP(hipMalloc(&b->ztab, (size_t)b->nown * p.nZ * 16 + 4096));
+  // S-pass items: chunks of s_chunk atoms per replica, largest replicas first
+  std::vector<int4> sitems;
+  std::vector<int2> rch(b->B);
+  std::vector<int> order(b->B);
+  for (int r = 0; r < b->B; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int u, int v) { return b->reps[u].n > b->reps[v].n; });
+  int slot = 0;
+  for (int r = 0; r < b->B; ++r) {
+    const RepDev &d = b->reps[r];
+    rch[r].x = slot;
+    int cnt = 0;
+    for (int a0 = 0; a0 < d.n; a0 += b->s_chunk) {
+      const int a1 = std::min(d.n, a0 + b->s_chunk);
+      sitems.push_back(make_int4(d.obase + a0, d.obase + a1, slot + cnt, r));
+      ++cnt;
+    }
+    rch[r].y = cnt;
+    slot += cnt;
+  }
+  b->n_chunks = slot;
+  // issue the items of the largest replicas first
+  std::vector<int4> sorted;
+  for (int r : order)
+    for (const int4 &it : sitems)
+      if (it.w == r) sorted.push_back(it);
+  b->n_sitems = (int)sorted.size();
+  const int SC = roundup(2 * p.nZ, 16);
+  BAT_HIP(hipMalloc(&b->s_items, sizeof(int4) * std::max(1, b->n_sitems)));
+  BAT_HIP(hipMemcpy(b->s_items, sorted.data(), sizeof(int4) * sorted.size(), hipMemcpyHostToDevice));
+  BAT_HIP(hipMalloc(&b->rchunks, sizeof(int2) * b->B));
+  BAT_HIP(hipMemcpy(b->rchunks, rch.data(), sizeof(int2) * b->B, hipMemcpyHostToDevice));
+  BAT_HIP(hipMalloc(&b->sk_part, sizeof(double) * (size_t)std::max(1, b->n_chunks) * p.Rpad * SC));
+  BAT_HIP(hipMalloc(&b->sk, sizeof(double) * (size_t)b->B * p.Rpad * 2 * p.nZ));
+  BAT_HIP(hipMalloc(&b->epart, sizeof(double) * (size_t)b->B * grid_for((long)p.Rpad * p.nZ)));
+  // force-pass operands per replica (padding entries stay zero)
+  b->bm_stride = (long)(p.Rpad / 16) * 4 * 64 * p.KSP;
+  BAT_HIP(hipMalloc(&b->bmat, b->bm_stride * tsz * b->B));
+  BAT_HIP(hipMemset(b->bmat, 0, b->bm_stride * tsz * b->B));
+  if (split) {
+    b->bmh_stride = (long)(p.Rpad / 16) * 4 * b->nkh * 2 * 64 * 8;
+    BAT_HIP(hipMalloc(&b->bmat_h, b->bmh_stride * sizeof(_Float16) * b->B));
+    BAT_HIP(hipMemset(b->bmat_h, 0, b->bmh_stride * sizeof(_Float16) * b->B));
+  }
+  BAT_HIP(hipMalloc(&b->d_W, sizeof(double) * b->W.size()));
+  BAT_HIP(hipMemcpy(b->d_W, b->W.data(), sizeof(double) * b->W.size(), hipMemcpyHostToDevice));
+  // force-pass items: groups of one workgroup's atoms per replica
+  const int wg = split ? 16 * b->f_nt * kWavesF : 16 * kforce_nt<double>() * kWavesF;
+  std::vector<int4> fitems;
+  for (int r : order) {
+    const RepDev &d = b->reps[r];
+    for (int a0 = 0; a0 < d.n; a0 += wg)
+      fitems.push_back(make_int4(d.obase + a0, d.obase + d.n, 0, r));
+  }
+  b->n_fitems = (int)fitems.size();
+  BAT_HIP(hipMalloc(&b->f_items, sizeof(int4) * std::max(1, b->n_fitems)));
+  BAT_HIP(hipMemcpy(b->f_items, fitems.data(), sizeof(int4) * fitems.size(), hipMemcpyHostToDevice));
+  if (split && split_force_lds_p(p, b->nkh) > 160 * 1024)
+    return bfail(b, CAV_ERR_ARG, "k lattice too wide for the batched split form");
+  return CAV_OK;
+}
+
+template <int RT>
+static void batch_sk_h(cav_batch *b, const SArgsH &a, int groups, size_t smem, int SCT, int waves)
+{
+  launch_sk_h<RT>(b->stream, a, dim3(b->n_sitems, groups), smem, SCT, waves);
+}
+
+template <typename T>
+static int batch_ewald_k_t(cav_batch *b)
+{
+  const EwaldKPlan &p = b->ek;
+  const int TS = p.nX + p.nY + p.nZ;
+  const int SC = roundup(2 * p.nZ, 16);
+  const dim3 tgrid(grid_for((long)b->maxn * TS), b->B);
+  int groups, tpg, waves;
+  s_layout_rt(p.Rpad, kRT, &groups, &tpg, &waves);
+  if (b->precision == CAV_FP64) {
+    kb_tables<T><<<tgrid, kBlock, 0, b->stream>>>(b->d_rep, p.kx, p.ky, p.kz, b->x, b->y, b->z,
+                                                  (Cx<T> *)b->tab);
+    SArgs a;
+    a.n = 0;
+    a.TS = TS;
+    a.nX = p.nX;
+    a.nY = p.nY;
+    a.nZ = p.nZ;
+    a.ky = p.ky;
+    a.kz = p.kz;
+    a.Rtiles = p.Rpad / 16;
+    a.chunk = b->s_chunk;
+    a.stage = 2 * 32 * (size_t)TS * 2 * sizeof(T) + 2 * 32 * sizeof(T) <= 150 * 1024 ? 32 : 16;
+    a.tiles_per_group = tpg;
+    a.q = b->q_o;
+    a.tab = b->tab;
+    a.part = b->sk_part;
+    a.Rpad = p.Rpad;
+    a.SC = SC;
+    a.items = b->s_items;
+    const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
+    launch_sk_rt<T, kRT>(b->stream, a, dim3(b->n_sitems, groups), smem, SC / 16, waves);
+  } else {
+    kb_tables_h<<<tgrid, kBlock, 0, b->stream>>>(b->d_rep, p.kx, p.ky, p.kz, b->x, b->y, b->z,
+                                                 (Cx<float> *)b->tab, (u4v *)b->ztab);
+    SArgsH a;
+    a.n = 0;
+    a.TS = p.nX + p.nY;
+    a.nX = p.nX;
+    a.nY = p.nY;
+    a.nZ = p.nZ;
+    a.Rtiles = p.Rpad / 16;
+    a.chunk = b->s_chunk;
+    const size_t per_atom = (size_t)a.TS * sizeof(Cx<float>) + (size_t)p.nZ * 16 + sizeof(float);
+    a.stage = b->s_stage;
+    if (2 * (size_t)a.stage * per_atom > 150 * 1024) a.stage = 16;
+    a.tiles_per_group = 0;
+    a.sa = (float)b->sA;
+    a.q = b->q_o;
+    a.tab = b->tab;
+    a.ztab = b->ztab;
+    a.part = b->sk_part;
+    a.Rpad = p.Rpad;
+    a.SC = SC;
+    a.items = b->s_items;
+    const int RT = b->s_rt;
+    s_layout_rt(p.Rpad, RT, &groups, &tpg, &waves);
+    a.tiles_per_group = tpg;
+    const size_t smem = 2 * (size_t)a.stage * per_atom;
+    if (RT == 1)
+      batch_sk_h<1>(b, a, groups, smem, SC / 16, waves);
+    else if (RT == 2)
+      batch_sk_h<2>(b, a, groups, smem, SC / 16, waves);
+    else
+      batch_sk_h<3>(b, a, groups, smem, SC / 16, waves);
+  }
+  const int nthr = p.Rpad * p.nZ;
+  k_sk_reduce<T><<<dim3(grid_for(nthr), b->B), kBlock, 0, b->stream>>>(
+      0, p.Rpad, SC, p.R, p.nZ, p.kz, p.KSP, b->sk_part, b->d_W, b->sk, (T *)b->bmat, b->epart,
+      1.0 / b->sA, batch_split(b) ? (_Float16 *)b->bmat_h : nullptr, b->nkh, b->sB, b->rchunks,
+      b->bm_stride, b->bmh_stride);
+  if (b->precision == CAV_FP64) {
+    FArgs f;
+    f.n = 0;
+    f.TS = TS;
+    f.nX = p.nX;
+    f.nY = p.nY;
+    f.nZ = p.nZ;
+    f.ky = p.ky;
+    f.R = p.R;
+    f.Rpad = p.Rpad;
+    f.KZ = p.KZ;
+    f.KSP = p.KSP;
+    f.tab = b->tab;
+    f.bm = b->bmat;
+    f.q = b->q_o;
+    f.minv = b->d_minv;
+    f.type = b->type_o;
+    f.c0 = f.kxs = f.kys = f.kzs = 0;
+    f.ax = b->ax;
+    f.ay = b->ay;
+    f.az = b->az;
+    f.out4 = nullptr;
+    f.items = b->f_items;
+    f.rep = b->d_rep;
+    f.bm_stride = b->bm_stride * (long)sizeof(T);
+    launch_kforce_t<T>(b->stream, f, p.KZ / 4, b->n_fitems);
+  } else {
+    FArgsH f;
+    f.n = 0;
+    f.TS = p.nX + p.nY;
+    f.nX = p.nX;
+    f.nY = p.nY;
+    f.nZ = p.nZ;
+    f.ky = p.ky;
+    f.R = p.R;
+    f.Rpad = p.Rpad;
+    f.nkh = b->nkh;
+    f.tab = b->tab;
+    f.ztab = b->ztab;
+    f.bmh = b->bmat_h;
+    f.q = b->q_o;
+    f.minv = b->d_minv;
+    f.type = b->type_o;
+    f.c0 = f.kxs = f.kys = f.kzs = 0;
+    f.ax = b->ax;
+    f.ay = b->ay;
+    f.az = b->az;
+    f.out4 = nullptr;
+    f.items = b->f_items;
+    f.rep = b->d_rep;
+    f.bm_stride = b->bmh_stride * (long)sizeof(_Float16);
+    f.inv_sb = 1.0 / b->sB;
+    const size_t smem = split_force_lds_p(p, b->nkh);
+    constexpr int NT = 2;
+    switch (b->nkh) {
+    case 1: allow_smem(k_kforce_h<1, NT>, smem); k_kforce_h<1, NT><<<b->n_fitems, 64 * kWavesF, smem, b->stream>>>(f); break;
+    case 2: allow_smem(k_kforce_h<2, NT>, smem); k_kforce_h<2, NT><<<b->n_fitems, 64 * kWavesF, smem, b->stream>>>(f); break;
+    case 3: allow_smem(k_kforce_h<3, NT>, smem); k_kforce_h<3, NT><<<b->n_fitems, 64 * kWavesF, smem, b->stream>>>(f); break;
+    default: allow_smem(k_kforce_h<4, NT>, smem); k_kforce_h<4, NT><<<b->n_fitems, 64 * kWavesF, smem, b->stream>>>(f); break;
+    }
+  }
+  BAT_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+// Ewald_k accelerations of every replica, added into the owned accelerations.
+int batch_ewald_k(cav_batch *b)
+{
+  if (!b->ek_on) return CAV_OK;
+  if (b->precision == CAV_FP64) return batch_ewald_k_t<double>(b);
+  return batch_ewald_k_t<float>(b);
 }
 
 }  // namespace cav

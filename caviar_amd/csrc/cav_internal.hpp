@@ -29,6 +29,7 @@ enum Flag : int {
   F_NGHOST_OLD = 5,// ghost count at the last list build
   F_MAXNB = 6,     // longest Verlet row seen at the last build
   F_FORCE = 7,     // host-requested rebuild
+  F_GOVER = 8,     // batched replicas: ghosts exceeded the replica's ghost capacity
   F_COUNT = 16
 };
 
@@ -52,6 +53,18 @@ struct EwaldKPlan {
   double lx = 1, ly = 1, lz = 1, lxyz_inv = 0, dip_coef = 0;
   std::vector<double> W;        // R*nZ: multiplicity(k) * f(k) over the half space
   std::vector<int> ref_r, ref_c, ref_conj;  // reference k order -> (row, col, conjugate?)
+};
+
+// One replica of a batch as the device kernels see it (see cav_batch.hip).
+// Slot arrays hold each replica's owned atoms then its ghosts from `base`;
+// owned-indexed arrays (velocities, accelerations, Verlet rows, phase tables)
+// hold its owned atoms from `obase`.
+struct RepDev {
+  double lo[3], hi[3], L[3], il[3];
+  double c0, kxs, kys, kzs;  // k-space force constants of this box
+  int base, n, gcap, obase;
+  int cbase, nbins, nb[3];
+  int pbase;                 // first per-block partial slot
 };
 
 }  // namespace cav
@@ -161,6 +174,89 @@ struct cav_ctx {
   cav::Timer t_pair, t_sk, t_force, t_tab, t_nb, t_int, t_list;
 };
 
+// Many independent single-domain systems (replicas) of one force field, each
+// with its own atoms and periodic box, stepped together: every kernel covers
+// all replicas through the replica table, and one step is one graph launch.
+struct cav_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int precision = CAV_FP64;
+
+  // shared settings
+  int nt = 0;
+  std::vector<double> mass, minv, charge;
+  double ghost_cutoff = 0;
+  double nl_cutoff = 0, nl_coef = 1.12347, rce = 0, rce2 = 0, thr_sq = 0, cinv = 0;
+  bool nl_on = false, lj_on = false, er_on = false, ek_on = false;
+  double lj_rc = 0;
+  std::vector<double> eps, sig;
+  double er_alpha = 1, er_ke = 1;
+  double ek_alpha = 0, ek_ke = 1;
+  int kx = 1, ky = 1, kz = 1;
+
+  // replicas as given (host)
+  struct HostRep {
+    double lo[3], hi[3];
+    int n = 0, gcap = 0;
+    std::vector<int> type;
+    std::vector<double> pos, vel;  // AoS
+  };
+  std::vector<HostRep> hreps;
+  std::vector<cav::RepDev> reps;
+  int B = 0;
+  long nslots = 0, nown = 0;
+  int ncells = 0, maxn = 0, pair_blocks = 0, scan_blocks = 0;
+  std::vector<int> h_cell_rep, h_st_off, h_st;
+
+  // device
+  cav::RepDev *d_rep = nullptr;
+  int *cell_rep = nullptr, *st_off = nullptr, *st = nullptr;
+  double *x = nullptr, *y = nullptr, *z = nullptr;     // slots
+  double *xo = nullptr, *yo = nullptr, *zo = nullptr;  // slots at the last list build
+  int *type = nullptr;                                 // slots
+  int *id = nullptr;                                   // slots: local owned index (ghost: its source)
+  double *q = nullptr;                                 // slots
+  double4 *xq = nullptr;
+  int2 *tid = nullptr;
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr;  // owned
+  double *ax = nullptr, *ay = nullptr, *az = nullptr;
+  double *aox = nullptr, *aoy = nullptr, *aoz = nullptr;
+  int *type_o = nullptr;                               // owned
+  double *q_o = nullptr;
+  double *snap = nullptr;  // [9][nown]: positions, velocities, accelerations
+  int *bin_of = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_items = nullptr;
+  int *gblk = nullptr;     // [B][scan_blocks] ghost-count block sums
+  int *nl = nullptr, *nn = nullptr;
+  int cap = 0;
+  int *flags = nullptr, *h_flags = nullptr;  // [B][F_COUNT]
+  double *d_minv = nullptr, *d_lj = nullptr;
+  double *part = nullptr;  // [B][pair_blocks] virial partials
+  double *vir = nullptr;   // [B]
+
+  // Ewald_k
+  cav::EwaldKPlan ek;      // lattice shared by the replicas (W of replica 0)
+  std::vector<double> W;   // [B][R*nZ]
+  double *d_W = nullptr;
+  void *tab = nullptr, *ztab = nullptr, *bmat = nullptr, *bmat_h = nullptr;
+  double *sk_part = nullptr, *sk = nullptr, *epart = nullptr;
+  int4 *s_items = nullptr, *f_items = nullptr;
+  int2 *rchunks = nullptr;
+  int n_sitems = 0, n_fitems = 0, n_chunks = 0, nkh = 0;
+  long bm_stride = 0, bmh_stride = 0;  // bytes (FP operand) / halves (split operand) per replica
+  double sA = 1, sB = 1;
+
+  // tiles chosen by the autotuner (cav_batch_autotune)
+  int s_rt = 3, s_stage = 32, s_chunk = 1024, f_nt = 2;
+  bool tuned = false;
+
+  // one step as a graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double graph_dt = 0;
+  bool setup_done = false;
+};
+
 // helpers shared by the translation units
 namespace cav {
 int fail(cav_ctx *c, int code, const std::string &msg);
@@ -182,6 +278,9 @@ int launch_drift(cav_ctx *c, double dt);
 int launch_kick(cav_ctx *c, double dt);
 int launch_begin_step(cav_ctx *c);
 int ewald_k_plan(cav_ctx *c);
+const char *plan_k_lattice(EwaldKPlan &p, int kx, int ky, int kz, int precision, double alpha,
+                           double lx, double ly, double lz);
+void split_scales(double qmax, double qsum, double wmax, double *sA, double *sB);
 int ewald_k_alloc(cav_ctx *c);
 int launch_energy_r(cav_ctx *c, double *e);
 int launch_energy_lj(cav_ctx *c, double *e);
@@ -189,6 +288,13 @@ int launch_energy_k(cav_ctx *c, double *e);
 int point_field_r(cav_ctx *c, int np, const double *pts, double *field, double *pot);
 int point_field_k(cav_ctx *c, int np, const double *pts, double *field, double *pot);
 int reduce_partials(cav_ctx *c, int count, int slot, bool add = false);
+int bfail(cav_batch *b, int code, const std::string &msg);
+int bhip_check(cav_batch *b, hipError_t e, const char *what);
+int batch_neighbor(cav_batch *b, bool forced);   // ghosts, rebuild test, cells, list
+int batch_pairs(cav_batch *b);                   // pack + LJ/Ewald_r + virial
+int batch_ewald_k_plan(cav_batch *b);
+int batch_ewald_k_alloc(cav_batch *b);
+int batch_ewald_k(cav_batch *b);
 
 inline int grid_for(long n, int block = kBlock) { return (int)((n + block - 1) / block); }
 
@@ -198,6 +304,12 @@ inline int grid_for(long n, int block = kBlock) { return (int)((n + block - 1) /
   do {                                                                       \
     hipError_t e_ = (call);                                                  \
     if (e_ != hipSuccess) return cav::hip_check(c, e_, #call);               \
+  } while (0)
+
+#define BAT_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) return cav::bhip_check(b, e_, #call);              \
   } while (0)
 
 #define CAV_TRY(call)                                                        \

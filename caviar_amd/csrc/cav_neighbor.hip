@@ -1,6 +1,8 @@
 // Ghost images, cell binning, rebuild test and Verlet-list construction on
 // gfx950.  Every kernel that depends on the per-step rebuild decision reads the
 // device flag and returns early, so a step never waits for the host.
+#include <algorithm>
+
 #include "cav_internal.hpp"
 
 namespace cav {
@@ -269,14 +271,10 @@ __global__ void k_bin_fill(int n, long cap_total, const int *bin_of, const int *
 // rho=0.3); a rare larger bin falls back to an insertion sort.
 constexpr int kSortMax = 256;
 
-__global__ __launch_bounds__(kSortMax) void k_bin_sort(int nbins, const int *start, int *items,
-                                                       const int *cond)
+__device__ void sort_bin(int s, int e, int *items)
 {
-  if (cond && !*cond) return;
-  const int b = blockIdx.x;
-  if (b >= nbins) return;
   const int tid = threadIdx.x;
-  const int s = start[b], e = start[b + 1], m = e - s;
+  const int m = e - s;
   if (m <= 1) return;
   if (m <= kSortMax) {
     __shared__ int v[kSortMax];
@@ -311,6 +309,15 @@ __global__ __launch_bounds__(kSortMax) void k_bin_sort(int nbins, const int *sta
     }
     items[j + 1] = val;
   }
+}
+
+__global__ __launch_bounds__(kSortMax) void k_bin_sort(int nbins, const int *start, int *items,
+                                                       const int *cond)
+{
+  if (cond && !*cond) return;
+  const int b = blockIdx.x;
+  if (b >= nbins) return;
+  sort_bin(start[b], start[b + 1], items);
 }
 
 // Owned atoms in cell order: stable compaction of bin_items (ghost entries
@@ -386,11 +393,24 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
                                                   const double *x, const double *y, const double *z,
                                                   const int *st_off, const int *st,
                                                   const int *start, const int *items, int *nl,
-                                                  int *nn, int *flags, const int *cond)
+                                                  int *nn, int *flags, const int *cond,
+                                                  const RepDev *rep, const int *cell_rep)
 {
-  if (cond && !*cond) return;
   const int b = blockIdx.x;
   if (b >= nbins) return;
+  // batched replicas: this cell's replica decides the owned range, the row base
+  // and the flags; entries are stored replica-local (owned i, ghost n+g)
+  int base = 0, obase = 0;
+  if (rep) {
+    const int r = cell_rep[b];
+    base = rep[r].base;
+    n = rep[r].n;
+    obase = rep[r].obase;
+    flags += 16 * r;
+    cond = flags + F_REBUILD;
+    start += r;  // every replica's cell starts carry one closing entry
+  }
+  if (cond && !*cond) return;
   __shared__ double sx[kCand], sy[kCand], sz[kCand];
   __shared__ int sj[kCand];
   __shared__ int scnt[64];
@@ -399,7 +419,7 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
   // owned atoms come first in a sorted bin
   int no = 0;
   for (int t = s0; t < e0; t += 64) {
-    const int v = (t + lane < e0) ? items[t + lane] : n;
+    const int v = (t + lane < e0) ? items[t + lane] - base : n;
     no += __popcll(__ballot(v < n));
   }
   if (no == 0) return;
@@ -457,7 +477,8 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
             const bool pass = in && ia[u] >= 0 && j != ia[u] && dot_rn(dx, dy, dz) < rce2;
             const unsigned long long mask = __ballot(pass);
             const int pre = __popcll(mask & below);
-            if (pass && c0[u] + pre < cap) nl[(long)ia[u] * cap + c0[u] + pre] = j;
+            if (pass && c0[u] + pre < cap)
+              nl[(long)(obase + ia[u] - base) * cap + c0[u] + pre] = j - base;
             c0[u] += __popcll(mask);
           }
         }
@@ -470,7 +491,7 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
       __syncthreads();
     }
     if (lane < gsize) {
-      const int i = items[s0 + og + lane];
+      const int i = items[s0 + og + lane] - base + obase;
       const int cnt = scnt[lane];
       nn[i] = cnt < cap ? cnt : cap;
       if (cnt > cap) flags[F_OVERFLOW] = 1;
@@ -508,7 +529,8 @@ int launch_list(cav_ctx *c, bool conditional)
   else
     k_list_wave<<<c->nbins, 64, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->x, c->y,
                                                 c->z, c->d_st_off, c->d_st, c->bin_start,
-                                                c->bin_items, c->nl, c->nn, c->flags, cond);
+                                                c->bin_items, c->nl, c->nn, c->flags, cond,
+                                                nullptr, nullptr);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }
@@ -535,6 +557,268 @@ int launch_save_old(cav_ctx *c, bool conditional)
   k_save_old<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->x, c->y, c->z, c->xo, c->yo,
                                                      c->zo, c->flags, cond);
   CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+
+// ---------------------------------------------------------- batched replicas --
+//
+// The same steps for every replica of a cav_batch at once: grid row
+// blockIdx.y is the replica for per-atom work; per-cell work maps a global
+// cell to its replica.  Each replica's decisions (ghost count, rebuild) live
+// in its own 16 flag words.
+
+__device__ __forceinline__ GhostRule ghost_rule_of(const RepDev &r, double gc, const double *x,
+                                                   const double *y, const double *z)
+{
+  GhostRule g;
+  g.x = x + r.base;
+  g.y = y + r.base;
+  g.z = z + r.base;
+  g.xl = r.lo[0] + gc;
+  g.xu = r.hi[0] - gc;
+  g.yl = r.lo[1] + gc;
+  g.yu = r.hi[1] - gc;
+  g.zl = r.lo[2] + gc;
+  g.zu = r.hi[2] - gc;
+  g.bx = g.by = g.bz = 1;
+  return g;
+}
+
+__global__ void kb_ghost_sums(const RepDev *rep, double gc, const double *x, const double *y,
+                              const double *z, int nblk, int *gblk)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const GhostRule g = ghost_rule_of(r, gc, x, y, z);
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  int tot;
+  block_exclusive_scan(i < r.n ? g(i) : 0, &tot);
+  if (threadIdx.x == 0) gblk[(long)blockIdx.y * nblk + blockIdx.x] = tot;
+}
+
+__global__ void kb_ghost_scan(const RepDev *rep, int nblk, int *gblk, int *flags)
+{
+  const int rr = blockIdx.x;
+  int *blk = gblk + (long)rr * nblk;
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nblk; base += kBlock) {
+    const int i = base + threadIdx.x;
+    const int v = i < nblk ? blk[i] : 0;
+    int tot;
+    const int ex = block_exclusive_scan(v, &tot);
+    if (i < nblk) blk[i] = ex + carry;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int *f = flags + 16 * rr;
+    f[F_NGHOST] = carry < rep[rr].gcap ? carry : rep[rr].gcap;
+    if (carry > rep[rr].gcap) f[F_GOVER] = 1;
+  }
+}
+
+__global__ void kb_ghost_scatter(const RepDev *rep, double gc, int nblk, const int *gblk, double *x,
+                                 double *y, double *z, int *type, int *id, double *q)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const GhostRule rule = ghost_rule_of(r, gc, x, y, z);
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int cnt = i < r.n ? rule(i) : 0;
+  const int off = block_exclusive_scan(cnt, nullptr) + gblk[(long)blockIdx.y * nblk + blockIdx.x];
+  if (i >= r.n || cnt == 0) return;
+  int sx, sy, sz;
+  rule.sides(i, sx, sy, sz);
+  const long own = r.base + i;
+  const double px = x[own], py = y[own], pz = z[own];
+  const double gx = px - sx * r.L[0], gy = py - sy * r.L[1], gz = pz - sz * r.L[2];
+  const int t = type[own];
+  const double qq = q[own];
+  int g = off;
+  auto put = [&](double a, double b, double c) {
+    if (g < r.gcap) {
+      const long s = (long)r.base + r.n + g;
+      x[s] = a;
+      y[s] = b;
+      z[s] = c;
+      type[s] = t;
+      id[s] = (int)i;
+      q[s] = qq;
+    }
+    ++g;
+  };
+  if (sx) put(gx, py, pz);
+  if (sy) put(px, gy, pz);
+  if (sz) put(px, py, gz);
+  if (sx && sy) put(gx, gy, pz);
+  if (sx && sz) put(gx, py, gz);
+  if (sy && sz) put(px, gy, gz);
+  if (sx && sy && sz) put(gx, gy, gz);
+}
+
+__global__ void kb_rebuild_check(const RepDev *rep, const double *x, const double *y,
+                                 const double *z, const double *xo, const double *yo,
+                                 const double *zo, double thr_sq, int *flags)
+{
+  const RepDev &r = rep[blockIdx.y];
+  int *f = flags + 16 * blockIdx.y;
+  if (f[F_REBUILD]) return;
+  const int G = f[F_NGHOST], Gold = f[F_NGHOST_OLD];
+  if (G != Gold) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) f[F_REBUILD] = 1;
+    return;
+  }
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= (long)r.n + G) return;
+  const long s = r.base + a;
+  const double dx = __dsub_rn(x[s], xo[s]), dy = __dsub_rn(y[s], yo[s]), dz = __dsub_rn(z[s], zo[s]);
+  if (dot_rn(dx, dy, dz) > thr_sq) f[F_REBUILD] = 1;
+}
+
+__global__ void kb_set_rebuild(int B, int *flags)
+{
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < B) flags[16 * r + F_REBUILD] = 1;
+}
+
+__device__ __forceinline__ BinRule bin_rule_of(const RepDev &r, double cinv)
+{
+  BinRule b;
+  b.lox = r.lo[0];
+  b.loy = r.lo[1];
+  b.loz = r.lo[2];
+  b.cinv = cinv;
+  b.nbx = r.nb[0];
+  b.nby = r.nb[1];
+  b.nbz = r.nb[2];
+  return b;
+}
+
+// cell counts of the rebuilding replicas (zeroed first by kb_zero_cells)
+__global__ void kb_zero_cells(const RepDev *rep, const int *flags, int *cnt, int *cur)
+{
+  const RepDev &r = rep[blockIdx.y];
+  if (!flags[16 * blockIdx.y + F_REBUILD]) return;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < r.nbins) cnt[r.cbase + k] = cur[r.cbase + k] = 0;
+}
+
+__global__ void kb_bin_count(const RepDev *rep, double cinv, const double *x, const double *y,
+                             const double *z, const int *flags, int *bin_of, int *cnt)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const int *f = flags + 16 * blockIdx.y;
+  if (!f[F_REBUILD]) return;
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= (long)r.n + f[F_NGHOST]) return;
+  const long s = r.base + a;
+  const int c = r.cbase + bin_rule_of(r, cinv)(x[s], y[s], z[s]);
+  bin_of[s] = c;
+  atomicAdd(cnt + c, 1);
+}
+
+// per replica: cell starts (absolute into the item array, replica r's starts
+// at index cbase + r .. cbase + r + nbins)
+__global__ void kb_bin_start(const RepDev *rep, const int *flags, const int *cnt, int *start)
+{
+  const int rr = blockIdx.x;
+  const RepDev &r = rep[rr];
+  if (!flags[16 * rr + F_REBUILD]) return;
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = r.base;
+  __syncthreads();
+  int *st = start + r.cbase + rr;
+  for (int k0 = 0; k0 < r.nbins; k0 += kBlock) {
+    const int k = k0 + threadIdx.x;
+    const int v = k < r.nbins ? cnt[r.cbase + k] : 0;
+    int tot;
+    const int ex = block_exclusive_scan(v, &tot);
+    if (k < r.nbins) st[k] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) st[r.nbins] = carry;
+}
+
+__global__ void kb_bin_fill(const RepDev *rep, const int *flags, const int *bin_of,
+                            const int *cell_rep, const int *start, int *cur, int *items)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const int *f = flags + 16 * blockIdx.y;
+  if (!f[F_REBUILD]) return;
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= (long)r.n + f[F_NGHOST]) return;
+  const long s = r.base + a;
+  const int c = bin_of[s];
+  items[start[c + blockIdx.y] + atomicAdd(cur + c, 1)] = (int)s;
+}
+
+__global__ __launch_bounds__(kSortMax) void kb_bin_sort(int ncells, const int *cell_rep,
+                                                        const int *flags, const int *start,
+                                                        int *items)
+{
+  const int c = blockIdx.x;
+  if (c >= ncells) return;
+  const int rr = cell_rep[c];
+  if (!flags[16 * rr + F_REBUILD]) return;
+  sort_bin(start[c + rr], start[c + rr + 1], items);
+}
+
+__global__ void kb_save_old(const RepDev *rep, const double *x, const double *y, const double *z,
+                            double *xo, double *yo, double *zo, int *flags)
+{
+  const RepDev &r = rep[blockIdx.y];
+  int *f = flags + 16 * blockIdx.y;
+  if (!f[F_REBUILD]) return;
+  const int G = f[F_NGHOST];
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a == 0) f[F_NGHOST_OLD] = G;
+  if (a >= (long)r.n + G) return;
+  const long s = r.base + a;
+  xo[s] = x[s];
+  yo[s] = y[s];
+  zo[s] = z[s];
+}
+
+// Ghosts, rebuild decision, cells and Verlet rows of every replica.
+int batch_neighbor(cav_batch *b, bool forced)
+{
+  const dim3 own_grid(b->scan_blocks, b->B);
+  kb_ghost_sums<<<own_grid, kBlock, 0, b->stream>>>(b->d_rep, b->ghost_cutoff, b->x, b->y, b->z,
+                                                     b->scan_blocks, b->gblk);
+  kb_ghost_scan<<<b->B, kBlock, 0, b->stream>>>(b->d_rep, b->scan_blocks, b->gblk, b->flags);
+  kb_ghost_scatter<<<own_grid, kBlock, 0, b->stream>>>(b->d_rep, b->ghost_cutoff, b->scan_blocks,
+                                                        b->gblk, b->x, b->y, b->z, b->type, b->id,
+                                                        b->q);
+  if (!b->nl_on) return CAV_OK;
+  int maxslots = 0, maxcells = 0;
+  for (const RepDev &r : b->reps) {
+    maxslots = std::max(maxslots, r.n + r.gcap);
+    maxcells = std::max(maxcells, r.nbins);
+  }
+  const dim3 slot_grid(grid_for(maxslots), b->B), cell_grid(grid_for(maxcells), b->B);
+  if (forced)
+    kb_set_rebuild<<<grid_for(b->B), kBlock, 0, b->stream>>>(b->B, b->flags);
+  else
+    kb_rebuild_check<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->x, b->y, b->z, b->xo, b->yo,
+                                                           b->zo, b->thr_sq, b->flags);
+  kb_zero_cells<<<cell_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_cnt, b->bin_cnt + b->ncells);
+  kb_bin_count<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->cinv, b->x, b->y, b->z, b->flags,
+                                                     b->bin_of, b->bin_cnt);
+  kb_bin_start<<<b->B, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_cnt, b->bin_start);
+  kb_bin_fill<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_of, b->cell_rep,
+                                                    b->bin_start, b->bin_cnt + b->ncells, b->bin_items);
+  kb_bin_sort<<<b->ncells, kSortMax, 0, b->stream>>>(b->ncells, b->cell_rep, b->flags, b->bin_start,
+                                                     b->bin_items);
+  k_list_wave<<<b->ncells, 64, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->x, b->y, b->z,
+                                               b->st_off, b->st, b->bin_start, b->bin_items, b->nl,
+                                               b->nn, b->flags, nullptr, b->d_rep, b->cell_rep);
+  kb_save_old<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->x, b->y, b->z, b->xo, b->yo, b->zo,
+                                                    b->flags);
+  BAT_HIP(hipGetLastError());
   return CAV_OK;
 }
 

@@ -3,6 +3,8 @@
 // real-space energy and point queries).  One lane per owned atom, no atomics:
 // every atom gathers its own row, so FP64 results are reproducible run to run.
 // No MFMA: the pair loop is a gather over irregular rows, not a contraction.
+#include <algorithm>
+
 #include "cav_internal.hpp"
 
 namespace cav {
@@ -48,6 +50,7 @@ struct PairArgs {
   double *ax, *ay, *az;
   double *part;
   int accumulate;  // add into the acceleration instead of overwriting it
+  const RepDev *rep;  // batched replicas (blockIdx.y) or nullptr
 };
 
 constexpr int kLPA = 16;  // lanes per atom in the pair kernel
@@ -59,13 +62,26 @@ constexpr int kLPA = 16;  // lanes per atom in the pair kernel
 template <typename T, bool LJ, bool ER, bool LAM>
 __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
 {
+  if (a.rep) {  // one replica per grid row: shift every array to it
+    const RepDev &r = a.rep[blockIdx.y];
+    const int base = r.base, obase = r.obase;
+    a.n = r.n;
+    a.xq += base;
+    a.tid += base;
+    a.nl += (long)obase * a.cap;
+    a.nn += obase;
+    a.ax += obase;
+    a.ay += obase;
+    a.az += obase;
+    a.part += r.pbase;
+  }
   const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
   const int t = (int)(gid / kLPA), sub = (int)(gid % kLPA);
   double vir = 0.0;
   T fx = 0, fy = 0, fz = 0;
   int i = 0, ti = 0;
   if (t < a.n) {
-    i = a.order[t];
+    i = a.order ? a.order[t] : t;
     const double4 pi4 = a.xq[i];
     const int2 tii = a.tid[i];
     ti = tii.x;
@@ -199,6 +215,8 @@ int reduce_partials(cav_ctx *c, int count, int slot, bool add)
 static PairArgs pair_args(cav_ctx *c)
 {
   PairArgs a;
+  a.rep = nullptr;
+  a.accumulate = 0;
   a.n = c->n;
   a.nt = c->nt;
   a.cap = c->nl_cap;
@@ -274,6 +292,20 @@ __global__ void k_zero_acc(int n, double *ax, double *ay, double *az)
 }
 
 __global__ void k_zero_scalar(double *p) { *p = 0.0; }
+
+__global__ void k_fill_d(double *p, long n, double v)
+{
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void kb_zero_own(const RepDev *rep, double *ax, double *ay, double *az)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= r.n) return;
+  ax[r.obase + i] = ay[r.obase + i] = az[r.obase + i] = 0.0;
+}
 
 int launch_zero_acc(cav_ctx *c)
 {
@@ -508,6 +540,100 @@ int launch_energy_lj(cav_ctx *c, double *e)
   CAV_TRY(reduce_partials(c, grid_for(c->n), 1));
   CAV_HIP(hipMemcpyAsync(e, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CAV_HIP(hipStreamSynchronize(c->stream));
+  return CAV_OK;
+}
+
+
+// ---------------------------------------------------------- batched replicas --
+
+__global__ void kb_pack(const RepDev *rep, const int *flags, const double *x, const double *y,
+                        const double *z, const double *q, const int *type, const int *id,
+                        double4 *xq, int2 *tid)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= (long)r.n + flags[16 * blockIdx.y + F_NGHOST]) return;
+  const long s = r.base + a;
+  xq[s] = make_double4(x[s], y[s], z[s], q[s]);
+  tid[s] = make_int2(type[s], id[s]);
+}
+
+// per replica: sum of its per-block partials in a fixed order
+__global__ void kb_reduce_rows(int nper, const double *part, double *out)
+{
+  __shared__ double red[kBlock];
+  const double *p = part + (long)blockIdx.x * nper;
+  double s = 0;
+  for (int i = threadIdx.x; i < nper; i += kBlock) s += p[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+template <typename T>
+static void batch_pairs_t(cav_batch *b, const PairArgs &a, dim3 g)
+{
+  if (b->lj_on && b->er_on)
+    k_pairs<T, true, true, false><<<g, kBlock, 0, b->stream>>>(a);
+  else if (b->lj_on)
+    k_pairs<T, true, false, false><<<g, kBlock, 0, b->stream>>>(a);
+  else
+    k_pairs<T, false, true, false><<<g, kBlock, 0, b->stream>>>(a);
+}
+
+// LJ + Ewald_r accelerations of every replica (overwriting), virial per replica.
+int batch_pairs(cav_batch *b)
+{
+  int maxslots = 0;
+  for (const RepDev &r : b->reps) maxslots = std::max(maxslots, r.n + r.gcap);
+  kb_pack<<<dim3(grid_for(maxslots), b->B), kBlock, 0, b->stream>>>(
+      b->d_rep, b->flags, b->x, b->y, b->z, b->q, b->type, b->id, b->xq, b->tid);
+  const dim3 g(b->pair_blocks, b->B);
+  if (!b->lj_on && !b->er_on) {
+    kb_zero_own<<<dim3(grid_for(b->maxn), b->B), kBlock, 0, b->stream>>>(b->d_rep, b->ax, b->ay,
+                                                                        b->az);
+    k_fill_d<<<grid_for((long)b->B * b->pair_blocks), kBlock, 0, b->stream>>>(
+        b->part, (long)b->B * b->pair_blocks, 0.0);
+  } else {
+    PairArgs a;
+    a.n = 0;
+    a.nt = b->nt;
+    a.cap = b->cap;
+    a.x = a.y = a.z = a.q = nullptr;
+    a.minv = b->d_minv;
+    a.type = a.id = a.mol = nullptr;
+    a.nl = b->nl;
+    a.nn = b->nn;
+    a.order = nullptr;
+    a.xq = b->xq;
+    a.tid = b->tid;
+    const int nt2 = b->nt * b->nt;
+    a.c12 = b->d_lj;
+    a.c6 = b->d_lj + nt2;
+    a.cshift = b->d_lj + 2 * nt2;
+    a.lj_rc2 = b->lj_rc * b->lj_rc;
+    a.alpha = b->er_alpha;
+    a.a2 = b->er_alpha * b->er_alpha;
+    a.ke = b->er_ke;
+    a.two_a_rpi = 2 * b->er_alpha * CAV_PIS_INV;
+    a.lambda = nullptr;
+    a.ax = b->ax;
+    a.ay = b->ay;
+    a.az = b->az;
+    a.part = b->part;
+    a.accumulate = 0;
+    a.rep = b->d_rep;
+    if (b->precision == CAV_FP64)
+      batch_pairs_t<double>(b, a, g);
+    else
+      batch_pairs_t<float>(b, a, g);
+  }
+  kb_reduce_rows<<<b->B, kBlock, 0, b->stream>>>(b->pair_blocks, b->part, b->vir);
+  BAT_HIP(hipGetLastError());
   return CAV_OK;
 }
 

@@ -156,6 +156,52 @@ int cav_point_field(cav_ctx *ctx, int which, int np, const double *pts, double *
 int cav_adjoint(cav_ctx *ctx, const double *weights, double *g_epsilon, double *g_sigma,
                 double *g_charge);
 
+/* ---- batched replicas (no counterpart in the reference) ------------------ */
+/* Many independent systems of one force field, each with its own atoms and
+ * periodic box, stepped together: every kernel covers all replicas and one
+ * velocity-Verlet step of all of them is one launch of a captured hipGraph.
+ * Each replica follows the single-system semantics of this header (the
+ * reference's Md_simulator step on its own Atom_data / Domain): its own
+ * ghosts, rebuild decisions and Verlet list.  Scope: periodic boxes, Verlet
+ * list from cells, Lj_cell_list + Electrostatic_ewald_r + Electrostatic_ewald_k
+ * (no lambda, dipole or slab). */
+typedef struct cav_batch cav_batch;
+int cav_batch_create(int device, int precision, cav_batch **out);
+int cav_batch_destroy(cav_batch *b);
+const char *cav_batch_last_error(const cav_batch *b);
+/* Settings shared by every replica (meaning as cav_set_types / cav_set_neighbor
+ * with make_verlet_list_from_cell_list + build_cell_list / cav_set_lj /
+ * cav_set_ewald_r without lambda / cav_set_ewald_k without dipole or slab). */
+int cav_batch_set_types(cav_batch *b, int ntypes, const double *mass, const double *charge);
+int cav_batch_set_neighbor(cav_batch *b, double ghost_cutoff, double cutoff, double cutoff_extra_coef);
+int cav_batch_set_lj(cav_batch *b, double cutoff, const double *epsilon, const double *sigma);
+int cav_batch_set_ewald_r(cav_batch *b, double alpha, double k_electrostatic);
+int cav_batch_set_ewald_k(cav_batch *b, double alpha, double k_electrostatic, int kx_max, int ky_max,
+                          int kz_max);
+/* One replica: its box, n atoms (types, AoS positions, AoS velocities or NULL);
+ * ids are 0..n-1.  *index (may be NULL) receives its replica number. */
+int cav_batch_add_replica(cav_batch *b, const double lower[3], const double upper[3], int n,
+                          const int *type, const double *pos, const double *vel, int *index);
+int cav_batch_num_replicas(const cav_batch *b, int *n);
+int cav_batch_num_atoms(const cav_batch *b, int replica, int *n);
+/* Md_simulator::setup for every replica; then nsteps steps of all replicas,
+ * each step one graph launch (use_graph != 0) or the same kernels launched
+ * one by one.  Capacities grow transparently; CAV_ERR_NAN restores the state
+ * before the offending step. */
+int cav_batch_setup(cav_batch *b);
+int cav_batch_step(cav_batch *b, double dt, int nsteps, int use_graph);
+/* Per replica: owned arrays (which = CAV_POS/VEL/ACC/ACC_OLD, AoS 3n doubles),
+ * virial, Verlet pairs with the reference's half-list membership. */
+int cav_batch_download(cav_batch *b, int replica, int which, double *out);
+int cav_batch_virial(cav_batch *b, int replica, double *virial);
+int cav_batch_get_pairs(cav_batch *b, int replica, int *ij, long capacity, long *npairs);
+/* Autotuner: times the batched Ewald_k passes for each candidate tile shape on
+ * this batch and keeps the fastest; cav_batch_tiles reports the choice (S-pass
+ * row tiles per wave, atoms per LDS stage, atoms per chunk; force-pass atom
+ * tiles per wave). */
+int cav_batch_autotune(cav_batch *b);
+int cav_batch_tiles(const cav_batch *b, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles);
+
 /* ---- measurement ----------------------------------------------------------- */
 /* Statistics of the current Verlet list: list entries per atom with the
  * reference's half-list membership, and those inside the LJ cutoff. */

@@ -182,7 +182,7 @@ static void free_all(cav_ctx *c)
                   c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
                   c->d_W,     c->kacc,      c->order,    c->ord_blk,  c->xq,
-                  c->tid,     c->d_raw_eps, c->d_raw_sig, c->ztab, c->bmat_h};
+                  c->tid,     c->d_raw_eps, c->d_raw_sig, c->ztab, c->bmat_h, c->cpos, c->sk_sum};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -608,6 +608,7 @@ int cav_upload_atoms(cav_ctx *c, int n, const int *id, const int *type, const in
   CAV_TRY(dmalloc(c, &c->gsrc, c->gcap));
   CAV_TRY(dmalloc(c, &c->bin_of, tot));
   CAV_TRY(dmalloc(c, &c->bin_items, tot));
+  CAV_TRY(dmalloc(c, &c->cpos, tot));
   CAV_TRY(dmalloc(c, &c->nn, n));
   CAV_TRY(dmalloc(c, &c->order, n));
   CAV_TRY(dmalloc(c, &c->xq, tot));

@@ -161,7 +161,7 @@ static void batch_free(cav_batch *b)
                   b->aox, b->aoy, b->aoz, b->type_o, b->q_o, b->snap, b->bin_of, b->bin_cnt,
                   b->bin_start, b->bin_items, b->gblk, b->nl, b->nn, b->flags, b->d_minv, b->d_lj,
                   b->part, b->vir, b->d_W, b->tab, b->ztab, b->bmat, b->bmat_h, b->sk_part, b->sk,
-                  b->epart, b->s_items, b->f_items, b->rchunks};
+                  b->epart, b->s_items, b->f_items, b->rchunks, b->cpos};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->h_flags) (void)hipHostFree(b->h_flags);
@@ -280,6 +280,7 @@ static int batch_layout(cav_batch *b)
   CAV_TRY(balloc(b, &b->tid, slots));
   CAV_TRY(balloc(b, &b->bin_of, slots));
   CAV_TRY(balloc(b, &b->bin_items, slots));
+  CAV_TRY(balloc(b, &b->cpos, slots));
   double **od[] = {&b->vx, &b->vy, &b->vz, &b->ax, &b->ay, &b->az, &b->aox, &b->aoy, &b->aoz, &b->q_o};
   for (double **p : od) CAV_TRY(balloc(b, p, own));
   CAV_TRY(balloc(b, &b->type_o, own));

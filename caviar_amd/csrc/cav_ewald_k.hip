@@ -134,6 +134,7 @@ struct SArgs {
   double *part;  // [chunk][Rpad][SC]
   int Rpad, SC;
   const int4 *items;  // batched: per workgroup column (first atom, end atom, chunk slot, replica)
+  int ncols, groups;  // logical grid: atom chunks (or items) x row-tile groups
 };
 
 // Issue the asynchronous copy of `bytes` contiguous bytes (a multiple of 16,
@@ -170,14 +171,24 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  int chunk = blockIdx.x;
-  const int rt0 = blockIdx.y * a.tiles_per_group + wave * RT;
-  const int rt_end = min(a.Rtiles, (int)(blockIdx.y + 1) * a.tiles_per_group);
+  // XCD-aware order: workgroups L and L+8 run on the same XCD (dispatch is
+  // round-robin over the 8 XCDs), so the row groups of one chunk share its
+  // records through one L2
+  int chunk, grp;
+  {
+    const int L = blockIdx.x, q = L >> 3;
+    grp = q % a.groups;
+    chunk = (q / a.groups) * 8 + (L & 7);
+    if (chunk >= a.ncols) return;
+  }
+  const int col = chunk;
+  const int rt0 = grp * a.tiles_per_group + wave * RT;
+  const int rt_end = min(a.Rtiles, (grp + 1) * a.tiles_per_group);
   long a0 = (long)chunk * a.chunk;
   long a1 = a0 + a.chunk;
   if (a1 > a.n) a1 = a.n;
   if (a.items) {
-    const int4 it = a.items[blockIdx.x];
+    const int4 it = a.items[col];
     a0 = it.x;
     a1 = it.y;
     chunk = it.z;
@@ -348,6 +359,23 @@ __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz
     __syncthreads();
   }
   if (threadIdx.x == 0) epart[blockIdx.x] = red[0];
+}
+
+// Fixed-order sum of the S-pass chunk partials ([chunk][plane] -> [plane]):
+// 64 entries per block, wave w sums chunks w, w+4, ..., then the four wave sums
+// are combined in one fixed order (coalesced, and 4x the parallelism of one
+// thread per entry walking every chunk).
+__global__ void k_chunk_sum(const double *part, int nchunks, long plane, double *out)
+{
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long e = (long)blockIdx.x * 64 + lane;
+  double s = 0;
+  if (e < plane)
+    for (int k = wave; k < nchunks; k += 4) s += part[(long)k * plane + e];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && e < plane) out[e] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
 // zero the padding rows of the operand (kappa >= 2 nZ)
@@ -587,6 +615,7 @@ struct SArgsH {
   const void *ztab;  // split Z records, nZ x 16 B per atom
   double *part;      // [chunk][Rpad][SC]
   const int4 *items; // batched: as SArgs::items
+  int ncols, groups; // logical grid, as SArgs
 };
 
 // Same work split as k_sk: a workgroup is an atom chunk x a group of row
@@ -605,14 +634,24 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   const unsigned char *ztab = reinterpret_cast<const unsigned char *>(a.ztab);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  int chunk = blockIdx.x;
-  const int rt0 = blockIdx.y * a.tiles_per_group + wave * RT;
-  const int rt_end = min(a.Rtiles, (int)(blockIdx.y + 1) * a.tiles_per_group);
+  // XCD-aware order: workgroups L and L+8 run on the same XCD (dispatch is
+  // round-robin over the 8 XCDs), so the row groups of one chunk share its
+  // records through one L2
+  int chunk, grp;
+  {
+    const int L = blockIdx.x, q = L >> 3;
+    grp = q % a.groups;
+    chunk = (q / a.groups) * 8 + (L & 7);
+    if (chunk >= a.ncols) return;
+  }
+  const int col = chunk;
+  const int rt0 = grp * a.tiles_per_group + wave * RT;
+  const int rt_end = min(a.Rtiles, (grp + 1) * a.tiles_per_group);
   long a0 = (long)chunk * a.chunk;
   long a1 = a0 + a.chunk;
   if (a1 > a.n) a1 = a.n;
   if (a.items) {
-    const int4 it = a.items[blockIdx.x];
+    const int4 it = a.items[col];
     a0 = it.x;
     a1 = it.y;
     chunk = it.z;
@@ -1107,6 +1146,8 @@ int ewald_k_alloc(cav_ctx *c)
                          c->stream));
   if (c->sk) (void)hipFree(c->sk);
   CAV_HIP(hipMalloc(&c->sk, sizeof(double) * (size_t)p.Rpad * 2 * p.nZ));
+  if (c->sk_sum) (void)hipFree(c->sk_sum);
+  CAV_HIP(hipMalloc(&c->sk_sum, sizeof(double) * (size_t)p.Rpad * SC));
   if (c->bmat) (void)hipFree(c->bmat);
   const size_t bm_elems = (size_t)(p.Rpad / 16) * 4 * 64 * p.KSP;
   CAV_HIP(hipMalloc(&c->bmat, bm_elems * tsz));
@@ -1140,8 +1181,11 @@ static void allow_smem(K kern, size_t smem)
 }
 
 template <typename T, int RT>
-static void launch_sk_rt(hipStream_t st, const SArgs &a, dim3 grid, size_t smem, int SCT, int waves)
+static void launch_sk_rt(hipStream_t st, SArgs a, dim3 lgrid, size_t smem, int SCT, int waves)
 {
+  a.ncols = (int)lgrid.x;
+  a.groups = (int)lgrid.y;
+  const dim3 grid((lgrid.x + 7) / 8 * 8 * lgrid.y);
   const int block = 64 * waves;
   allow_smem(k_sk<T, RT, 1>, smem);
   allow_smem(k_sk<T, RT, 2>, smem);
@@ -1229,8 +1273,10 @@ static int sk_pipeline(cav_ctx *c)
   launch_sk_rt<T, kRT>(c->stream, a, grid, smem, a.SC / 16, waves);
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
+  const long plane = (long)p.Rpad * a.SC;
+  k_chunk_sum<<<(plane + 63) / 64, kBlock, 0, c->stream>>>(c->sk_part, nch, plane, c->sk_sum);
   k_sk_reduce<T><<<grid_for(nthr), kBlock, 0, c->stream>>>(
-      nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_part, c->d_W, c->sk, (T *)c->bmat, c->part,
+      1, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_sum, c->d_W, c->sk, (T *)c->bmat, c->part,
       1.0, nullptr, 0, 1.0, nullptr, 0, 0);
   CAV_HIP(hipGetLastError());
   return reduce_partials(c, grid_for(nthr), 3);
@@ -1276,8 +1322,11 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
 }
 
 template <int RT>
-static void launch_sk_h(hipStream_t st, const SArgsH &a, dim3 grid, size_t smem, int SCT, int waves)
+static void launch_sk_h(hipStream_t st, SArgsH a, dim3 lgrid, size_t smem, int SCT, int waves)
 {
+  a.ncols = (int)lgrid.x;
+  a.groups = (int)lgrid.y;
+  const dim3 grid((lgrid.x + 7) / 8 * 8 * lgrid.y);
   const int block = 64 * waves;
 #define CAV_SKH(S)                                                      \
   case S:                                                               \
@@ -1331,8 +1380,10 @@ static int sk_pipeline_h(cav_ctx *c)
   launch_sk_h<kRT>(c->stream, a, dim3(nch, groups), smem, a.SC / 16, waves);
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
+  const long plane = (long)p.Rpad * a.SC;
+  k_chunk_sum<<<(plane + 63) / 64, kBlock, 0, c->stream>>>(c->sk_part, nch, plane, c->sk_sum);
   k_sk_reduce<float><<<grid_for(nthr), kBlock, 0, c->stream>>>(
-      nch, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_part, c->d_W, c->sk, (float *)c->bmat,
+      1, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_sum, c->d_W, c->sk, (float *)c->bmat,
       c->part, 1.0 / c->sA, (_Float16 *)c->bmat_h, c->nkh, c->sB, nullptr, 0, 0);
   CAV_HIP(hipGetLastError());
   return reduce_partials(c, grid_for(nthr), 3);
@@ -1465,8 +1516,10 @@ static int weighted_sk_t(cav_ctx *c, const double *weights, double *out)
   const int nch = (c->n + a.chunk - 1) / a.chunk;
   const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
   launch_sk_rt<T, kRT>(c->stream, a, dim3(nch, groups), smem, a.SC / 16, waves);
-  k_sk_sum<<<grid_for((long)p.Rpad * p.nZ), kBlock, 0, c->stream>>>(nch, p.Rpad, a.SC, p.nZ,
-                                                                    c->sk_part, out);
+  const long plane = (long)p.Rpad * a.SC;
+  k_chunk_sum<<<(plane + 63) / 64, kBlock, 0, c->stream>>>(c->sk_part, nch, plane, c->sk_sum);
+  k_sk_sum<<<grid_for((long)p.Rpad * p.nZ), kBlock, 0, c->stream>>>(1, p.Rpad, a.SC, p.nZ,
+                                                                    c->sk_sum, out);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }

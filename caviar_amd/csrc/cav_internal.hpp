@@ -132,6 +132,7 @@ struct cav_ctx {
   int *d_st_off = nullptr, *d_st = nullptr;
   int *scan_tmp = nullptr;
   int *order = nullptr;    // owned atoms in cell order (compaction of bin_items)
+  double4 *cpos = nullptr; // (x, y, z, index) in bin order, for the list build
   int *ord_blk = nullptr;  // scan block sums for the compaction
 
   // list (row-major: nl[i*nl_cap + k], nn[i] entries in row i)
@@ -157,6 +158,7 @@ struct cav_ctx {
   double *sk_part = nullptr; // [chunks][Rpad][2nZ]
   int sk_chunks = 0;
   double *sk = nullptr;      // [Rpad][2nZ] FP64
+  double *sk_sum = nullptr;  // [Rpad][SC] chunk partials summed (fixed order)
   void *bmat = nullptr;      // [KZ][4*Rpad] (T) force-pass operand
   // mixed form: split-FP16 Z records and force-pass operand (hi + lo halves)
   void *ztab = nullptr;      // [n][nZ] x 16 B: (Zr,-Zi) hi, (Zr,-Zi) lo, (Zi,Zr) hi, (Zi,Zr) lo
@@ -226,6 +228,7 @@ struct cav_batch {
   double *q_o = nullptr;
   double *snap = nullptr;  // [9][nown]: positions, velocities, accelerations
   int *bin_of = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_items = nullptr;
+  double4 *cpos = nullptr; // slots: (x, y, z, slot) in bin order
   int *gblk = nullptr;     // [B][scan_blocks] ghost-count block sums
   int *nl = nullptr, *nn = nullptr;
   int cap = 0;

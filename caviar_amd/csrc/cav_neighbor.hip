@@ -320,6 +320,17 @@ __global__ __launch_bounds__(kSortMax) void k_bin_sort(int nbins, const int *sta
   sort_bin(start[b], start[b + 1], items);
 }
 
+// (x, y, z, index) in bin order, for contiguous candidate staging in the list build
+__global__ void k_cell_pack(long cap_total, int n, const int *items, const double *x, const double *y,
+                            const double *z, double4 *cpos, const int *flags, const int *cond)
+{
+  if (cond && !*cond) return;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long)n + flags[F_NGHOST] || t >= cap_total) return;
+  const int j = items[t];
+  cpos[t] = make_double4(x[j], y[j], z[j], (double)j);
+}
+
 // Owned atoms in cell order: stable compaction of bin_items (ghost entries
 // dropped).  Consecutive lanes then walk overlapping stencils.
 __global__ void k_order_scatter(long cap_total, int n, const int *items, const int *blk, int *order,
@@ -361,6 +372,8 @@ int launch_bins(cav_ctx *c, bool conditional)
   k_bin_fill<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->bin_of, c->bin_start,
                                                      c->bin_cnt, c->bin_items, c->flags, cond);
   k_bin_sort<<<c->nbins, kSortMax, 0, c->stream>>>(c->nbins, c->bin_start, c->bin_items, cond);
+  k_cell_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(tot, c->n, c->bin_items, c->x, c->y, c->z,
+                                                      c->cpos, c->flags, cond);
   const int nbo = grid_for(tot);
   OwnedFlagBounded of{c->bin_items, c->flags, c->n};
   k_block_sums<<<nbo, kBlock, 0, c->stream>>>(tot, of, c->ord_blk, cond);
@@ -373,7 +386,10 @@ int launch_bins(cav_ctx *c, bool conditional)
 
 // ------------------------------------------------------------- Verlet list --
 
-constexpr int kCand = 512;  // stencil candidates staged in LDS per pass
+#ifndef CAV_LIST_CAND
+#define CAV_LIST_CAND 256
+#endif
+constexpr int kCand = CAV_LIST_CAND;  // stencil candidates staged in LDS per pass
 
 // Full Verlet rows: for owned i every owned j != i and every ghost within
 // cutoff_extra (strict, dr = pos_i - pos_j rounded as the reference rounds it,
@@ -384,17 +400,22 @@ constexpr int kCand = 512;  // stencil candidates staged in LDS per pass
 // One wave per cell: the candidates of the cell's de-duplicated stencil are
 // staged in LDS (in stencil order, each cell in bin order), then the cell's
 // owned atoms are tested kAPI at a time against 64 candidates per iteration
-// (each staged candidate is read once for kAPI independent tests); passing
+// (each staged candidate is read once for kAPI independent tests).  Candidates
+// come from the cell-ordered packed copy (x, y, z, index) written after the
+// bin sort, so staging reads are contiguous per stencil cell; passing
 // lanes append in lane order through a ballot and a prefix popcount, so rows
 // are written coalesced and in a deterministic order.
-constexpr int kAPI = 4;
+#ifndef CAV_LIST_API
+#define CAV_LIST_API 4
+#endif
+constexpr int kAPI = CAV_LIST_API;  // owned atoms tested per staged candidate
 
 __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, double rce2,
-                                                  const double *x, const double *y, const double *z,
-                                                  const int *st_off, const int *st,
-                                                  const int *start, const int *items, int *nl,
-                                                  int *nn, int *flags, const int *cond,
-                                                  const RepDev *rep, const int *cell_rep)
+                                                  const double4 *cpos, const int *st_off,
+                                                  const int *st, const int *start,
+                                                  const int *items, int *nl, int *nn, int *flags,
+                                                  const int *cond, const RepDev *rep,
+                                                  const int *cell_rep)
 {
   const int b = blockIdx.x;
   if (b >= nbins) return;
@@ -438,11 +459,11 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
         const int cb = start[cell], cn = start[cell + 1] - cb;
         const int take = (cn - ci) < (kCand - m) ? (cn - ci) : (kCand - m);
         for (int t = lane; t < take; t += 64) {
-          const int j = items[cb + ci + t];
-          sj[m + t] = j;
-          sx[m + t] = x[j];
-          sy[m + t] = y[j];
-          sz[m + t] = z[j];
+          const double4 c = cpos[cb + ci + t];
+          sj[m + t] = (int)c.w;
+          sx[m + t] = c.x;
+          sy[m + t] = c.y;
+          sz[m + t] = c.z;
         }
         m += take;
         ci += take;
@@ -458,11 +479,11 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
 #pragma unroll
         for (int u = 0; u < kAPI; ++u) {
           const bool ok = o0 + u < gsize;
-          ia[u] = ok ? items[s0 + og + o0 + u] : -1;
-          const int ii = ok ? ia[u] : items[s0 + og];
-          px[u] = x[ii];
-          py[u] = y[ii];
-          pz[u] = z[ii];
+          const double4 c = cpos[s0 + og + (ok ? o0 + u : 0)];
+          ia[u] = ok ? (int)c.w : -1;
+          px[u] = c.x;
+          py[u] = c.y;
+          pz[u] = c.z;
           c0[u] = ok ? scnt[o0 + u] : 0;
         }
         for (int cc = 0; cc < m; cc += 64) {
@@ -527,10 +548,9 @@ int launch_list(cav_ctx *c, bool conditional)
     k_list_all<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->nl_cap, c->nl, c->nn, c->flags,
                                                         cond);
   else
-    k_list_wave<<<c->nbins, 64, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->x, c->y,
-                                                c->z, c->d_st_off, c->d_st, c->bin_start,
-                                                c->bin_items, c->nl, c->nn, c->flags, cond,
-                                                nullptr, nullptr);
+    k_list_wave<<<c->nbins, 64, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->cpos,
+                                                c->d_st_off, c->d_st, c->bin_start, c->bin_items,
+                                                c->nl, c->nn, c->flags, cond, nullptr, nullptr);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }
@@ -767,6 +787,19 @@ __global__ __launch_bounds__(kSortMax) void kb_bin_sort(int ncells, const int *c
   sort_bin(start[c + rr], start[c + rr + 1], items);
 }
 
+__global__ void kb_cell_pack(const RepDev *rep, const int *flags, const int *items, const double *x,
+                             const double *y, const double *z, double4 *cpos)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const int *f = flags + 16 * blockIdx.y;
+  if (!f[F_REBUILD]) return;
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= (long)r.n + f[F_NGHOST]) return;
+  const long t = r.base + a;
+  const int j = items[t];
+  cpos[t] = make_double4(x[j], y[j], z[j], (double)j);
+}
+
 __global__ void kb_save_old(const RepDev *rep, const double *x, const double *y, const double *z,
                             double *xo, double *yo, double *zo, int *flags)
 {
@@ -813,9 +846,11 @@ int batch_neighbor(cav_batch *b, bool forced)
                                                     b->bin_start, b->bin_cnt + b->ncells, b->bin_items);
   kb_bin_sort<<<b->ncells, kSortMax, 0, b->stream>>>(b->ncells, b->cell_rep, b->flags, b->bin_start,
                                                      b->bin_items);
-  k_list_wave<<<b->ncells, 64, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->x, b->y, b->z,
-                                               b->st_off, b->st, b->bin_start, b->bin_items, b->nl,
-                                               b->nn, b->flags, nullptr, b->d_rep, b->cell_rep);
+  kb_cell_pack<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_items, b->x, b->y,
+                                                     b->z, b->cpos);
+  k_list_wave<<<b->ncells, 64, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->cpos, b->st_off,
+                                               b->st, b->bin_start, b->bin_items, b->nl, b->nn,
+                                               b->flags, nullptr, b->d_rep, b->cell_rep);
   kb_save_old<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->x, b->y, b->z, b->xo, b->yo, b->zo,
                                                     b->flags);
   BAT_HIP(hipGetLastError());

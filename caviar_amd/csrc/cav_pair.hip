@@ -75,7 +75,15 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
     a.az += obase;
     a.part += r.pbase;
   }
-  const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+  long blk = blockIdx.x;
+  if (!a.rep) {
+    // XCD-aware order (the grid is a multiple of 8 and dispatch is round-robin
+    // over the 8 XCDs): each XCD gets a contiguous run of cell-ordered atoms,
+    // so neighbouring atoms' gathers share one L2
+    const long per = gridDim.x >> 3;
+    blk = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  const long gid = blk * kBlock + threadIdx.x;
   const int t = (int)(gid / kLPA), sub = (int)(gid % kLPA);
   double vir = 0.0;
   T fx = 0, fy = 0, fz = 0;
@@ -250,10 +258,12 @@ static PairArgs pair_args(cav_ctx *c)
   return a;
 }
 
+static int pair_grid(const cav_ctx *c) { return (grid_for((long)c->n * kLPA) + 7) / 8 * 8; }
+
 template <typename T>
 static void launch_pairs_t(cav_ctx *c, const PairArgs &a, bool lj, bool er, bool lam)
 {
-  const int g = grid_for((long)c->n * kLPA);
+  const int g = pair_grid(c);
   if (lj && er && lam)
     k_pairs<T, true, true, true><<<g, kBlock, 0, c->stream>>>(a);
   else if (lj && er)
@@ -282,7 +292,7 @@ int launch_pairs(cav_ctx *c, bool want_lj, bool want_er, bool accumulate)
     launch_pairs_t<float>(c, a, lj, er, c->lambda_set);
   timer_end(c, c->t_pair);
   CAV_HIP(hipGetLastError());
-  return reduce_partials(c, grid_for((long)c->n * kLPA), 0, accumulate);
+  return reduce_partials(c, pair_grid(c), 0, accumulate);
 }
 
 __global__ void k_zero_acc(int n, double *ax, double *ay, double *az)

@@ -27,11 +27,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X dense peaks (vendor figures)
-PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}   # FP64 / FP32 matrix (and vector) peaks
+# MI355X peaks (vendor figures, dense): matrix cores by operand type, vector ALU
+PEAK = {"fp64_mfma": 78.6, "fp32_mfma": 157.3, "fp16_mfma": 2500.0,
+        "fp32_vector": 157.3, "fp64_vector": 78.6}          # TFLOP/s
+HBM_PEAK_GBS = 8000.0
 # Algorithmic FLOP convention per (k-vector, atom) term: S pass 16, force pass 22;
 # per Ewald_r list pair 43, LJ pair +12, integration 20 per atom.
 FLOP_SK, FLOP_KF, FLOP_PAIR, FLOP_LJ, FLOP_INT = 16, 22, 43, 12, 20
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_ions_N262144_kmax16.json")
 
 
 def log(msg):
@@ -90,6 +93,155 @@ def cpu_baseline(system, threads):
                        f"{na} of {n} atoms over all k ({threads} threads), scaled to one full step")}
 
 
+
+def copy_probe(mib):
+    """hipMemcpy device-to-device of ``mib`` MiB through the HIP runtime (its copy
+    kernel reads and writes exactly that many bytes; larger than the caches)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    nbytes = ctypes.c_size_t(mib << 20)
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, nbytes)
+    hip.hipDeviceSynchronize()
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: 3 x {mib} MiB device-to-device")
+
+
+def lattice(kmax):
+    nz = 2 * kmax + 1
+    rpad = -(-((kmax + 1) * (2 * kmax + 1)) // 16) * 16
+    sc = -(-(2 * nz) // 16) * 16
+    kz = -(-(2 * nz) // 4) * 4
+    nkh = -(-(2 * nz) // 32)
+    return rpad, sc, kz, nkh
+
+
+def kernel_rooflines(n, kmax, ppa, lj_ppa, tm32, tm64):
+    """Per kernel family: launch time, algorithmic work per launch, the bound and
+    the achieved fraction of the MI355X unit it runs on, and the MFMA FLOP the
+    kernel actually issues (counted from its tile shapes; rocprofv3's
+    SQ_INSTS_VALU_MFMA_MOPS_* x 512 gives the same totals)."""
+    nk = (2 * kmax) ** 3
+    rpad, sc, kz, nkh = lattice(kmax)
+    traffic = {}
+    if os.path.exists(TRAFFIC_FILE):
+        traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch", {})
+    out = {}
+    for form, tm in (("mixed", tm32), ("fp64", tm64)):
+        split = form == "mixed"
+        mfma = "fp16_mfma" if split else "fp64_mfma"
+        exec_kf = (4 * rpad * 32 * nkh * 2 * 3 if split else 2 * 4 * rpad * kz) * n
+        exec_sk = (2 * rpad * sc * 2 * 3 if split else 2 * rpad * sc * 2) * n
+        rows = {
+            "ewald_k_force": ("mfma", mfma, FLOP_KF * nk * n, exec_kf),
+            "ewald_k_sk": ("mfma", mfma, FLOP_SK * nk * n, exec_sk),
+            "pair": ("gather", "fp32_vector" if split else "fp64_vector",
+                     n * (FLOP_PAIR * ppa + FLOP_LJ * lj_ppa), None),
+        }
+        fo = {}
+        for k, (bound, unit, alg, ex) in rows.items():
+            ms = tm[k][0]
+            ach = alg / (ms * 1e-3) / 1e12
+            d = {"ms": round(ms, 4), "bound": bound, "unit": "TFLOP/s", "peak": PEAK[unit],
+                 "peak_unit": unit, "achieved": round(ach, 2), "frac": round(ach / PEAK[unit], 4)}
+            if ex is not None:
+                d["executed"] = round(ex / (ms * 1e-3) / 1e12, 2)
+                d["executed_frac"] = round(ex / (ms * 1e-3) / 1e12 / PEAK[unit], 4)
+            tb = traffic.get(f"{form}:{k}")
+            if tb:
+                d["traffic"] = tb
+                d["traffic_GBs"] = round(tb / (ms * 1e-3) / 1e9, 1)
+            fo[k] = d
+        for k in ("list", "neighbor", "ewald_k_tables", "integrate"):
+            fo[k] = {"ms": round(tm[k][0], 4)}
+        fo["list"]["bound"] = "fp64 compare + ballot, latency"
+        fo["neighbor"]["bound"] = "hbm / launch"
+        fo["ewald_k_tables"]["bound"] = "hbm write"
+        fo["integrate"]["bound"] = "hbm"
+        out[form] = fo
+    return out
+
+
+def run_batched(args):
+    """The batched-replica workload: replicas with N ~ U[200, 4000] and density
+    ~ U[0.2, 0.8], half +-1 / half neutral, kmax 6, mixed precision, one graph
+    launch per step, the force-matching adjoint every ``adjoint_every`` steps
+    (residual weights synthetic, seeded)."""
+    from caviar_amd import Batch, configs
+    t0 = time.perf_counter()
+    systems = configs.replica_set(args.replicas, (200, 4000), (0.2, 0.8), kmax=6, seed=1)
+    natoms = sum(s.n for s in systems)
+    log(f"batched: {len(systems)} replicas, {natoms} atoms, built in {time.perf_counter() - t0:.1f}s")
+    b = Batch(systems, precision="mixed")
+    b.setup()
+    tiles = b.autotune()
+    rng = np.random.default_rng(7)
+    weights = [rng.normal(size=(s.n, 3)) * 1e-3 for s in systems]
+    b.step(args.dt, 5)
+    steps, every = args.batch_steps, args.adjoint_every
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        b.step(args.dt, k)
+        done += k
+        if every > 0 and done % every == 0:
+            b.adjoint(weights)
+    t = time.perf_counter() - t0
+    # the same without the adjoint, for its share
+    t1 = time.perf_counter()
+    b.step(args.dt, 100)
+    t_plain = (time.perf_counter() - t1) / 100
+    res = {"metric": "atom-steps/s, batched replicas (mixed, adjoint every %d steps)" % every,
+           "value": round(natoms * steps / t, 1), "unit": "atom-steps/s",
+           "replicas": len(systems), "atoms": natoms, "steps": steps, "adjoint_every": every,
+           "ms_per_step": round(1e3 * t / steps, 4),
+           "ms_per_step_without_adjoint": round(1e3 * t_plain, 4),
+           "graph_launch_per_step": True, "tiles": tiles,
+           "config": {"workload": "half-charged LJ + Ewald replicas", "n_range": [200, 4000],
+                      "rho_range": [0.2, 0.8], "kmax": 6, "k_vectors_per_replica": 1728}}
+    log(f"batched: {res['value']:.4e} atom-steps/s, {res['ms_per_step']} ms/step "
+        f"({res['ms_per_step_without_adjoint']} ms without the adjoint), tiles {tiles}")
+    b.close()
+    return res
+
+
+def nve_drift(system, precision, steps, every, dt):
+    """Total energy (kinetic + shifted-force LJ + Ewald_r + Ewald_k) every
+    ``every`` steps of an NVE run: relative drift and fluctuation."""
+    from caviar_amd import Engine
+    e = Engine(system.copy(), precision=precision)
+    e.setup()
+    m = system.mass[system.types][:, None]
+    t, E = [], []
+    t0 = time.perf_counter()
+    for s in range(0, steps + 1, every):
+        if s:
+            e.step(dt, every)
+        v = e.get("vel")
+        ke = 0.5 * float((m * v * v).sum())
+        elj, er, ek = e.energy()
+        t.append(s * dt)
+        E.append(ke + elj + er + ek)
+    E = np.array(E)
+    slope = np.polyfit(np.array(t), E, 1)[0]
+    res = {"steps": steps, "dt": dt, "samples": len(E), "E0": E[0],
+           "rel_drift_end": float((E[-1] - E[0]) / abs(E[0])),
+           "rel_drift_per_1000_steps": float(slope * 1000 * dt / abs(E[0])),
+           "rel_fluctuation_rms": float(np.std(E - np.polyval(np.polyfit(t, E, 1), t)) / abs(E[0])),
+           "seconds": round(time.perf_counter() - t0, 1)}
+    log(f"nve {precision}: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,12 +254,27 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
                     help="run one form only (for profiling); the JSON line needs both")
+    ap.add_argument("--snapshots", type=int, default=10,
+                    help="seeded snapshots for the mixed-vs-FP64 force error")
+    ap.add_argument("--replicas", type=int, default=512,
+                    help="batched workload: replicas (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--adjoint-every", type=int, default=10)
+    ap.add_argument("--nve-steps", type=int, default=0,
+                    help="also measure NVE energy drift over this many steps (both forms)")
+    ap.add_argument("--nve-every", type=int, default=100)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device with the HIP runtime (a kernel of "
+                         "known traffic, to calibrate byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not part of this build)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     from caviar_amd import configs
     t0 = time.perf_counter()
     system = configs.ions(n=args.n, rho=0.3, kmax=args.kmax, seed=args.seed)
@@ -119,55 +286,34 @@ def main():
     e64, t64, tm64 = run_engine(system, "fp64", args.warmup, args.steps, args.dt)
     e32, t32, tm32 = run_engine(system, "mixed", args.warmup, args.steps, args.dt)
 
-    # force error of the mixed form on identical positions (initial and after the FP64 run)
+    # force error of the mixed form against FP64 on identical positions: 10 seeded snapshots
     errs = []
-    for pos in (system.pos, e64.get("pos")):
+    for k in range(args.snapshots):
+        snap = configs.ions(n=args.n, rho=0.3, kmax=args.kmax, seed=1000 + k).pos
         for e in (e64, e32):
-            e.set("pos", pos)
+            e.set("pos", snap)
             e.recalculate(force_rebuild=True)
         errs.append(force_rms(e32.get("acc"), e64.get("acc")))
     ppa, lj_ppa = e64.list_stats()
-    log(f"force rms err mixed vs fp64: {errs}; pairs/atom {ppa:.2f}, lj pairs/atom {lj_ppa:.2f}")
+    log(f"force rms err mixed vs fp64 over {len(errs)} snapshots: max {max(errs):.3e}, "
+        f"mean {np.mean(errs):.3e}; pairs/atom {ppa:.2f}, lj pairs/atom {lj_ppa:.2f}")
 
     n, K = system.n, args.steps
     value = n * K / t32
     value64 = n * K / t64
     nk = (2 * args.kmax) ** 3
-    # dominant kernel of the mixed step and its algorithmic roofline
-    fam = {"ewald_k_force": FLOP_KF * nk * n, "ewald_k_sk": FLOP_SK * nk * n}
-    dom = max(fam, key=lambda k: tm32[k][0])
-    dom_ms = tm32[dom][0]
-    achieved = fam[dom] / (dom_ms * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS["mixed"],
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS["mixed"], 4), "traffic": None,
-            "kernel": dom, "ms_per_launch": round(dom_ms, 4)}
+    kernels = kernel_rooflines(n, args.kmax, ppa, lj_ppa, tm32, tm64)
+    # dominant kernel of the mixed step
+    dom = max(("ewald_k_force", "ewald_k_sk", "pair", "list"), key=lambda k: tm32[k][0])
+    kd = kernels["mixed"][dom]
+    roof = {"bound": kd["bound"], "achieved": kd["achieved"], "peak": kd["peak"], "unit": kd["unit"],
+            "frac": kd["frac"], "traffic": kd.get("traffic"), "kernel": dom,
+            "ms_per_launch": kd["ms"], "executed": kd.get("executed"),
+            "executed_frac": kd.get("executed_frac"),
+            "note": "achieved = algorithmic FLOP per launch / measured launch time; peak = the "
+                    "MI355X unit the kernel runs on; executed = MFMA FLOP actually issued"}
     step_flop = n * (FLOP_PAIR * ppa + FLOP_LJ * lj_ppa + (FLOP_SK + FLOP_KF) * nk + FLOP_INT)
 
-    out = {
-        "metric": "atom-steps/s LJ+Ewald N=256k, mixed vs FP64 GPU at force RMS err <=1e-5",
-        "value": round(value, 1),
-        "unit": "atom-steps/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t32 / K, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32/fp64 mixed",
-        "data": "synthetic",
-        "config": {"workload": "charged LJ + Ewald ions, N=262144, rho=0.3, kmax=16, mixed precision",
-                   "n_atoms": n, "kmax": args.kmax, "k_vectors": nk, "dt": args.dt,
-                   "parallelism": "single GPU"},
-        "fp64": {"value": round(value64, 1), "ms_per_step": round(1e3 * t64 / K, 4),
-                 "mixed_over_fp64": round(value / value64, 3)},
-        "force_rms_rel_err": max(errs),
-        "algorithmic_tflops_per_step": {"mixed": round(step_flop / (t32 / K) / 1e12, 2),
-                                        "fp64": round(step_flop / (t64 / K) / 1e12, 2)},
-        "kernel_ms": {"mixed": {k: round(v[0], 4) for k, v in tm32.items()},
-                      "fp64": {k: round(v[0], 4) for k, v in tm64.items()}},
-        "roofline": roof,
-    }
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         threads = max(1, min(threads, 16))

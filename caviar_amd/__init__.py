@@ -109,6 +109,7 @@ def load_library():
         "cav_batch_get_pairs": [P, I, ip, L, ctypes.POINTER(L)],
         "cav_batch_autotune": [P],
         "cav_batch_tiles": [P, ip, ip, ip, ip],
+        "cav_batch_adjoint": [P, dp, dp, dp, dp],
         "cav_reset_acceleration": [P],
         "cav_reset_virial": [P],
         "cav_forces": [P, I],
@@ -449,6 +450,21 @@ class Batch:
     def autotune(self) -> dict:
         self._check(self.lib.cav_batch_autotune(self.h))
         return self.tiles()
+
+    def adjoint(self, weights):
+        """Force-matching adjoint of every replica: ``weights`` is a list of
+        (n_r, 3) arrays; returns a list of (g_eps, g_sigma, g_charge) per replica."""
+        nt = self.systems[0].ntypes
+        B = len(self.systems)
+        w = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3)
+                                                 for x in weights]))
+        ge = np.zeros(B * nt * nt)
+        gs = np.zeros(B * nt * nt)
+        gq = np.zeros(B * nt)
+        self._check(self.lib.cav_batch_adjoint(self.h, _dp(w), _dp(ge), _dp(gs), _dp(gq)))
+        return [(ge[r * nt * nt:(r + 1) * nt * nt].reshape(nt, nt),
+                 gs[r * nt * nt:(r + 1) * nt * nt].reshape(nt, nt), gq[r * nt:(r + 1) * nt])
+                for r in range(B)]
 
     def tiles(self) -> dict:
         v = [ctypes.c_int() for _ in range(4)]

@@ -10,6 +10,7 @@
 // reference has no differentiation; this is the gradient of its force formulas
 // (lj_cell_list.cpp:143-155, electrostatic_ewald_r.cpp:183-211,
 // electrostatic_ewald_k.cpp:197-226).
+#include <algorithm>
 #include <cmath>
 
 #include "cav_internal.hpp"
@@ -21,6 +22,8 @@ int ewald_k_weighted_sk(cav_ctx *c, const double *weights, double *sk_out);
 int ewald_k_field(cav_ctx *c, double *out4);
 
 constexpr int kLPAa = 16;
+
+static int roundup16(int v) { return (v + 15) / 16 * 16; }
 
 struct AdjArgs {
   int n, nt, cap;
@@ -34,6 +37,7 @@ struct AdjArgs {
   double alpha, a2, ke, two_a_rpi;
   const double *lambda, *charge;
   double *part;  // [block][nvals]: g_eps[nt2] | g_sig[nt2] | g_q[nt]
+  const RepDev *rep;  // batched replicas (blockIdx.y) or nullptr
 };
 
 // One list entry: force on i is coef * (r_j - r_i); its derivative along the
@@ -42,6 +46,16 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_pairs(AdjArgs a)
 {
   extern __shared__ double sh[];  // [kBlock/kLPAa][nvals] per-atom-group partials
   const int nt2 = a.nt * a.nt, nv = 2 * nt2 + a.nt;
+  if (a.rep) {  // one replica per grid row, arrays shifted to it
+    const RepDev &r = a.rep[blockIdx.y];
+    a.n = r.n;
+    a.xq += r.base;
+    a.tid += r.base;
+    a.nl += (long)r.obase * a.cap;
+    a.nn += r.obase;
+    a.w += 3L * r.obase;
+    a.part += (long)blockIdx.y * gridDim.x * nv;
+  }
   const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
   const int t = (int)(gid / kLPAa), sub = (int)(gid % kLPAa);
   double gl[2 * 16 + 4];  // per-lane accumulators (nt <= 4)
@@ -185,6 +199,89 @@ __global__ void k_fill_weights(int n, int mode, int s, const int *type, const do
   u[i] = mode < 0 ? (type[i] == s ? 1.0 : 0.0) : q[i] * w[3 * i + mode];
 }
 
+
+// ---------------------------------------------------------- batched replicas --
+
+__global__ void kb_reduce_cols(int rows, int cols, const double *part, double *out)
+{
+  const int v = threadIdx.x;
+  if (v >= cols) return;
+  const double *p = part + (long)blockIdx.x * rows * cols;
+  double x = 0;
+  for (int r = 0; r < rows; ++r) x += p[(long)r * cols + v];
+  out[(long)blockIdx.x * cols + v] = x;
+}
+
+// k_adjoint_kspace for every replica (grid row): planes [B][Rpad][SC]
+__global__ void kb_adjoint_kspace(const RepDev *rep, int R, int Rpad, int SC, int nY, int nZ, int ky,
+                                  int kz, int nt, const double *W, const double *ss,
+                                  const double *sw, double *part)
+{
+  const int b = blockIdx.y, B = gridDim.y;
+  const RepDev &rr = rep[b];
+  const double kxs = rr.kxs, kys = rr.kys, kzs = rr.kzs;
+  W += (long)b * R * nZ;
+  const long plane = (long)Rpad * SC;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ double red[kBlock];
+  for (int s = 0; s < nt; ++s) {
+    double v = 0;
+    if (t < R * nZ) {
+      const int r = t / nZ, c = t - r * nZ;
+      const double w = W[(long)r * nZ + c];
+      if (w != 0) {
+        const int nx = r / nY, ny = r % nY - ky, nz = c - kz;
+        const long o = (long)r * SC + 2 * c;
+        const double *w0 = sw + ((long)0 * B + b) * plane, *w1 = sw + ((long)1 * B + b) * plane,
+                     *w2 = sw + ((long)2 * B + b) * plane;
+        const double tr = kxs * nx * w0[o] + kys * ny * w1[o] + kzs * nz * w2[o];
+        const double ti = kxs * nx * w0[o + 1] + kys * ny * w1[o + 1] + kzs * nz * w2[o + 1];
+        const double *sp = ss + ((long)s * B + b) * plane;
+        v = w * (sp[o] * ti - sp[o + 1] * tr);
+      }
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int q = kBlock / 2; q > 0; q >>= 1) {
+      if (threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[((long)b * gridDim.x + blockIdx.x) * nt + s] = red[0];
+    __syncthreads();
+  }
+}
+
+// k_adjoint_field for every replica (grid row)
+__global__ void kb_adjoint_field(const RepDev *rep, int nt, const double *w, const double *f4,
+                                 const int *type_o, double *part)
+{
+  const RepDev &r = rep[blockIdx.y];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ double red[kBlock];
+  for (int s = 0; s < nt; ++s) {
+    double v = 0;
+    if (i < r.n) {
+      const long o = r.obase + i;
+      if (type_o[o] == s) v = w[3 * o] * f4[4 * o] + w[3 * o + 1] * f4[4 * o + 1] + w[3 * o + 2] * f4[4 * o + 2];
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int q = kBlock / 2; q > 0; q >>= 1) {
+      if (threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[((long)blockIdx.y * gridDim.x + blockIdx.x) * nt + s] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void kb_fill_weights(long nown, int mode, int s, const int *type_o, const double *q_o,
+                                const double *w, double *u)
+{
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nown) return;
+  u[i] = mode < 0 ? (type_o[i] == s ? 1.0 : 0.0) : q_o[i] * w[3 * i + mode];
+}
 }  // namespace cav
 
 using namespace cav;
@@ -233,6 +330,7 @@ extern "C" int cav_adjoint(cav_ctx *c, const double *w_host, double *g_eps, doub
     a.lambda = c->d_lambda;
     a.charge = c->d_charge;
     a.part = dpart;
+    a.rep = nullptr;
     const size_t sh = sizeof(double) * (kBlock / kLPAa) * nv;
     k_adjoint_pairs<<<nblk, kBlock, sh, c->stream>>>(a);
     k_reduce_cols<<<1, 64, 0, c->stream>>>(nblk, nv, dpart, dout);
@@ -287,5 +385,105 @@ extern "C" int cav_adjoint(cav_ctx *c, const double *w_host, double *g_eps, doub
     for (int p = 0; p < nt2; ++p) g_sigma[p] = hout[nt2 + p];
   if (g_charge)
     for (int s = 0; s < nt; ++s) g_charge[s] = hout[2 * nt2 + s];
+  return CAV_OK;
+}
+
+// Force-matching adjoint of every replica at its current positions: w holds
+// the replicas' residual weights one after another (3 n_r doubles each, in
+// replica order); outputs are [B][nt*nt], [B][nt*nt] and [B][nt].
+extern "C" int cav_batch_adjoint(cav_batch *b, const double *w_host, double *g_eps, double *g_sigma,
+                                 double *g_charge)
+{
+  if (!b || !w_host) return CAV_ERR_ARG;
+  if (!b->setup_done) return bfail(b, CAV_ERR_STATE, "cav_batch_setup must run first");
+  if (b->nt > 4) return bfail(b, CAV_ERR_ARG, "the adjoint supports up to 4 atom types");
+  (void)hipSetDevice(b->device);
+  const int nt = b->nt, nt2 = nt * nt, nv = 2 * nt2 + nt, B = b->B;
+  const long nown = b->nown;
+  const int pb = grid_for((long)b->maxn * kLPAa);
+  const EwaldKPlan &p = b->ek;
+  const long plane = b->ek_on ? (long)p.Rpad * roundup16(2 * p.nZ) : 1;
+  const int kb = b->ek_on ? grid_for((long)p.R * p.nZ) : 1, fb = grid_for(b->maxn);
+  double *dw = nullptr, *dpart = nullptr, *dout = nullptr, *u = nullptr, *ss = nullptr, *f4 = nullptr;
+  const size_t npart = (size_t)B * std::max<long>((long)pb * nv, (long)(kb + fb) * nt) + 64;
+  BAT_HIP(hipMalloc(&dw, sizeof(double) * 3 * nown));
+  BAT_HIP(hipMalloc(&dpart, sizeof(double) * npart));
+  BAT_HIP(hipMalloc(&dout, sizeof(double) * (size_t)B * (nv + 2 * nt)));
+  BAT_HIP(hipMemcpyAsync(dw, w_host, sizeof(double) * 3 * nown, hipMemcpyHostToDevice, b->stream));
+  std::vector<double> hpair((size_t)B * nv, 0.0), hk((size_t)B * nt, 0.0), hf((size_t)B * nt, 0.0);
+  if (b->lj_on || b->er_on) {
+    AdjArgs a;
+    a.n = 0;
+    a.nt = nt;
+    a.cap = b->cap;
+    a.xq = b->xq;
+    a.tid = b->tid;
+    a.mol = nullptr;
+    a.nl = b->nl;
+    a.nn = b->nn;
+    a.w = dw;
+    a.eps = b->d_raw_eps;
+    a.sig = b->d_raw_sig;
+    a.lj_rc = b->lj_rc;
+    a.lj_rc2 = b->lj_rc * b->lj_rc;
+    a.lj_on = b->lj_on;
+    a.er_on = b->er_on;
+    a.lambda_set = 0;
+    a.alpha = b->er_alpha;
+    a.a2 = b->er_alpha * b->er_alpha;
+    a.ke = b->er_ke;
+    a.two_a_rpi = 2 * b->er_alpha * CAV_PIS_INV;
+    a.lambda = nullptr;
+    a.charge = b->d_charge;
+    a.part = dpart;
+    a.rep = b->d_rep;
+    const size_t sh = sizeof(double) * (kBlock / kLPAa) * nv;
+    k_adjoint_pairs<<<dim3(pb, B), kBlock, sh, b->stream>>>(a);
+    kb_reduce_cols<<<B, 64, 0, b->stream>>>(pb, nv, dpart, dout);
+    BAT_HIP(hipGetLastError());
+    BAT_HIP(hipMemcpyAsync(hpair.data(), dout, sizeof(double) * B * nv, hipMemcpyDeviceToHost, b->stream));
+    BAT_HIP(hipStreamSynchronize(b->stream));
+  }
+  if (b->ek_on) {
+    BAT_HIP(hipMalloc(&u, sizeof(double) * nown));
+    BAT_HIP(hipMalloc(&ss, sizeof(double) * plane * B * (nt + 3)));
+    BAT_HIP(hipMalloc(&f4, sizeof(double) * 4 * nown));
+    CAV_TRY(batch_ewald_k_field(b, f4));
+    for (int s = 0; s < nt; ++s) {
+      kb_fill_weights<<<grid_for(nown), kBlock, 0, b->stream>>>(nown, -1, s, b->type_o, b->q_o, dw, u);
+      CAV_TRY(batch_weighted_sk(b, u, ss + (size_t)s * B * plane));
+    }
+    double *sw = ss + (size_t)nt * B * plane;
+    for (int d = 0; d < 3; ++d) {
+      kb_fill_weights<<<grid_for(nown), kBlock, 0, b->stream>>>(nown, d, 0, b->type_o, b->q_o, dw, u);
+      CAV_TRY(batch_weighted_sk(b, u, sw + (size_t)d * B * plane));
+    }
+    kb_adjoint_kspace<<<dim3(kb, B), kBlock, 0, b->stream>>>(b->d_rep, p.R, p.Rpad, roundup16(2 * p.nZ),
+                                                             p.nY, p.nZ, p.ky, p.kz, nt, b->d_W, ss,
+                                                             sw, dpart);
+    kb_reduce_cols<<<B, 64, 0, b->stream>>>(kb, nt, dpart, dout + (size_t)B * nv);
+    double *dpart2 = dpart + (size_t)B * kb * nt;
+    kb_adjoint_field<<<dim3(fb, B), kBlock, 0, b->stream>>>(b->d_rep, nt, dw, f4, b->type_o, dpart2);
+    kb_reduce_cols<<<B, 64, 0, b->stream>>>(fb, nt, dpart2, dout + (size_t)B * (nv + nt));
+    BAT_HIP(hipGetLastError());
+    BAT_HIP(hipMemcpyAsync(hk.data(), dout + (size_t)B * nv, sizeof(double) * B * nt, hipMemcpyDeviceToHost, b->stream));
+    BAT_HIP(hipMemcpyAsync(hf.data(), dout + (size_t)B * (nv + nt), sizeof(double) * B * nt,
+                           hipMemcpyDeviceToHost, b->stream));
+    BAT_HIP(hipStreamSynchronize(b->stream));
+  }
+  for (int r = 0; r < B; ++r) {
+    const double C = b->ek_on ? b->reps[r].c0 : 0.0;  // 4 pi ke / V of this replica
+    for (int q = 0; q < nt2; ++q) {
+      if (g_eps) g_eps[(size_t)r * nt2 + q] = hpair[(size_t)r * nv + q];
+      if (g_sigma) g_sigma[(size_t)r * nt2 + q] = hpair[(size_t)r * nv + nt2 + q];
+    }
+    for (int s = 0; s < nt; ++s)
+      if (g_charge)
+        g_charge[(size_t)r * nt + s] = hpair[(size_t)r * nv + 2 * nt2 + s] + C * hk[(size_t)r * nt + s] +
+                                       hf[(size_t)r * nt + s];
+  }
+  void *fr[] = {dw, dpart, dout, u, ss, f4};
+  for (void *q : fr)
+    if (q) (void)hipFree(q);
   return CAV_OK;
 }

@@ -161,7 +161,8 @@ static void batch_free(cav_batch *b)
                   b->aox, b->aoy, b->aoz, b->type_o, b->q_o, b->snap, b->bin_of, b->bin_cnt,
                   b->bin_start, b->bin_items, b->gblk, b->nl, b->nn, b->flags, b->d_minv, b->d_lj,
                   b->part, b->vir, b->d_W, b->tab, b->ztab, b->bmat, b->bmat_h, b->sk_part, b->sk,
-                  b->epart, b->s_items, b->f_items, b->rchunks, b->cpos};
+                  b->epart, b->s_items, b->f_items, b->rchunks, b->cpos, b->d_raw_eps,
+                  b->d_raw_sig, b->d_charge};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->h_flags) (void)hipHostFree(b->h_flags);
@@ -320,6 +321,17 @@ static int batch_layout(cav_batch *b)
   }
   CAV_TRY(balloc(b, &b->d_lj, lj.size()));
   CAV_TRY(balloc(b, &b->d_minv, nt));
+  std::vector<double> re(std::max(nt2, 1), 0.0), rs(std::max(nt2, 1), 0.0);
+  if (b->lj_on) {
+    re = b->eps;
+    rs = b->sig;
+  }
+  CAV_TRY(balloc(b, &b->d_raw_eps, re.size()));
+  CAV_TRY(balloc(b, &b->d_raw_sig, rs.size()));
+  CAV_TRY(balloc(b, &b->d_charge, nt));
+  BAT_HIP(hipMemcpy(b->d_raw_eps, re.data(), sizeof(double) * re.size(), hipMemcpyHostToDevice));
+  BAT_HIP(hipMemcpy(b->d_raw_sig, rs.data(), sizeof(double) * rs.size(), hipMemcpyHostToDevice));
+  BAT_HIP(hipMemcpy(b->d_charge, b->charge.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
   BAT_HIP(hipMemcpy(b->d_lj, lj.data(), sizeof(double) * lj.size(), hipMemcpyHostToDevice));
   BAT_HIP(hipMemcpy(b->d_minv, b->minv.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
 

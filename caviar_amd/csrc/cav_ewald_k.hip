@@ -1976,6 +1976,109 @@ static int batch_ewald_k_t(cav_batch *b)
   return CAV_OK;
 }
 
+
+// Per replica, fixed-order sum of its S-pass chunk partials into [B][plane].
+__global__ void kb_chunk_sum(const int2 *rchunks, const double *part, long plane, double *out)
+{
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long e = (long)blockIdx.x * 64 + lane;
+  const int2 rc = rchunks[blockIdx.y];
+  double s = 0;
+  if (e < plane)
+    for (int k = wave; k < rc.y; k += 4) s += part[(long)(rc.x + k) * plane + e];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && e < plane)
+    out[(long)blockIdx.y * plane + e] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+// Per-atom k-space field of every replica (C sum k Im(g e^{ik.r}) and the
+// potential, without charge or mass) into out4[owned][4], from the operand of
+// the last force evaluation; rebuilds full FP32/FP64 phase tables first.
+template <typename T>
+static int batch_field_t(cav_batch *b, double *out4)
+{
+  const EwaldKPlan &p = b->ek;
+  const int TS = p.nX + p.nY + p.nZ;
+  kb_tables<T><<<dim3(grid_for((long)b->maxn * TS), b->B), kBlock, 0, b->stream>>>(
+      b->d_rep, p.kx, p.ky, p.kz, b->x, b->y, b->z, (Cx<T> *)b->tab);
+  FArgs f;
+  f.n = 0;
+  f.TS = TS;
+  f.nX = p.nX;
+  f.nY = p.nY;
+  f.nZ = p.nZ;
+  f.ky = p.ky;
+  f.R = p.R;
+  f.Rpad = p.Rpad;
+  f.KZ = p.KZ;
+  f.KSP = p.KSP;
+  f.tab = b->tab;
+  f.bm = b->bmat;
+  f.q = b->q_o;
+  f.minv = b->d_minv;
+  f.type = b->type_o;
+  f.c0 = f.kxs = f.kys = f.kzs = 0;
+  f.ax = f.ay = f.az = nullptr;
+  f.out4 = out4;
+  f.items = b->f_items;
+  f.rep = b->d_rep;
+  f.bm_stride = b->bm_stride * (long)sizeof(T);
+  launch_kforce_t<T>(b->stream, f, p.KZ / 4, b->n_fitems);
+  BAT_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+int batch_ewald_k_field(cav_batch *b, double *out4)
+{
+  if (b->precision == CAV_FP64) return batch_field_t<double>(b, out4);
+  return batch_field_t<float>(b, out4);
+}
+
+// S over the half lattice with per-atom weights u (owned-indexed) for every
+// replica, into out[B][Rpad][SC] (re, im at columns 2c, 2c+1).  Uses the full
+// phase tables of batch_ewald_k_field.
+template <typename T>
+static int batch_wsk_t(cav_batch *b, const double *u, double *out)
+{
+  const EwaldKPlan &p = b->ek;
+  const int TS = p.nX + p.nY + p.nZ;
+  const int SC = roundup(2 * p.nZ, 16);
+  int groups, tpg, waves;
+  s_layout_rt(p.Rpad, kRT, &groups, &tpg, &waves);
+  SArgs a;
+  a.n = 0;
+  a.TS = TS;
+  a.nX = p.nX;
+  a.nY = p.nY;
+  a.nZ = p.nZ;
+  a.ky = p.ky;
+  a.kz = p.kz;
+  a.Rtiles = p.Rpad / 16;
+  a.chunk = b->s_chunk;
+  a.stage = 2 * 32 * (size_t)TS * 2 * sizeof(T) + 2 * 32 * sizeof(T) <= 150 * 1024 ? 32 : 16;
+  a.tiles_per_group = tpg;
+  a.q = u;
+  a.tab = b->tab;
+  a.part = b->sk_part;
+  a.Rpad = p.Rpad;
+  a.SC = SC;
+  a.items = b->s_items;
+  const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
+  launch_sk_rt<T, kRT>(b->stream, a, dim3(b->n_sitems, groups), smem, SC / 16, waves);
+  const long plane = (long)p.Rpad * SC;
+  kb_chunk_sum<<<dim3((plane + 63) / 64, b->B), kBlock, 0, b->stream>>>(b->rchunks, b->sk_part, plane, out);
+  BAT_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+int batch_weighted_sk(cav_batch *b, const double *u, double *out)
+{
+  if (b->precision == CAV_FP64) return batch_wsk_t<double>(b, u, out);
+  return batch_wsk_t<float>(b, u, out);
+}
+
 // Ewald_k accelerations of every replica, added into the owned accelerations.
 int batch_ewald_k(cav_batch *b)
 {

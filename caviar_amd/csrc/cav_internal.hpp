@@ -234,6 +234,7 @@ struct cav_batch {
   int cap = 0;
   int *flags = nullptr, *h_flags = nullptr;  // [B][F_COUNT]
   double *d_minv = nullptr, *d_lj = nullptr;
+  double *d_raw_eps = nullptr, *d_raw_sig = nullptr, *d_charge = nullptr;  // for the adjoint
   double *part = nullptr;  // [B][pair_blocks] virial partials
   double *vir = nullptr;   // [B]
 
@@ -298,6 +299,8 @@ int batch_pairs(cav_batch *b);                   // pack + LJ/Ewald_r + virial
 int batch_ewald_k_plan(cav_batch *b);
 int batch_ewald_k_alloc(cav_batch *b);
 int batch_ewald_k(cav_batch *b);
+int batch_ewald_k_field(cav_batch *b, double *out4);
+int batch_weighted_sk(cav_batch *b, const double *u, double *out);
 
 inline int grid_for(long n, int block = kBlock) { return (int)((n + block - 1) / block); }
 

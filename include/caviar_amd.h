@@ -200,6 +200,11 @@ int cav_batch_get_pairs(cav_batch *b, int replica, int *ij, long capacity, long 
  * row tiles per wave, atoms per LDS stage, atoms per chunk; force-pass atom
  * tiles per wave). */
 int cav_batch_autotune(cav_batch *b);
+/* cav_adjoint for every replica at its current positions: weights holds the
+ * replicas' residual weights one after another (3 n_r doubles each, replica
+ * order); outputs are [replica][nt*nt], [replica][nt*nt], [replica][nt]. */
+int cav_batch_adjoint(cav_batch *b, const double *weights, double *g_epsilon, double *g_sigma,
+                      double *g_charge);
 int cav_batch_tiles(const cav_batch *b, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles);
 
 /* ---- measurement ----------------------------------------------------------- */

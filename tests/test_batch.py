@@ -118,3 +118,23 @@ def test_autotune_keeps_results(gpu):
     b.step(0.002, 5)
     for r, s in enumerate(systems):
         assert rms(b.get(r, "acc"), single(s, steps=5).get("acc")) <= 1e-4, r
+
+
+@pytest.mark.parametrize("precision, tol", [("fp64", 1e-10), ("mixed", 1e-5)])
+def test_batch_adjoint_matches_single_systems(gpu, precision, tol):
+    """The batched force-matching adjoint of every replica against the
+    single-system adjoint (itself pinned by central differences and the FP64
+    CPU reference)."""
+    systems = ion_replicas()[:4]
+    print(f"[case] batch adjoint {precision}", flush=True)
+    b = Batch(systems, precision=precision)
+    b.setup()
+    rng = np.random.default_rng(2)
+    weights = [rng.normal(size=(s.n, 3)) for s in systems]
+    got = b.adjoint(weights)
+    for r, s in enumerate(systems):
+        e = Engine(s.copy(), precision="fp64")
+        e.setup()
+        want = e.adjoint(weights[r])
+        for g, w in zip(got[r], want):
+            assert np.linalg.norm(g - w) <= tol * max(np.linalg.norm(w), 1e-300), (r, g, w)

@@ -110,6 +110,10 @@ def load_library():
         "cav_batch_autotune": [P],
         "cav_batch_tiles": [P, ip, ip, ip, ip],
         "cav_batch_adjoint": [P, dp, dp, dp, dp],
+        "cav_autotune": [P],
+        "cav_tiles": [P, ip, ip, ip, ip],
+        "cav_set_pair_kernel": [P, I],
+        "cav_pair_kernel": [P, ip],
         "cav_reset_acceleration": [P],
         "cav_reset_virial": [P],
         "cav_forces": [P, I],
@@ -227,6 +231,28 @@ class Engine:
         r = ctypes.c_int()
         self._check(self.lib.cav_neighbor_build(self.h, int(update), ctypes.byref(r)))
         return bool(r.value)
+
+    # ---- tuning
+    def autotune(self) -> dict:
+        """Pick the Ewald_k tile shapes by timing the candidates on this system."""
+        self._check(self.lib.cav_autotune(self.h))
+        return self.tiles()
+
+    def tiles(self) -> dict:
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(self.lib.cav_tiles(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("s_rows", "s_stage", "s_chunk", "f_tiles"), (x.value for x in v)))
+
+    PAIR_KERNELS = {"auto": 0, "rows": 1, "cells": 2}
+
+    def set_pair_kernel(self, kind: str):
+        """'auto' (cell-staged where it applies), 'rows' or 'cells'."""
+        self._check(self.lib.cav_set_pair_kernel(self.h, self.PAIR_KERNELS[kind]))
+
+    def pair_kernel(self) -> str:
+        v = ctypes.c_int()
+        self._check(self.lib.cav_pair_kernel(self.h, ctypes.byref(v)))
+        return {1: "rows", 2: "cells"}[v.value]
 
     # ---- the pieces of a step, one object at a time
     def exchange(self) -> bool:

@@ -182,7 +182,8 @@ static void free_all(cav_ctx *c)
                   c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
                   c->d_W,     c->kacc,      c->order,    c->ord_blk,  c->xq,
-                  c->tid,     c->d_raw_eps, c->d_raw_sig, c->ztab, c->bmat_h, c->cpos, c->sk_sum};
+                  c->tid,     c->d_raw_eps, c->d_raw_sig, c->ztab, c->bmat_h, c->cpos, c->sk_sum,
+                  c->binpos,  c->xqs,    c->tids,       c->nll,    c->lstart, c->litems};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -292,6 +293,7 @@ static int neighbor_init(cav_ctx *c)
   CAV_HIP(hipMemcpyAsync(c->d_st, c->st.data(), sizeof(int) * c->st.size(), hipMemcpyHostToDevice, c->stream));
   CAV_TRY(dmalloc(c, &c->bin_cnt, c->nbins));
   CAV_TRY(dmalloc(c, &c->bin_start, c->nbins + 1));
+  CAV_TRY(dmalloc(c, &c->lstart, c->nbins + 1));
   CAV_TRY(dmalloc(c, &c->scan_tmp, grid_for(c->nbins) + 1));
   return CAV_OK;
 }
@@ -300,6 +302,7 @@ static int alloc_list(cav_ctx *c, int cap)
 {
   c->nl_cap = cap;
   CAV_TRY(dmalloc(c, &c->nl, (size_t)cap * c->n));
+  CAV_TRY(dmalloc(c, &c->nll, (size_t)cap * c->n));
   return CAV_OK;
 }
 
@@ -337,6 +340,8 @@ static int refresh_params(cav_ctx *c)
 }
 
 // The force phase: reset acceleration + every enabled field (md_simulator.cpp:370-373).
+static int forces(cav_ctx *c);
+int forces_all(cav_ctx *c) { return forces(c); }
 static int forces(cav_ctx *c)
 {
   const bool pair = (c->lj_on && !c->lj_cellwalk) || c->er_on;
@@ -609,6 +614,10 @@ int cav_upload_atoms(cav_ctx *c, int n, const int *id, const int *type, const in
   CAV_TRY(dmalloc(c, &c->bin_of, tot));
   CAV_TRY(dmalloc(c, &c->bin_items, tot));
   CAV_TRY(dmalloc(c, &c->cpos, tot));
+  CAV_TRY(dmalloc(c, &c->binpos, tot));
+  CAV_TRY(dmalloc(c, &c->litems, tot));
+  CAV_TRY(dmalloc(c, &c->xqs, tot));
+  CAV_TRY(dmalloc(c, &c->tids, tot));
   CAV_TRY(dmalloc(c, &c->nn, n));
   CAV_TRY(dmalloc(c, &c->order, n));
   CAV_TRY(dmalloc(c, &c->xq, tot));
@@ -716,8 +725,13 @@ int cav_setup(cav_ctx *c)
   // LJ over the list needs rc_lj <= the list's own cutoff (the skin guarantee of
   // the rebuild test); otherwise it walks the cells every step like the reference.
   c->lj_cellwalk = c->lj_on && (c->lj_rc > c->nl_cutoff || c->all_atom_test);
-  const int npart = (int)std::max<long>(3L * grid_for(std::max<long>((long)c->n, 1L << 16)),
-                                        (long)grid_for(16L * c->n)) + 64;
+  const long cells = (c->nbins + 7) / 8 * 8;  // one partial per cell of the cell-staged kernel
+  // one partial per block of the row-gather pair kernel (kLPA lanes per atom,
+  // grid rounded up to a multiple of 8), per cell of the cell-staged kernel
+  const long pair_blocks = (grid_for((long)kLPA * c->n) + 7) / 8 * 8;
+  const int npart = (int)std::max<long>(std::max<long>(3L * grid_for(std::max<long>((long)c->n, 1L << 16)),
+                                                       pair_blocks),
+                                        cells) + 64;
   CAV_TRY(dmalloc(c, &c->part, npart));
   c->npart = npart;
   if (c->ek_on) {
@@ -875,6 +889,45 @@ int cav_forces(cav_ctx *c, int mask)
   if (lj && c->lj_cellwalk) CAV_TRY(launch_lj_cellwalk(c));
   if ((mask & CAV_FF_EWALD_K) && c->ek_on) CAV_TRY(launch_ewald_k(c));
   CAV_HIP(hipStreamSynchronize(c->stream));
+  return CAV_OK;
+}
+
+int cav_autotune(cav_ctx *c)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  if (!c->ek_on) return CAV_OK;
+  (void)hipSetDevice(c->device);
+  CAV_TRY(ewald_k_autotune(c));
+  // the tuning launches added k-space terms: recompute the forces of the current state
+  CAV_TRY(forces(c));
+  CAV_TRY(sync_flags(c));
+  return CAV_OK;
+}
+
+int cav_tiles(const cav_ctx *c, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (s_rows) *s_rows = c->tune_rt;
+  if (s_stage) *s_stage = c->tune_stage;
+  if (s_chunk) *s_chunk = c->tune_chunk;
+  if (f_tiles) *f_tiles = c->tune_fnt;
+  return CAV_OK;
+}
+
+int cav_set_pair_kernel(cav_ctx *c, int kind)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (kind < CAV_PAIRS_AUTO || kind > CAV_PAIRS_CELLS)
+    return fail(c, CAV_ERR_ARG, "pair kernel must be CAV_PAIRS_AUTO, _ROWS or _CELLS");
+  c->pair_kind = kind;
+  return CAV_OK;
+}
+
+int cav_pair_kernel(const cav_ctx *c, int *kind)
+{
+  if (!c || !kind) return CAV_ERR_ARG;
+  *kind = cell_staged(c) ? CAV_PAIRS_CELLS : CAV_PAIRS_ROWS;
   return CAV_OK;
 }
 

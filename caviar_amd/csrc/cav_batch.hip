@@ -263,7 +263,7 @@ static int batch_layout(cav_batch *b)
   b->ncells = cells;
   b->maxn = maxn;
   b->scan_blocks = grid_for(maxn);
-  b->pair_blocks = grid_for((long)maxn * 16);
+  b->pair_blocks = grid_for((long)maxn * kLPA);
   for (int r = 0; r < b->B; ++r) b->reps[r].pbase = r * b->pair_blocks;
   if (b->cap < (int)list_need) b->cap = (int)((list_need + 3) / 4 * 4);
   CAV_TRY(batch_ewald_k_plan(b));

@@ -1108,6 +1108,18 @@ static void s_layout(const cav_ctx *c, int *groups, int *tiles_per_group, int *w
   *waves = tpg / kRT;
 }
 
+static void s_layout_rt(int Rpad, int RT, int *groups, int *tiles_per_group, int *waves)
+{
+  const int tiles = Rpad / 16;
+  const int per_max = kWavesMax * RT;
+  const int g = (tiles + per_max - 1) / per_max;
+  int tpg = (tiles + g - 1) / g;
+  tpg = (tpg + RT - 1) / RT * RT;
+  *groups = g;
+  *tiles_per_group = tpg;
+  *waves = tpg / RT;
+}
+
 static int s_stage(const cav_ctx *c, size_t tsz)
 {
   const int TS = c->ek.nX + c->ek.nY + c->ek.nZ;
@@ -1136,7 +1148,7 @@ int ewald_k_alloc(cav_ctx *c)
   CAV_HIP(hipMalloc(&c->tab, tab_bytes));
   c->tab_bytes = tab_bytes;
   c->tab_cap = c->n;
-  const int chunk = s_chunk(c, 512);
+  const int chunk = c->tune_chunk > 0 ? c->tune_chunk : s_chunk(c, 512);
   c->sk_chunks = (c->n + chunk - 1) / chunk;
   if (c->sk_chunks < 1) c->sk_chunks = 1;
   const int SC = roundup(2 * p.nZ, 16);
@@ -1364,6 +1376,7 @@ static int sk_pipeline_h(cav_ctx *c)
   a.chunk = (a.chunk + 31) / 32 * 32;
   const size_t per_atom = (size_t)a.TS * sizeof(Cx<float>) + (size_t)p.nZ * 16 + sizeof(float);
   a.stage = 2 * 32 * per_atom <= 150 * 1024 ? 32 : 16;
+  if (c->tune_stage == 16) a.stage = 16;
   a.sa = (float)c->sA;
   a.q = c->q;
   a.tab = c->tab;
@@ -1372,12 +1385,21 @@ static int sk_pipeline_h(cav_ctx *c)
   a.items = nullptr;
   a.Rpad = p.Rpad;
   a.SC = roundup(2 * p.nZ, 16);
+  const int RT = c->tune_rt;
   int groups, waves;
-  s_layout(c, &groups, &a.tiles_per_group, &waves);
+  s_layout_rt(p.Rpad, RT, &groups, &a.tiles_per_group, &waves);
   const int nch = (c->n + a.chunk - 1) / a.chunk;
   const size_t smem = 2 * (size_t)a.stage * per_atom;
+  const dim3 lg(nch, groups);
   timer_begin(c, c->t_sk);
-  launch_sk_h<kRT>(c->stream, a, dim3(nch, groups), smem, a.SC / 16, waves);
+  if (RT == 1)
+    launch_sk_h<1>(c->stream, a, lg, smem, a.SC / 16, waves);
+  else if (RT == 2)
+    launch_sk_h<2>(c->stream, a, lg, smem, a.SC / 16, waves);
+  else if (RT == 4)
+    launch_sk_h<4>(c->stream, a, lg, smem, a.SC / 16, waves);
+  else
+    launch_sk_h<3>(c->stream, a, lg, smem, a.SC / 16, waves);
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
   const long plane = (long)p.Rpad * a.SC;
@@ -1390,12 +1412,23 @@ static int sk_pipeline_h(cav_ctx *c)
 }
 
 // LDS of the split force pass: the workgroup's [X | Y] records + two operand blocks
-static size_t split_force_lds_p(const EwaldKPlan &p, int nkh)
+static size_t split_force_lds_p(const EwaldKPlan &p, int nkh, int nt = 2)
 {
-  return 2 * (size_t)4 * nkh * 2 * 64 * 16 + (size_t)16 * 2 * kWavesF * (p.nX + p.nY) * sizeof(Cx<float>);
+  return 2 * (size_t)4 * nkh * 2 * 64 * 16 + (size_t)16 * nt * kWavesF * (p.nX + p.nY) * sizeof(Cx<float>);
 }
 
-static size_t split_force_lds(const cav_ctx *c) { return split_force_lds_p(c->ek, c->nkh); }
+static size_t split_force_lds(const cav_ctx *c) { return split_force_lds_p(c->ek, c->nkh, c->tune_fnt); }
+
+template <int NT>
+static void launch_kforce_h(hipStream_t st, const FArgsH &f, int nkh, int g, size_t smem)
+{
+  switch (nkh) {
+  case 1: allow_smem(k_kforce_h<1, NT>, smem); k_kforce_h<1, NT><<<g, 64 * kWavesF, smem, st>>>(f); break;
+  case 2: allow_smem(k_kforce_h<2, NT>, smem); k_kforce_h<2, NT><<<g, 64 * kWavesF, smem, st>>>(f); break;
+  case 3: allow_smem(k_kforce_h<3, NT>, smem); k_kforce_h<3, NT><<<g, 64 * kWavesF, smem, st>>>(f); break;
+  default: allow_smem(k_kforce_h<4, NT>, smem); k_kforce_h<4, NT><<<g, 64 * kWavesF, smem, st>>>(f); break;
+  }
+}
 
 static int kforce_h(cav_ctx *c)
 {
@@ -1428,16 +1461,14 @@ static int kforce_h(cav_ctx *c)
   f.rep = nullptr;
   f.bm_stride = 0;
   f.inv_sb = 1.0;
-  constexpr int NT = 2;
+  const int NT = c->tune_fnt;
   const int g = (c->n + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF);
   const size_t smem = split_force_lds(c);
   timer_begin(c, c->t_force);
-  switch (c->nkh) {
-  case 1: allow_smem(k_kforce_h<1, NT>, smem); k_kforce_h<1, NT><<<g, 64 * kWavesF, smem, c->stream>>>(f); break;
-  case 2: allow_smem(k_kforce_h<2, NT>, smem); k_kforce_h<2, NT><<<g, 64 * kWavesF, smem, c->stream>>>(f); break;
-  case 3: allow_smem(k_kforce_h<3, NT>, smem); k_kforce_h<3, NT><<<g, 64 * kWavesF, smem, c->stream>>>(f); break;
-  default: allow_smem(k_kforce_h<4, NT>, smem); k_kforce_h<4, NT><<<g, 64 * kWavesF, smem, c->stream>>>(f); break;
-  }
+  if (NT == 1)
+    launch_kforce_h<1>(c->stream, f, c->nkh, g, smem);
+  else
+    launch_kforce_h<2>(c->stream, f, c->nkh, g, smem);
   timer_end(c, c->t_force);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
@@ -1700,17 +1731,6 @@ __global__ void kb_tables_h(const RepDev *rep, int kx, int ky, int kz, const dou
   }
 }
 
-static void s_layout_rt(int Rpad, int RT, int *groups, int *tiles_per_group, int *waves)
-{
-  const int tiles = Rpad / 16;
-  const int per_max = kWavesMax * RT;
-  const int g = (tiles + per_max - 1) / per_max;
-  int tpg = (tiles + g - 1) / g;
-  tpg = (tpg + RT - 1) / RT * RT;
-  *groups = g;
-  *tiles_per_group = tpg;
-  *waves = tpg / RT;
-}
 
 static bool batch_split(const cav_batch *b)
 {
@@ -1963,14 +1983,7 @@ static int batch_ewald_k_t(cav_batch *b)
     f.rep = b->d_rep;
     f.bm_stride = b->bmh_stride * (long)sizeof(_Float16);
     f.inv_sb = 1.0 / b->sB;
-    const size_t smem = split_force_lds_p(p, b->nkh);
-    constexpr int NT = 2;
-    switch (b->nkh) {
-    case 1: allow_smem(k_kforce_h<1, NT>, smem); k_kforce_h<1, NT><<<b->n_fitems, 64 * kWavesF, smem, b->stream>>>(f); break;
-    case 2: allow_smem(k_kforce_h<2, NT>, smem); k_kforce_h<2, NT><<<b->n_fitems, 64 * kWavesF, smem, b->stream>>>(f); break;
-    case 3: allow_smem(k_kforce_h<3, NT>, smem); k_kforce_h<3, NT><<<b->n_fitems, 64 * kWavesF, smem, b->stream>>>(f); break;
-    default: allow_smem(k_kforce_h<4, NT>, smem); k_kforce_h<4, NT><<<b->n_fitems, 64 * kWavesF, smem, b->stream>>>(f); break;
-    }
+    launch_kforce_h<2>(b->stream, f, b->nkh, b->n_fitems, split_force_lds_p(p, b->nkh, 2));
   }
   BAT_HIP(hipGetLastError());
   return CAV_OK;
@@ -2077,6 +2090,64 @@ int batch_weighted_sk(cav_batch *b, const double *u, double *out)
 {
   if (b->precision == CAV_FP64) return batch_wsk_t<double>(b, u, out);
   return batch_wsk_t<float>(b, u, out);
+}
+
+// ------------------------------------------------------------------ autotuner --
+
+// Times the Ewald_k passes (tables, S pass, reduction, force pass) for each
+// candidate tile shape on this context's own system and keeps the fastest:
+// S-pass row tiles per wave, atoms per LDS stage and atoms per chunk, and in
+// the split form the force pass's atom tiles per wave.
+int ewald_k_autotune(cav_ctx *c)
+{
+  struct Cand { int rt, stage, chunk, fnt; };
+  std::vector<Cand> cands;
+  const int chunks[3] = {256, 512, 1024};
+  if (c->precision == CAV_MIXED) {
+    for (int rt : {1, 2, 3, 4})
+      for (int st : {16, 32})
+        for (int ch : chunks)
+          for (int nt : {1, 2}) cands.push_back({rt, st, ch, nt});
+  } else {
+    for (int ch : chunks) cands.push_back({3, 0, ch, 2});
+  }
+  hipEvent_t e0, e1;
+  CAV_HIP(hipEventCreate(&e0));
+  CAV_HIP(hipEventCreate(&e1));
+  const bool timing = c->timing;
+  c->timing = false;
+  float best = 1e30f;
+  Cand pick = {c->tune_rt, c->tune_stage, c->tune_chunk, c->tune_fnt};
+  for (const Cand &k : cands) {
+    c->tune_rt = k.rt;
+    c->tune_stage = k.stage;
+    c->tune_fnt = k.fnt;
+    if (c->tune_chunk != k.chunk) {
+      c->tune_chunk = k.chunk;
+      CAV_TRY(ewald_k_alloc(c));
+    }
+    if (c->precision == CAV_MIXED && split_force_lds(c) > 160 * 1024) continue;
+    for (int r = 0; r < 2; ++r) CAV_TRY(launch_ewald_k(c));
+    CAV_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < 3; ++r) CAV_TRY(launch_ewald_k(c));
+    CAV_HIP(hipEventRecord(e1, c->stream));
+    CAV_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CAV_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      pick = k;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->timing = timing;
+  c->tune_rt = pick.rt;
+  c->tune_stage = pick.stage;
+  c->tune_fnt = pick.fnt;
+  c->tune_chunk = pick.chunk;
+  c->tuned = true;
+  return ewald_k_alloc(c);
 }
 
 // Ewald_k accelerations of every replica, added into the owned accelerations.

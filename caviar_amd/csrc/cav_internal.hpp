@@ -30,10 +30,15 @@ enum Flag : int {
   F_MAXNB = 6,     // longest Verlet row seen at the last build
   F_FORCE = 7,     // host-requested rebuild
   F_GOVER = 8,     // batched replicas: ghosts exceeded the replica's ghost capacity
+  F_MAXST = 9,     // largest cell stencil (candidate atoms) the cell-staged pair kernel met
   F_COUNT = 16
 };
 
 constexpr int kBlock = 256;
+#ifndef CAV_KLPA
+#define CAV_KLPA 8
+#endif
+constexpr int kLPA = CAV_KLPA;  // lanes per atom in the row-gather pair kernel
 
 struct Timer {
   hipEvent_t a = nullptr, b = nullptr;
@@ -133,6 +138,11 @@ struct cav_ctx {
   int *scan_tmp = nullptr;
   int *order = nullptr;    // owned atoms in cell order (compaction of bin_items)
   double4 *cpos = nullptr; // (x, y, z, index) in bin order, for the list build
+  int *binpos = nullptr;   // bin-order position of every slot at the last list build
+  int *lstart = nullptr, *litems = nullptr;  // cells of the last list build
+  double4 *xqs = nullptr;  // (x, y, z, q) in bin order, refreshed every step
+  int2 *tids = nullptr;    // (type, id) in bin order
+  unsigned short *nll = nullptr;  // Verlet entries as positions in the row's cell staging order
   int *ord_blk = nullptr;  // scan block sums for the compaction
 
   // list (row-major: nl[i*nl_cap + k], nn[i] entries in row i)
@@ -165,6 +175,12 @@ struct cav_ctx {
   void *bmat_h = nullptr;    // [Rpad/16][4][NKH][64] x 32 B: 8 hi halves, 8 lo halves
   int nkh = 0;               // 32-deep K-steps along the force-pass reduction axis
   double sA = 1, sB = 1;     // power-of-two operand scales of the split form
+  // Ewald_k tiles (cav_autotune): S-pass row tiles per wave, atoms per LDS
+  // stage (0 = largest that fits), atoms per chunk (0 = default), force-pass
+  // atom tiles per wave of the split form
+  int tune_rt = 3, tune_stage = 0, tune_chunk = 0, tune_fnt = 2;
+  int pair_kind = 0;  // cav_pair_kernel
+  bool tuned = false;
   double *d_W = nullptr;     // [Rpad][nZ]
   double *kacc = nullptr;    // per-atom k-space (fx, fy, fz, phi) scratch for point queries
 
@@ -274,10 +290,12 @@ int launch_rebuild_check(cav_ctx *c);
 int launch_bins(cav_ctx *c, bool conditional);
 int launch_list(cav_ctx *c, bool conditional);
 int launch_save_old(cav_ctx *c, bool conditional);
+bool cell_staged(const cav_ctx *c);
 int launch_pairs(cav_ctx *c, bool lj, bool er, bool accumulate);
 int launch_lj_cellwalk(cav_ctx *c);
 int launch_zero_acc(cav_ctx *c);
 int launch_ewald_k(cav_ctx *c);
+int forces_all(cav_ctx *c);
 int launch_drift(cav_ctx *c, double dt);
 int launch_kick(cav_ctx *c, double dt);
 int launch_begin_step(cav_ctx *c);
@@ -286,6 +304,7 @@ const char *plan_k_lattice(EwaldKPlan &p, int kx, int ky, int kz, int precision,
                            double lx, double ly, double lz);
 void split_scales(double qmax, double qsum, double wmax, double *sA, double *sB);
 int ewald_k_alloc(cav_ctx *c);
+int ewald_k_autotune(cav_ctx *c);
 int launch_energy_r(cav_ctx *c, double *e);
 int launch_energy_lj(cav_ctx *c, double *e);
 int launch_energy_k(cav_ctx *c, double *e);

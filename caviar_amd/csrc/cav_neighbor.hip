@@ -415,7 +415,7 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
                                                   const int *st, const int *start,
                                                   const int *items, int *nl, int *nn, int *flags,
                                                   const int *cond, const RepDev *rep,
-                                                  const int *cell_rep)
+                                                  const int *cell_rep, unsigned short *nll)
 {
   const int b = blockIdx.x;
   if (b >= nbins) return;
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
     const int gsize = (no - og) < 64 ? (no - og) : 64;
     scnt[lane] = 0;
     // candidate cursor: stencil cell index and position inside that cell
-    int cs = sb, ci = 0;
+    int cs = sb, ci = 0, mbase = 0;  // mbase: candidates staged before this stage
     while (cs < se) {
       // stage up to kCand candidates
       int m = 0;
@@ -498,8 +498,11 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
             const bool pass = in && ia[u] >= 0 && j != ia[u] && dot_rn(dx, dy, dz) < rce2;
             const unsigned long long mask = __ballot(pass);
             const int pre = __popcll(mask & below);
-            if (pass && c0[u] + pre < cap)
-              nl[(long)(obase + ia[u] - base) * cap + c0[u] + pre] = j - base;
+            if (pass && c0[u] + pre < cap) {
+              const long at = (long)(obase + ia[u] - base) * cap + c0[u] + pre;
+              nl[at] = j - base;
+              if (nll) nll[at] = (unsigned short)(mbase + idx);  // position in the stencil staging order
+            }
             c0[u] += __popcll(mask);
           }
         }
@@ -509,6 +512,7 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
             if (o0 + u < gsize) scnt[o0 + u] = c0[u];
         }
       }
+      mbase += m;
       __syncthreads();
     }
     if (lane < gsize) {
@@ -541,6 +545,23 @@ __global__ void k_list_all(int n, int cap, int *nl, int *nn, int *flags, const i
   atomicMax(flags + F_MAXNB, cnt);
 }
 
+// The cells the list was built from, kept for the cell-staged pair kernel:
+// later re-binnings (the LJ cell walk, energy and point queries) rewrite
+// bin_start/bin_items, while the list's staging positions refer to these.
+// binpos is the inverse permutation (slot -> position in bin order).
+__global__ void k_list_bins(int n, int nbins, long cap_total, const int *start, const int *items,
+                            int *lstart, int *litems, int *binpos, const int *flags, const int *cond)
+{
+  if (cond && !*cond) return;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t <= nbins) lstart[t] = start[t];
+  if (t < (long)n + flags[F_NGHOST] && t < cap_total) {
+    const int a = items[t];
+    litems[t] = a;
+    binpos[a] = (int)t;
+  }
+}
+
 int launch_list(cav_ctx *c, bool conditional)
 {
   const int *cond = conditional ? c->flags + F_REBUILD : nullptr;
@@ -550,7 +571,14 @@ int launch_list(cav_ctx *c, bool conditional)
   else
     k_list_wave<<<c->nbins, 64, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->cpos,
                                                 c->d_st_off, c->d_st, c->bin_start, c->bin_items,
-                                                c->nl, c->nn, c->flags, cond, nullptr, nullptr);
+                                                c->nl, c->nn, c->flags, cond, nullptr, nullptr,
+                                                c->nll);
+  if (c->nll) {
+    const long tot = (long)c->n + c->gcap;
+    k_list_bins<<<grid_for(tot > c->nbins + 1 ? tot : c->nbins + 1), kBlock, 0, c->stream>>>(
+        c->n, c->nbins, tot, c->bin_start, c->bin_items, c->lstart, c->litems, c->binpos, c->flags,
+        cond);
+  }
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }
@@ -850,7 +878,7 @@ int batch_neighbor(cav_batch *b, bool forced)
                                                      b->z, b->cpos);
   k_list_wave<<<b->ncells, 64, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->cpos, b->st_off,
                                                b->st, b->bin_start, b->bin_items, b->nl, b->nn,
-                                               b->flags, nullptr, b->d_rep, b->cell_rep);
+                                               b->flags, nullptr, b->d_rep, b->cell_rep, nullptr);
   kb_save_old<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->x, b->y, b->z, b->xo, b->yo, b->zo,
                                                     b->flags);
   BAT_HIP(hipGetLastError());

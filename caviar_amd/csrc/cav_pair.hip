@@ -21,7 +21,7 @@ __device__ __forceinline__ double dot_rn_p(double a, double b, double c)
 // Gaussian term of the Ewald force.
 __device__ __forceinline__ float erfc_hastings(float x, float expmx2)
 {
-  const float t = __frcp_rn(fmaf(0.47047f, x, 1.0f));
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.47047f, x, 1.0f));  // v_rcp_f32, 1 ulp
   float p = -5.325673686e-02f;
   p = fmaf(p, t, 2.980402004e-01f);
   p = fmaf(p, t, -6.267941092e-01f);
@@ -53,15 +53,120 @@ struct PairArgs {
   const RepDev *rep;  // batched replicas (blockIdx.y) or nullptr
 };
 
-constexpr int kLPA = 16;  // lanes per atom in the pair kernel
+constexpr int kPairU = 4;  // neighbours whose loads are issued together
 
-// Per-pair force coefficient (force on i = coef * (r_j - r_i)) for T = double
-// (reference arithmetic) or float (mixed form).  kLPA lanes share one atom's
-// row (coalesced row reads, balanced work); their partial forces are summed
-// with a fixed shuffle tree, so FP64 results stay reproducible.
+// LJ per type pair (c12, c6, cshift) staged in LDS as T at kernel start:
+// 3 nt^2 values, converted once instead of per pair.
+template <typename T>
+__device__ __forceinline__ void stage_lj_tables(const PairArgs &a, int nt2, T *tab)
+{
+  for (int t = threadIdx.x; t < 3 * nt2; t += blockDim.x)
+    tab[t] = (T)(t < nt2 ? a.c12[t] : (t < 2 * nt2 ? a.c6[t - nt2] : a.cshift[t - 2 * nt2]));
+}
+
+template <typename T>
+__device__ __forceinline__ T recip(T x)
+{
+  if constexpr (sizeof(T) == 8)
+    return 1.0 / x;
+  else
+    return __builtin_amdgcn_rcpf(x);  // mixed form: v_rcp_f32, 1 ulp
+}
+
+// Force coefficient of one listed pair (force on i = coef * (r_j - r_i)), in T =
+// double (the reference's arithmetic) or float (mixed form); adds the pair's
+// Ewald_r virial term for the reference's list entries.
+//   LJ: lj_cell_list.cpp:141-142 skips dr^2 > rc^2; shifted-force coefficients.
+//   ER: electrostatic_ewald_r.cpp:162-211, every listed pair, exact zero skipped,
+//       lambda of same-molecule pairs (0 skips the pair).
+template <typename T, bool LJ, bool ER, bool LAM>
+__device__ __forceinline__ T pair_coef(const PairArgs &a, const T *tab, int nt2, int i, int j,
+                                       int ti, int idi, int moli, const int2 &tj, T qi, T keqi,
+                                       const double4 &pi4, const double4 &pj, T &dx, T &dy, T &dz,
+                                       double &vir)
+{
+  const double dxd = pj.x - pi4.x, dyd = pj.y - pi4.y, dzd = pj.z - pi4.z;
+  dx = (T)dxd;
+  dy = (T)dyd;
+  dz = (T)dzd;
+  T r2;
+  if constexpr (sizeof(T) == 8)
+    r2 = dot_rn_p(dxd, dyd, dzd);  // the exact rounding order of the reference test
+  else
+    r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  T coef = 0;
+  // type-pair entry as the reference indexes it: [lower owned index][partner]
+  const int p = (j < a.n && j < i) ? tj.x * a.nt + ti : ti * a.nt + tj.x;
+  if constexpr (LJ) {
+    if (!(r2 > (T)a.lj_rc2)) {
+      const T ri2 = recip(r2);
+      const T ri4 = ri2 * ri2, ri8 = ri4 * ri4;
+      coef += -tab[p] * (ri8 * ri4 * ri2) + tab[nt2 + p] * ri8 + tab[2 * nt2 + p];
+    }
+  }
+  if constexpr (ER) {
+    if (!(dxd == 0 && dyd == 0 && dzd == 0)) {
+      T lam = 1;
+      bool skip = false;
+      if constexpr (LAM) {
+        if (moli == a.mol[j]) {
+          const T l = (T)a.lambda[p];
+          if (l == 0) skip = true;
+          lam = l;
+        }
+      }
+      if (!skip) {
+        const T qj = (T)pj.w;
+        T ce;
+        if constexpr (sizeof(T) == 8) {
+          const double rn = sqrt(r2);
+          const double s = (a.two_a_rpi * exp(-a.a2 * r2) + erfc(a.alpha * rn) / rn) * (1.0 / r2);
+          ce = -lam * a.ke * (double)qi * qj * s;
+        } else {
+          const float rinv = __frsqrt_rn(r2);
+          const float rn = r2 * rinv;
+          const float e = __expf(-(float)a.a2 * r2);
+          const float ec = erfc_hastings((float)a.alpha * rn, e);
+          const float s = fmaf((float)a.two_a_rpi, e, ec * rinv) * (rinv * rinv);
+          ce = -lam * keqi * qj * s;
+        }
+        coef += ce;
+        // virial once per reference list entry (owned j > i or ghost), id_i < id_j
+        if ((j >= a.n || j > i) && idi < tj.y) vir += -(double)ce * (double)r2;
+      }
+    }
+  }
+  return coef;
+}
+
+// Deterministic per-block sum of the virial: fixed shuffle tree per wave, then
+// the waves in order.
+__device__ __forceinline__ void block_virial(double vir, double *part)
+{
+  __shared__ double s_red[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) vir += __shfl_xor(vir, o, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = vir;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) v += s_red[w];
+    *part = v;
+  }
+}
+
+// Row-gather pair kernel: kLPA lanes share one atom's row (coalesced row
+// reads, balanced work); each lane fetches kPairU indices, then their (x, y,
+// z, q) and (type, id), before computing, so the gathers overlap.  The
+// partial forces of an atom's lanes are summed with a fixed shuffle tree, so
+// FP64 results stay reproducible.
 template <typename T, bool LJ, bool ER, bool LAM>
 __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
 {
+  extern __shared__ double dyn_pair[];
+  T *tab = reinterpret_cast<T *>(dyn_pair);
+  const int nt2 = LJ ? a.nt * a.nt : 0;
+  stage_lj_tables(a, nt2, tab);
+  __syncthreads();
   if (a.rep) {  // one replica per grid row: shift every array to it
     const RepDev &r = a.rep[blockIdx.y];
     const int base = r.base, obase = r.obase;
@@ -93,70 +198,36 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
     const double4 pi4 = a.xq[i];
     const int2 tii = a.tid[i];
     ti = tii.x;
-    const int idi = tii.y;
     const int moli = LAM ? a.mol[i] : 0;
     const T qi = (T)pi4.w;
     const T keqi = (T)(a.ke) * qi;
     const int cnt = a.nn[i];
     const int *row = a.nl + (long)i * a.cap;
-    for (int k = sub; k < cnt; k += kLPA) {
-      const int j = row[k];
-      const double4 pj = a.xq[j];
-      const double dxd = pj.x - pi4.x, dyd = pj.y - pi4.y, dzd = pj.z - pi4.z;
-      const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd;
-      T r2;
-      if constexpr (sizeof(T) == 8)
-        r2 = dot_rn_p(dxd, dyd, dzd);  // the exact rounding order of the reference test
-      else
-        r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-      T coef = 0;
-      int2 tj = make_int2(0, 0);
-      if constexpr (LJ || ER) tj = a.tid[j];
-      // type-pair entry as the reference indexes it: [lower owned index][partner]
-      const int p = (j < a.n && j < i) ? tj.x * a.nt + ti : ti * a.nt + tj.x;
-      if constexpr (LJ) {
-        if (!(r2 > (T)a.lj_rc2)) {  // lj_cell_list.cpp:141-142: skip dr^2 > rc^2
-          const T ri2 = (T)1 / r2;
-          const T ri4 = ri2 * ri2, ri8 = ri4 * ri4;
-          coef += -(T)a.c12[p] * (ri8 * ri4 * ri2) + (T)a.c6[p] * ri8 + (T)a.cshift[p];
-        }
+    for (int k0 = sub; k0 < cnt; k0 += kPairU * kLPA) {
+      int jj[kPairU];
+      double4 pj[kPairU];
+      int2 tj[kPairU];
+#pragma unroll
+      for (int u = 0; u < kPairU; ++u) {
+        const int k = k0 + u * kLPA;
+        jj[u] = k < cnt ? row[k] : -1;
       }
-      if constexpr (ER) {
-        // electrostatic_ewald_r.cpp:162-211: every listed pair, exact zero skipped
-        if (!(dxd == 0 && dyd == 0 && dzd == 0)) {
-          T lam = 1;
-          bool skip = false;
-          if constexpr (LAM) {
-            if (moli == a.mol[j]) {
-              const T l = (T)a.lambda[p];
-              if (l == 0) skip = true;
-              lam = l;
-            }
-          }
-          if (!skip) {
-            const T qj = (T)pj.w;
-            T ce;
-            if constexpr (sizeof(T) == 8) {
-              const double rn = sqrt(r2);
-              const double s = (a.two_a_rpi * exp(-a.a2 * r2) + erfc(a.alpha * rn) / rn) * (1.0 / r2);
-              ce = -lam * a.ke * (double)qi * qj * s;
-            } else {
-              const float rinv = __frsqrt_rn(r2);
-              const float rn = r2 * rinv;
-              const float e = __expf(-(float)a.a2 * r2);
-              const float ec = erfc_hastings((float)a.alpha * rn, e);
-              const float s = fmaf((float)a.two_a_rpi, e, ec * rinv) * (rinv * rinv);
-              ce = -lam * keqi * qj * s;
-            }
-            coef += ce;
-            // virial once per reference list entry (owned j > i or ghost), id_i < id_j
-            if ((j >= a.n || j > i) && idi < tj.y) vir += -(double)ce * (double)r2;
-          }
-        }
+#pragma unroll
+      for (int u = 0; u < kPairU; ++u) {
+        const int j = jj[u] < 0 ? i : jj[u];
+        pj[u] = a.xq[j];
+        tj[u] = a.tid[j];
       }
-      fx = fma(coef, dx, fx);
-      fy = fma(coef, dy, fy);
-      fz = fma(coef, dz, fz);
+#pragma unroll
+      for (int u = 0; u < kPairU; ++u) {
+        if (jj[u] < 0) break;
+        T dx, dy, dz;
+        const T coef = pair_coef<T, LJ, ER, LAM>(a, tab, nt2, i, jj[u], ti, tii.y, moli, tj[u], qi,
+                                                 keqi, pi4, pj[u], dx, dy, dz, vir);
+        fx = fma(coef, dx, fx);
+        fy = fma(coef, dy, fy);
+        fz = fma(coef, dz, fz);
+      }
     }
   }
 #pragma unroll
@@ -177,26 +248,217 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
       a.az[i] = (double)fz * mi;
     }
   }
-  // deterministic per-block virial partial
-  __shared__ double red[kBlock];
-  red[threadIdx.x] = vir;
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+  block_virial(vir, a.part + blockIdx.x);
+}
+
+// ----------------------------------------------------- cell-staged pair kernel --
+//
+// One workgroup per cell: the cell's de-duplicated 27-stencil is staged in LDS
+// in the order the list build walked it (stencil cells in order, each in bin
+// order), from the bin-ordered copies (x, y, z, q) and (type, id) refreshed
+// every step; each Verlet entry carries its position in that order, so every
+// j of the cell's owned atoms is an LDS read.  Lanes per atom adapt to the
+// cell's owned count; the partial forces of an atom's lanes are summed with a
+// fixed shuffle tree.  A cell whose stencil exceeds the LDS tile falls back to
+// gathering its rows from the global copies.  The tile is sized per launch
+// from the largest stencil met so far (a flag word the kernel raises).
+constexpr int kPairCand = 1024;
+constexpr int kPairLds = sizeof(double4) + sizeof(int2) + sizeof(int);  // bytes per staged atom
+
+struct CellPairArgs {
+  int nbins, cap, seen;
+  const int *start, *st_off, *st, *items;
+  const double4 *xqs;
+  const int2 *tids;
+  const unsigned short *nll;
+  int *flags;
+};
+
+template <typename T, bool LJ, bool ER>
+__global__ __launch_bounds__(kBlock) void k_pairs_cells(PairArgs a, CellPairArgs ca)
+{
+  // staged stencil: (x, y, z, q)[cap], then (type, id)[cap], then slot index[cap],
+  // then the LJ tables (3 nt^2 values of T)
+  extern __shared__ double4 lds[];
+  double4 *lq = lds;
+  int2 *lt = reinterpret_cast<int2 *>(lq + ca.cap);
+  int *lj = reinterpret_cast<int *>(lt + ca.cap);
+  T *tab = reinterpret_cast<T *>(lj + ca.cap);
+  const int nt2 = LJ ? a.nt * a.nt : 0;
+  stage_lj_tables(a, nt2, tab);
+  __shared__ int s_base[32], s_off[33];
+  __shared__ int sh_nc, sh_self;
+  __shared__ T s_f[3][kBlock];  // partial forces of the atoms' lanes
+  const long per = gridDim.x >> 3;  // XCD-aware cell order, as k_pairs
+  const int b = (int)((long)(blockIdx.x & 7) * per + (blockIdx.x >> 3));
+  double vir = 0.0;
+  if (b < ca.nbins) {
+    const int s0 = ca.start[b], e0 = ca.start[b + 1];
+    const int sb = ca.st_off[b], ns = ca.st_off[b + 1] - sb;  // <= 27 stencil cells
+    if (threadIdx.x < 64) {
+      // the stencil's cell ranges and their running offsets in staging order
+      const int lane = threadIdx.x;
+      if (lane == 0) sh_self = -1;
+      int cell = -1, cb = 0, cn = 0;
+      if (lane < ns) {
+        cell = ca.st[sb + lane];
+        cb = ca.start[cell];
+        cn = ca.start[cell + 1] - cb;
+      }
+      int incl = cn;
+      for (int o = 1; o < 32; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+      }
+      if (lane < ns) {
+        s_base[lane] = cb;
+        s_off[lane + 1] = incl;
+        if (cell == b) sh_self = incl - cn;  // this cell's own atoms in the staging order
+      }
+      if (lane == 0) s_off[0] = 0;
+      if (lane == (ns > 0 ? ns - 1 : 0)) {
+        sh_nc = ns > 0 ? incl : 0;
+        if (ns > 0 && incl > ca.seen) atomicMax(ca.flags + F_MAXST, incl);
+      }
+    }
+    // owned atoms come first in a sorted bin
+    int no = 0;
+    for (int t0 = s0; t0 < e0; t0 += kBlock) {
+      const int t = t0 + (int)threadIdx.x;
+      no += __syncthreads_count(t < e0 && ca.items[t] < a.n);
+    }
     __syncthreads();
+    const int nc = sh_nc, self = sh_self;
+    const bool staged = nc <= ca.cap && self >= 0;
+    if (no > 0) {
+      // lanes per atom: as many as fill the block (any count; the partial
+      // forces are summed through LDS in lane order)
+      const int L = no >= kBlock ? 1 : (kBlock / no < 64 ? kBlock / no : 64);
+      const int per_pass = kBlock / L;
+      const int slot = threadIdx.x / L, sub = threadIdx.x - slot * L;
+      constexpr int U = 4;  // neighbour indices fetched together
+      for (int a0 = 0; a0 < no; a0 += per_pass) {
+        const int ai = a0 + slot;
+        const bool act = slot < per_pass && ai < no;
+        // the row's first indices are fetched before the stencil is staged:
+        // rows hold cap entries, so reads below cap stay in the row
+        const int i = act ? ca.items[s0 + ai] : 0;
+        const long row = (long)i * a.cap;
+        const int cnt = act ? a.nn[i] : 0;
+        int loc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = sub + u * L;
+          loc[u] = (act && k < a.cap) ? (staged ? (int)ca.nll[row + k] : a.nl[row + k]) : 0;
+        }
+        if (a0 == 0) {
+          if (staged) {
+            for (int t = threadIdx.x; t < nc; t += kBlock) {
+              int c = 0;
+              while (s_off[c + 1] <= t) ++c;
+              const int src = s_base[c] + t - s_off[c];
+              lq[t] = ca.xqs[src];
+              lt[t] = ca.tids[src];
+              lj[t] = ca.items[src];
+            }
+          }
+          __syncthreads();
+        }
+        T fx = 0, fy = 0, fz = 0;
+        int ti = 0;
+        if (act) {
+          double4 pi4;
+          int2 tii;
+          if (staged) {
+            pi4 = lq[self + ai];
+            tii = lt[self + ai];
+          } else {
+            pi4 = a.xq[i];
+            tii = a.tid[i];
+          }
+          ti = tii.x;
+          const T qi = (T)pi4.w, keqi = (T)a.ke * qi;
+          for (int k0 = sub; k0 < cnt; k0 += U * L) {
+            if (k0 != sub) {
+#pragma unroll
+              for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * L;
+                loc[u] = k < cnt ? (staged ? (int)ca.nll[row + k] : a.nl[row + k]) : 0;
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              if (k0 + u * L >= cnt) break;
+              int j;
+              double4 pj;
+              int2 tj;
+              if (staged) {
+                pj = lq[loc[u]];
+                tj = lt[loc[u]];
+                j = lj[loc[u]];
+              } else {
+                j = loc[u];
+                pj = a.xq[j];
+                tj = a.tid[j];
+              }
+              T dx, dy, dz;
+              const T coef = pair_coef<T, LJ, ER, false>(a, tab, nt2, i, j, ti, tii.y, 0, tj, qi,
+                                                         keqi, pi4, pj, dx, dy, dz, vir);
+              fx = fma(coef, dx, fx);
+              fy = fma(coef, dy, fy);
+              fz = fma(coef, dz, fz);
+            }
+          }
+        }
+        s_f[0][threadIdx.x] = fx;
+        s_f[1][threadIdx.x] = fy;
+        s_f[2][threadIdx.x] = fz;
+        __syncthreads();
+        if (act && sub == 0) {
+          fx = fy = fz = 0;
+          for (int l = 0; l < L; ++l) {
+            fx += s_f[0][threadIdx.x + l];
+            fy += s_f[1][threadIdx.x + l];
+            fz += s_f[2][threadIdx.x + l];
+          }
+          const double mi = a.minv[ti];
+          if (a.accumulate) {
+            a.ax[i] += (double)fx * mi;
+            a.ay[i] += (double)fy * mi;
+            a.az[i] += (double)fz * mi;
+          } else {
+            a.ax[i] = (double)fx * mi;
+            a.ay[i] = (double)fy * mi;
+            a.az[i] = (double)fz * mi;
+          }
+        }
+        __syncthreads();
+      }
+    }
   }
-  if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
+  block_virial(vir, a.part + blockIdx.x);
 }
 
 // Packed copies for the pair gathers: (x, y, z, q) and (type, id).
 __global__ void k_pack(int n, long cap_total, const double *x, const double *y, const double *z,
                        const double *q, const int *type, const int *id, const int *flags,
-                       double4 *xq, int2 *tid)
+                       double4 *xq, int2 *tid, const int *binpos, double4 *xqs, int2 *tids)
 {
   const long a = (long)blockIdx.x * kBlock + threadIdx.x;
   if (a >= (long)n + flags[F_NGHOST] || a >= cap_total) return;
-  xq[a] = make_double4(x[a], y[a], z[a], q[a]);
-  tid[a] = make_int2(type[a], id[a]);
+  const double4 v = make_double4(x[a], y[a], z[a], q[a]);
+  const int2 w = make_int2(type[a], id[a]);
+  xq[a] = v;
+  tid[a] = w;
+  // the same in bin order, for the cell-staged pair kernel: only slots that
+  // existed at the last list build have a bin position
+  if (binpos && a < (long)n + flags[F_NGHOST_OLD]) {
+    const int t = binpos[a];
+    if (t >= 0 && t < cap_total) {
+      xqs[t] = v;
+      tids[t] = w;
+    }
+  }
 }
 
 __global__ void k_reduce_part(int count, const double *part, double *out, int add)
@@ -215,6 +477,7 @@ __global__ void k_reduce_part(int count, const double *part, double *out, int ad
 
 int reduce_partials(cav_ctx *c, int count, int slot, bool add)
 {
+  if (count > c->npart) return fail(c, CAV_ERR_STATE, "partial-sum buffer smaller than the grid");
   k_reduce_part<<<1, kBlock, 0, c->stream>>>(count, c->part, c->scal + slot, add ? 1 : 0);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
@@ -264,16 +527,57 @@ template <typename T>
 static void launch_pairs_t(cav_ctx *c, const PairArgs &a, bool lj, bool er, bool lam)
 {
   const int g = pair_grid(c);
+  const size_t lds = 3 * sizeof(T) * a.nt * a.nt;  // LJ tables
   if (lj && er && lam)
-    k_pairs<T, true, true, true><<<g, kBlock, 0, c->stream>>>(a);
+    k_pairs<T, true, true, true><<<g, kBlock, lds, c->stream>>>(a);
   else if (lj && er)
-    k_pairs<T, true, true, false><<<g, kBlock, 0, c->stream>>>(a);
+    k_pairs<T, true, true, false><<<g, kBlock, lds, c->stream>>>(a);
   else if (lj)
-    k_pairs<T, true, false, false><<<g, kBlock, 0, c->stream>>>(a);
+    k_pairs<T, true, false, false><<<g, kBlock, lds, c->stream>>>(a);
   else if (er && lam)
     k_pairs<T, false, true, true><<<g, kBlock, 0, c->stream>>>(a);
   else
     k_pairs<T, false, true, false><<<g, kBlock, 0, c->stream>>>(a);
+}
+
+// The cell-staged kernel runs on request (CAV_PAIRS_CELLS) where it applies:
+// the cell-built list (its entries' staging positions), no per-molecule
+// lambda.  Otherwise, and by default, the row-gather kernel runs (faster on
+// the ion workloads measured at N=262144, rho=0.3: 0.165 vs 0.232 ms mixed).
+bool cell_staged(const cav_ctx *c)
+{
+  return c->pair_kind == CAV_PAIRS_CELLS && c->from_cells && c->build_cells && !c->all_atom_test &&
+         !c->lambda_set && c->nll;
+}
+
+static int cell_grid(const cav_ctx *c) { return (c->nbins + 7) / 8 * 8; }
+
+template <typename T>
+static void launch_pairs_cells_t(cav_ctx *c, const PairArgs &a, bool lj, bool er)
+{
+  CellPairArgs ca;
+  ca.nbins = c->nbins;
+  ca.start = c->lstart;
+  ca.st_off = c->d_st_off;
+  ca.st = c->d_st;
+  ca.items = c->litems;
+  ca.xqs = c->xqs;
+  ca.tids = c->tids;
+  ca.nll = c->nll;
+  ca.flags = c->flags;
+  // LDS tile: the largest stencil met so far (last synchronised flags), in
+  // steps of 64 atoms; cells beyond it gather from global memory and raise it
+  const int seen = c->h_flags ? c->h_flags[F_MAXST] : 0;
+  ca.seen = seen;
+  ca.cap = seen > 0 ? std::min(kPairCand, (seen + 63) / 64 * 64) : kPairCand;
+  const size_t lds = (size_t)ca.cap * kPairLds + 3 * sizeof(T) * a.nt * a.nt;
+  const int g = cell_grid(c);
+  if (lj && er)
+    k_pairs_cells<T, true, true><<<g, kBlock, lds, c->stream>>>(a, ca);
+  else if (lj)
+    k_pairs_cells<T, true, false><<<g, kBlock, lds, c->stream>>>(a, ca);
+  else
+    k_pairs_cells<T, false, true><<<g, kBlock, lds, c->stream>>>(a, ca);
 }
 
 int launch_pairs(cav_ctx *c, bool want_lj, bool want_er, bool accumulate)
@@ -283,16 +587,27 @@ int launch_pairs(cav_ctx *c, bool want_lj, bool want_er, bool accumulate)
   PairArgs a = pair_args(c);
   a.accumulate = accumulate ? 1 : 0;
   const long tot = (long)c->n + c->gcap;
+  const bool cells = cell_staged(c);
+  // every block writes one virial partial: the buffer must cover the grid
+  if ((cells ? cell_grid(c) : pair_grid(c)) > c->npart)
+    return fail(c, CAV_ERR_STATE, "partial-sum buffer smaller than the pair grid");
   k_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->x, c->y, c->z, c->q, c->type,
-                                                 c->id, c->flags, c->xq, c->tid);
+                                                 c->id, c->flags, c->xq, c->tid,
+                                                 cells ? c->binpos : nullptr, c->xqs, c->tids);
   timer_begin(c, c->t_pair);
-  if (c->precision == CAV_FP64)
+  if (cells) {
+    if (c->precision == CAV_FP64)
+      launch_pairs_cells_t<double>(c, a, lj, er);
+    else
+      launch_pairs_cells_t<float>(c, a, lj, er);
+  } else if (c->precision == CAV_FP64) {
     launch_pairs_t<double>(c, a, lj, er, c->lambda_set);
-  else
+  } else {
     launch_pairs_t<float>(c, a, lj, er, c->lambda_set);
+  }
   timer_end(c, c->t_pair);
   CAV_HIP(hipGetLastError());
-  return reduce_partials(c, pair_grid(c), 0, accumulate);
+  return reduce_partials(c, cells ? cell_grid(c) : pair_grid(c), 0, accumulate);
 }
 
 __global__ void k_zero_acc(int n, double *ax, double *ay, double *az)
@@ -587,10 +902,11 @@ __global__ void kb_reduce_rows(int nper, const double *part, double *out)
 template <typename T>
 static void batch_pairs_t(cav_batch *b, const PairArgs &a, dim3 g)
 {
+  const size_t lds = 3 * sizeof(T) * a.nt * a.nt;  // LJ tables
   if (b->lj_on && b->er_on)
-    k_pairs<T, true, true, false><<<g, kBlock, 0, b->stream>>>(a);
+    k_pairs<T, true, true, false><<<g, kBlock, lds, b->stream>>>(a);
   else if (b->lj_on)
-    k_pairs<T, true, false, false><<<g, kBlock, 0, b->stream>>>(a);
+    k_pairs<T, true, false, false><<<g, kBlock, lds, b->stream>>>(a);
   else
     k_pairs<T, false, true, false><<<g, kBlock, 0, b->stream>>>(a);
 }

@@ -207,6 +207,24 @@ int cav_batch_adjoint(cav_batch *b, const double *weights, double *g_epsilon, do
                       double *g_charge);
 int cav_batch_tiles(const cav_batch *b, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles);
 
+/* ---- tuning (no counterpart in the reference) ------------------------------ */
+/* Autotuner: times the Ewald_k passes for each candidate tile shape on this
+ * context's own system (S-pass row tiles per wave, atoms per LDS stage, atoms
+ * per chunk; split form: force-pass atom tiles per wave) and keeps the
+ * fastest; the forces of the current state are recomputed afterwards.
+ * cav_tiles reports the tiles in use (0 = the built-in choice). */
+int cav_autotune(cav_ctx *ctx);
+int cav_tiles(const cav_ctx *ctx, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles);
+/* Short-range pair kernel.  CAV_PAIRS_ROWS gathers every neighbour of an
+ * atom's Verlet row through L2 (8 lanes per atom); CAV_PAIRS_CELLS stages each
+ * cell's 27-cell stencil in LDS and reads the neighbours from there (where it
+ * applies: list built from cells, no molecule lambda; otherwise rows).
+ * CAV_PAIRS_AUTO (default) is the faster of the two on the measured
+ * workloads: ROWS.  Both give the same forces to rounding. */
+enum cav_pair_kernel { CAV_PAIRS_AUTO = 0, CAV_PAIRS_ROWS = 1, CAV_PAIRS_CELLS = 2 };
+int cav_set_pair_kernel(cav_ctx *ctx, int kind);
+int cav_pair_kernel(const cav_ctx *ctx, int *kind_in_use);
+
 /* ---- measurement ----------------------------------------------------------- */
 /* Statistics of the current Verlet list: list entries per atom with the
  * reference's half-list membership, and those inside the LJ cutoff. */

@@ -220,3 +220,61 @@ def test_per_field_calculate_acceleration(gpu, name):
     e.exchange()
     e.neighbor_build(False)
     assert np.array_equal(F.sorted_pairs(e.pairs()), F.sorted_pairs(ref["pairs"]))
+
+
+def test_autotune_keeps_results(gpu):
+    """The tile autotuner changes only the Ewald_k tiling: FP64 forces stay
+    within 1e-12 of the oracle, mixed within 1e-5 of FP64."""
+    s = configs.ions(n=1500, kmax=5, seed=9)
+    print("[case] autotune", flush=True)
+    o = Oracle(s)
+    o.setup()
+    want = o.get("acc")
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-5)):
+        e = Engine(s, precision=prec)
+        e.setup()
+        tiles = e.autotune()
+        assert tiles["s_chunk"] in (256, 512, 1024)
+        a = e.get("acc")
+        err = F.rel_err(a, want) if prec == "fp64" else np.sqrt(((a - want) ** 2).sum(1).mean() / (want ** 2).sum(1).mean())
+        assert err <= tol, (prec, tiles, err)
+
+
+@pytest.mark.parametrize("name", ["ions2000_setup", "aniso400_setup", "lj4000_setup",
+                                  "lambda_dipole300_setup"])
+def test_pair_kernels_agree(gpu, name):
+    """The cell-staged pair kernel (neighbours read from the cell's LDS tile)
+    and the row-gather kernel: both within 1e-12 of the reference in FP64 and
+    within 1e-5 force RMS in the mixed form, over a 20-step trajectory with
+    energy queries (which re-bin the cells between list builds) interleaved."""
+    s, ref, meta = F.load(name)
+    print(f"[case] pair kernels {name}", flush=True)
+    o = Oracle(s)
+    o.setup()
+    want0 = o.get("acc")
+    assert o.run(0.002, 20) == 0
+    want20 = o.get("acc")
+    for prec in ("fp64", "mixed"):
+        got = {}
+        for kind in ("rows", "cells"):
+            e = Engine(s.copy(), precision=prec)
+            e.set_pair_kernel(kind)
+            e.setup()
+            lam = s.ewald_r is not None and s.ewald_r.lambda_ is not None
+            expect = "rows" if (kind == "rows" or lam) else "cells"
+            assert e.pair_kernel() == expect, (kind, e.pair_kernel())
+            a0 = e.get("acc")
+            for _ in range(4):
+                e.step(0.002, 5)
+                e.energy(lj=s.lj is not None, r=s.ewald_r is not None, k=False)
+            got[kind] = (a0, e.get("acc"))
+        for kind, (a0, a20) in got.items():
+            if prec == "fp64":
+                assert F.rel_err(a0, ref["acc"]) <= 1e-12, kind
+                assert F.rel_err(a0, want0) <= 1e-12, kind
+                assert F.rel_err(a20, want20) <= 1e-9, kind
+            else:
+                rms = np.sqrt(((a0 - want0) ** 2).sum(1).mean() / (want0 ** 2).sum(1).mean())
+                assert rms <= 1e-5, kind
+        if prec == "fp64":
+            assert F.rel_err(got["cells"][0], got["rows"][0]) <= 1e-13

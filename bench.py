@@ -271,6 +271,8 @@ def main():
     ap.add_argument("--nve-steps", type=int, default=0,
                     help="also measure NVE energy drift over this many steps (both forms)")
     ap.add_argument("--nve-every", type=int, default=100)
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the batched-replica workload and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device with the HIP runtime (a kernel of "
                          "known traffic, to calibrate byte counters) and exit")
@@ -282,6 +284,9 @@ def main():
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.only_batched:
+        print(json.dumps(run_batched(args)))
         return
     from caviar_amd import configs
     t0 = time.perf_counter()

@@ -67,14 +67,14 @@ __global__ void k_drift(int n, double dt, const double *lo, const double *hi, co
   bool wrapped = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const double h = __dmul_rn(__dmul_rn(__dmul_rn(0.5, a[d]), dt), dt);
-    double r = __dadd_rn(*p[d], __dadd_rn(__dmul_rn(v[d], dt), h));
+    const double h = xmul(xmul(xmul(0.5, a[d]), dt), dt);
+    double r = xadd(*p[d], xadd(xmul(v[d], dt), h));
     if (b[d] == 1) {
       if (r < lo[d]) {
-        r = __dadd_rn(r, L[d]);
+        r = xadd(r, L[d]);
         wrapped = true;
       } else if (r > hi[d]) {
-        r = __dsub_rn(r, L[d]);
+        r = xsub(r, L[d]);
         wrapped = true;
       }
     }
@@ -103,10 +103,10 @@ __global__ void k_wrap(int n, const double *lo, const double *hi, const double *
     if (b[d] != 1) continue;
     double r = *p[d];
     if (r < lo[d]) {
-      r = __dadd_rn(r, L[d]);
+      r = xadd(r, L[d]);
       wrapped = true;
     } else if (r > hi[d]) {
-      r = __dsub_rn(r, L[d]);
+      r = xsub(r, L[d]);
       wrapped = true;
     }
     *p[d] = r;
@@ -124,9 +124,9 @@ __global__ void k_kick(int n, double hdt, double *vx, double *vy, double *vz, co
 {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  vx[i] = __dadd_rn(vx[i], __dmul_rn(hdt, __dadd_rn(ax[i], aox[i])));
-  vy[i] = __dadd_rn(vy[i], __dmul_rn(hdt, __dadd_rn(ay[i], aoy[i])));
-  vz[i] = __dadd_rn(vz[i], __dmul_rn(hdt, __dadd_rn(az[i], aoz[i])));
+  vx[i] = xadd(vx[i], xmul(hdt, xadd(ax[i], aox[i])));
+  vy[i] = xadd(vy[i], xmul(hdt, xadd(ay[i], aoy[i])));
+  vz[i] = xadd(vz[i], xmul(hdt, xadd(az[i], aoz[i])));
 }
 
 __global__ void k_begin_step(int *flags)

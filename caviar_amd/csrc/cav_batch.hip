@@ -65,10 +65,10 @@ __global__ void kb_wrap(const RepDev *rep, double *x, double *y, double *z, int 
   for (int d = 0; d < 3; ++d) {
     double v = *p[d];
     if (v < r.lo[d]) {
-      v = __dadd_rn(v, r.L[d]);
+      v = xadd(v, r.L[d]);
       wrapped = true;
     } else if (v > r.hi[d]) {
-      v = __dsub_rn(v, r.L[d]);
+      v = xsub(v, r.L[d]);
       wrapped = true;
     }
     *p[d] = v;
@@ -97,13 +97,13 @@ __global__ void kb_drift(const RepDev *rep, double dt, double *x, double *y, dou
   bool wrapped = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const double h = __dmul_rn(__dmul_rn(__dmul_rn(0.5, a[d]), dt), dt);
-    double q = __dadd_rn(*p[d], __dadd_rn(__dmul_rn(v[d], dt), h));
+    const double h = xmul(xmul(xmul(0.5, a[d]), dt), dt);
+    double q = xadd(*p[d], xadd(xmul(v[d], dt), h));
     if (q < r.lo[d]) {
-      q = __dadd_rn(q, r.L[d]);
+      q = xadd(q, r.L[d]);
       wrapped = true;
     } else if (q > r.hi[d]) {
-      q = __dsub_rn(q, r.L[d]);
+      q = xsub(q, r.L[d]);
       wrapped = true;
     }
     *p[d] = q;
@@ -123,9 +123,9 @@ __global__ void kb_kick(long nown, double hdt, double *vx, double *vy, double *v
 {
   const long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i >= nown) return;
-  vx[i] = __dadd_rn(vx[i], __dmul_rn(hdt, __dadd_rn(ax[i], aox[i])));
-  vy[i] = __dadd_rn(vy[i], __dmul_rn(hdt, __dadd_rn(ay[i], aoy[i])));
-  vz[i] = __dadd_rn(vz[i], __dmul_rn(hdt, __dadd_rn(az[i], aoz[i])));
+  vx[i] = xadd(vx[i], xmul(hdt, xadd(ax[i], aox[i])));
+  vy[i] = xadd(vy[i], xmul(hdt, xadd(ay[i], aoy[i])));
+  vz[i] = xadd(vz[i], xmul(hdt, xadd(az[i], aoz[i])));
 }
 
 // positions (slots) and velocities/accelerations (owned) <-> the snapshot

@@ -34,6 +34,25 @@ enum Flag : int {
   F_COUNT = 16
 };
 
+// IEEE double operations rounded one at a time, never contracted into an FMA
+// (hipcc contracts by default): where a decision or a trajectory must follow
+// the reference's rounding exactly.
+__device__ __forceinline__ double xadd(double a, double b)
+{
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ double xsub(double a, double b)
+{
+#pragma clang fp contract(off)
+  return a - b;
+}
+__device__ __forceinline__ double xmul(double a, double b)
+{
+#pragma clang fp contract(off)
+  return a * b;
+}
+
 constexpr int kBlock = 256;
 #ifndef CAV_KLPA
 #define CAV_KLPA 8

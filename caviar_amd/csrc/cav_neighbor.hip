@@ -156,8 +156,10 @@ int launch_ghosts(cav_ctx *c)
 
 __device__ __forceinline__ double dot_rn(double a, double b, double c)
 {
-  // Vector::operator* as the reference evaluates it: (a*a + b*b) + c*c, no FMA.
-  return __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
+  // Vector::operator* as the reference evaluates it: (a*a + b*b) + c*c, every
+  // product and sum rounded on its own (hipcc contracts to FMA by default).
+#pragma clang fp contract(off)
+  return (a * a + b * b) + c * c;
 }
 
 // Neighborlist::update_is_needed (neighborlist.cpp:91-139): size change, or any
@@ -174,7 +176,7 @@ __global__ void k_rebuild_check(int n, long cap_total, const double *x, const do
   }
   const long a = (long)blockIdx.x * kBlock + threadIdx.x;
   if (a >= (long)n + G || a >= cap_total) return;
-  const double dx = __dsub_rn(x[a], xo[a]), dy = __dsub_rn(y[a], yo[a]), dz = __dsub_rn(z[a], zo[a]);
+  const double dx = xsub(x[a], xo[a]), dy = xsub(y[a], yo[a]), dz = xsub(z[a], zo[a]);
   if (dot_rn(dx, dy, dz) > thr_sq) flags[F_REBUILD] = 1;
 }
 
@@ -197,9 +199,9 @@ struct BinRule {
   // x-fastest linear index.
   __device__ __forceinline__ int operator()(double px, double py, double pz) const
   {
-    int ix = (int)floor(__dmul_rn(__dsub_rn(px, lox), cinv));
-    int iy = (int)floor(__dmul_rn(__dsub_rn(py, loy), cinv));
-    int iz = (int)floor(__dmul_rn(__dsub_rn(pz, loz), cinv));
+    int ix = (int)floor(xmul(xsub(px, lox), cinv));
+    int iy = (int)floor(xmul(xsub(py, loy), cinv));
+    int iz = (int)floor(xmul(xsub(pz, loz), cinv));
     ix = ix < 0 ? 0 : (ix >= nbx ? nbx - 1 : ix);
     iy = iy < 0 ? 0 : (iy >= nby ? nby - 1 : iy);
     iz = iz < 0 ? 0 : (iz >= nbz ? nbz - 1 : iz);
@@ -266,37 +268,42 @@ __global__ void k_bin_fill(int n, long cap_total, const int *bin_of, const int *
 
 // Insertion order of update_cell_list (owned ascending, then ghosts n+g
 // ascending, neighborlist.cpp:457-471) is ascending index order.  Each bin is
-// sorted by one 256-thread block with a bitonic network in LDS (bins hold up
-// to 256 atoms; ghost-heavy corner bins of a periodic box reach ~150 at
-// rho=0.3); a rare larger bin falls back to an insertion sort.
+// sorted by one 256-thread block with a bitonic network in LDS: bins up to
+// kSortMax atoms one element per thread, up to kSortBig (dense small boxes,
+// whose few bins hold hundreds to thousands of atoms) several compare-exchange
+// pairs per thread; a larger bin falls back to an insertion sort.
 constexpr int kSortMax = 256;
+constexpr int kSortBig = 4096;
 
 __device__ void sort_bin(int s, int e, int *items)
 {
   const int tid = threadIdx.x;
   const int m = e - s;
   if (m <= 1) return;
-  if (m <= kSortMax) {
-    __shared__ int v[kSortMax];
+  if (m <= kSortBig) {
+    __shared__ int v[kSortBig];
     int p2 = 2;
     while (p2 < m) p2 <<= 1;
-    if (tid < p2) v[tid] = tid < m ? items[s + tid] : 0x7fffffff;
+    for (int t = tid; t < p2; t += kSortMax) v[t] = t < m ? items[s + t] : 0x7fffffff;
     __syncthreads();
+    const int half = p2 >> 1;
     for (int k = 2; k <= p2; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
-        const int o = tid ^ j;
-        if (tid < p2 && o > tid) {
-          const int a = v[tid], c = v[o];
-          const bool up = (tid & k) == 0;
+        for (int t = tid; t < half; t += kSortMax) {
+          // t-th pair of this stage: lower index with a zero at bit j
+          const int lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+          const int hi = lo | j;
+          const int a = v[lo], c = v[hi];
+          const bool up = (lo & k) == 0;
           if ((a > c) == up) {
-            v[tid] = c;
-            v[o] = a;
+            v[lo] = c;
+            v[hi] = a;
           }
         }
         __syncthreads();
       }
     }
-    if (tid < m) items[s + tid] = v[tid];
+    for (int t = tid; t < m; t += kSortMax) items[s + t] = v[t];
     return;
   }
   if (tid != 0) return;
@@ -493,8 +500,8 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
           const double qx = in ? sx[idx] : 0.0, qy = in ? sy[idx] : 0.0, qz = in ? sz[idx] : 0.0;
 #pragma unroll
           for (int u = 0; u < kAPI; ++u) {
-            const double dx = __dsub_rn(px[u], qx), dy = __dsub_rn(py[u], qy),
-                         dz = __dsub_rn(pz[u], qz);
+            const double dx = xsub(px[u], qx), dy = xsub(py[u], qy),
+                         dz = xsub(pz[u], qz);
             const bool pass = in && ia[u] >= 0 && j != ia[u] && dot_rn(dx, dy, dz) < rce2;
             const unsigned long long mask = __ballot(pass);
             const int pre = __popcll(mask & below);
@@ -721,7 +728,7 @@ __global__ void kb_rebuild_check(const RepDev *rep, const double *x, const doubl
   const long a = (long)blockIdx.x * kBlock + threadIdx.x;
   if (a >= (long)r.n + G) return;
   const long s = r.base + a;
-  const double dx = __dsub_rn(x[s], xo[s]), dy = __dsub_rn(y[s], yo[s]), dz = __dsub_rn(z[s], zo[s]);
+  const double dx = xsub(x[s], xo[s]), dy = xsub(y[s], yo[s]), dz = xsub(z[s], zo[s]);
   if (dot_rn(dx, dy, dz) > thr_sq) f[F_REBUILD] = 1;
 }
 

@@ -11,7 +11,10 @@ namespace cav {
 
 __device__ __forceinline__ double dot_rn_p(double a, double b, double c)
 {
-  return __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
+  // (a*a + b*b) + c*c with every product and sum rounded on its own, as the
+  // reference's Vector::operator* (hipcc contracts to FMA by default)
+#pragma clang fp contract(off)
+  return (a * a + b * b) + c * c;
 }
 
 // erfc(x) = t * P(t) * exp(-x^2), t = 1/(1 + p x): the Hastings form of
@@ -652,9 +655,9 @@ struct CellArgs {
 
 __device__ __forceinline__ int cell_of(const CellArgs &c, double px, double py, double pz)
 {
-  int ix = (int)floor(__dmul_rn(__dsub_rn(px, c.lox), c.cinv));
-  int iy = (int)floor(__dmul_rn(__dsub_rn(py, c.loy), c.cinv));
-  int iz = (int)floor(__dmul_rn(__dsub_rn(pz, c.loz), c.cinv));
+  int ix = (int)floor(xmul(xsub(px, c.lox), c.cinv));
+  int iy = (int)floor(xmul(xsub(py, c.loy), c.cinv));
+  int iz = (int)floor(xmul(xsub(pz, c.loz), c.cinv));
   ix = ix < 0 ? 0 : (ix >= c.nbx ? c.nbx - 1 : ix);
   iy = iy < 0 ? 0 : (iy >= c.nby ? c.nby - 1 : iy);
   iz = iz < 0 ? 0 : (iz >= c.nbz ? c.nbz - 1 : iz);
@@ -703,7 +706,7 @@ __global__ void k_lj_cellwalk(CellArgs c, int nt, const double *c12, const doubl
     for (int t = c.start[cell]; t < c.start[cell + 1]; ++t) {
       const int j = c.items[t];
       if (j == i) continue;
-      const double dx = __dsub_rn(c.x[j], xi), dy = __dsub_rn(c.y[j], yi), dz = __dsub_rn(c.z[j], zi);
+      const double dx = xsub(c.x[j], xi), dy = xsub(c.y[j], yi), dz = xsub(c.z[j], zi);
       const double r2 = dot_rn_p(dx, dy, dz);
       if (r2 > rc2) continue;
       const int tj = c.type[j];

@@ -220,12 +220,15 @@ def run_batched(args):
     return res
 
 
-def nve_drift(system, precision, steps, every, dt):
+def nve_drift(system, precision, steps, every, dt, equilibrate=0):
     """Total energy (kinetic + shifted-force LJ + Ewald_r + Ewald_k) every
-    ``every`` steps of an NVE run: relative drift and fluctuation."""
+    ``every`` steps of an NVE run, after ``equilibrate`` unsampled steps from
+    the random start: relative drift and fluctuation, and the sampled series."""
     from caviar_amd import Engine
     e = Engine(system.copy(), precision=precision)
     e.setup()
+    if equilibrate:
+        e.step(dt, equilibrate)
     m = system.mass[system.types][:, None]
     t, E = [], []
     t0 = time.perf_counter()
@@ -239,12 +242,13 @@ def nve_drift(system, precision, steps, every, dt):
         E.append(ke + elj + er + ek)
     E = np.array(E)
     slope = np.polyfit(np.array(t), E, 1)[0]
-    res = {"steps": steps, "dt": dt, "samples": len(E), "E0": E[0],
+    res = {"steps": steps, "dt": dt, "equilibrate": equilibrate, "samples": len(E), "E0": float(E[0]),
            "rel_drift_end": float((E[-1] - E[0]) / abs(E[0])),
            "rel_drift_per_1000_steps": float(slope * 1000 * dt / abs(E[0])),
            "rel_fluctuation_rms": float(np.std(E - np.polyval(np.polyfit(t, E, 1), t)) / abs(E[0])),
-           "seconds": round(time.perf_counter() - t0, 1)}
-    log(f"nve {precision}: {res}")
+           "seconds": round(time.perf_counter() - t0, 1),
+           "E_rel": [float(v) for v in (E - E[0]) / abs(E[0])]}
+    log(f"nve {precision}: " + str({k: v for k, v in res.items() if k != "E_rel"}))
     return res
 
 
@@ -271,6 +275,8 @@ def main():
     ap.add_argument("--nve-steps", type=int, default=0,
                     help="also measure NVE energy drift over this many steps (both forms)")
     ap.add_argument("--nve-every", type=int, default=100)
+    ap.add_argument("--nve-equilibrate", type=int, default=0,
+                    help="unsampled steps from the random start before the NVE measurement")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched-replica workload and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -360,7 +366,8 @@ def main():
         "roofline": roof,
     }
     if args.nve_steps:
-        out["nve"] = {f: nve_drift(system, f, args.nve_steps, args.nve_every, args.dt)
+        out["nve"] = {f: nve_drift(system, f, args.nve_steps, args.nve_every, args.dt,
+                                   args.nve_equilibrate)
                       for f in ("mixed", "fp64")}
     if args.replicas:
         out["batched"] = run_batched(args)

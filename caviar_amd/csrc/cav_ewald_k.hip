@@ -170,7 +170,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
                reinterpret_cast<T *>(smem_raw + 2 * stage_bytes) + stage};
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwaves = blockDim.x >> 6;
   // XCD-aware order: workgroups L and L+8 run on the same XCD (dispatch is
   // round-robin over the 8 XCDs), so the row groups of one chunk share its
   // records through one L2
@@ -368,7 +368,7 @@ __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz
 __global__ void k_chunk_sum(const double *part, int nchunks, long plane, double *out)
 {
   __shared__ double red[4][64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long e = (long)blockIdx.x * 64 + lane;
   double s = 0;
   if (e < plane)
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
   const int blk_elems = 4 * 64 * a.KSP;
   T *lb[2] = {reinterpret_cast<T *>(smem_raw), reinterpret_cast<T *>(smem_raw) + blk_elems};
   const int nwaves = blockDim.x >> 6;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   long base = ((long)blockIdx.x * nwaves + wave) * (16 * NT);
   if (a.items) {
@@ -633,7 +633,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
   const unsigned char *ztab = reinterpret_cast<const unsigned char *>(a.ztab);
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwaves = blockDim.x >> 6;
   // XCD-aware order: workgroups L and L+8 run on the same XCD (dispatch is
   // round-robin over the 8 XCDs), so the row groups of one chunk share its
   // records through one L2
@@ -797,7 +797,7 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
   const int xy_bytes = wg_atoms * a.TS * (int)sizeof(Cx<float>);
   const unsigned char *lb[2] = {smem_raw + xy_bytes, smem_raw + xy_bytes + blk_bytes};
   const Cx<float> *lxy = reinterpret_cast<const Cx<float> *>(smem_raw);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   long wg_base = (long)blockIdx.x * wg_atoms;
   if (a.items) {
@@ -1994,7 +1994,7 @@ static int batch_ewald_k_t(cav_batch *b)
 __global__ void kb_chunk_sum(const int2 *rchunks, const double *part, long plane, double *out)
 {
   __shared__ double red[4][64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long e = (long)blockIdx.x * 64 + lane;
   const int2 rc = rchunks[blockIdx.y];
   double s = 0;

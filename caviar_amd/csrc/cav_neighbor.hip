@@ -417,14 +417,27 @@ constexpr int kCand = CAV_LIST_CAND;  // stencil candidates staged in LDS per pa
 #endif
 constexpr int kAPI = CAV_LIST_API;  // owned atoms tested per staged candidate
 
-__global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, double rce2,
+// Cells per workgroup, one wave each (measured at N=262144: 1 -> 0.307 ms,
+// 2 -> 0.318, 4 -> 0.323, 8 -> 0.374; the build is not occupancy bound).
+#ifndef CAV_LIST_WPB
+#define CAV_LIST_WPB 1
+#endif
+constexpr int kListWaves = CAV_LIST_WPB;
+
+static int list_grid(long cells) { return (int)((cells + kListWaves - 1) / kListWaves); }
+
+__global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins, int cap, double rce2,
                                                   const double4 *cpos, const int *st_off,
                                                   const int *st, const int *start,
                                                   const int *items, int *nl, int *nn, int *flags,
                                                   const int *cond, const RepDev *rep,
                                                   const int *cell_rep, unsigned short *nll)
 {
-  const int b = blockIdx.x;
+  // Each wave builds the rows of one cell on its own (wave-local LDS, no
+  // block barrier).  The wave index is read as a scalar so that everything
+  // derived from the cell stays in scalar registers.
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = (int)((long)blockIdx.x * kListWaves + wv);
   if (b >= nbins) return;
   // batched replicas: this cell's replica decides the owned range, the row base
   // and the flags; entries are stored replica-local (owned i, ghost n+g)
@@ -439,10 +452,12 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
     start += r;  // every replica's cell starts carry one closing entry
   }
   if (cond && !*cond) return;
-  __shared__ double sx[kCand], sy[kCand], sz[kCand];
-  __shared__ int sj[kCand];
-  __shared__ int scnt[64];
-  const int lane = threadIdx.x;
+  __shared__ double sx_[kListWaves][kCand], sy_[kListWaves][kCand], sz_[kListWaves][kCand];
+  __shared__ int sj_[kListWaves][kCand];
+  __shared__ int scnt_[kListWaves][64];
+  double *sx = sx_[wv], *sy = sy_[wv], *sz = sz_[wv];
+  int *sj = sj_[wv], *scnt = scnt_[wv];
+  const int lane = threadIdx.x & 63;
   const int s0 = start[b], e0 = start[b + 1];
   // owned atoms come first in a sorted bin
   int no = 0;
@@ -479,7 +494,7 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
           ci = 0;
         }
       }
-      __syncthreads();
+      __syncwarp();
       for (int o0 = 0; o0 < gsize; o0 += kAPI) {
         int ia[kAPI], c0[kAPI];
         double px[kAPI], py[kAPI], pz[kAPI];
@@ -520,7 +535,7 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
         }
       }
       mbase += m;
-      __syncthreads();
+      __syncwarp();
     }
     if (lane < gsize) {
       const int i = items[s0 + og + lane] - base + obase;
@@ -529,7 +544,7 @@ __global__ __launch_bounds__(64) void k_list_wave(int n, int nbins, int cap, dou
       if (cnt > cap) flags[F_OVERFLOW] = 1;
       atomicMax(flags + F_MAXNB, cnt);
     }
-    __syncthreads();
+    __syncwarp();
   }
 }
 
@@ -576,7 +591,7 @@ int launch_list(cav_ctx *c, bool conditional)
     k_list_all<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->nl_cap, c->nl, c->nn, c->flags,
                                                         cond);
   else
-    k_list_wave<<<c->nbins, 64, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->cpos,
+    k_list_wave<<<list_grid(c->nbins), 64 * kListWaves, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->cpos,
                                                 c->d_st_off, c->d_st, c->bin_start, c->bin_items,
                                                 c->nl, c->nn, c->flags, cond, nullptr, nullptr,
                                                 c->nll);
@@ -883,7 +898,7 @@ int batch_neighbor(cav_batch *b, bool forced)
                                                      b->bin_items);
   kb_cell_pack<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_items, b->x, b->y,
                                                      b->z, b->cpos);
-  k_list_wave<<<b->ncells, 64, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->cpos, b->st_off,
+  k_list_wave<<<list_grid(b->ncells), 64 * kListWaves, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->cpos, b->st_off,
                                                b->st, b->bin_start, b->bin_items, b->nl, b->nn,
                                                b->flags, nullptr, b->d_rep, b->cell_rep, nullptr);
   kb_save_old<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->x, b->y, b->z, b->xo, b->yo, b->zo,

@@ -155,7 +155,7 @@ __device__ __forceinline__ void copy_to_lds_async(const unsigned char *src, unsi
 // the staged X, Y tables and charges, B ((atom,re/im) x (nz,re/im)) from Z.
 // Stages of atom records are double-buffered in LDS: the next stage streams
 // in from HBM (direct-to-LDS loads) while the current one feeds the MFMAs.
-// In the mixed form the MFMAs accumulate in FP32 over one chunk (<= 1024
+// In the mixed form the MFMAs accumulate in FP32 over one chunk (<= 4096
 // atoms) and the chunk partials are summed in FP64.
 template <typename T, int RT, int SCT>
 __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
@@ -2102,7 +2102,9 @@ int ewald_k_autotune(cav_ctx *c)
 {
   struct Cand { int rt, stage, chunk, fnt; };
   std::vector<Cand> cands;
-  const int chunks[3] = {256, 512, 1024};
+  // atoms per S-pass chunk: FP32 accumulation within a chunk (its rounding
+  // error grows as sqrt(chunk)), FP64 across chunks; fewer chunks, fewer partials
+  const int chunks[5] = {256, 512, 1024, 2048, 4096};
   if (c->precision == CAV_MIXED) {
     for (int rt : {1, 2, 3, 4})
       for (int st : {16, 32})

@@ -27,7 +27,7 @@ enum Flag : int {
   F_OVERFLOW = 3,  // a Verlet row exceeded the list capacity
   F_NGHOST = 4,    // ghost count of the current step
   F_NGHOST_OLD = 5,// ghost count at the last list build
-  F_MAXNB = 6,     // longest Verlet row seen at the last build
+  F_MAXNB = 6,     // longest overflowing Verlet row at the last build (set with F_OVERFLOW)
   F_FORCE = 7,     // host-requested rebuild
   F_GOVER = 8,     // batched replicas: ghosts exceeded the replica's ghost capacity
   F_MAXST = 9,     // largest cell stencil (candidate atoms) the cell-staged pair kernel met

@@ -154,6 +154,24 @@ int launch_ghosts(cav_ctx *c)
 
 // ------------------------------------------------------------ rebuild test --
 
+// Lane `src`'s double, as a scalar.
+__device__ __forceinline__ double lane_d(double v, int src)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// A double that is the same in every lane, moved to scalar registers.
+__device__ __forceinline__ double uniform_d(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ double dot_rn(double a, double b, double c)
 {
   // Vector::operator* as the reference evaluates it: (a*a + b*b) + c*c, every
@@ -394,7 +412,7 @@ int launch_bins(cav_ctx *c, bool conditional)
 // ------------------------------------------------------------- Verlet list --
 
 #ifndef CAV_LIST_CAND
-#define CAV_LIST_CAND 256
+#define CAV_LIST_CAND 512
 #endif
 constexpr int kCand = CAV_LIST_CAND;  // stencil candidates staged in LDS per pass
 
@@ -452,11 +470,10 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
     start += r;  // every replica's cell starts carry one closing entry
   }
   if (cond && !*cond) return;
-  __shared__ double sx_[kListWaves][kCand], sy_[kListWaves][kCand], sz_[kListWaves][kCand];
-  __shared__ int sj_[kListWaves][kCand];
-  __shared__ int scnt_[kListWaves][64];
-  double *sx = sx_[wv], *sy = sy_[wv], *sz = sz_[wv];
-  int *sj = sj_[wv], *scnt = scnt_[wv];
+  // staged candidates (x, y, z, slot) in the cell's stencil order; a stage is
+  // padded to a multiple of 64 with far-away entries so every lane tests one
+  __shared__ double4 sc_[kListWaves][kCand];
+  double4 *sc = sc_[wv];
   const int lane = threadIdx.x & 63;
   const int s0 = start[b], e0 = start[b + 1];
   // owned atoms come first in a sorted bin
@@ -466,85 +483,108 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
     no += __popcll(__ballot(v < n));
   }
   if (no == 0) return;
-  const int sb = st_off[b], se = st_off[b + 1];
+  const int sb = st_off[b], ns = st_off[b + 1] - sb;  // <= 27 stencil cells
   const unsigned long long below = (1ull << lane) - 1ull;
+  // padding and empty owned slots sit at +-1e300: their squared distance to
+  // anything is +inf, so they never pass the cutoff test
+  const double4 far_cand = make_double4(1e300, 1e300, 1e300, -1.0);
+  // the stencil's cell ranges, loaded by lanes 0..ns-1 at once, and their
+  // running offsets in staging order (stencil cells in order, each in bin order)
+  __shared__ int s_base_[kListWaves][32], s_off_[kListWaves][33];
+  int *s_base = s_base_[wv], *s_off = s_off_[wv];
+  int total;
+  {
+    int cb = 0, cn = 0;
+    if (lane < ns) {
+      const int cell = st[sb + lane];
+      cb = start[cell];
+      cn = start[cell + 1] - cb;
+    }
+    int incl = cn;
+    for (int o = 1; o < 32; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    if (lane < ns) {
+      s_base[lane] = cb;
+      s_off[lane + 1] = incl;
+    }
+    if (lane == 0) s_off[0] = 0;
+    total = __shfl(incl, ns > 0 ? ns - 1 : 0, 64);
+    if (ns == 0) total = 0;
+    __syncwarp();
+  }
   for (int og = 0; og < no; og += 64) {
     const int gsize = (no - og) < 64 ? (no - og) : 64;
-    scnt[lane] = 0;
-    // candidate cursor: stencil cell index and position inside that cell
-    int cs = sb, ci = 0, mbase = 0;  // mbase: candidates staged before this stage
-    while (cs < se) {
-      // stage up to kCand candidates
-      int m = 0;
-      while (cs < se && m < kCand) {
-        const int cell = st[cs];
-        const int cb = start[cell], cn = start[cell + 1] - cb;
-        const int take = (cn - ci) < (kCand - m) ? (cn - ci) : (kCand - m);
-        for (int t = lane; t < take; t += 64) {
-          const double4 c = cpos[cb + ci + t];
-          sj[m + t] = (int)c.w;
-          sx[m + t] = c.x;
-          sy[m + t] = c.y;
-          sz[m + t] = c.z;
-        }
-        m += take;
-        ci += take;
-        if (ci == cn) {
-          ++cs;
-          ci = 0;
-        }
+    // lane t holds owned atom og+t (position, slot) and its row count; the
+    // kAPI atoms of a group are read from those lanes, never from memory
+    const double4 mine = lane < gsize ? cpos[s0 + og + lane] : make_double4(-1e300, -1e300, -1e300, -1.0);
+    int cnt = 0;
+    for (int mbase = 0; mbase < total; mbase += kCand) {
+      // stage candidates mbase .. mbase+m-1 of the flattened stencil: each
+      // lane finds its cells by walking the offsets forward, all loads at once
+      const int m = (total - mbase) < kCand ? (total - mbase) : kCand;
+      int c = 0;
+      for (int t = lane; t < m; t += 64) {
+        const int g = mbase + t;
+        while (s_off[c + 1] <= g) ++c;
+        sc[t] = cpos[s_base[c] + g - s_off[c]];
       }
+      const int mpad = (m + 63) & ~63;
+      if (m + lane < mpad) sc[m + lane] = far_cand;
       __syncwarp();
       for (int o0 = 0; o0 < gsize; o0 += kAPI) {
+        // the kAPI owned atoms are the same for every lane: positions, slots,
+        // row offsets and row counters in scalar registers (empty slots of
+        // the group carry the far sentinel and never pass)
         int ia[kAPI], c0[kAPI];
+        long row[kAPI];
         double px[kAPI], py[kAPI], pz[kAPI];
 #pragma unroll
         for (int u = 0; u < kAPI; ++u) {
-          const bool ok = o0 + u < gsize;
-          const double4 c = cpos[s0 + og + (ok ? o0 + u : 0)];
-          ia[u] = ok ? (int)c.w : -1;
-          px[u] = c.x;
-          py[u] = c.y;
-          pz[u] = c.z;
-          c0[u] = ok ? scnt[o0 + u] : 0;
+          const int src = o0 + u;  // < 64: o0 is a multiple of kAPI below 64
+          ia[u] = __builtin_amdgcn_readlane((int)mine.w, src);
+          px[u] = lane_d(mine.x, src);
+          py[u] = lane_d(mine.y, src);
+          pz[u] = lane_d(mine.z, src);
+          c0[u] = __builtin_amdgcn_readlane(cnt, src);
+          row[u] = (long)(obase + ia[u] - base) * cap;
         }
-        for (int cc = 0; cc < m; cc += 64) {
+        for (int cc = 0; cc < mpad; cc += 64) {
           const int idx = cc + lane;
-          const bool in = idx < m;
-          const int j = in ? sj[idx] : -1;
-          const double qx = in ? sx[idx] : 0.0, qy = in ? sy[idx] : 0.0, qz = in ? sz[idx] : 0.0;
+          const double4 q = sc[idx];
+          const int j = (int)q.w;
 #pragma unroll
           for (int u = 0; u < kAPI; ++u) {
-            const double dx = xsub(px[u], qx), dy = xsub(py[u], qy),
-                         dz = xsub(pz[u], qz);
-            const bool pass = in && ia[u] >= 0 && j != ia[u] && dot_rn(dx, dy, dz) < rce2;
+            const double dx = xsub(px[u], q.x), dy = xsub(py[u], q.y), dz = xsub(pz[u], q.z);
+            const bool pass = j != ia[u] && dot_rn(dx, dy, dz) < rce2;
             const unsigned long long mask = __ballot(pass);
-            const int pre = __popcll(mask & below);
-            if (pass && c0[u] + pre < cap) {
-              const long at = (long)(obase + ia[u] - base) * cap + c0[u] + pre;
-              nl[at] = j - base;
-              if (nll) nll[at] = (unsigned short)(mbase + idx);  // position in the stencil staging order
+            if (mask) {
+              const int pre = __popcll(mask & below);
+              if (pass && c0[u] + pre < cap) {
+                const long at = row[u] + c0[u] + pre;
+                nl[at] = j - base;
+                if (nll) nll[at] = (unsigned short)(mbase + idx);  // position in the stencil staging order
+              }
+              c0[u] = __builtin_amdgcn_readfirstlane(c0[u] + (int)__popcll(mask));
             }
-            c0[u] += __popcll(mask);
           }
         }
-        if (lane == 0) {
 #pragma unroll
-          for (int u = 0; u < kAPI; ++u)
-            if (o0 + u < gsize) scnt[o0 + u] = c0[u];
-        }
+        for (int u = 0; u < kAPI; ++u) cnt = lane == o0 + u ? c0[u] : cnt;
       }
-      mbase += m;
       __syncwarp();
     }
     if (lane < gsize) {
-      const int i = items[s0 + og + lane] - base + obase;
-      const int cnt = scnt[lane];
+      const int i = (int)mine.w - base + obase;
       nn[i] = cnt < cap ? cnt : cap;
-      if (cnt > cap) flags[F_OVERFLOW] = 1;
-      atomicMax(flags + F_MAXNB, cnt);
+      // the longest row is needed only to regrow the list after an overflow:
+      // one device-wide atomic per overflowing row, none otherwise
+      if (cnt > cap) {
+        flags[F_OVERFLOW] = 1;
+        atomicMax(flags + F_MAXNB, cnt);
+      }
     }
-    __syncwarp();
   }
 }
 
@@ -563,8 +603,10 @@ __global__ void k_list_all(int n, int cap, int *nl, int *nn, int *flags, const i
     ++cnt;
   }
   nn[i] = cnt < cap ? cnt : cap;
-  if (cnt > cap) flags[F_OVERFLOW] = 1;
-  atomicMax(flags + F_MAXNB, cnt);
+  if (cnt > cap) {
+    flags[F_OVERFLOW] = 1;
+    atomicMax(flags + F_MAXNB, cnt);
+  }
 }
 
 // The cells the list was built from, kept for the cell-staged pair kernel:

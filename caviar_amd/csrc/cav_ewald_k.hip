@@ -164,10 +164,8 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
   extern __shared__ unsigned char smem_raw[];
   const int stage = a.stage;
   const size_t stage_bytes = (size_t)stage * a.TS * sizeof(Cx<T>);
-  Cx<T> *buf[2] = {reinterpret_cast<Cx<T> *>(smem_raw),
-                   reinterpret_cast<Cx<T> *>(smem_raw + stage_bytes)};
-  T *sqb[2] = {reinterpret_cast<T *>(smem_raw + 2 * stage_bytes),
-               reinterpret_cast<T *>(smem_raw + 2 * stage_bytes) + stage};
+  // stage buffers: records at smem_raw + b * stage_bytes, charges after both;
+  // addressed from the __shared__ array itself (LDS, not flat, accesses)
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
 
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwaves = blockDim.x >> 6;
@@ -227,10 +225,11 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
   auto issue = [&](long base, int b) {
     const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
     const int bytes = (int)(((size_t)cnt * a.TS * sizeof(Cx<T>) + 15) / 16 * 16);
-    copy_to_lds_async(tab + (size_t)base * a.TS * sizeof(Cx<T>),
-                      reinterpret_cast<unsigned char *>(buf[b]), bytes, wave, nwaves, lane);
+    copy_to_lds_async(tab + (size_t)base * a.TS * sizeof(Cx<T>), smem_raw + b * stage_bytes,
+                      bytes, wave, nwaves, lane);
     if (threadIdx.x < stage)
-      sqb[b][threadIdx.x] = threadIdx.x < cnt ? (T)a.q[base + threadIdx.x] : (T)0;
+      reinterpret_cast<T *>(smem_raw + 2 * stage_bytes)[b * stage + threadIdx.x] =
+          threadIdx.x < cnt ? (T)a.q[base + threadIdx.x] : (T)0;
   };
 
   int b = 0;
@@ -240,8 +239,8 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
   for (long base = a0; base < a1; base += stage) {
     if (base + stage < a1) issue(base + stage, b ^ 1);  // streams in during the MFMAs below
     const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
-    const Cx<T> *st = buf[b];
-    const T *sq = sqb[b];
+    const Cx<T> *st = reinterpret_cast<const Cx<T> *>(smem_raw + b * stage_bytes);
+    const T *sq = reinterpret_cast<const T *>(smem_raw + 2 * stage_bytes) + b * stage;
     if (active) {
 #pragma unroll 2
       for (int ks = 0; ks < stage / 2; ++ks) {
@@ -414,7 +413,8 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
   const unsigned char *bm = reinterpret_cast<const unsigned char *>(a.bm);
   extern __shared__ unsigned char smem_raw[];
   const int blk_elems = 4 * 64 * a.KSP;
-  T *lb[2] = {reinterpret_cast<T *>(smem_raw), reinterpret_cast<T *>(smem_raw) + blk_elems};
+  // operand buffers at smem_raw + {0, blk_elems}: addressed from the __shared__
+  // array itself so the compiler emits LDS (not flat) accesses
   const int nwaves = blockDim.x >> 6;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
@@ -460,14 +460,14 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
   const int nks = a.KZ >> 2;  // K-steps in use (<= KS)
   const int blk_bytes = blk_elems * (int)sizeof(T);
   // the operand of row block rb streams into LDS while rb-1 is multiplied
-  copy_to_lds_async(bm, reinterpret_cast<unsigned char *>(lb[0]), blk_bytes, wave, nwaves, lane);
+  copy_to_lds_async(bm, smem_raw, blk_bytes, wave, nwaves, lane);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (int rb = 0; rb < nrb; ++rb) {
     const int cur = rb & 1;
     if (rb + 1 < nrb)
       copy_to_lds_async(bm + (size_t)(rb + 1) * blk_bytes,
-                        reinterpret_cast<unsigned char *>(lb[cur ^ 1]), blk_bytes, wave, nwaves,
+                        smem_raw + (cur ^ 1) * blk_bytes, blk_bytes, wave, nwaves,
                         lane);
     V acc[4][NT];
 #pragma unroll
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
 #pragma unroll
       for (int tau = 0; tau < NT; ++tau) acc[t][tau] = V{0, 0, 0, 0};
     // operand fragments of this row block: component t, this lane, KSP values
-    const T *bfr = lb[cur] + lane * a.KSP;
+    const T *bfr = reinterpret_cast<const T *>(smem_raw) + cur * blk_elems + lane * a.KSP;
 #pragma unroll
     for (int q = 0; q < (KS + VW - 1) / VW; ++q) {
       if (VW * q >= nks) break;
@@ -795,7 +795,8 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
   const int blk_bytes = 4 * a.nkh * 2 * 64 * 16;
   const int wg_atoms = 16 * NT * nwaves;
   const int xy_bytes = wg_atoms * a.TS * (int)sizeof(Cx<float>);
-  const unsigned char *lb[2] = {smem_raw + xy_bytes, smem_raw + xy_bytes + blk_bytes};
+  // operand buffers at smem_raw + xy_bytes + {0, blk_bytes}, addressed from the
+  // __shared__ array itself so the compiler emits LDS (not flat) accesses
   const Cx<float> *lxy = reinterpret_cast<const Cx<float> *>(smem_raw);
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
@@ -848,14 +849,22 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
     for (int e = 0; e < 4; ++e) fx[tau][e] = fy[tau][e] = fz[tau][e] = ph[tau][e] = 0;
 
   const int nrb = a.Rpad >> 4;
-  copy_to_lds_async(bm, const_cast<unsigned char *>(lb[0]), blk_bytes, wave, nwaves, lane);
+  copy_to_lds_async(bm, smem_raw + xy_bytes, blk_bytes, wave, nwaves, lane);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (int rb = 0; rb < nrb; ++rb) {
     const int cur = rb & 1;
-    if (rb + 1 < nrb)
-      copy_to_lds_async(bm + (size_t)(rb + 1) * blk_bytes, const_cast<unsigned char *>(lb[cur ^ 1]),
-                        blk_bytes, wave, nwaves, lane);
+    // The next row block's operand (blk_bytes = 512 threads x NK x 16 B) is
+    // fetched into registers here and stored to the other LDS buffer after
+    // this block is consumed: with no LDS write in flight, reads of the
+    // current block need no wait on the prefetch.
+    const bool more = rb + 1 < nrb;
+    u4v pf[NK];
+    if (more) {
+      const u4v *src = reinterpret_cast<const u4v *>(bm + (size_t)(rb + 1) * blk_bytes);
+#pragma unroll
+      for (int k = 0; k < NK; ++k) pf[k] = src[threadIdx.x + k * (64 * kWavesF)];
+    }
     f4 acc[4][NT];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -866,7 +875,7 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
       if (s >= a.nkh) break;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const unsigned char *u = lb[cur] + (size_t)((t * a.nkh + s) * 2) * 1024 + lane * 16;
+        const unsigned char *u = smem_raw + xy_bytes + cur * blk_bytes + ((t * a.nkh + s) * 2) * 1024 + lane * 16;
         const u4v bh = *reinterpret_cast<const u4v *>(u);
         const u4v bl = *reinterpret_cast<const u4v *>(u + 1024);
 #pragma unroll
@@ -900,7 +909,11 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
           ph[tau][e] += pr * ur - pi * ui;
         }
     }
-    __builtin_amdgcn_s_waitcnt(0);
+    if (more) {
+      u4v *dst = reinterpret_cast<u4v *>(smem_raw + xy_bytes + (cur ^ 1) * blk_bytes);
+#pragma unroll
+      for (int k = 0; k < NK; ++k) dst[threadIdx.x + k * (64 * kWavesF)] = pf[k];
+    }
     __syncthreads();
   }
 #pragma unroll

@@ -562,9 +562,14 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
             if (mask) {
               const int pre = __popcll(mask & below);
               if (pass && c0[u] + pre < cap) {
-                const long at = row[u] + c0[u] + pre;
-                nl[at] = j - base;
-                if (nll) nll[at] = (unsigned short)(mbase + idx);  // position in the stencil staging order
+                // uniform row pointers plus a 32-bit lane offset
+                const long at0 = row[u] + c0[u];
+                int *rp = nl + at0;
+                rp[pre] = j - base;
+                if (nll) {
+                  unsigned short *lp = nll + at0;
+                  lp[pre] = (unsigned short)(mbase + idx);  // position in the stencil staging order
+                }
               }
               c0[u] = __builtin_amdgcn_readfirstlane(c0[u] + (int)__popcll(mask));
             }

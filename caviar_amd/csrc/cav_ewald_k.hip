@@ -684,23 +684,35 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   const int offY = a.nX;
   const bool active = rt0 < rt_end;
 
+  // The last wave of the workgroup is the producer: it streams each stage's
+  // records, split Z rows and scaled charges into the other LDS buffer while
+  // the compute waves multiply the current one.  The compute waves issue no
+  // asynchronous LDS writes themselves, so their LDS reads never wait on them.
+  const bool producer = wave == nwaves - 1;
   auto issue = [&](long base, int b) {
     unsigned char *dst = smem_raw + b * buf_bytes;
     const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
     const int rb = (int)(((size_t)cnt * a.TS * sizeof(Cx<float>) + 15) / 16 * 16);
-    copy_to_lds_async(tab + (size_t)base * a.TS * sizeof(Cx<float>), dst, rb, wave, nwaves, lane);
-    copy_to_lds_async(ztab + (size_t)base * a.nZ * 16, dst + rec_bytes, cnt * a.nZ * 16, wave,
-                      nwaves, lane);
-    float *sq = reinterpret_cast<float *>(dst + rec_bytes + z_bytes);
-    if (threadIdx.x < stage) sq[threadIdx.x] = threadIdx.x < cnt ? (float)a.q[base + threadIdx.x] * a.sa : 0.f;
+    copy_to_lds_async(tab + (size_t)base * a.TS * sizeof(Cx<float>), dst, rb, 0, 1, lane);
+    copy_to_lds_async(ztab + (size_t)base * a.nZ * 16, dst + rec_bytes, cnt * a.nZ * 16, 0, 1,
+                      lane);
+    if (lane < stage)
+      reinterpret_cast<float *>(dst + rec_bytes + z_bytes)[lane] =
+          lane < cnt ? (float)a.q[base + lane] * a.sa : 0.f;
   };
 
   int b = 0;
-  if (a0 < a1) issue(a0, 0);
+  if (producer && a0 < a1) issue(a0, 0);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (long base = a0; base < a1; base += stage) {
-    if (base + stage < a1) issue(base + stage, b ^ 1);
+    if (producer) {
+      if (base + stage < a1) issue(base + stage, b ^ 1);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      b ^= 1;
+      continue;
+    }
     const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
     const unsigned char *cur = smem_raw + b * buf_bytes;
     const Cx<float> *recs = reinterpret_cast<const Cx<float> *>(cur);
@@ -747,7 +759,6 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
         }
       }
     }
-    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     b ^= 1;
   }
@@ -1121,6 +1132,20 @@ static void s_layout(const cav_ctx *c, int *groups, int *tiles_per_group, int *w
   *waves = tpg / kRT;
 }
 
+// Split S pass: at most kWavesMax - 1 compute waves (the producer wave is
+// added at launch).
+static void s_layout_h(int Rpad, int RT, int *groups, int *tiles_per_group, int *waves)
+{
+  const int tiles = Rpad / 16;
+  const int per_max = (kWavesMax - 1) * RT;
+  const int g = (tiles + per_max - 1) / per_max;
+  int tpg = (tiles + g - 1) / g;
+  tpg = (tpg + RT - 1) / RT * RT;
+  *groups = g;
+  *tiles_per_group = tpg;
+  *waves = tpg / RT;
+}
+
 static void s_layout_rt(int Rpad, int RT, int *groups, int *tiles_per_group, int *waves)
 {
   const int tiles = Rpad / 16;
@@ -1352,7 +1377,7 @@ static void launch_sk_h(hipStream_t st, SArgsH a, dim3 lgrid, size_t smem, int S
   a.ncols = (int)lgrid.x;
   a.groups = (int)lgrid.y;
   const dim3 grid((lgrid.x + 7) / 8 * 8 * lgrid.y);
-  const int block = 64 * waves;
+  const int block = 64 * (waves + 1);  // compute waves + the producer wave
 #define CAV_SKH(S)                                                      \
   case S:                                                               \
     allow_smem(k_sk_h<RT, S>, smem);                                    \
@@ -1400,7 +1425,7 @@ static int sk_pipeline_h(cav_ctx *c)
   a.SC = roundup(2 * p.nZ, 16);
   const int RT = c->tune_rt;
   int groups, waves;
-  s_layout_rt(p.Rpad, RT, &groups, &a.tiles_per_group, &waves);
+  s_layout_h(p.Rpad, RT, &groups, &a.tiles_per_group, &waves);
   const int nch = (c->n + a.chunk - 1) / a.chunk;
   const size_t smem = 2 * (size_t)a.stage * per_atom;
   const dim3 lg(nch, groups);
@@ -1929,7 +1954,7 @@ static int batch_ewald_k_t(cav_batch *b)
     a.SC = SC;
     a.items = b->s_items;
     const int RT = b->s_rt;
-    s_layout_rt(p.Rpad, RT, &groups, &tpg, &waves);
+    s_layout_h(p.Rpad, RT, &groups, &tpg, &waves);
     a.tiles_per_group = tpg;
     const size_t smem = 2 * (size_t)a.stage * per_atom;
     if (RT == 1)

@@ -388,16 +388,30 @@ static int check_capacity(cav_ctx *c, bool *redo)
   return CAV_OK;
 }
 
-static int snapshot(cav_ctx *c, bool save)
+struct Nine {
+  double *p[9];
+};
+
+// The owned positions, velocities and accelerations to or from the snapshot,
+// in one launch (nine separate copies cost nine launches per step).
+__global__ void k_snapshot(int n, Nine a, double *snap, int save)
 {
-  const size_t b = sizeof(double) * c->n;
-  double *arr[9] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az};
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
   for (int k = 0; k < 9; ++k) {
     if (save)
-      CAV_HIP(hipMemcpyAsync(c->snap + (size_t)k * c->n, arr[k], b, hipMemcpyDeviceToDevice, c->stream));
+      snap[(long)k * n + i] = a.p[k][i];
     else
-      CAV_HIP(hipMemcpyAsync(arr[k], c->snap + (size_t)k * c->n, b, hipMemcpyDeviceToDevice, c->stream));
+      a.p[k][i] = snap[(long)k * n + i];
   }
+}
+
+static int snapshot(cav_ctx *c, bool save)
+{
+  const Nine arr = {{c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az}};
+  k_snapshot<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, arr, c->snap, save ? 1 : 0);
+  CAV_HIP(hipGetLastError());
   return CAV_OK;
 }
 

@@ -881,21 +881,30 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int tau = 0; tau < NT; ++tau) acc[t][tau] = f4{0, 0, 0, 0};
+    // Per K-step the four components' fragments are loaded first; each split
+    // product then sweeps all 4 x NT accumulators, so consecutive MFMAs are
+    // independent (per accumulator the order stays hi.hi, hi.lo, lo.hi).
 #pragma unroll
     for (int s = 0; s < NK; ++s) {
-      if (s >= a.nkh) break;
+      u4v bh[4], bl[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const unsigned char *u = smem_raw + xy_bytes + cur * blk_bytes + ((t * a.nkh + s) * 2) * 1024 + lane * 16;
-        const u4v bh = *reinterpret_cast<const u4v *>(u);
-        const u4v bl = *reinterpret_cast<const u4v *>(u + 1024);
-#pragma unroll
-        for (int tau = 0; tau < NT; ++tau) {
-          acc[t][tau] = mfma_h(ahi[tau][s], bh, acc[t][tau]);
-          acc[t][tau] = mfma_h(ahi[tau][s], bl, acc[t][tau]);
-          acc[t][tau] = mfma_h(alo[tau][s], bh, acc[t][tau]);
-        }
+        const unsigned char *u = smem_raw + xy_bytes + cur * blk_bytes + ((t * NK + s) * 2) * 1024 + lane * 16;
+        bh[t] = *reinterpret_cast<const u4v *>(u);
+        bl[t] = *reinterpret_cast<const u4v *>(u + 1024);
       }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int tau = 0; tau < NT; ++tau) acc[t][tau] = mfma_h(ahi[tau][s], bh[t], acc[t][tau]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int tau = 0; tau < NT; ++tau) acc[t][tau] = mfma_h(ahi[tau][s], bl[t], acc[t][tau]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int tau = 0; tau < NT; ++tau) acc[t][tau] = mfma_h(alo[tau][s], bh[t], acc[t][tau]);
     }
     const int r = rb * 16 + lr;
     if (r < a.R) {

@@ -767,22 +767,24 @@ int cav_batch_autotune(cav_batch *b)
   if (!b->setup_done) return bfail(b, CAV_ERR_STATE, "cav_batch_setup must run first");
   if (!b->ek_on) return CAV_OK;
   (void)hipSetDevice(b->device);
-  struct Cand { int rt, stage, chunk; };
+  struct Cand { int rt, stage, chunk, fnt; };
   std::vector<Cand> cands;
   const int rts[3] = {1, 2, 3}, stages[2] = {16, 32}, chunks[3] = {256, 512, 1024};
   if (b->precision == CAV_MIXED) {
     for (int rt : rts)
       for (int st : stages)
-        for (int ch : chunks) cands.push_back({rt, st, ch});
+        for (int ch : chunks)
+          for (int nt : {1, 2}) cands.push_back({rt, st, ch, nt});
   } else {
-    for (int ch : chunks) cands.push_back({b->s_rt, b->s_stage, ch});
+    for (int ch : chunks) cands.push_back({b->s_rt, b->s_stage, ch, b->f_nt});
   }
   float best = 1e30f;
-  Cand pick = {b->s_rt, b->s_stage, b->s_chunk};
+  Cand pick = {b->s_rt, b->s_stage, b->s_chunk, b->f_nt};
   for (const Cand &c : cands) {
     b->s_rt = c.rt;
     b->s_stage = c.stage;
     b->s_chunk = c.chunk;
+    b->f_nt = c.fnt;
     CAV_TRY(batch_ewald_k_alloc(b));
     const float ms = time_ewald_k(b, 5);
     BAT_HIP(hipGetLastError());
@@ -794,6 +796,7 @@ int cav_batch_autotune(cav_batch *b)
   b->s_rt = pick.rt;
   b->s_stage = pick.stage;
   b->s_chunk = pick.chunk;
+  b->f_nt = pick.fnt;
   b->tuned = true;
   drop_graph(b);
   // rebuild buffers for the chosen tiles and restore the forces of the current state

@@ -2030,7 +2030,10 @@ static int batch_ewald_k_t(cav_batch *b)
     f.rep = b->d_rep;
     f.bm_stride = b->bmh_stride * (long)sizeof(_Float16);
     f.inv_sb = 1.0 / b->sB;
-    launch_kforce_h<2>(b->stream, f, b->nkh, b->n_fitems, split_force_lds_p(p, b->nkh, 2));
+    if (b->f_nt == 1)
+      launch_kforce_h<1>(b->stream, f, b->nkh, b->n_fitems, split_force_lds_p(p, b->nkh, 1));
+    else
+      launch_kforce_h<2>(b->stream, f, b->nkh, b->n_fitems, split_force_lds_p(p, b->nkh, 2));
   }
   BAT_HIP(hipGetLastError());
   return CAV_OK;

@@ -42,10 +42,18 @@ struct AdjArgs {
 
 // One list entry: force on i is coef * (r_j - r_i); its derivative along the
 // residual weight w_i is (w_i . r_ij) dcoef/dtheta.
+//
+// NT (the type count, <= 4) is a template parameter so that every per-lane
+// accumulator has a compile-time index and lives in registers: for a fixed
+// atom i the type-pair entry depends only on the partner's type u and on
+// whether the partner is the lower owned index (the reference's row rule), so
+// the LJ terms accumulate per (lower, u) and the charge terms per u; they are
+// mapped to the [eps | sigma | charge] entries once per atom.
+template <int NT>
 __global__ __launch_bounds__(kBlock) void k_adjoint_pairs(AdjArgs a)
 {
-  extern __shared__ double sh[];  // [kBlock/kLPAa][nvals] per-atom-group partials
-  const int nt2 = a.nt * a.nt, nv = 2 * nt2 + a.nt;
+  constexpr int NT2 = NT * NT, NV = 2 * NT2 + NT;
+  extern __shared__ double sh[];  // [kBlock/kLPAa][NV] per-atom-group partials
   if (a.rep) {  // one replica per grid row, arrays shifted to it
     const RepDev &r = a.rep[blockIdx.y];
     a.n = r.n;
@@ -54,19 +62,26 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_pairs(AdjArgs a)
     a.nl += (long)r.obase * a.cap;
     a.nn += r.obase;
     a.w += 3L * r.obase;
-    a.part += (long)blockIdx.y * gridDim.x * nv;
+    a.part += (long)blockIdx.y * gridDim.x * NV;
   }
   const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
   const int t = (int)(gid / kLPAa), sub = (int)(gid % kLPAa);
-  double gl[2 * 16 + 4];  // per-lane accumulators (nt <= 4)
-  for (int v = 0; v < nv; ++v) gl[v] = 0;
+  // [lower][u]: d/d eps and d/d sigma of entry (lower ? u*NT+ti : ti*NT+u)
+  double ge[2][NT], gs[2][NT];
+  double gqi = 0, gqj[NT];  // d/dq of type ti (this atom) and of partner type u
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    ge[0][u] = ge[1][u] = gs[0][u] = gs[1][u] = 0;
+    gqj[u] = 0;
+  }
+  int ti = 0;
   if (t < a.n) {
     const int i = t;
     const double4 pi4 = a.xq[i];
-    const int2 tii = a.tid[i];
-    const int ti = tii.x;
+    ti = a.tid[i].x;
     const int moli = a.lambda_set ? a.mol[i] : 0;
     const double wx = a.w[3 * i], wy = a.w[3 * i + 1], wz = a.w[3 * i + 2];
+    const double qti = a.charge[ti];
     const int cnt = a.nn[i];
     const int *row = a.nl + (long)i * a.cap;
     for (int k = sub; k < cnt; k += kLPAa) {
@@ -77,7 +92,8 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_pairs(AdjArgs a)
       const double wr = wx * dx + wy * dy + wz * dz;  // w_i . r_ij
       const int tj = a.tid[j].x;
       // type-pair entry as the reference indexes it: [lower owned index][partner]
-      const int p = (j < a.n && j < i) ? tj * a.nt + ti : ti * a.nt + tj;
+      const bool lower = j < a.n && j < i;
+      const int p = lower ? tj * NT + ti : ti * NT + tj;
       if (a.lj_on && !(r2 > a.lj_rc2)) {
         const double e = a.eps[p], s = a.sig[p];
         const double ri2 = 1.0 / r2, rci2 = 1.0 / a.lj_rc2;
@@ -85,10 +101,16 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_pairs(AdjArgs a)
         const double r6 = ri2 * ri2 * ri2, r8 = r6 * ri2, r14 = r6 * r6 * ri2;
         const double c6 = rci2 * rci2 * rci2, c8 = c6 * rci2, c14 = c6 * c6 * rci2;
         // coef = 4e(-12 s^12 r^-14 + 6 s^6 r^-8 + 12 s^12 rc^-14 - 6 s^6 rc^-8)
-        const double dce = 4.0 * (-12 * s12 * r14 + 6 * s6 * r8 + 12 * s12 * c14 - 6 * s6 * c8);
-        const double dcs = 4.0 * e * (-144 * s11 * r14 + 36 * s5 * r8 + 144 * s11 * c14 - 36 * s5 * c8);
-        gl[p] += wr * dce;
-        gl[nt2 + p] += wr * dcs;
+        const double dce = wr * 4.0 * (-12 * s12 * r14 + 6 * s6 * r8 + 12 * s12 * c14 - 6 * s6 * c8);
+        const double dcs = wr * 4.0 * e * (-144 * s11 * r14 + 36 * s5 * r8 + 144 * s11 * c14 - 36 * s5 * c8);
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+          const bool hit = tj == u;
+          ge[0][u] += (hit && !lower) ? dce : 0.0;
+          ge[1][u] += (hit && lower) ? dce : 0.0;
+          gs[0][u] += (hit && !lower) ? dcs : 0.0;
+          gs[1][u] += (hit && lower) ? dcs : 0.0;
+        }
       }
       if (a.er_on && !(dx == 0 && dy == 0 && dz == 0)) {
         double lam = 1.0;
@@ -102,27 +124,59 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_pairs(AdjArgs a)
           const double sr = (a.two_a_rpi * exp(-a.a2 * r2) + erfc(a.alpha * rn) / rn) / r2;
           // coef = -lam ke q_a q_b sr: d/dq_a = -lam ke q_b sr, d/dq_b = -lam ke q_a sr
           const double base = -lam * a.ke * sr * wr;
-          gl[2 * nt2 + ti] += base * a.charge[tj];
-          gl[2 * nt2 + tj] += base * a.charge[ti];
+          double qtj = 0;
+#pragma unroll
+          for (int u = 0; u < NT; ++u) qtj = tj == u ? a.charge[u] : qtj;
+          gqi += base * qtj;
+#pragma unroll
+          for (int u = 0; u < NT; ++u) gqj[u] += tj == u ? base * qti : 0.0;
         }
       }
     }
   }
-  // fixed-order reduction: 16 lanes of an atom, then the block's atoms
-  for (int v = 0; v < nv; ++v) {
-    double x = gl[v];
+  // fixed-order reduction: the 16 lanes of an atom (shuffle tree), then the
+  // block's atoms in group order
 #pragma unroll
-    for (int o = 1; o < kLPAa; o <<= 1) x += __shfl_xor(x, o, 64);
-    gl[v] = x;
+  for (int o = 1; o < kLPAa; o <<= 1) {
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      ge[0][u] += __shfl_xor(ge[0][u], o, 64);
+      ge[1][u] += __shfl_xor(ge[1][u], o, 64);
+      gs[0][u] += __shfl_xor(gs[0][u], o, 64);
+      gs[1][u] += __shfl_xor(gs[1][u], o, 64);
+      gqj[u] += __shfl_xor(gqj[u], o, 64);
+    }
+    gqi += __shfl_xor(gqi, o, 64);
   }
   const int grp = threadIdx.x / kLPAa;
-  if (sub == 0)
-    for (int v = 0; v < nv; ++v) sh[grp * nv + v] = gl[v];
+  if (sub == 0) {
+    double *g = sh + grp * NV;
+    for (int v = 0; v < NV; ++v) g[v] = 0;
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      g[ti * NT + u] += ge[0][u];
+      g[u * NT + ti] += ge[1][u];
+      g[NT2 + ti * NT + u] += gs[0][u];
+      g[NT2 + u * NT + ti] += gs[1][u];
+      g[2 * NT2 + u] += gqj[u];
+    }
+    g[2 * NT2 + ti] += gqi;
+  }
   __syncthreads();
-  if (threadIdx.x < nv) {
+  if (threadIdx.x < NV) {
     double x = 0;
-    for (int g = 0; g < kBlock / kLPAa; ++g) x += sh[g * nv + threadIdx.x];
-    a.part[(long)blockIdx.x * nv + threadIdx.x] = x;
+    for (int gi = 0; gi < kBlock / kLPAa; ++gi) x += sh[gi * NV + threadIdx.x];
+    a.part[(long)blockIdx.x * NV + threadIdx.x] = x;
+  }
+}
+
+static void launch_adjoint_pairs(int nt, dim3 grid, size_t sh, hipStream_t st, const AdjArgs &a)
+{
+  switch (nt) {
+  case 1: k_adjoint_pairs<1><<<grid, kBlock, sh, st>>>(a); break;
+  case 2: k_adjoint_pairs<2><<<grid, kBlock, sh, st>>>(a); break;
+  case 3: k_adjoint_pairs<3><<<grid, kBlock, sh, st>>>(a); break;
+  default: k_adjoint_pairs<4><<<grid, kBlock, sh, st>>>(a); break;
   }
 }
 
@@ -332,7 +386,7 @@ extern "C" int cav_adjoint(cav_ctx *c, const double *w_host, double *g_eps, doub
     a.part = dpart;
     a.rep = nullptr;
     const size_t sh = sizeof(double) * (kBlock / kLPAa) * nv;
-    k_adjoint_pairs<<<nblk, kBlock, sh, c->stream>>>(a);
+    launch_adjoint_pairs(c->nt, dim3(nblk), sh, c->stream, a);
     k_reduce_cols<<<1, 64, 0, c->stream>>>(nblk, nv, dpart, dout);
     CAV_HIP(hipGetLastError());
     CAV_HIP(hipMemcpyAsync(hout.data(), dout, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
@@ -438,7 +492,7 @@ extern "C" int cav_batch_adjoint(cav_batch *b, const double *w_host, double *g_e
     a.part = dpart;
     a.rep = b->d_rep;
     const size_t sh = sizeof(double) * (kBlock / kLPAa) * nv;
-    k_adjoint_pairs<<<dim3(pb, B), kBlock, sh, b->stream>>>(a);
+    launch_adjoint_pairs(b->nt, dim3(pb, B), sh, b->stream, a);
     kb_reduce_cols<<<B, 64, 0, b->stream>>>(pb, nv, dpart, dout);
     BAT_HIP(hipGetLastError());
     BAT_HIP(hipMemcpyAsync(hpair.data(), dout, sizeof(double) * B * nv, hipMemcpyDeviceToHost, b->stream));

@@ -56,7 +56,11 @@ struct PairArgs {
   const RepDev *rep;  // batched replicas (blockIdx.y) or nullptr
 };
 
-constexpr int kPairU = 4;  // neighbours whose loads are issued together
+// Neighbours whose loads a lane issues together in the row-gather kernel
+// (measured at N=262144: FP32 2/4/6/8 -> 0.187/0.166/0.157/0.164 ms, FP64
+// 0.204/0.220/0.208/0.254 ms; the FP64 pair math needs the registers).
+template <typename T>
+constexpr int pair_u() { return sizeof(T) == 8 ? 2 : 6; }
 
 // LJ per type pair (c12, c6, cshift) staged in LDS as T at kernel start:
 // 3 nt^2 values, converted once instead of per pair.
@@ -158,7 +162,7 @@ __device__ __forceinline__ void block_virial(double vir, double *part)
 }
 
 // Row-gather pair kernel: kLPA lanes share one atom's row (coalesced row
-// reads, balanced work); each lane fetches kPairU indices, then their (x, y,
+// reads, balanced work); each lane fetches pair_u<T>() indices, then their (x, y,
 // z, q) and (type, id), before computing, so the gathers overlap.  The
 // partial forces of an atom's lanes are summed with a fixed shuffle tree, so
 // FP64 results stay reproducible.
@@ -206,23 +210,24 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
     const T keqi = (T)(a.ke) * qi;
     const int cnt = a.nn[i];
     const int *row = a.nl + (long)i * a.cap;
-    for (int k0 = sub; k0 < cnt; k0 += kPairU * kLPA) {
-      int jj[kPairU];
-      double4 pj[kPairU];
-      int2 tj[kPairU];
+    constexpr int U = pair_u<T>();
+    for (int k0 = sub; k0 < cnt; k0 += U * kLPA) {
+      int jj[U];
+      double4 pj[U];
+      int2 tj[U];
 #pragma unroll
-      for (int u = 0; u < kPairU; ++u) {
+      for (int u = 0; u < U; ++u) {
         const int k = k0 + u * kLPA;
         jj[u] = k < cnt ? row[k] : -1;
       }
 #pragma unroll
-      for (int u = 0; u < kPairU; ++u) {
+      for (int u = 0; u < U; ++u) {
         const int j = jj[u] < 0 ? i : jj[u];
         pj[u] = a.xq[j];
         tj[u] = a.tid[j];
       }
 #pragma unroll
-      for (int u = 0; u < kPairU; ++u) {
+      for (int u = 0; u < U; ++u) {
         if (jj[u] < 0) break;
         T dx, dy, dz;
         const T coef = pair_coef<T, LJ, ER, LAM>(a, tab, nt2, i, jj[u], ti, tii.y, moli, tj[u], qi,

@@ -111,7 +111,7 @@ def load_library():
         "cav_batch_tiles": [P, ip, ip, ip, ip],
         "cav_batch_adjoint": [P, dp, dp, dp, dp],
         "cav_autotune": [P],
-        "cav_tiles": [P, ip, ip, ip, ip],
+        "cav_tiles": [P, ip, ip, ip, ip, ip],
         "cav_set_pair_kernel": [P, I],
         "cav_pair_kernel": [P, ip],
         "cav_reset_acceleration": [P],
@@ -239,9 +239,9 @@ class Engine:
         return self.tiles()
 
     def tiles(self) -> dict:
-        v = [ctypes.c_int() for _ in range(4)]
+        v = [ctypes.c_int() for _ in range(5)]
         self._check(self.lib.cav_tiles(self.h, *[ctypes.byref(x) for x in v]))
-        return dict(zip(("s_rows", "s_stage", "s_chunk", "f_tiles"), (x.value for x in v)))
+        return dict(zip(("s_rows", "s_stage", "s_chunk", "f_tiles", "f_waves"), (x.value for x in v)))
 
     PAIR_KERNELS = {"auto": 0, "rows": 1, "cells": 2}
 

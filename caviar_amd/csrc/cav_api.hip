@@ -919,13 +919,15 @@ int cav_autotune(cav_ctx *c)
   return CAV_OK;
 }
 
-int cav_tiles(const cav_ctx *c, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles)
+int cav_tiles(const cav_ctx *c, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles,
+              int *f_waves)
 {
   if (!c) return CAV_ERR_ARG;
   if (s_rows) *s_rows = c->tune_rt;
   if (s_stage) *s_stage = c->tune_stage;
   if (s_chunk) *s_chunk = c->tune_chunk;
   if (f_tiles) *f_tiles = c->tune_fnt;
+  if (f_waves) *f_waves = c->tune_fw;
   return CAV_OK;
 }
 

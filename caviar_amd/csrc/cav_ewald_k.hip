@@ -795,8 +795,8 @@ struct FArgsH {
 // the split records), B = the operand blocks staged in LDS per row block and
 // shared by the workgroup's waves.  The workgroup's [X | Y] records are staged
 // in LDS once, so the epilogue's per-(atom, row) phase factors are LDS reads.
-template <int NK, int NT>
-__global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
+template <int NK, int NT, int W>
+__global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
 {
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
   const u4v *ztab = reinterpret_cast<const u4v *>(a.ztab);
@@ -865,16 +865,20 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
   __syncthreads();
   for (int rb = 0; rb < nrb; ++rb) {
     const int cur = rb & 1;
-    // The next row block's operand (blk_bytes = 512 threads x NK x 16 B) is
-    // fetched into registers here and stored to the other LDS buffer after
-    // this block is consumed: with no LDS write in flight, reads of the
-    // current block need no wait on the prefetch.
+    // The next row block's operand (CH = 512 NK pieces of 16 B) is fetched
+    // into registers here and stored to the other LDS buffer after this block
+    // is consumed: with no LDS write in flight, reads of the current block
+    // need no wait on the prefetch.
+    constexpr int CH = 512 * NK, PER = (CH + 64 * W - 1) / (64 * W);
     const bool more = rb + 1 < nrb;
-    u4v pf[NK];
+    u4v pf[PER];
     if (more) {
       const u4v *src = reinterpret_cast<const u4v *>(bm + (size_t)(rb + 1) * blk_bytes);
 #pragma unroll
-      for (int k = 0; k < NK; ++k) pf[k] = src[threadIdx.x + k * (64 * kWavesF)];
+      for (int k = 0; k < PER; ++k) {
+        const int idx = threadIdx.x + k * (64 * W);
+        if (idx < CH) pf[k] = src[idx];
+      }
     }
     f4 acc[4][NT];
 #pragma unroll
@@ -932,7 +936,10 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce_h(FArgsH a)
     if (more) {
       u4v *dst = reinterpret_cast<u4v *>(smem_raw + xy_bytes + (cur ^ 1) * blk_bytes);
 #pragma unroll
-      for (int k = 0; k < NK; ++k) dst[threadIdx.x + k * (64 * kWavesF)] = pf[k];
+      for (int k = 0; k < PER; ++k) {
+        const int idx = threadIdx.x + k * (64 * W);
+        if (idx < CH) dst[idx] = pf[k];
+      }
     }
     __syncthreads();
   }
@@ -1459,21 +1466,26 @@ static int sk_pipeline_h(cav_ctx *c)
 }
 
 // LDS of the split force pass: the workgroup's [X | Y] records + two operand blocks
-static size_t split_force_lds_p(const EwaldKPlan &p, int nkh, int nt = 2)
+// LDS of a split force-pass workgroup: two operand blocks plus the [X | Y]
+// records of its 16 nt w atoms (w waves).
+static size_t split_force_lds_p(const EwaldKPlan &p, int nkh, int nt = 2, int w = kWavesF)
 {
-  return 2 * (size_t)4 * nkh * 2 * 64 * 16 + (size_t)16 * nt * kWavesF * (p.nX + p.nY) * sizeof(Cx<float>);
+  return 2 * (size_t)4 * nkh * 2 * 64 * 16 + (size_t)16 * nt * w * (p.nX + p.nY) * sizeof(Cx<float>);
 }
 
-static size_t split_force_lds(const cav_ctx *c) { return split_force_lds_p(c->ek, c->nkh, c->tune_fnt); }
+static size_t split_force_lds(const cav_ctx *c)
+{
+  return split_force_lds_p(c->ek, c->nkh, c->tune_fnt, c->tune_fw);
+}
 
-template <int NT>
+template <int NT, int W>
 static void launch_kforce_h(hipStream_t st, const FArgsH &f, int nkh, int g, size_t smem)
 {
   switch (nkh) {
-  case 1: allow_smem(k_kforce_h<1, NT>, smem); k_kforce_h<1, NT><<<g, 64 * kWavesF, smem, st>>>(f); break;
-  case 2: allow_smem(k_kforce_h<2, NT>, smem); k_kforce_h<2, NT><<<g, 64 * kWavesF, smem, st>>>(f); break;
-  case 3: allow_smem(k_kforce_h<3, NT>, smem); k_kforce_h<3, NT><<<g, 64 * kWavesF, smem, st>>>(f); break;
-  default: allow_smem(k_kforce_h<4, NT>, smem); k_kforce_h<4, NT><<<g, 64 * kWavesF, smem, st>>>(f); break;
+  case 1: allow_smem(k_kforce_h<1, NT, W>, smem); k_kforce_h<1, NT, W><<<g, 64 * W, smem, st>>>(f); break;
+  case 2: allow_smem(k_kforce_h<2, NT, W>, smem); k_kforce_h<2, NT, W><<<g, 64 * W, smem, st>>>(f); break;
+  case 3: allow_smem(k_kforce_h<3, NT, W>, smem); k_kforce_h<3, NT, W><<<g, 64 * W, smem, st>>>(f); break;
+  default: allow_smem(k_kforce_h<4, NT, W>, smem); k_kforce_h<4, NT, W><<<g, 64 * W, smem, st>>>(f); break;
   }
 }
 
@@ -1508,14 +1520,20 @@ static int kforce_h(cav_ctx *c)
   f.rep = nullptr;
   f.bm_stride = 0;
   f.inv_sb = 1.0;
-  const int NT = c->tune_fnt;
-  const int g = (c->n + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF);
+  const int NT = c->tune_fnt, W = c->tune_fw;
+  const int g = (c->n + 16 * NT * W - 1) / (16 * NT * W);
   const size_t smem = split_force_lds(c);
   timer_begin(c, c->t_force);
-  if (NT == 1)
-    launch_kforce_h<1>(c->stream, f, c->nkh, g, smem);
-  else
-    launch_kforce_h<2>(c->stream, f, c->nkh, g, smem);
+  if (W == 16) {
+    if (NT == 1)
+      launch_kforce_h<1, 16>(c->stream, f, c->nkh, g, smem);
+    else
+      launch_kforce_h<2, 16>(c->stream, f, c->nkh, g, smem);
+  } else if (NT == 1) {
+    launch_kforce_h<1, kWavesF>(c->stream, f, c->nkh, g, smem);
+  } else {
+    launch_kforce_h<2, kWavesF>(c->stream, f, c->nkh, g, smem);
+  }
   timer_end(c, c->t_force);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
@@ -2031,9 +2049,9 @@ static int batch_ewald_k_t(cav_batch *b)
     f.bm_stride = b->bmh_stride * (long)sizeof(_Float16);
     f.inv_sb = 1.0 / b->sB;
     if (b->f_nt == 1)
-      launch_kforce_h<1>(b->stream, f, b->nkh, b->n_fitems, split_force_lds_p(p, b->nkh, 1));
+      launch_kforce_h<1, kWavesF>(b->stream, f, b->nkh, b->n_fitems, split_force_lds_p(p, b->nkh, 1));
     else
-      launch_kforce_h<2>(b->stream, f, b->nkh, b->n_fitems, split_force_lds_p(p, b->nkh, 2));
+      launch_kforce_h<2, kWavesF>(b->stream, f, b->nkh, b->n_fitems, split_force_lds_p(p, b->nkh, 2));
   }
   BAT_HIP(hipGetLastError());
   return CAV_OK;
@@ -2150,7 +2168,7 @@ int batch_weighted_sk(cav_batch *b, const double *u, double *out)
 // the split form the force pass's atom tiles per wave.
 int ewald_k_autotune(cav_ctx *c)
 {
-  struct Cand { int rt, stage, chunk, fnt; };
+  struct Cand { int rt, stage, chunk, fnt, fw; };
   std::vector<Cand> cands;
   // atoms per S-pass chunk: FP32 accumulation within a chunk (its rounding
   // error grows as sqrt(chunk)), FP64 across chunks; fewer chunks, fewer partials
@@ -2159,9 +2177,10 @@ int ewald_k_autotune(cav_ctx *c)
     for (int rt : {1, 2, 3, 4})
       for (int st : {16, 32})
         for (int ch : chunks)
-          for (int nt : {1, 2}) cands.push_back({rt, st, ch, nt});
+          for (int nt : {1, 2})
+            for (int fw : {kWavesF, 16}) cands.push_back({rt, st, ch, nt, fw});
   } else {
-    for (int ch : chunks) cands.push_back({3, 0, ch, 2});
+    for (int ch : chunks) cands.push_back({3, 0, ch, 2, kWavesF});
   }
   hipEvent_t e0, e1;
   CAV_HIP(hipEventCreate(&e0));
@@ -2169,11 +2188,12 @@ int ewald_k_autotune(cav_ctx *c)
   const bool timing = c->timing;
   c->timing = false;
   float best = 1e30f;
-  Cand pick = {c->tune_rt, c->tune_stage, c->tune_chunk, c->tune_fnt};
+  Cand pick = {c->tune_rt, c->tune_stage, c->tune_chunk, c->tune_fnt, c->tune_fw};
   for (const Cand &k : cands) {
     c->tune_rt = k.rt;
     c->tune_stage = k.stage;
     c->tune_fnt = k.fnt;
+    c->tune_fw = k.fw;
     if (c->tune_chunk != k.chunk) {
       c->tune_chunk = k.chunk;
       CAV_TRY(ewald_k_alloc(c));
@@ -2197,6 +2217,7 @@ int ewald_k_autotune(cav_ctx *c)
   c->tune_rt = pick.rt;
   c->tune_stage = pick.stage;
   c->tune_fnt = pick.fnt;
+  c->tune_fw = pick.fw;
   c->tune_chunk = pick.chunk;
   c->tuned = true;
   return ewald_k_alloc(c);

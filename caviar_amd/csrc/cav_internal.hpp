@@ -197,7 +197,7 @@ struct cav_ctx {
   // Ewald_k tiles (cav_autotune): S-pass row tiles per wave, atoms per LDS
   // stage (0 = largest that fits), atoms per chunk (0 = default), force-pass
   // atom tiles per wave of the split form
-  int tune_rt = 3, tune_stage = 0, tune_chunk = 0, tune_fnt = 2;
+  int tune_rt = 3, tune_stage = 0, tune_chunk = 0, tune_fnt = 2, tune_fw = 8;
   int pair_kind = 0;  // cav_pair_kernel
   bool tuned = false;
   double *d_W = nullptr;     // [Rpad][nZ]

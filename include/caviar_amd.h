@@ -210,11 +210,13 @@ int cav_batch_tiles(const cav_batch *b, int *s_rows, int *s_stage, int *s_chunk,
 /* ---- tuning (no counterpart in the reference) ------------------------------ */
 /* Autotuner: times the Ewald_k passes for each candidate tile shape on this
  * context's own system (S-pass row tiles per wave, atoms per LDS stage, atoms
- * per chunk; split form: force-pass atom tiles per wave) and keeps the
- * fastest; the forces of the current state are recomputed afterwards.
+ * per chunk; split form: force-pass atom tiles per wave and waves per
+ * workgroup) and keeps the fastest; the forces of the current state are
+ * recomputed afterwards.
  * cav_tiles reports the tiles in use (0 = the built-in choice). */
 int cav_autotune(cav_ctx *ctx);
-int cav_tiles(const cav_ctx *ctx, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles);
+int cav_tiles(const cav_ctx *ctx, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles,
+              int *f_waves);
 /* Short-range pair kernel.  CAV_PAIRS_ROWS gathers every neighbour of an
  * atom's Verlet row through L2 (8 lanes per atom); CAV_PAIRS_CELLS stages each
  * cell's 27-cell stencil in LDS and reads the neighbours from there (where it

@@ -285,46 +285,50 @@ __global__ void k_bin_fill(int n, long cap_total, const int *bin_of, const int *
 }
 
 // Insertion order of update_cell_list (owned ascending, then ghosts n+g
-// ascending, neighborlist.cpp:457-471) is ascending index order.  Each bin is
-// sorted by one 256-thread block with a bitonic network in LDS: bins up to
-// kSortMax atoms one element per thread, up to kSortBig (dense small boxes,
-// whose few bins hold hundreds to thousands of atoms) several compare-exchange
-// pairs per thread; a larger bin falls back to an insertion sort.
-constexpr int kSortMax = 256;
-constexpr int kSortBig = 4096;
+// ascending, neighborlist.cpp:457-471) is ascending index order: every bin is
+// sorted after the (atomic, unordered) fill.
+// One bin per wave (kSortWaves bins per workgroup, no block barrier): a bin of
+// at most 64 atoms is sorted by rank (each lane counts the smaller indices of
+// the others through shuffles; indices are distinct), a larger one with a
+// bitonic network in the wave's LDS, and beyond kSortWaveMax atoms by lane 0.
+constexpr int kSortWaves = 4, kSortWaveMax = 2048;
 
-__device__ void sort_bin(int s, int e, int *items)
+__device__ void sort_bin_wave(int s, int e, int *items, int *v)
 {
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int m = e - s;
   if (m <= 1) return;
-  if (m <= kSortBig) {
-    __shared__ int v[kSortBig];
-    int p2 = 2;
+  if (m <= 64) {
+    const int x = lane < m ? items[s + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int k = 0; k < m; ++k) rank += __shfl(x, k, 64) < x ? 1 : 0;
+    if (lane < m) items[s + rank] = x;
+    return;
+  }
+  if (m <= kSortWaveMax) {
+    int p2 = 128;
     while (p2 < m) p2 <<= 1;
-    for (int t = tid; t < p2; t += kSortMax) v[t] = t < m ? items[s + t] : 0x7fffffff;
-    __syncthreads();
+    for (int t = lane; t < p2; t += 64) v[t] = t < m ? items[s + t] : 0x7fffffff;
+    __syncwarp();
     const int half = p2 >> 1;
     for (int k = 2; k <= p2; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int t = tid; t < half; t += kSortMax) {
-          // t-th pair of this stage: lower index with a zero at bit j
+        for (int t = lane; t < half; t += 64) {
           const int lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
           const int hi = lo | j;
           const int a = v[lo], c = v[hi];
-          const bool up = (lo & k) == 0;
-          if ((a > c) == up) {
+          if ((a > c) == ((lo & k) == 0)) {
             v[lo] = c;
             v[hi] = a;
           }
         }
-        __syncthreads();
+        __syncwarp();
       }
     }
-    for (int t = tid; t < m; t += kSortMax) items[s + t] = v[t];
+    for (int t = lane; t < m; t += 64) items[s + t] = v[t];
     return;
   }
-  if (tid != 0) return;
+  if (lane != 0) return;
   for (int i = s + 1; i < e; ++i) {
     const int val = items[i];
     int j = i - 1;
@@ -336,13 +340,15 @@ __device__ void sort_bin(int s, int e, int *items)
   }
 }
 
-__global__ __launch_bounds__(kSortMax) void k_bin_sort(int nbins, const int *start, int *items,
-                                                       const int *cond)
+__global__ __launch_bounds__(64 * kSortWaves) void k_bin_sort(int nbins, const int *start, int *items,
+                                                              const int *cond)
 {
+  __shared__ int v_[kSortWaves][kSortWaveMax];
   if (cond && !*cond) return;
-  const int b = blockIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = blockIdx.x * kSortWaves + wv;
   if (b >= nbins) return;
-  sort_bin(start[b], start[b + 1], items);
+  sort_bin_wave(start[b], start[b + 1], items, v_[wv]);
 }
 
 // (x, y, z, index) in bin order, for contiguous candidate staging in the list build
@@ -396,7 +402,8 @@ int launch_bins(cav_ctx *c, bool conditional)
   k_zero_int<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->bin_cnt, c->nbins, cond);
   k_bin_fill<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->bin_of, c->bin_start,
                                                      c->bin_cnt, c->bin_items, c->flags, cond);
-  k_bin_sort<<<c->nbins, kSortMax, 0, c->stream>>>(c->nbins, c->bin_start, c->bin_items, cond);
+  k_bin_sort<<<(c->nbins + kSortWaves - 1) / kSortWaves, 64 * kSortWaves, 0, c->stream>>>(
+      c->nbins, c->bin_start, c->bin_items, cond);
   k_cell_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(tot, c->n, c->bin_items, c->x, c->y, c->z,
                                                       c->cpos, c->flags, cond);
   const int nbo = grid_for(tot);
@@ -873,15 +880,17 @@ __global__ void kb_bin_fill(const RepDev *rep, const int *flags, const int *bin_
   items[start[c + blockIdx.y] + atomicAdd(cur + c, 1)] = (int)s;
 }
 
-__global__ __launch_bounds__(kSortMax) void kb_bin_sort(int ncells, const int *cell_rep,
-                                                        const int *flags, const int *start,
-                                                        int *items)
+__global__ __launch_bounds__(64 * kSortWaves) void kb_bin_sort(int ncells, const int *cell_rep,
+                                                               const int *flags, const int *start,
+                                                               int *items)
 {
-  const int c = blockIdx.x;
+  __shared__ int v_[kSortWaves][kSortWaveMax];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = blockIdx.x * kSortWaves + wv;
   if (c >= ncells) return;
   const int rr = cell_rep[c];
   if (!flags[16 * rr + F_REBUILD]) return;
-  sort_bin(start[c + rr], start[c + rr + 1], items);
+  sort_bin_wave(start[c + rr], start[c + rr + 1], items, v_[wv]);
 }
 
 __global__ void kb_cell_pack(const RepDev *rep, const int *flags, const int *items, const double *x,
@@ -941,8 +950,8 @@ int batch_neighbor(cav_batch *b, bool forced)
   kb_bin_start<<<b->B, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_cnt, b->bin_start);
   kb_bin_fill<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_of, b->cell_rep,
                                                     b->bin_start, b->bin_cnt + b->ncells, b->bin_items);
-  kb_bin_sort<<<b->ncells, kSortMax, 0, b->stream>>>(b->ncells, b->cell_rep, b->flags, b->bin_start,
-                                                     b->bin_items);
+  kb_bin_sort<<<(b->ncells + kSortWaves - 1) / kSortWaves, 64 * kSortWaves, 0, b->stream>>>(
+      b->ncells, b->cell_rep, b->flags, b->bin_start, b->bin_items);
   kb_cell_pack<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_items, b->x, b->y,
                                                      b->z, b->cpos);
   k_list_wave<<<list_grid(b->ncells), 64 * kListWaves, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->cpos, b->st_off,

@@ -310,6 +310,9 @@ static int sync_flags(cav_ctx *c)
 {
   CAV_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int) * F_COUNT, hipMemcpyDeviceToHost, c->stream));
   CAV_HIP(hipStreamSynchronize(c->stream));
+  // a conditional list build that ran decides whether the staging positions
+  // match the current list
+  if (c->h_flags[F_REBUILD]) c->nll_built = c->nll_pending;
   return CAV_OK;
 }
 
@@ -370,6 +373,7 @@ static int neighbor_phase(cav_ctx *c, bool forced)
   timer_end(c, c->t_nb);
   timer_begin(c, c->t_list);
   CAV_TRY(launch_list(c, true));
+  if (forced) c->nll_built = c->nll_pending;  // the build is known to run
   CAV_TRY(launch_save_old(c, true));
   timer_end(c, c->t_list);
   return CAV_OK;

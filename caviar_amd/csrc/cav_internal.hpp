@@ -199,6 +199,9 @@ struct cav_ctx {
   // atom tiles per wave of the split form
   int tune_rt = 3, tune_stage = 0, tune_chunk = 0, tune_fnt = 2, tune_fw = 8;
   int pair_kind = 0;  // cav_pair_kernel
+  // staging positions (nll) valid for the current list / written by the last
+  // conditional build launched
+  bool nll_built = false, nll_pending = false;
   bool tuned = false;
   double *d_W = nullptr;     // [Rpad][nZ]
   double *kacc = nullptr;    // per-atom k-space (fx, fy, fz, phi) scratch for point queries

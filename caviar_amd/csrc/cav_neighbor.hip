@@ -641,6 +641,13 @@ __global__ void k_list_bins(int n, int nbins, long cap_total, const int *start, 
 int launch_list(cav_ctx *c, bool conditional)
 {
   const int *cond = conditional ? c->flags + F_REBUILD : nullptr;
+  // staging positions (and the cells they refer to) only for the cell-staged
+  // pair kernel; a build that writes them makes them valid for its list, one
+  // that does not invalidates them (known at once for an unconditional build,
+  // after the flags are read back for a conditional one)
+  const bool with_nll = c->pair_kind == CAV_PAIRS_CELLS && c->nll && !c->all_atom_test;
+  c->nll_pending = with_nll;
+  if (!conditional) c->nll_built = with_nll;
   if (c->all_atom_test)
     k_list_all<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->nl_cap, c->nl, c->nn, c->flags,
                                                         cond);
@@ -648,8 +655,8 @@ int launch_list(cav_ctx *c, bool conditional)
     k_list_wave<<<list_grid(c->nbins), 64 * kListWaves, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->cpos,
                                                 c->d_st_off, c->d_st, c->bin_start, c->bin_items,
                                                 c->nl, c->nn, c->flags, cond, nullptr, nullptr,
-                                                c->nll);
-  if (c->nll) {
+                                                with_nll ? c->nll : nullptr);
+  if (with_nll) {
     const long tot = (long)c->n + c->gcap;
     k_list_bins<<<grid_for(tot > c->nbins + 1 ? tot : c->nbins + 1), kBlock, 0, c->stream>>>(
         c->n, c->nbins, tot, c->bin_start, c->bin_items, c->lstart, c->litems, c->binpos, c->flags,

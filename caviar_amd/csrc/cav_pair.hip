@@ -555,7 +555,7 @@ static void launch_pairs_t(cav_ctx *c, const PairArgs &a, bool lj, bool er, bool
 bool cell_staged(const cav_ctx *c)
 {
   return c->pair_kind == CAV_PAIRS_CELLS && c->from_cells && c->build_cells && !c->all_atom_test &&
-         !c->lambda_set && c->nll;
+         !c->lambda_set && c->nll && c->nll_built;
 }
 
 static int cell_grid(const cav_ctx *c) { return (c->nbins + 7) / 8 * 8; }

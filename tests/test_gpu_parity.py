@@ -278,3 +278,23 @@ def test_pair_kernels_agree(gpu, name):
                 assert rms <= 1e-5, kind
         if prec == "fp64":
             assert F.rel_err(got["cells"][0], got["rows"][0]) <= 1e-13
+
+
+def test_pair_kernel_switch_mid_run(gpu):
+    """Switching to the cell-staged kernel after setup: it takes over once a
+    list build has written the staging positions (the row kernel runs until
+    then); the 20-step trajectory matches the oracle's."""
+    s, ref, meta = F.load("ions2000_setup")
+    print("[case] pair kernel switch mid-run", flush=True)
+    o = Oracle(s)
+    o.setup()
+    assert o.run(0.002, 20) == 0
+    e = Engine(s.copy())
+    e.set_pair_kernel("rows")
+    e.setup()
+    e.step(0.002, 5)
+    e.set_pair_kernel("cells")
+    e.step(0.002, 15)
+    assert e.pair_kernel() == "cells"
+    assert F.rel_err(e.get("acc"), o.get("acc")) <= 1e-9
+    assert F.rel_err(e.get("pos"), o.get("pos")) <= 1e-9

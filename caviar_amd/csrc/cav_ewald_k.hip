@@ -76,6 +76,22 @@ __device__ __forceinline__ void split_pair(float a, float b, unsigned &hi, unsig
   lo = pack_h2((_Float16)(a - (float)ah), (_Float16)(b - (float)bh));
 }
 
+// The same split in three instructions: hi by one packed round-to-nearest
+// conversion, each half of lo by a mixed-precision fma, fp16(a * 1 - hi.x),
+// reading hi's half directly (a - hi is exact in FP32, so the result equals
+// split_pair's).
+__device__ __forceinline__ void split_pair_mix(float a, float b, unsigned &hi, unsigned &lo)
+{
+  const _Float16 ah = (_Float16)a, bh = (_Float16)b;
+  hi = pack_h2(ah, bh);
+  unsigned l;
+  asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]" : "=v"(l) : "v"(a), "v"(hi));
+  asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+               : "+v"(l)
+               : "v"(b), "v"(hi));
+  lo = l;
+}
+
 // ztab (optional, mixed form): for every Z entry a 16-byte record of the two
 // real-embedding columns split into halves, (Zr, -Zi) hi, lo, (Zi, Zr) hi, lo.
 template <typename T>
@@ -683,6 +699,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   }
   const int offY = a.nX;
   const bool active = rt0 < rt_end;
+  const int rec_stride = a.TS * (int)sizeof(Cx<float>), z_stride = a.nZ * 16;
 
   // The last wave of the workgroup is the producer: it streams each stage's
   // records, split Z rows and scaled charges into the other LDS buffer while
@@ -701,40 +718,69 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
           lane < cnt ? (float)a.q[base + lane] * a.sa : 0.f;
   };
 
+  // Once a stage's copies have landed the producer folds each atom's scaled
+  // charge into its X entries, so a compute wave forms q X Y with one complex
+  // product, and it clears a partial stage's missing atoms (zero records, zero
+  // Z), so the compute waves need no per-atom bounds (the copy is rounded up
+  // to 16 bytes and may have touched the first of them).
+  auto finish = [&](long base, int b) {
+    const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
+    unsigned char *dst = smem_raw + b * buf_bytes;
+    const float *sq = reinterpret_cast<const float *>(dst + rec_bytes + z_bytes);
+    for (int e = lane; e < stage * a.nX; e += 64) {
+      const int j = e / a.nX, c = e - j * a.nX;
+      Cx<float> *x = reinterpret_cast<Cx<float> *>(dst + j * rec_stride) + c;
+      const float qj = sq[j];
+      Cx<float> v = *x;
+      v.re = j < cnt ? v.re * qj : 0.f;
+      v.im = j < cnt ? v.im * qj : 0.f;
+      *x = v;
+    }
+    if (cnt == stage) return;
+    for (int o = cnt * a.TS * 8 + lane * 4; o < stage * a.TS * 8; o += 64 * 4)
+      *reinterpret_cast<float *>(dst + o) = 0.f;
+    for (int o = cnt * a.nZ * 16 + lane * 16; o < stage * a.nZ * 16; o += 64 * 16)
+      *reinterpret_cast<u4v *>(dst + rec_bytes + o) = u4v{0u, 0u, 0u, 0u};
+  };
+
   int b = 0;
-  if (producer && a0 < a1) issue(a0, 0);
+  if (producer && a0 < a1) {
+    issue(a0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    finish(a0, 0);
+  }
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (long base = a0; base < a1; base += stage) {
     if (producer) {
-      if (base + stage < a1) issue(base + stage, b ^ 1);
+      if (base + stage < a1) {
+        issue(base + stage, b ^ 1);
+        __builtin_amdgcn_s_waitcnt(0);
+        finish(base + stage, b ^ 1);
+      }
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       b ^= 1;
       continue;
     }
-    const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
-    const unsigned char *cur = smem_raw + b * buf_bytes;
-    const Cx<float> *recs = reinterpret_cast<const Cx<float> *>(cur);
-    const unsigned char *zs = cur + rec_bytes;
-    const float *sq = reinterpret_cast<const float *>(cur + rec_bytes + z_bytes);
+    // LDS offsets of this stage, 32-bit; rows and columns beyond the lattice
+    // read clamped (finite) entries and are masked when the partials are written
+    const unsigned cur = (unsigned)(b * buf_bytes);
     if (active) {
       for (int ks = 0; ks < stage / 16; ++ks) {
         const int j0 = 16 * ks + 4 * lk;
         u4v ahi[RT], alo[RT];
 #pragma unroll
-        for (int r = 0; r < RT; ++r) {
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + u;
+          const Cx<float> *rec = reinterpret_cast<const Cx<float> *>(smem_raw + cur + j * rec_stride);
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int j = j0 + u;
-            const bool ok = j < cnt && rvalid[r];
-            const Cx<float> *rec = recs + (ok ? j : 0) * a.TS;
+          for (int r = 0; r < RT; ++r) {
             const Cx<float> X = rec[rnx[r]], Y = rec[offY + rny[r]];
-            const float qj = ok ? sq[j] : 0.f;
             const float pr = X.re * Y.re - X.im * Y.im;
             const float pi = X.re * Y.im + X.im * Y.re;
             unsigned h, l;
-            split_pair(qj * pr, qj * pi, h, l);
+            split_pair_mix(pr, pi, h, l);
             ahi[r][u] = h;
             alo[r][u] = l;
           }
@@ -744,11 +790,10 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
           u4v bhi, blo;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const int j = j0 + u;
-            const bool ok = j < cnt && zvalid[s];
-            const uint2 w = *reinterpret_cast<const uint2 *>(zs + (size_t)(ok ? j : 0) * a.nZ * 16 + zoff[s]);
-            bhi[u] = ok ? w.x : 0u;
-            blo[u] = ok ? w.y : 0u;
+            const uint2 w = *reinterpret_cast<const uint2 *>(smem_raw + cur + rec_bytes +
+                                                             (j0 + u) * z_stride + zoff[s]);
+            bhi[u] = w.x;
+            blo[u] = w.y;
           }
 #pragma unroll
           for (int r = 0; r < RT; ++r) {
@@ -772,7 +817,8 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
       for (int e = 0; e < 4; ++e) {
         const int row = (rt0 + r) * 16 + 4 * lk + e;
         const int col = s * 16 + lr;
-        a.part[((long)chunk * a.Rpad + row) * a.SC + col] = (double)acc[r][s][e];
+        const bool ok = row < a.nX * a.nY && (col >> 1) < a.nZ;
+        a.part[((long)chunk * a.Rpad + row) * a.SC + col] = ok ? (double)acc[r][s][e] : 0.0;
       }
   }
 }

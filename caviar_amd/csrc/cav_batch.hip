@@ -161,7 +161,7 @@ static void batch_free(cav_batch *b)
                   b->aox, b->aoy, b->aoz, b->type_o, b->q_o, b->snap, b->bin_of, b->bin_cnt,
                   b->bin_start, b->bin_items, b->gblk, b->nl, b->nn, b->flags, b->d_minv, b->d_lj,
                   b->part, b->vir, b->d_W, b->tab, b->ztab, b->bmat, b->bmat_h, b->sk_part, b->sk,
-                  b->epart, b->s_items, b->f_items, b->rchunks, b->cpos, b->d_raw_eps,
+                  b->epart, b->s_items, b->s_items_o, b->f_items, b->rchunks, b->cpos, b->d_raw_eps,
                   b->d_raw_sig, b->d_charge};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -196,7 +196,7 @@ static int batch_layout(cav_batch *b)
   b->rce2 = b->rce * b->rce;
   b->thr_sq = 0.25 * (b->rce - b->nl_cutoff) * (b->rce - b->nl_cutoff);
   b->cinv = 1.0 / b->rce;
-  long slots = 0, own = 0;
+  long slots = 0, own = 0, tabs = 0;
   int cells = 0, maxn = 0;
   b->h_cell_rep.clear();
   b->h_st_off.clear();
@@ -222,8 +222,10 @@ static int batch_layout(cav_batch *b)
     d.gcap = h.gcap;
     d.base = (int)slots;
     d.obase = (int)own;
+    d.tbase = (int)tabs;
     slots += ((long)h.n + h.gcap + 63) / 64 * 64;
     own += h.n;
+    tabs += ((long)h.n + 31) / 32 * 32;
     maxn = std::max(maxn, h.n);
     // cells and the de-duplicated periodic 27-stencil (neighborlist.cpp:581-697)
     d.cbase = cells;
@@ -260,6 +262,7 @@ static int batch_layout(cav_batch *b)
   b->h_st_off.push_back((int)b->h_st.size());
   b->nslots = slots;
   b->nown = own;
+  b->ntab = tabs;
   b->ncells = cells;
   b->maxn = maxn;
   b->scan_blocks = grid_for(maxn);
@@ -769,12 +772,12 @@ int cav_batch_autotune(cav_batch *b)
   (void)hipSetDevice(b->device);
   struct Cand { int rt, stage, chunk, fnt; };
   std::vector<Cand> cands;
-  const int rts[3] = {1, 2, 3}, stages[2] = {16, 32}, chunks[3] = {256, 512, 1024};
+  // the split S pass stages one 32-atom table block at a time
+  const int rts[3] = {1, 2, 3}, chunks[3] = {256, 512, 1024};
   if (b->precision == CAV_MIXED) {
     for (int rt : rts)
-      for (int st : stages)
-        for (int ch : chunks)
-          for (int nt : {1, 2}) cands.push_back({rt, st, ch, nt});
+      for (int ch : chunks)
+        for (int nt : {1, 2}) cands.push_back({rt, 32, ch, nt});
   } else {
     for (int ch : chunks) cands.push_back({b->s_rt, b->s_stage, ch, b->f_nt});
   }

@@ -82,8 +82,7 @@ __device__ __forceinline__ void split_pair(float a, float b, unsigned &hi, unsig
 // split_pair's).
 __device__ __forceinline__ void split_pair_mix(float a, float b, unsigned &hi, unsigned &lo)
 {
-  const _Float16 ah = (_Float16)a, bh = (_Float16)b;
-  hi = pack_h2(ah, bh);
+  asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(hi) : "v"(a), "v"(b));
   unsigned l;
   asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]" : "=v"(l) : "v"(a), "v"(hi));
   asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
@@ -580,41 +579,71 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
 // sB chosen from the bound |g| <= max W * sum |q|) so that no half overflows
 // and the scales are removed exactly afterwards.
 
-// Tables of the split form: per atom an FP32 record [X(0..kx) | Y(-ky..ky)]
-// (TXY entries) and the split Z records (nZ x 16 B).
-__global__ void k_tables_h(long npts, int kx, int ky, int kz, double ilx, double ily, double ilz,
-                           const double *x, const double *y, const double *z, Cx<float> *xy,
-                           u4v *ztab)
+// Tables of the split form, blocked: atoms in blocks of kTabAtoms = 32.  Per
+// block, each component of the FP32 record [X(0..kx) | Y(-ky..ky)] is one row
+// of kTabRow = 272 bytes (32 complex values, then 16 bytes of padding so that
+// the rows a wave reads side by side start in different LDS banks), and each
+// real-embedding column of Z (2 nZ of them: (Zr, -Zi) and (Zi, Zr) per nz) is
+// one row [32 hi words | 32 lo words | 16 bytes].  A block is copied to LDS as
+// it lies and read there with compile-time offsets; atoms past the end of a
+// block range are zero.
+constexpr int kTabAtoms = 32, kTabRow = 272;
+
+__device__ __forceinline__ void put_split(unsigned char *xy, unsigned char *zt, long blk, int e,
+                                          int j, int TXY, int nZ, float cs, float sn)
 {
-  const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= npts * TS) return;
-  const long a = t / TS;
-  const int e = (int)(t - a * TS);
-  double u;
-  int m;
-  if (e < nX) {
-    m = e;
-    u = x[a] * ilx;
-  } else if (e < nX + nY) {
-    m = e - nX - ky;
-    u = y[a] * ily;
-  } else {
-    m = e - nX - nY - kz;
-    u = z[a] * ilz;
-  }
-  double f = (double)m * u;
-  f -= rint(f);
-  float sn, cs;
-  sincospif((float)(2.0 * f), &sn, &cs);
-  if (e < nX + nY) {
-    xy[a * (nX + nY) + e] = Cx<float>{cs, sn};
+  if (e < TXY) {
+    *reinterpret_cast<Cx<float> *>(xy + (blk * TXY + e) * kTabRow + j * 8) = Cx<float>{cs, sn};
   } else {
     unsigned h0, l0, h1, l1;
     split_pair(cs, -sn, h0, l0);
     split_pair(sn, cs, h1, l1);
-    ztab[a * nZ + (e - nX - nY)] = u4v{h0, l0, h1, l1};
+    unsigned char *row = zt + (blk * 2 * nZ + 2 * (e - TXY)) * kTabRow + j * 4;
+    *reinterpret_cast<unsigned *>(row) = h0;
+    *reinterpret_cast<unsigned *>(row + 128) = l0;
+    *reinterpret_cast<unsigned *>(row + kTabRow) = h1;
+    *reinterpret_cast<unsigned *>(row + kTabRow + 128) = l1;
   }
+}
+
+// (cos, sin) of 2 pi m u for table entry e of an atom at (px, py, pz) / box.
+__device__ __forceinline__ void split_phase(int e, int nX, int nY, int ky, int kz, double ux,
+                                            double uy, double uz, float &cs, float &sn)
+{
+  double u;
+  int m;
+  if (e < nX) {
+    m = e;
+    u = ux;
+  } else if (e < nX + nY) {
+    m = e - nX - ky;
+    u = uy;
+  } else {
+    m = e - nX - nY - kz;
+    u = uz;
+  }
+  double f = (double)m * u;
+  f -= rint(f);
+  sincospif((float)(2.0 * f), &sn, &cs);
+}
+
+// One thread per (block, entry, atom of the block): consecutive threads write
+// consecutive atoms of one row.
+__global__ void k_tables_h(long npts, int kx, int ky, int kz, double ilx, double ily, double ilz,
+                           const double *x, const double *y, const double *z, unsigned char *xy,
+                           unsigned char *zt)
+{
+  const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
+  const long nblk = (npts + kTabAtoms - 1) / kTabAtoms;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nblk * TS * kTabAtoms) return;
+  const int j = (int)(t & (kTabAtoms - 1));
+  const long be = t / kTabAtoms, blk = be / TS;
+  const int e = (int)(be - blk * TS);
+  const long a = blk * kTabAtoms + j;
+  float cs = 0.f, sn = 0.f;
+  if (a < npts) split_phase(e, nX, nY, ky, kz, x[a] * ilx, y[a] * ily, z[a] * ilz, cs, sn);
+  put_split(xy, zt, blk, e, j, nX + nY, nZ, cs, sn);
 }
 
 __device__ __forceinline__ f4 mfma_h(u4v a, u4v b, f4 c)
@@ -627,25 +656,31 @@ struct SArgsH {
   int n, TS, nX, nY, nZ, Rtiles, chunk, stage, tiles_per_group, Rpad, SC;
   float sa;
   const double *q;
-  const void *tab;   // FP32 records [X | Y] per atom (TS = nX + nY entries)
-  const void *ztab;  // split Z records, nZ x 16 B per atom
+  const void *tab;   // blocked FP32 records [X | Y] (TS = nX + nY rows per block)
+  const void *ztab;  // blocked split Z (2 nZ rows per block)
   double *part;      // [chunk][Rpad][SC]
-  const int4 *items; // batched: as SArgs::items
+  const int4 *items; // batched: (first, end) table position, chunk slot, replica
+  const RepDev *rep; // batched: charges of replica r at q[obase + position - tbase]
   int ncols, groups; // logical grid, as SArgs
 };
 
 // Same work split as k_sk: a workgroup is an atom chunk x a group of row
-// tiles, wave w owns RT row tiles and every column tile; a K-step is 16 atoms.
-// Lane l feeds rows / columns l%16 with atoms 4(l/16)..4(l/16)+3 of the step,
-// (re, im) of each atom adjacent along the reduction axis.
+// tiles, wave w owns RT row tiles and every column tile; a K-step is 16 atoms
+// and an LDS stage is one 32-atom table block.  Lane l feeds rows / columns
+// l%16 with atoms 4(l/16)..4(l/16)+3 of the step, (re, im) of each atom
+// adjacent along the reduction axis.  In the blocked layout a lane's four
+// atoms are adjacent in a row, so each operand is read with two (records) or
+// one (Z half) 16-byte LDS read at a compile-time offset from a per-lane base.
 template <int RT, int SCT>
 __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
 {
   extern __shared__ unsigned char smem_raw[];
-  const int stage = a.stage;
-  const size_t rec_bytes = (size_t)stage * a.TS * sizeof(Cx<float>);
-  const size_t z_bytes = (size_t)stage * a.nZ * 16;
-  const size_t buf_bytes = rec_bytes + z_bytes + (size_t)stage * sizeof(float);
+  constexpr int S = kTabAtoms;
+  const unsigned rec_bytes = (unsigned)a.TS * kTabRow;     // [X | Y] rows of a block
+  const unsigned zrow_bytes = (unsigned)(2 * a.nZ) * kTabRow;  // Z rows of a block
+  // per buffer: records, SC Z rows (rows past 2 nZ are never written: their
+  // columns are masked on output), the block's scaled charges
+  const unsigned buf_bytes = rec_bytes + (unsigned)a.SC * kTabRow + S * 4;
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
   const unsigned char *ztab = reinterpret_cast<const unsigned char *>(a.ztab);
 
@@ -666,11 +701,13 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   long a0 = (long)chunk * a.chunk;
   long a1 = a0 + a.chunk;
   if (a1 > a.n) a1 = a.n;
+  long qoff = 0;
   if (a.items) {
     const int4 it = a.items[col];
     a0 = it.x;
     a1 = it.y;
     chunk = it.z;
+    qoff = (long)a.rep[it.w].obase - a.rep[it.w].tbase;
   }
 
   f4 acc[RT][SCT];
@@ -680,134 +717,114 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
     for (int s = 0; s < SCT; ++s) acc[r][s] = f4{0, 0, 0, 0};
 
   const int lr = lane & 15, lk = lane >> 4;
-  int rnx[RT], rny[RT];
-  bool rvalid[RT];
+  // per-lane LDS offsets inside a buffer; rows past the lattice read row 0
+  // (finite) and are masked on output
+  unsigned xo[RT], yo[RT];
 #pragma unroll
   for (int r = 0; r < RT; ++r) {
     const int row = (rt0 + r) * 16 + lr;
-    rvalid[r] = (rt0 + r) < rt_end && row < a.nX * a.nY;
-    rnx[r] = rvalid[r] ? row / a.nY : 0;
-    rny[r] = rvalid[r] ? row % a.nY : 0;
+    const bool ok = (rt0 + r) < rt_end && row < a.nX * a.nY;
+    xo[r] = (unsigned)(ok ? row / a.nY : 0) * kTabRow + 32 * lk;
+    yo[r] = (unsigned)(a.nX + (ok ? row % a.nY : 0)) * kTabRow + 32 * lk;
   }
-  int zoff[SCT];  // byte offset of this lane's column inside an atom's split-Z block
-  bool zvalid[SCT];
-#pragma unroll
-  for (int s = 0; s < SCT; ++s) {
-    const int col = s * 16 + lr, zc = col >> 1;
-    zvalid[s] = zc < a.nZ;
-    zoff[s] = (zvalid[s] ? zc : 0) * 16 + (col & 1) * 8;
-  }
-  const int offY = a.nX;
+  const unsigned zo = rec_bytes + (unsigned)lr * kTabRow + 16 * lk;
   const bool active = rt0 < rt_end;
-  const int rec_stride = a.TS * (int)sizeof(Cx<float>), z_stride = a.nZ * 16;
 
-  // The last wave of the workgroup is the producer: it streams each stage's
-  // records, split Z rows and scaled charges into the other LDS buffer while
-  // the compute waves multiply the current one.  The compute waves issue no
-  // asynchronous LDS writes themselves, so their LDS reads never wait on them.
+  // The last wave of the workgroup is the producer: it streams each block's
+  // record and Z rows into the other LDS buffer while the compute waves
+  // multiply the current one, then folds each atom's scaled charge into its X
+  // entries (zero for atoms outside the chunk), so a compute wave forms q X Y
+  // with one complex product.  The compute waves issue no asynchronous LDS
+  // writes themselves, so their LDS reads never wait on them.
   const bool producer = wave == nwaves - 1;
   auto issue = [&](long base, int b) {
     unsigned char *dst = smem_raw + b * buf_bytes;
-    const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
-    const int rb = (int)(((size_t)cnt * a.TS * sizeof(Cx<float>) + 15) / 16 * 16);
-    copy_to_lds_async(tab + (size_t)base * a.TS * sizeof(Cx<float>), dst, rb, 0, 1, lane);
-    copy_to_lds_async(ztab + (size_t)base * a.nZ * 16, dst + rec_bytes, cnt * a.nZ * 16, 0, 1,
-                      lane);
-    if (lane < stage)
-      reinterpret_cast<float *>(dst + rec_bytes + z_bytes)[lane] =
-          lane < cnt ? (float)a.q[base + lane] * a.sa : 0.f;
+    const long blk = base / S;
+    copy_to_lds_async(tab + blk * rec_bytes, dst, rec_bytes, 0, 1, lane);
+    copy_to_lds_async(ztab + blk * zrow_bytes, dst + rec_bytes, zrow_bytes, 0, 1, lane);
+    if (lane < S)
+      reinterpret_cast<float *>(dst + rec_bytes + a.SC * kTabRow)[lane] =
+          base + lane < a1 ? (float)a.q[base + lane + qoff] * a.sa : 0.f;
   };
-
-  // Once a stage's copies have landed the producer folds each atom's scaled
-  // charge into its X entries, so a compute wave forms q X Y with one complex
-  // product, and it clears a partial stage's missing atoms (zero records, zero
-  // Z), so the compute waves need no per-atom bounds (the copy is rounded up
-  // to 16 bytes and may have touched the first of them).
-  auto finish = [&](long base, int b) {
-    const int cnt = (int)((a1 - base) < stage ? (a1 - base) : stage);
+  auto finish = [&](int b) {
     unsigned char *dst = smem_raw + b * buf_bytes;
-    const float *sq = reinterpret_cast<const float *>(dst + rec_bytes + z_bytes);
-    for (int e = lane; e < stage * a.nX; e += 64) {
-      const int j = e / a.nX, c = e - j * a.nX;
-      Cx<float> *x = reinterpret_cast<Cx<float> *>(dst + j * rec_stride) + c;
+    const float *sq = reinterpret_cast<const float *>(dst + rec_bytes + a.SC * kTabRow);
+    for (int e = lane; e < a.nX * S; e += 64) {
+      const int c = e / S, j = e % S;
+      Cx<float> *x = reinterpret_cast<Cx<float> *>(dst + c * kTabRow + j * 8);
       const float qj = sq[j];
       Cx<float> v = *x;
-      v.re = j < cnt ? v.re * qj : 0.f;
-      v.im = j < cnt ? v.im * qj : 0.f;
+      v.re = qj != 0.f ? v.re * qj : 0.f;
+      v.im = qj != 0.f ? v.im * qj : 0.f;
       *x = v;
     }
-    if (cnt == stage) return;
-    for (int o = cnt * a.TS * 8 + lane * 4; o < stage * a.TS * 8; o += 64 * 4)
-      *reinterpret_cast<float *>(dst + o) = 0.f;
-    for (int o = cnt * a.nZ * 16 + lane * 16; o < stage * a.nZ * 16; o += 64 * 16)
-      *reinterpret_cast<u4v *>(dst + rec_bytes + o) = u4v{0u, 0u, 0u, 0u};
   };
 
   int b = 0;
   if (producer && a0 < a1) {
     issue(a0, 0);
     __builtin_amdgcn_s_waitcnt(0);
-    finish(a0, 0);
+    finish(0);
   }
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  for (long base = a0; base < a1; base += stage) {
-    if (producer) {
-      if (base + stage < a1) {
-        issue(base + stage, b ^ 1);
+  // three loops with the same barriers: producer, compute waves, and compute
+  // waves with no row tile in this group (no accumulators live across them)
+  if (producer) {
+    for (long base = a0; base < a1; base += S) {
+      if (base + S < a1) {
+        issue(base + S, b ^ 1);
         __builtin_amdgcn_s_waitcnt(0);
-        finish(base + stage, b ^ 1);
+        finish(b ^ 1);
       }
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       b ^= 1;
-      continue;
     }
-    // LDS offsets of this stage, 32-bit; rows and columns beyond the lattice
-    // read clamped (finite) entries and are masked when the partials are written
-    const unsigned cur = (unsigned)(b * buf_bytes);
-    if (active) {
-      for (int ks = 0; ks < stage / 16; ++ks) {
-        const int j0 = 16 * ks + 4 * lk;
-        u4v ahi[RT], alo[RT];
+    return;
+  }
+  if (!active) {
+    for (long base = a0; base < a1; base += S) __syncthreads();
+    return;
+  }
+  for (long base = a0; base < a1; base += S) {
+    const unsigned cur = (unsigned)b * buf_bytes;
+#pragma unroll
+    for (int ks = 0; ks < S / 16; ++ks) {
+      u4v ahi[RT], alo[RT];
+#pragma unroll
+      for (int r = 0; r < RT; ++r) {
+        const float4 *px = reinterpret_cast<const float4 *>(smem_raw + cur + xo[r] + 128 * ks);
+        const float4 *py = reinterpret_cast<const float4 *>(smem_raw + cur + yo[r] + 128 * ks);
+        const float4 x01 = px[0], x23 = px[1], y01 = py[0], y23 = py[1];
+        const float xr[4] = {x01.x, x01.z, x23.x, x23.z}, xi[4] = {x01.y, x01.w, x23.y, x23.w};
+        const float yr[4] = {y01.x, y01.z, y23.x, y23.z}, yi[4] = {y01.y, y01.w, y23.y, y23.w};
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int j = j0 + u;
-          const Cx<float> *rec = reinterpret_cast<const Cx<float> *>(smem_raw + cur + j * rec_stride);
-#pragma unroll
-          for (int r = 0; r < RT; ++r) {
-            const Cx<float> X = rec[rnx[r]], Y = rec[offY + rny[r]];
-            const float pr = X.re * Y.re - X.im * Y.im;
-            const float pi = X.re * Y.im + X.im * Y.re;
-            unsigned h, l;
-            split_pair_mix(pr, pi, h, l);
-            ahi[r][u] = h;
-            alo[r][u] = l;
-          }
+          const float pr = xr[u] * yr[u] - xi[u] * yi[u];
+          const float pi = xr[u] * yi[u] + xi[u] * yr[u];
+          unsigned h, l;
+          split_pair_mix(pr, pi, h, l);
+          ahi[r][u] = h;
+          alo[r][u] = l;
         }
+      }
 #pragma unroll
-        for (int s = 0; s < SCT; ++s) {
-          u4v bhi, blo;
+      for (int s = 0; s < SCT; ++s) {
+        const unsigned char *pz = smem_raw + cur + zo + s * 16 * kTabRow + 64 * ks;
+        const u4v bhi = *reinterpret_cast<const u4v *>(pz);
+        const u4v blo = *reinterpret_cast<const u4v *>(pz + 128);
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const uint2 w = *reinterpret_cast<const uint2 *>(smem_raw + cur + rec_bytes +
-                                                             (j0 + u) * z_stride + zoff[s]);
-            bhi[u] = w.x;
-            blo[u] = w.y;
-          }
-#pragma unroll
-          for (int r = 0; r < RT; ++r) {
-            acc[r][s] = mfma_h(ahi[r], bhi, acc[r][s]);
-            acc[r][s] = mfma_h(ahi[r], blo, acc[r][s]);
-            acc[r][s] = mfma_h(alo[r], bhi, acc[r][s]);
-          }
+        for (int r = 0; r < RT; ++r) {
+          acc[r][s] = mfma_h(ahi[r], bhi, acc[r][s]);
+          acc[r][s] = mfma_h(ahi[r], blo, acc[r][s]);
+          acc[r][s] = mfma_h(alo[r], bhi, acc[r][s]);
         }
       }
     }
     __syncthreads();
     b ^= 1;
   }
-  if (!active) return;
 #pragma unroll
   for (int r = 0; r < RT; ++r) {
     if (rt0 + r >= rt_end) break;
@@ -839,29 +856,31 @@ struct FArgsH {
 
 // Force pass of the split form: A = the atoms' Z (kappa = 2 nz + re/im, from
 // the split records), B = the operand blocks staged in LDS per row block and
-// shared by the workgroup's waves.  The workgroup's [X | Y] records are staged
+// shared by the workgroup's waves.  The workgroup's blocked [X | Y] rows are staged
 // in LDS once, so the epilogue's per-(atom, row) phase factors are LDS reads.
 template <int NK, int NT, int W>
 __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
 {
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
-  const u4v *ztab = reinterpret_cast<const u4v *>(a.ztab);
+  const unsigned char *ztab = reinterpret_cast<const unsigned char *>(a.ztab);
   const unsigned char *bm = reinterpret_cast<const unsigned char *>(a.bmh);
   extern __shared__ unsigned char smem_raw[];
   const int nwaves = blockDim.x >> 6;
   const int blk_bytes = 4 * a.nkh * 2 * 64 * 16;
-  const int wg_atoms = 16 * NT * nwaves;
-  const int xy_bytes = wg_atoms * a.TS * (int)sizeof(Cx<float>);
+  const int wg_atoms = 16 * NT * nwaves;  // a multiple of kTabAtoms
+  const int xy_bytes = wg_atoms / kTabAtoms * a.TS * kTabRow;
   // operand buffers at smem_raw + xy_bytes + {0, blk_bytes}, addressed from the
   // __shared__ array itself so the compiler emits LDS (not flat) accesses
   const Cx<float> *lxy = reinterpret_cast<const Cx<float> *>(smem_raw);
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   long wg_base = (long)blockIdx.x * wg_atoms;
+  long toff = 0;  // table position - atom index
   if (a.items) {
     const int4 it = a.items[blockIdx.x];
     const RepDev &r = a.rep[it.w];
     wg_base = it.x;
+    toff = (long)r.tbase - r.obase;
     a.n = it.y;
     bm += it.w * a.bm_stride;
     a.c0 = r.c0 * a.inv_sb;
@@ -872,9 +891,11 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
   const long base = wg_base + (long)wave * (16 * NT);
   const int offY = a.nX;
   {
+    // the workgroup's table blocks (its first atom starts a block)
     const long cnt = min((long)wg_atoms, a.n - wg_base);
-    copy_to_lds_async(tab + (size_t)wg_base * a.TS * sizeof(Cx<float>), smem_raw,
-                      (int)(cnt * a.TS * sizeof(Cx<float>) + 15) / 16 * 16, wave, nwaves, lane);
+    const int nb = (int)((cnt + kTabAtoms - 1) / kTabAtoms);
+    copy_to_lds_async(tab + (wg_base + toff) / kTabAtoms * a.TS * kTabRow, smem_raw,
+                      nb * a.TS * kTabRow, wave, nwaves, lane);
   }
 
   // A fragments: atom l%16 of tile tau, kappa = 32 s + 8 (l/16) + i, i.e.
@@ -891,9 +912,11 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
         const int c = 16 * s + 4 * lk + u;
         unsigned h = 0, l = 0;
         if (atom < a.n && c < a.nZ && s < a.nkh) {
-          const u4v rec = ztab[atom * a.nZ + c];
-          h = rec[0] ^ 0x80000000u;
-          l = rec[1] ^ 0x80000000u;
+          const long t = atom + toff;
+          const unsigned char *row =
+              ztab + ((t / kTabAtoms) * 2 * a.nZ + 2 * c) * kTabRow + (t % kTabAtoms) * 4;
+          h = *reinterpret_cast<const unsigned *>(row) ^ 0x80000000u;
+          l = *reinterpret_cast<const unsigned *>(row + 128) ^ 0x80000000u;
         }
         ahi[tau][s][u] = h;
         alo[tau][s][u] = l;
@@ -966,8 +989,9 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
         for (int e = 0; e < 4; ++e) {
           const long atom = base + tau * 16 + 4 * lk + e;
           if (atom >= a.n) continue;
-          const Cx<float> *rec = lxy + (atom - wg_base) * a.TS;
-          const Cx<float> X = rec[nxi], Y = rec[offY + nyi];
+          const int rel = (int)(atom - wg_base);
+          const Cx<float> *rec = lxy + (rel / kTabAtoms) * a.TS * (kTabRow / 8) + rel % kTabAtoms;
+          const Cx<float> X = rec[nxi * (kTabRow / 8)], Y = rec[(offY + nyi) * (kTabRow / 8)];
           const float pr = X.re * Y.re - X.im * Y.im;
           const float pi = X.re * Y.im + X.im * Y.re;
           const float ur = acc[0][tau][e], ui = acc[1][tau][e];
@@ -1242,8 +1266,11 @@ int ewald_k_alloc(cav_ctx *c)
   const EwaldKPlan &p = c->ek;
   const size_t tsz = c->precision == CAV_FP64 ? sizeof(double) : sizeof(float);
   const int TS = p.nX + p.nY + p.nZ;
-  // padded: the S pass streams whole 16-byte chunks past the last record
-  const size_t tab_bytes = (size_t)c->n * TS * 2 * tsz + 4096;
+  // padded: the S pass streams whole 16-byte chunks past the last record; the
+  // split form's blocked rows (whole 32-atom blocks) fit in the same buffer
+  const size_t nblk = ((size_t)c->n + kTabAtoms - 1) / kTabAtoms;
+  const size_t tab_bytes =
+      std::max((size_t)c->n * TS * 2 * tsz, nblk * (p.nX + p.nY) * kTabRow) + 4096;
   if (c->tab) (void)hipFree(c->tab);
   CAV_HIP(hipMalloc(&c->tab, tab_bytes));
   c->tab_bytes = tab_bytes;
@@ -1271,7 +1298,7 @@ int ewald_k_alloc(cav_ctx *c)
   if (c->bmat_h) (void)hipFree(c->bmat_h);
   c->ztab = c->bmat_h = nullptr;
   if (c->precision == CAV_MIXED) {
-    CAV_HIP(hipMalloc(&c->ztab, (size_t)c->n * p.nZ * 16 + 4096));
+    CAV_HIP(hipMalloc(&c->ztab, nblk * 2 * p.nZ * kTabRow + 4096));
     const size_t bh_elems = (size_t)(p.Rpad / 16) * 4 * c->nkh * 2 * 64 * 8;
     CAV_HIP(hipMalloc(&c->bmat_h, bh_elems * sizeof(_Float16)));
     k_zero_t<_Float16><<<grid_for(bh_elems), kBlock, 0, c->stream>>>(bh_elems, (_Float16 *)c->bmat_h);
@@ -1433,6 +1460,13 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
   return CAV_OK;
 }
 
+// LDS of a split S-pass workgroup: two buffers of one table block each
+// ([X | Y] rows, SC Z rows, 32 scaled charges).
+static size_t split_sk_lds(const EwaldKPlan &p)
+{
+  return 2 * ((size_t)(p.nX + p.nY + roundup(2 * p.nZ, 16)) * kTabRow + kTabAtoms * sizeof(float));
+}
+
 template <int RT>
 static void launch_sk_h(hipStream_t st, SArgsH a, dim3 lgrid, size_t smem, int SCT, int waves)
 {
@@ -1460,10 +1494,11 @@ static int sk_pipeline_h(cav_ctx *c)
 {
   const EwaldKPlan &p = c->ek;
   const int TS = p.nX + p.nY + p.nZ;
+  const long nblk = (c->n + kTabAtoms - 1) / kTabAtoms;
   timer_begin(c, c->t_tab);
-  k_tables_h<<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
+  k_tables_h<<<grid_for(nblk * TS * kTabAtoms), kBlock, 0, c->stream>>>(
       c->n, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
-      (Cx<float> *)c->tab, (u4v *)c->ztab);
+      (unsigned char *)c->tab, (unsigned char *)c->ztab);
   timer_end(c, c->t_tab);
   SArgsH a;
   a.n = c->n;
@@ -1473,23 +1508,22 @@ static int sk_pipeline_h(cav_ctx *c)
   a.nZ = p.nZ;
   a.Rtiles = p.Rpad / 16;
   a.chunk = (c->n + c->sk_chunks - 1) / c->sk_chunks;
-  a.chunk = (a.chunk + 31) / 32 * 32;
-  const size_t per_atom = (size_t)a.TS * sizeof(Cx<float>) + (size_t)p.nZ * 16 + sizeof(float);
-  a.stage = 2 * 32 * per_atom <= 150 * 1024 ? 32 : 16;
-  if (c->tune_stage == 16) a.stage = 16;
+  a.chunk = (a.chunk + kTabAtoms - 1) / kTabAtoms * kTabAtoms;
+  a.stage = kTabAtoms;
   a.sa = (float)c->sA;
   a.q = c->q;
   a.tab = c->tab;
   a.ztab = c->ztab;
   a.part = c->sk_part;
   a.items = nullptr;
+  a.rep = nullptr;
   a.Rpad = p.Rpad;
   a.SC = roundup(2 * p.nZ, 16);
   const int RT = c->tune_rt;
   int groups, waves;
   s_layout_h(p.Rpad, RT, &groups, &a.tiles_per_group, &waves);
   const int nch = (c->n + a.chunk - 1) / a.chunk;
-  const size_t smem = 2 * (size_t)a.stage * per_atom;
+  const size_t smem = split_sk_lds(p);
   const dim3 lg(nch, groups);
   timer_begin(c, c->t_sk);
   if (RT == 1)
@@ -1511,12 +1545,12 @@ static int sk_pipeline_h(cav_ctx *c)
   return reduce_partials(c, grid_for(nthr), 3);
 }
 
-// LDS of the split force pass: the workgroup's [X | Y] records + two operand blocks
-// LDS of a split force-pass workgroup: two operand blocks plus the [X | Y]
-// records of its 16 nt w atoms (w waves).
+// LDS of a split force-pass workgroup: two operand blocks plus the blocked
+// [X | Y] rows of its 16 nt w atoms (w waves).
 static size_t split_force_lds_p(const EwaldKPlan &p, int nkh, int nt = 2, int w = kWavesF)
 {
-  return 2 * (size_t)4 * nkh * 2 * 64 * 16 + (size_t)16 * nt * w * (p.nX + p.nY) * sizeof(Cx<float>);
+  return 2 * (size_t)4 * nkh * 2 * 64 * 16 +
+         (size_t)16 * nt * w / kTabAtoms * (p.nX + p.nY) * kTabRow;
 }
 
 static size_t split_force_lds(const cav_ctx *c)
@@ -1591,7 +1625,7 @@ int launch_ewald_k(cav_ctx *c)
   if (c->precision == CAV_FP64) {
     CAV_TRY(sk_pipeline<double>(c));
     CAV_TRY(kforce<double>(c, c->n, c->tab, nullptr));
-  } else if (split_force_lds(c) <= 160 * 1024) {
+  } else if (split_force_lds(c) <= 160 * 1024 && split_sk_lds(c->ek) <= 160 * 1024) {
     CAV_TRY(sk_pipeline_h(c));
     CAV_TRY(kforce_h(c));
   } else {  // k lattices too wide for the split form's LDS tiles: FP32 tiles
@@ -1806,40 +1840,23 @@ __global__ void kb_tables(const RepDev *rep, int kx, int ky, int kz, const doubl
   tab[(r.obase + a) * TS + e] = v;
 }
 
+// Batched split tables: replica r's atoms fill the blocks from r.tbase.
 __global__ void kb_tables_h(const RepDev *rep, int kx, int ky, int kz, const double *x,
-                            const double *y, const double *z, Cx<float> *xy, u4v *ztab)
+                            const double *y, const double *z, unsigned char *xy, unsigned char *zt)
 {
   const RepDev &r = rep[blockIdx.y];
   const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
+  const long nblk = (r.n + kTabAtoms - 1) / kTabAtoms;
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= (long)r.n * TS) return;
-  const long a = t / TS;
-  const int e = (int)(t - a * TS);
-  const long s = r.base + a, o = r.obase + a;
-  double u;
-  int m;
-  if (e < nX) {
-    m = e;
-    u = x[s] * r.il[0];
-  } else if (e < nX + nY) {
-    m = e - nX - ky;
-    u = y[s] * r.il[1];
-  } else {
-    m = e - nX - nY - kz;
-    u = z[s] * r.il[2];
-  }
-  double f = (double)m * u;
-  f -= rint(f);
-  float sn, cs;
-  sincospif((float)(2.0 * f), &sn, &cs);
-  if (e < nX + nY) {
-    xy[o * (nX + nY) + e] = Cx<float>{cs, sn};
-  } else {
-    unsigned h0, l0, h1, l1;
-    split_pair(cs, -sn, h0, l0);
-    split_pair(sn, cs, h1, l1);
-    ztab[o * nZ + (e - nX - nY)] = u4v{h0, l0, h1, l1};
-  }
+  if (t >= nblk * TS * kTabAtoms) return;
+  const int j = (int)(t & (kTabAtoms - 1));
+  const long be = t / kTabAtoms, blk = be / TS;
+  const int e = (int)(be - blk * TS);
+  const long a = blk * kTabAtoms + j, s = r.base + a;
+  float cs = 0.f, sn = 0.f;
+  if (a < r.n)
+    split_phase(e, nX, nY, ky, kz, x[s] * r.il[0], y[s] * r.il[1], z[s] * r.il[2], cs, sn);
+  put_split(xy, zt, r.tbase / kTabAtoms + blk, e, j, nX + nY, nZ, cs, sn);
 }
 
 
@@ -1882,12 +1899,12 @@ int batch_ewald_k_plan(cav_batch *b)
 static int batch_free_k(cav_batch *b)
 {
   void *ptrs[] = {b->d_W, b->tab, b->ztab, b->bmat, b->bmat_h, b->sk_part, b->sk, b->epart,
-                  b->s_items, b->f_items, b->rchunks};
+                  b->s_items, b->s_items_o, b->f_items, b->rchunks};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   b->d_W = b->sk_part = b->sk = b->epart = nullptr;
   b->tab = b->ztab = b->bmat = b->bmat_h = nullptr;
-  b->s_items = b->f_items = nullptr;
+  b->s_items = b->s_items_o = b->f_items = nullptr;
   b->rchunks = nullptr;
   return CAV_OK;
 }
@@ -1901,8 +1918,10 @@ int batch_ewald_k_alloc(cav_batch *b)
   const bool split = batch_split(b);
   const size_t tsz = b->precision == CAV_FP64 ? sizeof(double) : sizeof(float);
   const int TS = p.nX + p.nY + p.nZ;
-  BAT_HIP(hipMalloc(&b->tab, (size_t)b->nown * TS * 2 * tsz + 4096));
-  if (split) BAT_HIP(hipMalloc(&b->ztab, (size_t)b->nown * p.nZ * 16 + 4096));
+  const size_t nblk = (size_t)b->ntab / kTabAtoms;
+  BAT_HIP(hipMalloc(&b->tab, std::max((size_t)b->nown * TS * 2 * tsz,
+                                      nblk * (p.nX + p.nY) * kTabRow) + 4096));
+  if (split) BAT_HIP(hipMalloc(&b->ztab, nblk * 2 * p.nZ * kTabRow + 4096));
   // S-pass items: chunks of s_chunk atoms per replica, largest replicas first
   std::vector<int4> sitems;
   std::vector<int2> rch(b->B);
@@ -1931,6 +1950,14 @@ int batch_ewald_k_alloc(cav_batch *b)
       if (it.w == r) sorted.push_back(it);
   b->n_sitems = (int)sorted.size();
   const int SC = roundup(2 * p.nZ, 16);
+  BAT_HIP(hipMalloc(&b->s_items_o, sizeof(int4) * std::max(1, b->n_sitems)));
+  BAT_HIP(hipMemcpy(b->s_items_o, sorted.data(), sizeof(int4) * sorted.size(), hipMemcpyHostToDevice));
+  if (split)  // the split form's blocked tables hold replica r from tbase
+    for (int4 &it : sorted) {
+      const int sh = b->reps[it.w].tbase - b->reps[it.w].obase;
+      it.x += sh;
+      it.y += sh;
+    }
   BAT_HIP(hipMalloc(&b->s_items, sizeof(int4) * std::max(1, b->n_sitems)));
   BAT_HIP(hipMemcpy(b->s_items, sorted.data(), sizeof(int4) * sorted.size(), hipMemcpyHostToDevice));
   BAT_HIP(hipMalloc(&b->rchunks, sizeof(int2) * b->B));
@@ -1960,7 +1987,7 @@ int batch_ewald_k_alloc(cav_batch *b)
   b->n_fitems = (int)fitems.size();
   BAT_HIP(hipMalloc(&b->f_items, sizeof(int4) * std::max(1, b->n_fitems)));
   BAT_HIP(hipMemcpy(b->f_items, fitems.data(), sizeof(int4) * fitems.size(), hipMemcpyHostToDevice));
-  if (split && split_force_lds_p(p, b->nkh) > 160 * 1024)
+  if (split && (split_force_lds_p(p, b->nkh) > 160 * 1024 || split_sk_lds(p) > 160 * 1024))
     return bfail(b, CAV_ERR_ARG, "k lattice too wide for the batched split form");
   return CAV_OK;
 }
@@ -2004,8 +2031,9 @@ static int batch_ewald_k_t(cav_batch *b)
     const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
     launch_sk_rt<T, kRT>(b->stream, a, dim3(b->n_sitems, groups), smem, SC / 16, waves);
   } else {
-    kb_tables_h<<<tgrid, kBlock, 0, b->stream>>>(b->d_rep, p.kx, p.ky, p.kz, b->x, b->y, b->z,
-                                                 (Cx<float> *)b->tab, (u4v *)b->ztab);
+    const dim3 hgrid(grid_for((long)(b->maxn + kTabAtoms - 1) / kTabAtoms * kTabAtoms * TS), b->B);
+    kb_tables_h<<<hgrid, kBlock, 0, b->stream>>>(b->d_rep, p.kx, p.ky, p.kz, b->x, b->y, b->z,
+                                                 (unsigned char *)b->tab, (unsigned char *)b->ztab);
     SArgsH a;
     a.n = 0;
     a.TS = p.nX + p.nY;
@@ -2014,9 +2042,7 @@ static int batch_ewald_k_t(cav_batch *b)
     a.nZ = p.nZ;
     a.Rtiles = p.Rpad / 16;
     a.chunk = b->s_chunk;
-    const size_t per_atom = (size_t)a.TS * sizeof(Cx<float>) + (size_t)p.nZ * 16 + sizeof(float);
-    a.stage = b->s_stage;
-    if (2 * (size_t)a.stage * per_atom > 150 * 1024) a.stage = 16;
+    a.stage = kTabAtoms;
     a.tiles_per_group = 0;
     a.sa = (float)b->sA;
     a.q = b->q_o;
@@ -2026,10 +2052,11 @@ static int batch_ewald_k_t(cav_batch *b)
     a.Rpad = p.Rpad;
     a.SC = SC;
     a.items = b->s_items;
+    a.rep = b->d_rep;
     const int RT = b->s_rt;
     s_layout_h(p.Rpad, RT, &groups, &tpg, &waves);
     a.tiles_per_group = tpg;
-    const size_t smem = 2 * (size_t)a.stage * per_atom;
+    const size_t smem = split_sk_lds(p);
     if (RT == 1)
       batch_sk_h<1>(b, a, groups, smem, SC / 16, waves);
     else if (RT == 2)
@@ -2191,7 +2218,7 @@ static int batch_wsk_t(cav_batch *b, const double *u, double *out)
   a.part = b->sk_part;
   a.Rpad = p.Rpad;
   a.SC = SC;
-  a.items = b->s_items;
+  a.items = b->s_items_o;
   const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
   launch_sk_rt<T, kRT>(b->stream, a, dim3(b->n_sitems, groups), smem, SC / 16, waves);
   const long plane = (long)p.Rpad * SC;
@@ -2220,11 +2247,10 @@ int ewald_k_autotune(cav_ctx *c)
   // error grows as sqrt(chunk)), FP64 across chunks; fewer chunks, fewer partials
   const int chunks[5] = {256, 512, 1024, 2048, 4096};
   if (c->precision == CAV_MIXED) {
-    for (int rt : {1, 2, 3, 4})
-      for (int st : {16, 32})
-        for (int ch : chunks)
-          for (int nt : {1, 2})
-            for (int fw : {kWavesF, 16}) cands.push_back({rt, st, ch, nt, fw});
+    for (int rt : {1, 2, 3, 4})  // the split S pass stages one 32-atom table block
+      for (int ch : chunks)
+        for (int nt : {1, 2})
+          for (int fw : {kWavesF, 16}) cands.push_back({rt, kTabAtoms, ch, nt, fw});
   } else {
     for (int ch : chunks) cands.push_back({3, 0, ch, 2, kWavesF});
   }

@@ -81,12 +81,13 @@ struct EwaldKPlan {
 
 // One replica of a batch as the device kernels see it (see cav_batch.hip).
 // Slot arrays hold each replica's owned atoms then its ghosts from `base`;
-// owned-indexed arrays (velocities, accelerations, Verlet rows, phase tables)
-// hold its owned atoms from `obase`.
+// owned-indexed arrays (velocities, accelerations, Verlet rows, FP64 phase
+// tables) hold its owned atoms from `obase`; the split form's blocked phase
+// tables hold them from `tbase`, a multiple of 32.
 struct RepDev {
   double lo[3], hi[3], L[3], il[3];
   double c0, kxs, kys, kzs;  // k-space force constants of this box
-  int base, n, gcap, obase;
+  int base, n, gcap, obase, tbase;
   int cbase, nbins, nb[3];
   int pbase;                 // first per-block partial slot
 };
@@ -245,7 +246,7 @@ struct cav_batch {
   std::vector<HostRep> hreps;
   std::vector<cav::RepDev> reps;
   int B = 0;
-  long nslots = 0, nown = 0;
+  long nslots = 0, nown = 0, ntab = 0;
   int ncells = 0, maxn = 0, pair_blocks = 0, scan_blocks = 0;
   std::vector<int> h_cell_rep, h_st_off, h_st;
 
@@ -282,7 +283,9 @@ struct cav_batch {
   double *d_W = nullptr;
   void *tab = nullptr, *ztab = nullptr, *bmat = nullptr, *bmat_h = nullptr;
   double *sk_part = nullptr, *sk = nullptr, *epart = nullptr;
-  int4 *s_items = nullptr, *f_items = nullptr;
+  // S-pass items in table positions (blocked in the split form) and in owned
+  // positions (FP32/FP64 tables)
+  int4 *s_items = nullptr, *s_items_o = nullptr, *f_items = nullptr;
   int2 *rchunks = nullptr;
   int n_sitems = 0, n_fitems = 0, n_chunks = 0, nkh = 0;
   long bm_stride = 0, bmh_stride = 0;  // bytes (FP operand) / halves (split operand) per replica

@@ -310,6 +310,14 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
 // order: for row block rb and component t, lane (kappa%4)*16 + rr holds its
 // KS values (kappa/4 = 0..) contiguously, so the force pass loads them as
 // 16-byte vectors.  Also the per-thread k-space energy term W |S|^2.
+// Split force-pass operand of one 16-row block: 4 components x nkh K-steps x
+// (hi, lo) x 64 lanes x 8 halves, then the z = 0 column's values of
+// components 0 and 1 (FP32, 16 rows each).  The z = 0 column of Z is exactly
+// (1, 0) for every atom, so its product is that per-row constant: it seeds
+// the accumulators instead of taking a K slot, and the K axis holds the
+// 2 (nZ - 1) other kappa (kappa' = 2 c' + re/im, c' = nz below kz, nz - 1 above).
+__host__ __device__ constexpr int split_blk_bytes(int nkh) { return 4 * nkh * 2 * 64 * 16 + 128; }
+
 template <typename T>
 __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz, int KSP,
                             const double *part, const double *W, double *sk, T *bm,
@@ -354,14 +362,21 @@ __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz
         const int lanef = (kap & 3) * 16 + rr;
         bm[(((long)rb * 4 + comp) * 64 + lanef) * KSP + (kap >> 2)] = (T)v[comp][sidx];
         if (bmh) {
-          // split form: K-step kap/32, lane (kap%32/8)*16 + rr, element kap%8;
-          // 64 lanes x 8 hi halves, then 64 lanes x 8 lo halves
-          const long unit = (((long)rb * 4 + comp) * nkh + (kap >> 5)) * 2;
-          const int ln = ((kap & 31) >> 3) * 16 + rr, el = kap & 7;
+          // split form: K-step kap'/32, lane (kap'%32/8)*16 + rr, element
+          // kap'%8; 64 lanes x 8 hi halves, then 64 lanes x 8 lo halves
+          _Float16 *blk = bmh + (long)rb * (split_blk_bytes(nkh) / 2);
           const float vf = (float)(v[comp][sidx] * sb);
-          const _Float16 h = (_Float16)vf;
-          bmh[(unit * 64 + ln) * 8 + el] = h;
-          bmh[((unit + 1) * 64 + ln) * 8 + el] = (_Float16)(vf - (float)h);
+          if (c == kz) {
+            if (comp < 2 && sidx == 0)
+              reinterpret_cast<float *>(blk + 4 * nkh * 2 * 64 * 8)[comp * 16 + rr] = vf;
+          } else {
+            const int kq = 2 * (c < kz ? c : c - 1) + sidx;
+            const int unit = (comp * nkh + (kq >> 5)) * 2;
+            const int ln = ((kq & 31) >> 3) * 16 + rr, el = kq & 7;
+            const _Float16 h = (_Float16)vf;
+            blk[(unit * 64 + ln) * 8 + el] = h;
+            blk[((unit + 1) * 64 + ln) * 8 + el] = (_Float16)(vf - (float)h);
+          }
         }
       }
   }
@@ -841,7 +856,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
 }
 
 struct FArgsH {
-  int n, TS, nX, nY, nZ, ky, R, Rpad, nkh;  // TS = nX + nY: FP32 [X | Y] record length
+  int n, TS, nX, nY, nZ, ky, kz, R, Rpad, nkh;  // TS = nX + nY: FP32 [X | Y] record length
   const void *tab, *ztab, *bmh;
   const double *q, *minv;
   const int *type;
@@ -866,7 +881,7 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
   const unsigned char *bm = reinterpret_cast<const unsigned char *>(a.bmh);
   extern __shared__ unsigned char smem_raw[];
   const int nwaves = blockDim.x >> 6;
-  const int blk_bytes = 4 * a.nkh * 2 * 64 * 16;
+  const int blk_bytes = split_blk_bytes(NK);
   const int wg_atoms = 16 * NT * nwaves;  // a multiple of kTabAtoms
   const int xy_bytes = wg_atoms / kTabAtoms * a.TS * kTabRow;
   // operand buffers at smem_raw + xy_bytes + {0, blk_bytes}, addressed from the
@@ -909,12 +924,13 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
     for (int s = 0; s < NK; ++s)
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int c = 16 * s + 4 * lk + u;
+        const int c = 16 * s + 4 * lk + u;  // c' (the z = 0 column is left out)
         unsigned h = 0, l = 0;
-        if (atom < a.n && c < a.nZ && s < a.nkh) {
+        if (atom < a.n && c < a.nZ - 1) {
           const long t = atom + toff;
+          const int zc = c < a.kz ? c : c + 1;
           const unsigned char *row =
-              ztab + ((t / kTabAtoms) * 2 * a.nZ + 2 * c) * kTabRow + (t % kTabAtoms) * 4;
+              ztab + ((t / kTabAtoms) * 2 * a.nZ + 2 * zc) * kTabRow + (t % kTabAtoms) * 4;
           h = *reinterpret_cast<const unsigned *>(row) ^ 0x80000000u;
           l = *reinterpret_cast<const unsigned *>(row + 128) ^ 0x80000000u;
         }
@@ -938,7 +954,7 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
     // into registers here and stored to the other LDS buffer after this block
     // is consumed: with no LDS write in flight, reads of the current block
     // need no wait on the prefetch.
-    constexpr int CH = 512 * NK, PER = (CH + 64 * W - 1) / (64 * W);
+    constexpr int CH = split_blk_bytes(NK) / 16, PER = (CH + 64 * W - 1) / (64 * W);
     const bool more = rb + 1 < nrb;
     u4v pf[PER];
     if (more) {
@@ -949,11 +965,17 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
         if (idx < CH) pf[k] = src[idx];
       }
     }
+    // accumulators seeded with the z = 0 column's products (components 0, 1)
+    const float *z0 = reinterpret_cast<const float *>(smem_raw + xy_bytes + cur * blk_bytes +
+                                                      4 * NK * 2 * 1024);
+    const float b0 = z0[lr], b1 = z0[16 + lr];
     f4 acc[4][NT];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int tau = 0; tau < NT; ++tau) acc[t][tau] = f4{0, 0, 0, 0};
+    for (int tau = 0; tau < NT; ++tau) {
+      acc[0][tau] = f4{b0, b0, b0, b0};
+      acc[1][tau] = f4{b1, b1, b1, b1};
+      acc[2][tau] = acc[3][tau] = f4{0, 0, 0, 0};
+    }
     // Per K-step the four components' fragments are loaded first; each split
     // product then sweeps all 4 x NT accumulators, so consecutive MFMAs are
     // independent (per accumulator the order stays hi.hi, hi.lo, lo.hi).
@@ -1189,8 +1211,8 @@ int ewald_k_plan(cav_ctx *c)
   if (const char *e = plan_k_lattice(p, c->kx, c->ky, c->kz, c->precision, c->ek_alpha, l[0], l[1], l[2]))
     return fail(c, CAV_ERR_ARG, e);
   if (c->dipole) p.dip_coef = c->ek_ke * CAV_4PI * p.lxyz_inv / (1 + 2.0 * c->eps_dipole);
-  // split form: K-steps of 32 along kappa
-  c->nkh = (2 * p.nZ + 31) / 32;
+  // split form: K-steps of 32 along the 2 (nZ - 1) kappa other than z = 0
+  c->nkh = std::max(1, (2 * (p.nZ - 1) + 31) / 32);
   double qmax = 0, qsum = 0, wmax = 0;
   for (int i = 0; i < c->n; ++i) {
     const double qa = std::fabs(c->charge[c->h_type[i]]);
@@ -1299,7 +1321,7 @@ int ewald_k_alloc(cav_ctx *c)
   c->ztab = c->bmat_h = nullptr;
   if (c->precision == CAV_MIXED) {
     CAV_HIP(hipMalloc(&c->ztab, nblk * 2 * p.nZ * kTabRow + 4096));
-    const size_t bh_elems = (size_t)(p.Rpad / 16) * 4 * c->nkh * 2 * 64 * 8;
+    const size_t bh_elems = (size_t)(p.Rpad / 16) * (split_blk_bytes(c->nkh) / 2);
     CAV_HIP(hipMalloc(&c->bmat_h, bh_elems * sizeof(_Float16)));
     k_zero_t<_Float16><<<grid_for(bh_elems), kBlock, 0, c->stream>>>(bh_elems, (_Float16 *)c->bmat_h);
   }
@@ -1549,7 +1571,7 @@ static int sk_pipeline_h(cav_ctx *c)
 // [X | Y] rows of its 16 nt w atoms (w waves).
 static size_t split_force_lds_p(const EwaldKPlan &p, int nkh, int nt = 2, int w = kWavesF)
 {
-  return 2 * (size_t)4 * nkh * 2 * 64 * 16 +
+  return 2 * (size_t)split_blk_bytes(nkh) +
          (size_t)16 * nt * w / kTabAtoms * (p.nX + p.nY) * kTabRow;
 }
 
@@ -1579,6 +1601,7 @@ static int kforce_h(cav_ctx *c)
   f.nY = p.nY;
   f.nZ = p.nZ;
   f.ky = p.ky;
+  f.kz = p.kz;
   f.R = p.R;
   f.Rpad = p.Rpad;
   f.nkh = c->nkh;
@@ -1891,7 +1914,7 @@ int batch_ewald_k_plan(cav_batch *b)
     for (double w : p.W) wmax = std::max(wmax, std::fabs(w));
     gb = std::max(gb, wmax * qsum);
   }
-  b->nkh = (2 * b->ek.nZ + 31) / 32;
+  b->nkh = std::max(1, (2 * (b->ek.nZ - 1) + 31) / 32);
   split_scales(qmax, gb > 0 ? 1.0 : 0.0, gb, &b->sA, &b->sB);
   return CAV_OK;
 }
@@ -1970,7 +1993,7 @@ int batch_ewald_k_alloc(cav_batch *b)
   BAT_HIP(hipMalloc(&b->bmat, b->bm_stride * tsz * b->B));
   BAT_HIP(hipMemset(b->bmat, 0, b->bm_stride * tsz * b->B));
   if (split) {
-    b->bmh_stride = (long)(p.Rpad / 16) * 4 * b->nkh * 2 * 64 * 8;
+    b->bmh_stride = (long)(p.Rpad / 16) * (split_blk_bytes(b->nkh) / 2);
     BAT_HIP(hipMalloc(&b->bmat_h, b->bmh_stride * sizeof(_Float16) * b->B));
     BAT_HIP(hipMemset(b->bmat_h, 0, b->bmh_stride * sizeof(_Float16) * b->B));
   }
@@ -2103,6 +2126,7 @@ static int batch_ewald_k_t(cav_batch *b)
     f.nY = p.nY;
     f.nZ = p.nZ;
     f.ky = p.ky;
+    f.kz = p.kz;
     f.R = p.R;
     f.Rpad = p.Rpad;
     f.nkh = b->nkh;

@@ -129,7 +129,7 @@ def lattice(kmax):
     rpad = -(-((kmax + 1) * (2 * kmax + 1)) // 16) * 16
     sc = -(-(2 * nz) // 16) * 16
     kz = -(-(2 * nz) // 4) * 4
-    nkh = -(-(2 * nz) // 32)
+    nkh = max(1, -(-(2 * (nz - 1)) // 32))  # split force pass: z = 0 seeds the accumulators
     return rpad, sc, kz, nkh
 
 

@@ -944,6 +944,18 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
 #pragma unroll
     for (int e = 0; e < 4; ++e) fx[tau][e] = fy[tau][e] = fz[tau][e] = ph[tau][e] = 0;
 
+  // LDS offset (in complex values) of the lane's atoms 4 (l/16) .. +3 of each
+  // tile in the blocked [X | Y] rows: adjacent, so one 32-byte read per row
+  int xoff[NT];
+#pragma unroll
+  for (int tau = 0; tau < NT; ++tau) {
+    const int rel = wave * 16 * NT + tau * 16 + 4 * lk;
+    xoff[tau] = (rel / kTabAtoms) * a.TS * (kTabRow / 8) + rel % kTabAtoms;
+  }
+  // lattice row of the lane, r = rb*16 + l%16 = nxi nY + nyi, kept incrementally;
+  // rows past R have zero operands, so they add nothing
+  int nxi = lr / a.nY, nyi = lr % a.nY;
+
   const int nrb = a.Rpad >> 4;
   copy_to_lds_async(bm, smem_raw + xy_bytes, blk_bytes, wave, nwaves, lane);
   __builtin_amdgcn_s_waitcnt(0);
@@ -963,6 +975,23 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
       for (int k = 0; k < PER; ++k) {
         const int idx = threadIdx.x + k * (64 * W);
         if (idx < CH) pf[k] = src[idx];
+      }
+    }
+    // phase factors X Y of the lane's (atom, row) pairs, formed while the
+    // products below run (atoms past n give sums that are never written)
+    const float wx = (float)nxi, wy = (float)(nyi - a.ky);
+    float pr[NT][4], pi[NT][4];
+#pragma unroll
+    for (int tau = 0; tau < NT; ++tau) {
+      const float4 *px = reinterpret_cast<const float4 *>(lxy + xoff[tau] + nxi * (kTabRow / 8));
+      const float4 *py = reinterpret_cast<const float4 *>(lxy + xoff[tau] + (offY + nyi) * (kTabRow / 8));
+      const float4 x01 = px[0], x23 = px[1], y01 = py[0], y23 = py[1];
+      const float xr[4] = {x01.x, x01.z, x23.x, x23.z}, xi[4] = {x01.y, x01.w, x23.y, x23.w};
+      const float yr[4] = {y01.x, y01.z, y23.x, y23.z}, yi[4] = {y01.y, y01.w, y23.y, y23.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        pr[tau][e] = xr[e] * yr[e] - xi[e] * yi[e];
+        pi[tau][e] = xr[e] * yi[e] + xi[e] * yr[e];
       }
     }
     // accumulators seeded with the z = 0 column's products (components 0, 1)
@@ -1001,29 +1030,22 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
 #pragma unroll
         for (int tau = 0; tau < NT; ++tau) acc[t][tau] = mfma_h(alo[tau][s], bh[t], acc[t][tau]);
     }
-    const int r = rb * 16 + lr;
-    if (r < a.R) {
-      const int nxi = r / a.nY, nyi = r - (r / a.nY) * a.nY;
-      const float wx = (float)nxi, wy = (float)(nyi - a.ky);
 #pragma unroll
-      for (int tau = 0; tau < NT; ++tau)
+    for (int tau = 0; tau < NT; ++tau)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const long atom = base + tau * 16 + 4 * lk + e;
-          if (atom >= a.n) continue;
-          const int rel = (int)(atom - wg_base);
-          const Cx<float> *rec = lxy + (rel / kTabAtoms) * a.TS * (kTabRow / 8) + rel % kTabAtoms;
-          const Cx<float> X = rec[nxi * (kTabRow / 8)], Y = rec[(offY + nyi) * (kTabRow / 8)];
-          const float pr = X.re * Y.re - X.im * Y.im;
-          const float pi = X.re * Y.im + X.im * Y.re;
-          const float ur = acc[0][tau][e], ui = acc[1][tau][e];
-          const float uzr = acc[2][tau][e], uzi = acc[3][tau][e];
-          const float h = pr * ui + pi * ur;
-          fx[tau][e] += wx * h;
-          fy[tau][e] += wy * h;
-          fz[tau][e] += pr * uzi + pi * uzr;
-          ph[tau][e] += pr * ur - pi * ui;
-        }
+      for (int e = 0; e < 4; ++e) {
+        const float ur = acc[0][tau][e], ui = acc[1][tau][e];
+        const float uzr = acc[2][tau][e], uzi = acc[3][tau][e];
+        const float h = pr[tau][e] * ui + pi[tau][e] * ur;
+        fx[tau][e] += wx * h;
+        fy[tau][e] += wy * h;
+        fz[tau][e] += pr[tau][e] * uzi + pi[tau][e] * uzr;
+        ph[tau][e] += pr[tau][e] * ur - pi[tau][e] * ui;
+      }
+    nyi += 16;
+    while (nyi >= a.nY) {
+      nyi -= a.nY;
+      ++nxi;
     }
     if (more) {
       u4v *dst = reinterpret_cast<u4v *>(smem_raw + xy_bytes + (cur ^ 1) * blk_bytes);

@@ -112,6 +112,7 @@ def load_library():
         "cav_batch_adjoint": [P, dp, dp, dp, dp],
         "cav_autotune": [P],
         "cav_tiles": [P, ip, ip, ip, ip, ip],
+        "cav_set_tiles": [P, I, I, I, I],
         "cav_set_pair_kernel": [P, I],
         "cav_pair_kernel": [P, ip],
         "cav_reset_acceleration": [P],
@@ -242,6 +243,12 @@ class Engine:
         v = [ctypes.c_int() for _ in range(5)]
         self._check(self.lib.cav_tiles(self.h, *[ctypes.byref(x) for x in v]))
         return dict(zip(("s_rows", "s_stage", "s_chunk", "f_tiles", "f_waves"), (x.value for x in v)))
+
+    def set_tiles(self, s_rows: int, s_chunk: int, f_tiles: int = 1, f_waves: int = 16) -> dict:
+        """Pin the Ewald_k tiles instead of tuning them."""
+        self._check(self.lib.cav_set_tiles(self.h, int(s_rows), int(s_chunk), int(f_tiles),
+                                           int(f_waves)))
+        return self.tiles()
 
     PAIR_KERNELS = {"auto": 0, "rows": 1, "cells": 2}
 

@@ -935,6 +935,29 @@ int cav_tiles(const cav_ctx *c, int *s_rows, int *s_stage, int *s_chunk, int *f_
   return CAV_OK;
 }
 
+int cav_set_tiles(cav_ctx *c, int s_rows, int s_chunk, int f_tiles, int f_waves)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  if (s_rows < 1 || s_rows > 4 || s_chunk <= 0 || s_chunk % 32 || f_tiles < 1 || f_tiles > 2 ||
+      (f_waves != 8 && f_waves != 16))
+    return fail(c, CAV_ERR_ARG, "tiles out of range");
+  if (!c->ek_on) return CAV_OK;
+  (void)hipSetDevice(c->device);
+  if (c->precision == CAV_MIXED) {
+    c->tune_rt = s_rows;
+    c->tune_stage = 32;
+    c->tune_fnt = f_tiles;
+    c->tune_fw = f_waves;
+  }
+  c->tune_chunk = s_chunk;
+  c->tuned = true;
+  CAV_TRY(ewald_k_alloc(c));
+  CAV_TRY(forces(c));
+  CAV_TRY(sync_flags(c));
+  return CAV_OK;
+}
+
 int cav_set_pair_kernel(cav_ctx *c, int kind)
 {
   if (!c) return CAV_ERR_ARG;

@@ -318,9 +318,9 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
 // 2 (nZ - 1) other kappa (kappa' = 2 c' + re/im, c' = nz below kz, nz - 1 above).
 __host__ __device__ constexpr int split_blk_bytes(int nkh) { return 4 * nkh * 2 * 64 * 16 + 128; }
 
-template <typename T>
+template <typename T, typename P>
 __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz, int KSP,
-                            const double *part, const double *W, double *sk, T *bm,
+                            const P *part, const double *W, double *sk, T *bm,
                             double *epart, double inv_sa, _Float16 *bmh, int nkh, double sb,
                             const int2 *rchunks, long bm_stride, long bmh_stride)
 {
@@ -340,9 +340,9 @@ __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz
     const int r = t / nZ, c = t - r * nZ;
     double sr = 0, si = 0;
     for (int k = 0; k < nchunks; ++k) {
-      const double *p = part + ((long)k * Rpad + r) * SC + 2 * c;
-      sr += p[0];
-      si += p[1];
+      const P *p = part + ((long)k * Rpad + r) * SC + 2 * c;
+      sr += (double)p[0];
+      si += (double)p[1];
     }
     sr *= inv_sa;
     si *= inv_sa;
@@ -394,14 +394,15 @@ __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz
 // 64 entries per block, wave w sums chunks w, w+4, ..., then the four wave sums
 // are combined in one fixed order (coalesced, and 4x the parallelism of one
 // thread per entry walking every chunk).
-__global__ void k_chunk_sum(const double *part, int nchunks, long plane, double *out)
+template <typename P>
+__global__ void k_chunk_sum(const P *part, int nchunks, long plane, double *out)
 {
   __shared__ double red[4][64];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long e = (long)blockIdx.x * 64 + lane;
   double s = 0;
   if (e < plane)
-    for (int k = wave; k < nchunks; k += 4) s += part[(long)k * plane + e];
+    for (int k = wave; k < nchunks; k += 4) s += (double)part[(long)k * plane + e];
   red[wave][lane] = s;
   __syncthreads();
   if (wave == 0 && e < plane) out[e] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
@@ -598,10 +599,11 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
 // block, each component of the FP32 record [X(0..kx) | Y(-ky..ky)] is one row
 // of kTabRow = 272 bytes (32 complex values, then 16 bytes of padding so that
 // the rows a wave reads side by side start in different LDS banks), and each
-// real-embedding column of Z (2 nZ of them: (Zr, -Zi) and (Zi, Zr) per nz) is
-// one row [32 hi words | 32 lo words | 16 bytes].  A block is copied to LDS as
-// it lies and read there with compile-time offsets; atoms past the end of a
-// block range are zero.
+// nz one row [32 hi words | 32 lo words | 16 bytes] of the split pair
+// (Zr, -Zi).  The second real-embedding column, (Zi, Zr), is the same halves
+// swapped with one sign flipped, so the S pass derives it in LDS.  A block is
+// copied to LDS as it lies and read there with compile-time offsets; atoms
+// past the end of a block range are zero.
 constexpr int kTabAtoms = 32, kTabRow = 272;
 
 __device__ __forceinline__ void put_split(unsigned char *xy, unsigned char *zt, long blk, int e,
@@ -610,14 +612,11 @@ __device__ __forceinline__ void put_split(unsigned char *xy, unsigned char *zt, 
   if (e < TXY) {
     *reinterpret_cast<Cx<float> *>(xy + (blk * TXY + e) * kTabRow + j * 8) = Cx<float>{cs, sn};
   } else {
-    unsigned h0, l0, h1, l1;
+    unsigned h0, l0;
     split_pair(cs, -sn, h0, l0);
-    split_pair(sn, cs, h1, l1);
-    unsigned char *row = zt + (blk * 2 * nZ + 2 * (e - TXY)) * kTabRow + j * 4;
+    unsigned char *row = zt + (blk * nZ + (e - TXY)) * kTabRow + j * 4;
     *reinterpret_cast<unsigned *>(row) = h0;
     *reinterpret_cast<unsigned *>(row + 128) = l0;
-    *reinterpret_cast<unsigned *>(row + kTabRow) = h1;
-    *reinterpret_cast<unsigned *>(row + kTabRow + 128) = l1;
   }
 }
 
@@ -673,7 +672,7 @@ struct SArgsH {
   const double *q;
   const void *tab;   // blocked FP32 records [X | Y] (TS = nX + nY rows per block)
   const void *ztab;  // blocked split Z (2 nZ rows per block)
-  double *part;      // [chunk][Rpad][SC]
+  float *part;       // [chunk][Rpad][SC]: FP32 chunk sums, summed over chunks in FP64
   const int4 *items; // batched: (first, end) table position, chunk slot, replica
   const RepDev *rep; // batched: charges of replica r at q[obase + position - tbase]
   int ncols, groups; // logical grid, as SArgs
@@ -691,11 +690,13 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
 {
   extern __shared__ unsigned char smem_raw[];
   constexpr int S = kTabAtoms;
-  const unsigned rec_bytes = (unsigned)a.TS * kTabRow;     // [X | Y] rows of a block
-  const unsigned zrow_bytes = (unsigned)(2 * a.nZ) * kTabRow;  // Z rows of a block
-  // per buffer: records, SC Z rows (rows past 2 nZ are never written: their
-  // columns are masked on output), the block's scaled charges
-  const unsigned buf_bytes = rec_bytes + (unsigned)a.SC * kTabRow + S * 4;
+  const unsigned rec_bytes = (unsigned)a.TS * kTabRow;  // [X | Y] rows of a block
+  const unsigned zrow_bytes = (unsigned)a.nZ * kTabRow;  // (Zr, -Zi) rows of a block
+  // per buffer: records; nZ (Zr, -Zi) rows then nZ (Zi, Zr) rows, padded to
+  // nZ + 8 SCT rows (rows past them are never written: their columns are
+  // masked on output); the block's scaled charges
+  const unsigned zrows = (unsigned)(a.nZ + 8 * SCT);
+  const unsigned buf_bytes = rec_bytes + zrows * kTabRow + S * 4;
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
   const unsigned char *ztab = reinterpret_cast<const unsigned char *>(a.ztab);
 
@@ -742,7 +743,8 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
     xo[r] = (unsigned)(ok ? row / a.nY : 0) * kTabRow + 32 * lk;
     yo[r] = (unsigned)(a.nX + (ok ? row % a.nY : 0)) * kTabRow + 32 * lk;
   }
-  const unsigned zo = rec_bytes + (unsigned)lr * kTabRow + 16 * lk;
+  // column s*16 + l%16: even columns are (Zr, -Zi) rows, odd ones (Zi, Zr)
+  const unsigned zo = rec_bytes + (unsigned)((lr & 1) * a.nZ + (lr >> 1)) * kTabRow + 16 * lk;
   const bool active = rt0 < rt_end;
 
   // The last wave of the workgroup is the producer: it streams each block's
@@ -758,12 +760,23 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
     copy_to_lds_async(tab + blk * rec_bytes, dst, rec_bytes, 0, 1, lane);
     copy_to_lds_async(ztab + blk * zrow_bytes, dst + rec_bytes, zrow_bytes, 0, 1, lane);
     if (lane < S)
-      reinterpret_cast<float *>(dst + rec_bytes + a.SC * kTabRow)[lane] =
+      reinterpret_cast<float *>(dst + rec_bytes + zrows * kTabRow)[lane] =
           base + lane < a1 ? (float)a.q[base + lane + qoff] * a.sa : 0.f;
   };
   auto finish = [&](int b) {
     unsigned char *dst = smem_raw + b * buf_bytes;
-    const float *sq = reinterpret_cast<const float *>(dst + rec_bytes + a.SC * kTabRow);
+    const float *sq = reinterpret_cast<const float *>(dst + rec_bytes + zrows * kTabRow);
+    // (Zi, Zr) halves from (Zr, -Zi): swap the two fp16 halves of each word
+    // and negate the new low one (rounding to fp16 is odd-symmetric, so this
+    // is exactly the split of (Zi, Zr))
+    for (int e = lane; e < a.nZ * 16; e += 64) {
+      const int row = e >> 4, q = e & 15;
+      const u4v w = *reinterpret_cast<const u4v *>(dst + rec_bytes + row * kTabRow + q * 16);
+      u4v o;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o[k] = ((w[k] >> 16) | (w[k] << 16)) ^ 0x8000u;
+      *reinterpret_cast<u4v *>(dst + rec_bytes + (a.nZ + row) * kTabRow + q * 16) = o;
+    }
     for (int e = lane; e < a.nX * S; e += 64) {
       const int c = e / S, j = e % S;
       Cx<float> *x = reinterpret_cast<Cx<float> *>(dst + c * kTabRow + j * 8);
@@ -781,7 +794,6 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
     __builtin_amdgcn_s_waitcnt(0);
     finish(0);
   }
-  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   // three loops with the same barriers: producer, compute waves, and compute
   // waves with no row tile in this group (no accumulators live across them)
@@ -792,7 +804,6 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
         __builtin_amdgcn_s_waitcnt(0);
         finish(b ^ 1);
       }
-      __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       b ^= 1;
     }
@@ -826,7 +837,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
       }
 #pragma unroll
       for (int s = 0; s < SCT; ++s) {
-        const unsigned char *pz = smem_raw + cur + zo + s * 16 * kTabRow + 64 * ks;
+        const unsigned char *pz = smem_raw + cur + zo + s * 8 * kTabRow + 64 * ks;
         const u4v bhi = *reinterpret_cast<const u4v *>(pz);
         const u4v blo = *reinterpret_cast<const u4v *>(pz + 128);
 #pragma unroll
@@ -850,7 +861,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
         const int row = (rt0 + r) * 16 + 4 * lk + e;
         const int col = s * 16 + lr;
         const bool ok = row < a.nX * a.nY && (col >> 1) < a.nZ;
-        a.part[((long)chunk * a.Rpad + row) * a.SC + col] = ok ? (double)acc[r][s][e] : 0.0;
+        a.part[((long)chunk * a.Rpad + row) * a.SC + col] = ok ? acc[r][s][e] : 0.f;
       }
   }
 }
@@ -930,7 +941,7 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
           const long t = atom + toff;
           const int zc = c < a.kz ? c : c + 1;
           const unsigned char *row =
-              ztab + ((t / kTabAtoms) * 2 * a.nZ + 2 * zc) * kTabRow + (t % kTabAtoms) * 4;
+              ztab + ((t / kTabAtoms) * a.nZ + zc) * kTabRow + (t % kTabAtoms) * 4;
           h = *reinterpret_cast<const unsigned *>(row) ^ 0x80000000u;
           l = *reinterpret_cast<const unsigned *>(row + 128) ^ 0x80000000u;
         }
@@ -1342,7 +1353,7 @@ int ewald_k_alloc(cav_ctx *c)
   if (c->bmat_h) (void)hipFree(c->bmat_h);
   c->ztab = c->bmat_h = nullptr;
   if (c->precision == CAV_MIXED) {
-    CAV_HIP(hipMalloc(&c->ztab, nblk * 2 * p.nZ * kTabRow + 4096));
+    CAV_HIP(hipMalloc(&c->ztab, nblk * p.nZ * kTabRow + 4096));
     const size_t bh_elems = (size_t)(p.Rpad / 16) * (split_blk_bytes(c->nkh) / 2);
     CAV_HIP(hipMalloc(&c->bmat_h, bh_elems * sizeof(_Float16)));
     k_zero_t<_Float16><<<grid_for(bh_elems), kBlock, 0, c->stream>>>(bh_elems, (_Float16 *)c->bmat_h);
@@ -1457,8 +1468,8 @@ static int sk_pipeline(cav_ctx *c)
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
   const long plane = (long)p.Rpad * a.SC;
-  k_chunk_sum<<<(plane + 63) / 64, kBlock, 0, c->stream>>>(c->sk_part, nch, plane, c->sk_sum);
-  k_sk_reduce<T><<<grid_for(nthr), kBlock, 0, c->stream>>>(
+  k_chunk_sum<double><<<(plane + 63) / 64, kBlock, 0, c->stream>>>(c->sk_part, nch, plane, c->sk_sum);
+  k_sk_reduce<T, double><<<grid_for(nthr), kBlock, 0, c->stream>>>(
       1, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_sum, c->d_W, c->sk, (T *)c->bmat, c->part,
       1.0, nullptr, 0, 1.0, nullptr, 0, 0);
   CAV_HIP(hipGetLastError());
@@ -1505,10 +1516,12 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
 }
 
 // LDS of a split S-pass workgroup: two buffers of one table block each
-// ([X | Y] rows, SC Z rows, 32 scaled charges).
+// ([X | Y] rows, nZ + 8 SCT Z rows, 32 scaled charges).  (A third buffer, to
+// request blocks two ahead, measured slower: it halves the workgroups per CU.)
 static size_t split_sk_lds(const EwaldKPlan &p)
 {
-  return 2 * ((size_t)(p.nX + p.nY + roundup(2 * p.nZ, 16)) * kTabRow + kTabAtoms * sizeof(float));
+  const int zrows = p.nZ + 8 * (roundup(2 * p.nZ, 16) / 16);
+  return 2 * ((size_t)(p.nX + p.nY + zrows) * kTabRow + kTabAtoms * sizeof(float));
 }
 
 template <int RT>
@@ -1558,7 +1571,7 @@ static int sk_pipeline_h(cav_ctx *c)
   a.q = c->q;
   a.tab = c->tab;
   a.ztab = c->ztab;
-  a.part = c->sk_part;
+  a.part = reinterpret_cast<float *>(c->sk_part);
   a.items = nullptr;
   a.rep = nullptr;
   a.Rpad = p.Rpad;
@@ -1581,8 +1594,9 @@ static int sk_pipeline_h(cav_ctx *c)
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
   const long plane = (long)p.Rpad * a.SC;
-  k_chunk_sum<<<(plane + 63) / 64, kBlock, 0, c->stream>>>(c->sk_part, nch, plane, c->sk_sum);
-  k_sk_reduce<float><<<grid_for(nthr), kBlock, 0, c->stream>>>(
+  k_chunk_sum<float><<<(plane + 63) / 64, kBlock, 0, c->stream>>>(
+      reinterpret_cast<const float *>(c->sk_part), nch, plane, c->sk_sum);
+  k_sk_reduce<float, double><<<grid_for(nthr), kBlock, 0, c->stream>>>(
       1, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_sum, c->d_W, c->sk, (float *)c->bmat,
       c->part, 1.0 / c->sA, (_Float16 *)c->bmat_h, c->nkh, c->sB, nullptr, 0, 0);
   CAV_HIP(hipGetLastError());
@@ -1738,7 +1752,7 @@ static int weighted_sk_t(cav_ctx *c, const double *weights, double *out)
   const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
   launch_sk_rt<T, kRT>(c->stream, a, dim3(nch, groups), smem, a.SC / 16, waves);
   const long plane = (long)p.Rpad * a.SC;
-  k_chunk_sum<<<(plane + 63) / 64, kBlock, 0, c->stream>>>(c->sk_part, nch, plane, c->sk_sum);
+  k_chunk_sum<double><<<(plane + 63) / 64, kBlock, 0, c->stream>>>(c->sk_part, nch, plane, c->sk_sum);
   k_sk_sum<<<grid_for((long)p.Rpad * p.nZ), kBlock, 0, c->stream>>>(1, p.Rpad, a.SC, p.nZ,
                                                                     c->sk_sum, out);
   CAV_HIP(hipGetLastError());
@@ -1966,7 +1980,7 @@ int batch_ewald_k_alloc(cav_batch *b)
   const size_t nblk = (size_t)b->ntab / kTabAtoms;
   BAT_HIP(hipMalloc(&b->tab, std::max((size_t)b->nown * TS * 2 * tsz,
                                       nblk * (p.nX + p.nY) * kTabRow) + 4096));
-  if (split) BAT_HIP(hipMalloc(&b->ztab, nblk * 2 * p.nZ * kTabRow + 4096));
+  if (split) BAT_HIP(hipMalloc(&b->ztab, nblk * p.nZ * kTabRow + 4096));
   // S-pass items: chunks of s_chunk atoms per replica, largest replicas first
   std::vector<int4> sitems;
   std::vector<int2> rch(b->B);
@@ -2093,7 +2107,7 @@ static int batch_ewald_k_t(cav_batch *b)
     a.q = b->q_o;
     a.tab = b->tab;
     a.ztab = b->ztab;
-    a.part = b->sk_part;
+    a.part = reinterpret_cast<float *>(b->sk_part);
     a.Rpad = p.Rpad;
     a.SC = SC;
     a.items = b->s_items;
@@ -2110,8 +2124,9 @@ static int batch_ewald_k_t(cav_batch *b)
       batch_sk_h<3>(b, a, groups, smem, SC / 16, waves);
   }
   const int nthr = p.Rpad * p.nZ;
-  k_sk_reduce<T><<<dim3(grid_for(nthr), b->B), kBlock, 0, b->stream>>>(
-      0, p.Rpad, SC, p.R, p.nZ, p.kz, p.KSP, b->sk_part, b->d_W, b->sk, (T *)b->bmat, b->epart,
+  // partials: FP32 from the split S pass, FP64 from the FP64 one
+  k_sk_reduce<T, T><<<dim3(grid_for(nthr), b->B), kBlock, 0, b->stream>>>(
+      0, p.Rpad, SC, p.R, p.nZ, p.kz, p.KSP, reinterpret_cast<const T *>(b->sk_part), b->d_W, b->sk, (T *)b->bmat, b->epart,
       1.0 / b->sA, batch_split(b) ? (_Float16 *)b->bmat_h : nullptr, b->nkh, b->sB, b->rchunks,
       b->bm_stride, b->bmh_stride);
   if (b->precision == CAV_FP64) {

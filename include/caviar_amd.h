@@ -217,6 +217,12 @@ int cav_batch_tiles(const cav_batch *b, int *s_rows, int *s_stage, int *s_chunk,
 int cav_autotune(cav_ctx *ctx);
 int cav_tiles(const cav_ctx *ctx, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles,
               int *f_waves);
+/* Pins the Ewald_k tiles instead of tuning them (reproducible timings): S-pass
+ * row tiles per wave 1..4, atoms per chunk (a multiple of 32, > 0); split form:
+ * force-pass atom tiles per wave 1..2 and waves per workgroup 8 or 16.  After
+ * cav_setup; the forces of the current state are recomputed.  FP64 uses the
+ * chunk only. */
+int cav_set_tiles(cav_ctx *ctx, int s_rows, int s_chunk, int f_tiles, int f_waves);
 /* Short-range pair kernel.  CAV_PAIRS_ROWS gathers every neighbour of an
  * atom's Verlet row through L2 (8 lanes per atom); CAV_PAIRS_CELLS stages each
  * cell's 27-cell stencil in LDS and reads the neighbours from there (where it

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import fixtures as F
-from caviar_amd import Engine, NaNAccelerationError, configs
+from caviar_amd import CaviarError, Engine, NaNAccelerationError, configs
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -238,6 +238,34 @@ def test_autotune_keeps_results(gpu):
         a = e.get("acc")
         err = F.rel_err(a, want) if prec == "fp64" else np.sqrt(((a - want) ** 2).sum(1).mean() / (want ** 2).sum(1).mean())
         assert err <= tol, (prec, tiles, err)
+
+
+def test_pinned_tiles_keep_results(gpu):
+    """Every pinned Ewald_k tile shape (row tiles 1..4, chunks down to one
+    32-atom table block and past n, force tiles 1/2 x 8/16 waves) gives the
+    oracle's forces: FP64 within 1e-12, mixed within 1e-5; an out-of-range
+    shape is refused."""
+    s = configs.ions(n=1500, kmax=5, seed=9)
+    print("[case] pinned tiles", flush=True)
+    o = Oracle(s)
+    o.setup()
+    want = o.get("acc")
+    e = Engine(s, precision="mixed")
+    e.setup()
+    for rt, ch, nt, fw in ((1, 32, 1, 8), (2, 96, 2, 8), (3, 512, 1, 16), (4, 4096, 2, 16)):
+        tiles = e.set_tiles(rt, ch, nt, fw)
+        assert (tiles["s_rows"], tiles["s_chunk"], tiles["f_tiles"], tiles["f_waves"]) == (rt, ch, nt, fw)
+        a = e.get("acc")
+        err = np.sqrt(((a - want) ** 2).sum(1).mean() / (want ** 2).sum(1).mean())
+        assert err <= 1e-5, (rt, ch, nt, fw, err)
+    with pytest.raises(CaviarError):
+        e.set_tiles(5, 1024, 1, 16)
+    with pytest.raises(CaviarError):
+        e.set_tiles(3, 1000, 1, 16)
+    e = Engine(s, precision="fp64")
+    e.setup()
+    e.set_tiles(3, 256, 1, 8)
+    assert F.rel_err(e.get("acc"), want) <= 1e-12
 
 
 @pytest.mark.parametrize("name", ["ions2000_setup", "aniso400_setup", "lj4000_setup",

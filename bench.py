@@ -56,12 +56,15 @@ def run_engine(system, precision, warmup, steps, dt, pair_kernel="auto"):
     e.step(dt, steps)          # every step ends with a stream synchronisation
     t = time.perf_counter() - t0
     # per-kernel launch times from HIP events, in a separate pass so that the
-    # event records do not weigh on the throughput above
+    # event records do not weigh on the throughput above; Ewald_k runs in line
+    # here (not on its concurrent stream) so each time is the kernel's own
+    e.set_overlap(False)
     e.enable_timing(True)
     e.step(dt, min(steps, 10))
     timings = {k: e.kernel_time(k) for k in ("pair", "ewald_k_sk", "ewald_k_force",
                                               "ewald_k_tables", "neighbor", "list", "integrate")}
     e.enable_timing(False)
+    e.set_overlap(True)
     log(f"{precision}: {steps} steps in {t:.3f}s -> {system.n * steps / t:.4e} atom-steps/s; "
         + ", ".join(f"{k} {v[0]:.3f}ms" for k, v in timings.items()))
     timings["tiles"] = tiles
@@ -305,10 +308,23 @@ def main():
     if args.precision != "both":
         run_engine(system, args.precision, args.warmup, args.steps, args.dt, args.pair_kernel)
         return
+    # one engine at a time: each context's two HIP streams (short-range passes,
+    # concurrent Ewald_k) then get hardware queues of their own
     e64, t64, tm64 = run_engine(system, "fp64", args.warmup, args.steps, args.dt, args.pair_kernel)
+    ppa, lj_ppa = e64.list_stats()
+    e64.close()
     e32, t32, tm32 = run_engine(system, "mixed", args.warmup, args.steps, args.dt, args.pair_kernel)
+    e32.close()
 
-    # force error of the mixed form against FP64 on identical positions: 10 seeded snapshots
+    # force error of the mixed form (at the tiles timed above) against FP64 on
+    # identical positions: seeded snapshots
+    from caviar_amd import Engine
+    e64 = Engine(system.copy(), precision="fp64")
+    e64.setup()
+    e32 = Engine(system.copy(), precision="mixed")
+    e32.setup()
+    tl = tm32["tiles"]
+    e32.set_tiles(tl["s_rows"], tl["s_chunk"], tl["f_tiles"], tl["f_waves"])
     errs = []
     for k in range(args.snapshots):
         snap = configs.ions(n=args.n, rho=0.3, kmax=args.kmax, seed=1000 + k).pos
@@ -316,7 +332,8 @@ def main():
             e.set("pos", snap)
             e.recalculate(force_rebuild=True)
         errs.append(force_rms(e32.get("acc"), e64.get("acc")))
-    ppa, lj_ppa = e64.list_stats()
+    e64.close()
+    e32.close()
     log(f"force rms err mixed vs fp64 over {len(errs)} snapshots: max {max(errs):.3e}, "
         f"mean {np.mean(errs):.3e}; pairs/atom {ppa:.2f}, lj pairs/atom {lj_ppa:.2f}")
 

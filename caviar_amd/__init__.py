@@ -113,6 +113,7 @@ def load_library():
         "cav_autotune": [P],
         "cav_tiles": [P, ip, ip, ip, ip, ip],
         "cav_set_tiles": [P, I, I, I, I],
+        "cav_set_overlap": [P, I],
         "cav_set_pair_kernel": [P, I],
         "cav_pair_kernel": [P, ip],
         "cav_reset_acceleration": [P],
@@ -249,6 +250,10 @@ class Engine:
         self._check(self.lib.cav_set_tiles(self.h, int(s_rows), int(s_chunk), int(f_tiles),
                                            int(f_waves)))
         return self.tiles()
+
+    def set_overlap(self, on: bool):
+        """Ewald_k of a step concurrent with the short-range passes (default on)."""
+        self._check(self.lib.cav_set_overlap(self.h, int(bool(on))))
 
     PAIR_KERNELS = {"auto": 0, "rows": 1, "cells": 2}
 

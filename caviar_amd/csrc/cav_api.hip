@@ -183,7 +183,8 @@ static void free_all(cav_ctx *c)
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
                   c->d_W,     c->kacc,      c->order,    c->ord_blk,  c->xq,
                   c->tid,     c->d_raw_eps, c->d_raw_sig, c->ztab, c->bmat_h, c->cpos, c->sk_sum,
-                  c->binpos,  c->xqs,    c->tids,       c->nll,    c->lstart, c->litems};
+                  c->binpos,  c->xqs,    c->tids,       c->nll,    c->lstart, c->litems,
+                  c->ekx,     c->eky,    c->ekz,        c->part_k};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -342,10 +343,61 @@ static int refresh_params(cav_ctx *c)
   return CAV_OK;
 }
 
+__global__ void k_add3(int n, double *ax, double *ay, double *az, const double *bx,
+                       const double *by, const double *bz)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    ax[i] += bx[i];
+    ay[i] += by[i];
+    az[i] += bz[i];
+  }
+}
+
+// Starts the step's Ewald_k on the second stream once the drift has run: the
+// passes read only the owned positions and charges, and write their own
+// accelerations (zeroed first) and partial sums, so they run concurrently
+// with the ghost exchange, neighbour build and pair kernel of the main stream.
+static int ewald_k_fork(cav_ctx *c)
+{
+  CAV_HIP(hipEventRecord(c->ev_fork, c->stream));
+  CAV_HIP(hipStreamWaitEvent(c->kstream, c->ev_fork, 0));
+  struct Side {  // the Ewald_k launchers see the side stream and buffers
+    cav_ctx *c;
+    explicit Side(cav_ctx *c) : c(c) { flip(); }
+    ~Side() { flip(); }
+    void flip()
+    {
+      std::swap(c->stream, c->kstream);
+      std::swap(c->ax, c->ekx);
+      std::swap(c->ay, c->eky);
+      std::swap(c->az, c->ekz);
+      std::swap(c->part, c->part_k);
+    }
+  } side(c);
+  k_zero3<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->ax, c->ay, c->az);
+  CAV_HIP(hipGetLastError());
+  CAV_TRY(launch_ewald_k(c));
+  CAV_HIP(hipEventRecord(c->ev_join, c->stream));
+  return CAV_OK;
+}
+
+// Main stream waits for the side Ewald_k and adds its accelerations: the sum
+// per atom is the short-range one plus the k-space one, as in forces().
+static int ewald_k_join(cav_ctx *c)
+{
+  CAV_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  k_add3<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->ax, c->ay, c->az, c->ekx, c->eky,
+                                                   c->ekz);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
 // The force phase: reset acceleration + every enabled field (md_simulator.cpp:370-373).
-static int forces(cav_ctx *c);
+// `kspace` false leaves Ewald_k to the concurrent side stream.
+static int forces(cav_ctx *c, bool kspace = true);
 int forces_all(cav_ctx *c) { return forces(c); }
-static int forces(cav_ctx *c)
+static int forces(cav_ctx *c, bool kspace)
 {
   const bool pair = (c->lj_on && !c->lj_cellwalk) || c->er_on;
   if (pair)
@@ -353,7 +405,7 @@ static int forces(cav_ctx *c)
   else
     CAV_TRY(launch_zero_acc(c));
   if (c->lj_on && c->lj_cellwalk) CAV_TRY(launch_lj_cellwalk(c));
-  if (c->ek_on) CAV_TRY(launch_ewald_k(c));
+  if (c->ek_on && kspace) CAV_TRY(launch_ewald_k(c));
   return CAV_OK;
 }
 
@@ -465,7 +517,11 @@ int cav_destroy(cav_ctx *c)
   if (!c) return CAV_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->kstream) (void)hipStreamSynchronize(c->kstream);
   free_all(c);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->kstream) (void)hipStreamDestroy(c->kstream);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return CAV_OK;
@@ -760,6 +816,17 @@ int cav_setup(cav_ctx *c)
       c->npart = need;
     }
     CAV_TRY(ewald_k_alloc(c));
+    // the concurrent Ewald_k of a step: second stream, fork/join events, its
+    // accelerations and partial sums
+    CAV_TRY(dmalloc(c, &c->part_k, c->npart));
+    CAV_TRY(dmalloc(c, &c->ekx, std::max(c->n, 1)));
+    CAV_TRY(dmalloc(c, &c->eky, std::max(c->n, 1)));
+    CAV_TRY(dmalloc(c, &c->ekz, std::max(c->n, 1)));
+    if (!c->kstream) {
+      CAV_HIP(hipStreamCreate(&c->kstream));
+      CAV_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+      CAV_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
   }
   // initial list capacity from the mean density, refined by the first build
   if (c->nl_on) {
@@ -807,9 +874,12 @@ int cav_step(cav_ctx *c, double dt, int nsteps)
           c->ax, c->ay, c->az, c->aox, c->aoy, c->aoz, c->flags);
       timer_end(c, c->t_int);
       CAV_HIP(hipGetLastError());
+      const bool side = c->overlap && c->ek_on && c->n > 0;
+      if (side) CAV_TRY(ewald_k_fork(c));
       CAV_TRY(launch_ghosts(c));
       CAV_TRY(neighbor_phase(c, false));
-      CAV_TRY(forces(c));
+      CAV_TRY(forces(c, !side));
+      if (side) CAV_TRY(ewald_k_join(c));
       k_kick<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, 0.5 * dt, c->vx, c->vy, c->vz, c->ax,
                                                        c->ay, c->az, c->aox, c->aoy, c->aoz);
       CAV_HIP(hipGetLastError());
@@ -955,6 +1025,13 @@ int cav_set_tiles(cav_ctx *c, int s_rows, int s_chunk, int f_tiles, int f_waves)
   CAV_TRY(ewald_k_alloc(c));
   CAV_TRY(forces(c));
   CAV_TRY(sync_flags(c));
+  return CAV_OK;
+}
+
+int cav_set_overlap(cav_ctx *c, int on)
+{
+  if (!c) return CAV_ERR_ARG;
+  c->overlap = on != 0;
   return CAV_OK;
 }
 

@@ -178,6 +178,12 @@ struct cav_ctx {
   // reductions: per-block partials and results
   double *part = nullptr;  // per-block partial sums (virial, energies)
   int npart = 0;
+  // Ewald_k of a step on a second stream, concurrent with the short-range
+  // passes: its own accelerations and partial sums, added after the join
+  bool overlap = true;
+  hipStream_t kstream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  double *ekx = nullptr, *eky = nullptr, *ekz = nullptr, *part_k = nullptr;
   double *scal = nullptr;  // [0] virial, [1] e_lj, [2] e_r, [3] e_k, [4..] scratch
   double *h_scal = nullptr;
 

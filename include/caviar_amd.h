@@ -223,6 +223,11 @@ int cav_tiles(const cav_ctx *ctx, int *s_rows, int *s_stage, int *s_chunk, int *
  * cav_setup; the forces of the current state are recomputed.  FP64 uses the
  * chunk only. */
 int cav_set_tiles(cav_ctx *ctx, int s_rows, int s_chunk, int f_tiles, int f_waves);
+/* cav_step runs Ewald_k on a second HIP stream, concurrently with the ghost
+ * exchange, neighbour build and pair kernel (on = 1, the default); 0 runs
+ * every pass in order on one stream (per-kernel timings without contention).
+ * The accelerations are the same either way. */
+int cav_set_overlap(cav_ctx *ctx, int on);
 /* Short-range pair kernel.  CAV_PAIRS_ROWS gathers every neighbour of an
  * atom's Verlet row through L2 (8 lanes per atom); CAV_PAIRS_CELLS stages each
  * cell's 27-cell stencil in LDS and reads the neighbours from there (where it

@@ -240,6 +240,25 @@ def test_autotune_keeps_results(gpu):
         assert err <= tol, (prec, tiles, err)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_concurrent_ewald_k_same_trajectory(gpu, precision):
+    """Ewald_k on its own stream, concurrent with the ghost exchange, list
+    build and pair kernel, gives the in-order trajectory bit for bit (the side
+    accelerations are added to the short-range ones once, as in order), over
+    steps that rebuild the list."""
+    s = configs.ions(n=3000, kmax=6, seed=4)
+    print(f"[case] concurrent Ewald_k {precision}", flush=True)
+    out = []
+    for overlap in (True, False):
+        e = Engine(s.copy(), precision=precision)
+        e.set_overlap(overlap)
+        e.setup()
+        e.step(0.002, 25)
+        out.append([e.get(k) for k in ("pos", "vel", "acc")])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_pinned_tiles_keep_results(gpu):
     """Every pinned Ewald_k tile shape (row tiles 1..4, chunks down to one
     32-atom table block and past n, force tiles 1/2 x 8/16 waves) gives the

@@ -129,6 +129,24 @@ __global__ void k_kick(int n, double hdt, double *vx, double *vy, double *vz, co
   vz[i] = xadd(vz[i], xmul(hdt, xadd(az[i], aoz[i])));
 }
 
+// The kick after a concurrent Ewald_k: first a += ek (the k-space
+// accelerations, the single addition the in-order schedule makes), then the kick.
+__global__ void k_kick_join(int n, double hdt, double *vx, double *vy, double *vz, double *ax,
+                            double *ay, double *az, const double *ekx, const double *eky,
+                            const double *ekz, const double *aox, const double *aoy,
+                            const double *aoz)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double a[3] = {xadd(ax[i], ekx[i]), xadd(ay[i], eky[i]), xadd(az[i], ekz[i])};
+  ax[i] = a[0];
+  ay[i] = a[1];
+  az[i] = a[2];
+  vx[i] = xadd(vx[i], xmul(hdt, xadd(a[0], aox[i])));
+  vy[i] = xadd(vy[i], xmul(hdt, xadd(a[1], aoy[i])));
+  vz[i] = xadd(vz[i], xmul(hdt, xadd(a[2], aoz[i])));
+}
+
 __global__ void k_begin_step(int *flags)
 {
   flags[F_WRAPPED] = 0;
@@ -343,17 +361,6 @@ static int refresh_params(cav_ctx *c)
   return CAV_OK;
 }
 
-__global__ void k_add3(int n, double *ax, double *ay, double *az, const double *bx,
-                       const double *by, const double *bz)
-{
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) {
-    ax[i] += bx[i];
-    ay[i] += by[i];
-    az[i] += bz[i];
-  }
-}
-
 // Starts the step's Ewald_k on the second stream once the drift has run: the
 // passes read only the owned positions and charges, and write their own
 // accelerations (zeroed first) and partial sums, so they run concurrently
@@ -382,20 +389,32 @@ static int ewald_k_fork(cav_ctx *c)
   return CAV_OK;
 }
 
-// Main stream waits for the side Ewald_k and adds its accelerations: the sum
-// per atom is the short-range one plus the k-space one, as in forces().
+// The main stream waits for the side Ewald_k; the kick then adds its
+// accelerations (k_kick_join): per atom the short-range sum plus the k-space
+// one, as in forces().
 static int ewald_k_join(cav_ctx *c)
 {
   CAV_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-  k_add3<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->ax, c->ay, c->az, c->ekx, c->eky,
-                                                   c->ekz);
-  CAV_HIP(hipGetLastError());
   return CAV_OK;
 }
 
 // The force phase: reset acceleration + every enabled field (md_simulator.cpp:370-373).
 // `kspace` false leaves Ewald_k to the concurrent side stream.
 static int forces(cav_ctx *c, bool kspace = true);
+static int neighbor_phase(cav_ctx *c, bool forced);
+
+// The autotuner's unit of work: the Ewald_k passes as cav_step runs them,
+// i.e. on the side stream concurrently with a list build and the pair kernel
+// (in order when the overlap is off).  Accelerations are left inconsistent:
+// the caller recomputes them.
+int cav::tuning_phase(cav_ctx *c)
+{
+  if (!(c->overlap && c->n > 0)) return launch_ewald_k(c);
+  CAV_TRY(ewald_k_fork(c));
+  CAV_TRY(neighbor_phase(c, true));
+  CAV_TRY(forces(c, false));
+  return ewald_k_join(c);
+}
 int forces_all(cav_ctx *c) { return forces(c); }
 static int forces(cav_ctx *c, bool kspace)
 {
@@ -879,9 +898,16 @@ int cav_step(cav_ctx *c, double dt, int nsteps)
       CAV_TRY(launch_ghosts(c));
       CAV_TRY(neighbor_phase(c, false));
       CAV_TRY(forces(c, !side));
-      if (side) CAV_TRY(ewald_k_join(c));
-      k_kick<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, 0.5 * dt, c->vx, c->vy, c->vz, c->ax,
-                                                       c->ay, c->az, c->aox, c->aoy, c->aoz);
+      if (side) {
+        CAV_TRY(ewald_k_join(c));
+        k_kick_join<<<grid_for(c->n), kBlock, 0, c->stream>>>(
+            c->n, 0.5 * dt, c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->ekx, c->eky, c->ekz,
+            c->aox, c->aoy, c->aoz);
+      } else {
+        k_kick<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, 0.5 * dt, c->vx, c->vy, c->vz,
+                                                         c->ax, c->ay, c->az, c->aox, c->aoy,
+                                                         c->aoz);
+      }
       CAV_HIP(hipGetLastError());
       CAV_TRY(sync_flags(c));
       harvest_timers(c);

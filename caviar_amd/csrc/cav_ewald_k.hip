@@ -2297,7 +2297,8 @@ int batch_weighted_sk(cav_batch *b, const double *u, double *out)
 // ------------------------------------------------------------------ autotuner --
 
 // Times the Ewald_k passes (tables, S pass, reduction, force pass) for each
-// candidate tile shape on this context's own system and keeps the fastest:
+// candidate tile shape on this context's own system, as cav_step runs them
+// (concurrently with a list build and the pair kernel), and keeps the fastest:
 // S-pass row tiles per wave, atoms per LDS stage and atoms per chunk, and in
 // the split form the force pass's atom tiles per wave.
 int ewald_k_autotune(cav_ctx *c)
@@ -2332,9 +2333,9 @@ int ewald_k_autotune(cav_ctx *c)
       CAV_TRY(ewald_k_alloc(c));
     }
     if (c->precision == CAV_MIXED && split_force_lds(c) > 160 * 1024) continue;
-    for (int r = 0; r < 2; ++r) CAV_TRY(launch_ewald_k(c));
+    for (int r = 0; r < 2; ++r) CAV_TRY(tuning_phase(c));
     CAV_HIP(hipEventRecord(e0, c->stream));
-    for (int r = 0; r < 3; ++r) CAV_TRY(launch_ewald_k(c));
+    for (int r = 0; r < 3; ++r) CAV_TRY(tuning_phase(c));
     CAV_HIP(hipEventRecord(e1, c->stream));
     CAV_HIP(hipEventSynchronize(e1));
     float ms = 0;

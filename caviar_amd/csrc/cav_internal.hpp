@@ -313,6 +313,7 @@ namespace cav {
 int fail(cav_ctx *c, int code, const std::string &msg);
 int hip_check(cav_ctx *c, hipError_t e, const char *what);
 void timer_begin(cav_ctx *c, Timer &t);
+int tuning_phase(cav_ctx *c);
 void timer_end(cav_ctx *c, Timer &t);
 
 // launchers (defined in the .hip files)

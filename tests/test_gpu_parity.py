@@ -96,6 +96,30 @@ def test_mixed_precision_force_error(gpu, name):
     assert abs(v32 - v64) <= 1e-5 * max(abs(v64), 1.0)
 
 
+@pytest.mark.parametrize("k", [(1, 1, 1), (2, 5, 0), (8, 3, 8), (12, 12, 12), (5, 9, 16),
+                               (20, 7, 20), (16, 16, 24), (31, 2, 31), (31, 31, 31)])
+def test_mixed_lattice_shapes(gpu, k):
+    """The split Ewald_k over lattice shapes that exercise its geometry: kz = 0
+    (only the accumulator seed), K axes that are and are not multiples of 32
+    (2 (nZ - 1) = 4 kz = 0 .. 124 -> 1..4 K-steps), 1..8 column tiles, record
+    rows from 6 to 95, and the widest lattice (whose force-pass tiles exceed
+    LDS, so the FP32 tiles take over).  Mixed forces within 1e-5 RMS
+    of FP64 on the same positions; the k-space energy within 1e-6."""
+    kx, ky, kz = k
+    s = configs.ions(n=1200, rho=0.3, kmax=1, seed=31 + kx + ky + kz)
+    s.ewald_k.kx_max, s.ewald_k.ky_max, s.ewald_k.kz_max = kx, ky, kz
+    print(f"[case] mixed lattice {k}", flush=True)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        e = Engine(s.copy(), precision=prec)
+        e.setup()
+        out[prec] = (e.get("acc"), e.energy(lj=False)[2])
+    a64, a32 = out["fp64"][0], out["mixed"][0]
+    rms = np.sqrt(((a32 - a64) ** 2).sum(axis=1).mean() / (a64 ** 2).sum(axis=1).mean())
+    assert rms <= 1e-5, (k, rms)
+    assert abs(out["mixed"][1] - out["fp64"][1]) <= 1e-6 * abs(out["fp64"][1]), k
+
+
 def test_fp64_is_bitwise_reproducible(gpu):
     s = configs.ions(n=3000, kmax=6, seed=9)
     runs = []

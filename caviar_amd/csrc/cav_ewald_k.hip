@@ -2321,9 +2321,9 @@ int ewald_k_autotune(cav_ctx *c)
   CAV_HIP(hipEventCreate(&e1));
   const bool timing = c->timing;
   c->timing = false;
-  float best = 1e30f;
-  Cand pick = {c->tune_rt, c->tune_stage, c->tune_chunk, c->tune_fnt, c->tune_fw};
-  for (const Cand &k : cands) {
+  // Two rounds: every candidate timed over 3 launches, then the 4 fastest
+  // again over 10 each (single timings of ~1 ms phases vary by a few percent)
+  auto apply = [&](const Cand &k) -> int {
     c->tune_rt = k.rt;
     c->tune_stage = k.stage;
     c->tune_fnt = k.fnt;
@@ -2332,14 +2332,34 @@ int ewald_k_autotune(cav_ctx *c)
       c->tune_chunk = k.chunk;
       CAV_TRY(ewald_k_alloc(c));
     }
-    if (c->precision == CAV_MIXED && split_force_lds(c) > 160 * 1024) continue;
+    return CAV_OK;
+  };
+  auto time_it = [&](int reps, float *ms) -> int {
     for (int r = 0; r < 2; ++r) CAV_TRY(tuning_phase(c));
     CAV_HIP(hipEventRecord(e0, c->stream));
-    for (int r = 0; r < 3; ++r) CAV_TRY(tuning_phase(c));
+    for (int r = 0; r < reps; ++r) CAV_TRY(tuning_phase(c));
     CAV_HIP(hipEventRecord(e1, c->stream));
     CAV_HIP(hipEventSynchronize(e1));
+    CAV_HIP(hipEventElapsedTime(ms, e0, e1));
+    *ms /= (float)reps;
+    return CAV_OK;
+  };
+  std::vector<std::pair<float, int>> timed;
+  for (int q = 0; q < (int)cands.size(); ++q) {
+    CAV_TRY(apply(cands[q]));
+    if (c->precision == CAV_MIXED && split_force_lds(c) > 160 * 1024) continue;
     float ms = 0;
-    CAV_HIP(hipEventElapsedTime(&ms, e0, e1));
+    CAV_TRY(time_it(3, &ms));
+    timed.push_back({ms, q});
+  }
+  std::sort(timed.begin(), timed.end());
+  Cand pick = {c->tune_rt, c->tune_stage, c->tune_chunk, c->tune_fnt, c->tune_fw};
+  float best = 1e30f;
+  for (int q = 0; q < (int)timed.size() && q < 4; ++q) {
+    const Cand &k = cands[timed[q].second];
+    CAV_TRY(apply(k));
+    float ms = 0;
+    CAV_TRY(time_it(10, &ms));
     if (ms < best) {
       best = ms;
       pick = k;

@@ -761,9 +761,9 @@ int cav_batch_get_pairs(cav_batch *b, int replica, int *ij, long capacity, long 
 }
 
 // Autotuner: times the batched Ewald_k passes for each candidate tile shape
-// (split form: row tiles per S-pass wave, atoms per LDS stage, atoms per S
-// chunk; plain form: atoms per S chunk) on the batch itself and keeps the
-// fastest.  Results: cav_batch_tiles.
+// (split form: row tiles per S-pass wave, atoms per S chunk, force-pass atom
+// tiles per wave; plain form: atoms per S chunk) on the batch itself and keeps
+// the fastest.  Results: cav_batch_tiles.
 int cav_batch_autotune(cav_batch *b)
 {
   if (!b) return CAV_ERR_ARG;

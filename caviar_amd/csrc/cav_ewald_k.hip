@@ -91,12 +91,12 @@ __device__ __forceinline__ void split_pair_mix(float a, float b, unsigned &hi, u
   lo = l;
 }
 
-// ztab (optional, mixed form): for every Z entry a 16-byte record of the two
-// real-embedding columns split into halves, (Zr, -Zi) hi, lo, (Zi, Zr) hi, lo.
+// Atom-major phase tables of the FP64 form and of the FP32 tiles: per atom
+// [X(0..kx) | Y(-ky..ky) | Z(-kz..kz)], TS complex values.
 template <typename T>
 __global__ void k_tables(long npts, int TS, int kx, int ky, int kz, double ilx, double ily,
                          double ilz, const double *x, const double *y, const double *z,
-                         Cx<T> *tab, u4v *ztab)
+                         Cx<T> *tab)
 {
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= npts * TS) return;
@@ -128,12 +128,6 @@ __global__ void k_tables(long npts, int TS, int kx, int ky, int kz, double ilx, 
     sincospif((float)(2.0 * f), &s, &c);
     v.re = c;
     v.im = s;
-    if (ztab && e >= nX + nY) {
-      unsigned h0, l0, h1, l1;
-      split_pair(c, -s, h0, l0);
-      split_pair(s, c, h1, l1);
-      ztab[a * (2 * kz + 1) + (e - nX - nY)] = u4v{h0, l0, h1, l1};
-    }
   }
   tab[t] = v;
 }
@@ -671,7 +665,7 @@ struct SArgsH {
   float sa;
   const double *q;
   const void *tab;   // blocked FP32 records [X | Y] (TS = nX + nY rows per block)
-  const void *ztab;  // blocked split Z (2 nZ rows per block)
+  const void *ztab;  // blocked split (Zr, -Zi) rows (nZ per block)
   float *part;       // [chunk][Rpad][SC]: FP32 chunk sums, summed over chunks in FP64
   const int4 *items; // batched: (first, end) table position, chunk slot, replica
   const RepDev *rep; // batched: charges of replica r at q[obase + position - tbase]
@@ -1438,7 +1432,7 @@ static int sk_pipeline(cav_ctx *c)
   timer_begin(c, c->t_tab);
   k_tables<T><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
       c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
-      (Cx<T> *)c->tab, nullptr);
+      (Cx<T> *)c->tab);
   timer_end(c, c->t_tab);
   SArgs a;
   a.n = c->n;
@@ -1727,7 +1721,7 @@ static int weighted_sk_t(cav_ctx *c, const double *weights, double *out)
   const int TS = p.nX + p.nY + p.nZ;
   k_tables<T><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
       c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
-      (Cx<T> *)c->tab, nullptr);
+      (Cx<T> *)c->tab);
   SArgs a;
   a.n = c->n;
   a.TS = TS;
@@ -1775,7 +1769,7 @@ int ewald_k_field(cav_ctx *c, double *out4)
   const int TS = p.nX + p.nY + p.nZ;
   k_tables<float><<<grid_for((long)c->n * TS), kBlock, 0, c->stream>>>(
       c->n, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
-      (Cx<float> *)c->tab, nullptr);
+      (Cx<float> *)c->tab);
   CAV_HIP(hipGetLastError());
   return kforce<float>(c, c->n, c->tab, out4);
 }
@@ -1820,12 +1814,12 @@ int point_field_k(cav_ctx *c, int np, const double *pts, double *field, double *
   if (c->precision == CAV_FP64) {
     k_tables<double><<<grid_for((long)np * TS), kBlock, 0, c->stream>>>(
         np, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, d, d + np, d + 2 * np,
-        (Cx<double> *)tab, nullptr);
+        (Cx<double> *)tab);
     CAV_TRY(kforce<double>(c, np, tab, d + 3 * np));
   } else {
     k_tables<float><<<grid_for((long)np * TS), kBlock, 0, c->stream>>>(
         np, TS, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, d, d + np, d + 2 * np,
-        (Cx<float> *)tab, nullptr);
+        (Cx<float> *)tab);
     CAV_TRY(kforce<float>(c, np, tab, d + 3 * np));
   }
   std::vector<double> o4(4 * (size_t)np);

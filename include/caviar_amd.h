@@ -197,8 +197,8 @@ int cav_batch_virial(cav_batch *b, int replica, double *virial);
 int cav_batch_get_pairs(cav_batch *b, int replica, int *ij, long capacity, long *npairs);
 /* Autotuner: times the batched Ewald_k passes for each candidate tile shape on
  * this batch and keeps the fastest; cav_batch_tiles reports the choice (S-pass
- * row tiles per wave, atoms per LDS stage, atoms per chunk; force-pass atom
- * tiles per wave). */
+ * row tiles per wave, atoms per LDS stage (32 in the split form: one table
+ * block), atoms per chunk; force-pass atom tiles per wave). */
 int cav_batch_autotune(cav_batch *b);
 /* cav_adjoint for every replica at its current positions: weights holds the
  * replicas' residual weights one after another (3 n_r doubles each, replica
@@ -209,10 +209,10 @@ int cav_batch_tiles(const cav_batch *b, int *s_rows, int *s_stage, int *s_chunk,
 
 /* ---- tuning (no counterpart in the reference) ------------------------------ */
 /* Autotuner: times the Ewald_k passes for each candidate tile shape on this
- * context's own system (S-pass row tiles per wave, atoms per LDS stage, atoms
+ * context's own system, as cav_step runs them (S-pass row tiles per wave, atoms
  * per chunk; split form: force-pass atom tiles per wave and waves per
- * workgroup) and keeps the fastest; the forces of the current state are
- * recomputed afterwards.
+ * workgroup; the split S pass stages 32 atoms, one table block) and keeps the
+ * fastest; the forces of the current state are recomputed afterwards.
  * cav_tiles reports the tiles in use (0 = the built-in choice). */
 int cav_autotune(cav_ctx *ctx);
 int cav_tiles(const cav_ctx *ctx, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles,

@@ -958,7 +958,8 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
     xoff[tau] = (rel / kTabAtoms) * a.TS * (kTabRow / 8) + rel % kTabAtoms;
   }
   // lattice row of the lane, r = rb*16 + l%16 = nxi nY + nyi, kept incrementally;
-  // rows past R have zero operands, so they add nothing
+  // rows past R have zero operands (U = 0) and read the phase factors of row 0
+  // (finite), so they add exactly nothing
   int nxi = lr / a.nY, nyi = lr % a.nY;
 
   const int nrb = a.Rpad >> 4;
@@ -984,12 +985,14 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
     }
     // phase factors X Y of the lane's (atom, row) pairs, formed while the
     // products below run (atoms past n give sums that are never written)
-    const float wx = (float)nxi, wy = (float)(nyi - a.ky);
+    const bool rin = rb * 16 + lr < a.R;
+    const int rx = rin ? nxi : 0, ry = rin ? nyi : 0;
+    const float wx = (float)rx, wy = (float)(ry - a.ky);
     float pr[NT][4], pi[NT][4];
 #pragma unroll
     for (int tau = 0; tau < NT; ++tau) {
-      const float4 *px = reinterpret_cast<const float4 *>(lxy + xoff[tau] + nxi * (kTabRow / 8));
-      const float4 *py = reinterpret_cast<const float4 *>(lxy + xoff[tau] + (offY + nyi) * (kTabRow / 8));
+      const float4 *px = reinterpret_cast<const float4 *>(lxy + xoff[tau] + rx * (kTabRow / 8));
+      const float4 *py = reinterpret_cast<const float4 *>(lxy + xoff[tau] + (offY + ry) * (kTabRow / 8));
       const float4 x01 = px[0], x23 = px[1], y01 = py[0], y23 = py[1];
       const float xr[4] = {x01.x, x01.z, x23.x, x23.z}, xi[4] = {x01.y, x01.w, x23.y, x23.w};
       const float yr[4] = {y01.x, y01.z, y23.x, y23.z}, yi[4] = {y01.y, y01.w, y23.y, y23.w};

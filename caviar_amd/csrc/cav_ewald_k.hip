@@ -402,6 +402,32 @@ __global__ void k_chunk_sum(const P *part, int nchunks, long plane, double *out)
   if (wave == 0 && e < plane) out[e] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
+// The same sums for FP32 partials, four adjacent entries per lane (16-byte
+// loads; plane is a multiple of 4); per entry the summation order is
+// k_chunk_sum's.
+__global__ void k_chunk_sum4(const float *part, int nchunks, long plane, double *out)
+{
+  __shared__ double red[4][4][64];
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long e = ((long)blockIdx.x * 64 + lane) * 4;
+  double s[4] = {0, 0, 0, 0};
+  if (e < plane)
+    for (int k = wave; k < nchunks; k += 4) {
+      const float4 v = *reinterpret_cast<const float4 *>(part + (long)k * plane + e);
+      s[0] += (double)v.x;
+      s[1] += (double)v.y;
+      s[2] += (double)v.z;
+      s[3] += (double)v.w;
+    }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) red[wave][i][lane] = s[i];
+  __syncthreads();
+  if (wave == 0 && e < plane)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      out[e + i] = (red[0][i][lane] + red[1][i][lane]) + (red[2][i][lane] + red[3][i][lane]);
+}
+
 // zero the padding rows of the operand (kappa >= 2 nZ)
 template <typename T>
 __global__ void k_zero_t(long n, T *p)
@@ -867,7 +893,6 @@ struct FArgsH {
   const int *type;
   double c0, kxs, kys, kzs;  // c0 includes 1/sB
   double *ax, *ay, *az;
-  double *out4;
   const int4 *items;  // batched, as FArgs
   const RepDev *rep;
   long bm_stride;
@@ -943,11 +968,13 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
         alo[tau][s][u] = l;
       }
   }
-  float fx[NT][4], fy[NT][4], fz[NT][4], ph[NT][4];
+  // the split pass computes accelerations only (point fields use the FP32 /
+  // FP64 tiles), so the potential column is not accumulated
+  float fx[NT][4], fy[NT][4], fz[NT][4];
 #pragma unroll
   for (int tau = 0; tau < NT; ++tau)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) fx[tau][e] = fy[tau][e] = fz[tau][e] = ph[tau][e] = 0;
+    for (int e = 0; e < 4; ++e) fx[tau][e] = fy[tau][e] = fz[tau][e] = 0;
 
   // LDS offset (in complex values) of the lane's atoms 4 (l/16) .. +3 of each
   // tile in the blocked [X | Y] rows: adjacent, so one 32-byte read per row
@@ -1048,7 +1075,6 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
         fx[tau][e] += wx * h;
         fy[tau][e] += wy * h;
         fz[tau][e] += pr[tau][e] * uzi + pi[tau][e] * uzr;
-        ph[tau][e] += pr[tau][e] * ur - pi[tau][e] * ui;
       }
     nyi += 16;
     while (nyi >= a.nY) {
@@ -1069,28 +1095,20 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
   for (int tau = 0; tau < NT; ++tau)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      double vx = fx[tau][e], vy = fy[tau][e], vz = fz[tau][e], vp = ph[tau][e];
+      double vx = fx[tau][e], vy = fy[tau][e], vz = fz[tau][e];
 #pragma unroll
       for (int o = 1; o < 16; o <<= 1) {
         vx += __shfl_xor(vx, o, 64);
         vy += __shfl_xor(vy, o, 64);
         vz += __shfl_xor(vz, o, 64);
-        vp += __shfl_xor(vp, o, 64);
       }
       const long atom = base + tau * 16 + 4 * lk + e;
       if (lr == 0 && atom < a.n) {
         const double gx = a.c0 * a.kxs * vx, gy = a.c0 * a.kys * vy, gz = a.c0 * a.kzs * vz;
-        if (a.out4) {
-          a.out4[4 * atom + 0] = gx;
-          a.out4[4 * atom + 1] = gy;
-          a.out4[4 * atom + 2] = gz;
-          a.out4[4 * atom + 3] = a.c0 * vp;
-        } else {
-          const double qm = a.q[atom] * a.minv[a.type[atom]];
-          a.ax[atom] += gx * qm;
-          a.ay[atom] += gy * qm;
-          a.az[atom] += gz * qm;
-        }
+        const double qm = a.q[atom] * a.minv[a.type[atom]];
+        a.ax[atom] += gx * qm;
+        a.ay[atom] += gy * qm;
+        a.az[atom] += gz * qm;
       }
     }
 }
@@ -1591,7 +1609,7 @@ static int sk_pipeline_h(cav_ctx *c)
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
   const long plane = (long)p.Rpad * a.SC;
-  k_chunk_sum<float><<<(plane + 63) / 64, kBlock, 0, c->stream>>>(
+  k_chunk_sum4<<<(plane / 4 + 63) / 64, kBlock, 0, c->stream>>>(
       reinterpret_cast<const float *>(c->sk_part), nch, plane, c->sk_sum);
   k_sk_reduce<float, double><<<grid_for(nthr), kBlock, 0, c->stream>>>(
       1, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_sum, c->d_W, c->sk, (float *)c->bmat,
@@ -1651,7 +1669,6 @@ static int kforce_h(cav_ctx *c)
   f.ax = c->ax;
   f.ay = c->ay;
   f.az = c->az;
-  f.out4 = nullptr;
   f.items = nullptr;
   f.rep = nullptr;
   f.bm_stride = 0;
@@ -2174,7 +2191,6 @@ static int batch_ewald_k_t(cav_batch *b)
     f.ax = b->ax;
     f.ay = b->ay;
     f.az = b->az;
-    f.out4 = nullptr;
     f.items = b->f_items;
     f.rep = b->d_rep;
     f.bm_stride = b->bmh_stride * (long)sizeof(_Float16);

@@ -626,6 +626,14 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
 // past the end of a block range are zero.
 constexpr int kTabAtoms = 32, kTabRow = 272;
 
+// Bytes between the (Zr, -Zi) rows and the (Zi, Zr) rows of an S-pass LDS
+// buffer: a multiple of 16 that puts the second set 32 banks (of 64 x 4 B)
+// from the first.
+__host__ __device__ constexpr unsigned split_zpad(int nZ)
+{
+  return 4u * (unsigned)((32 - (nZ * (kTabRow / 4)) % 64 + 64) % 64);
+}
+
 __device__ __forceinline__ void put_split(unsigned char *xy, unsigned char *zt, long blk, int e,
                                           int j, int TXY, int nZ, float cs, float sn)
 {
@@ -716,7 +724,11 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   // nZ + 8 SCT rows (rows past them are never written: their columns are
   // masked on output); the block's scaled charges
   const unsigned zrows = (unsigned)(a.nZ + 8 * SCT);
-  const unsigned buf_bytes = rec_bytes + zrows * kTabRow + S * 4;
+  // the (Zi, Zr) rows start zpad bytes later, so that they sit 32 banks away
+  // from the (Zr, -Zi) rows: a column tile's even and odd lanes then read
+  // disjoint banks
+  const unsigned zpad = split_zpad(a.nZ);
+  const unsigned buf_bytes = rec_bytes + zrows * kTabRow + zpad + S * 4;
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
   const unsigned char *ztab = reinterpret_cast<const unsigned char *>(a.ztab);
 
@@ -764,7 +776,8 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
     yo[r] = (unsigned)(a.nX + (ok ? row % a.nY : 0)) * kTabRow + 32 * lk;
   }
   // column s*16 + l%16: even columns are (Zr, -Zi) rows, odd ones (Zi, Zr)
-  const unsigned zo = rec_bytes + (unsigned)((lr & 1) * a.nZ + (lr >> 1)) * kTabRow + 16 * lk;
+  const unsigned zo = rec_bytes + (lr & 1) * zpad +
+                      (unsigned)((lr & 1) * a.nZ + (lr >> 1)) * kTabRow + 16 * lk;
   const bool active = rt0 < rt_end;
 
   // The last wave of the workgroup is the producer: it streams each block's
@@ -780,12 +793,12 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
     copy_to_lds_async(tab + blk * rec_bytes, dst, rec_bytes, 0, 1, lane);
     copy_to_lds_async(ztab + blk * zrow_bytes, dst + rec_bytes, zrow_bytes, 0, 1, lane);
     if (lane < S)
-      reinterpret_cast<float *>(dst + rec_bytes + zrows * kTabRow)[lane] =
+      reinterpret_cast<float *>(dst + rec_bytes + zrows * kTabRow + zpad)[lane] =
           base + lane < a1 ? (float)a.q[base + lane + qoff] * a.sa : 0.f;
   };
   auto finish = [&](int b) {
     unsigned char *dst = smem_raw + b * buf_bytes;
-    const float *sq = reinterpret_cast<const float *>(dst + rec_bytes + zrows * kTabRow);
+    const float *sq = reinterpret_cast<const float *>(dst + rec_bytes + zrows * kTabRow + zpad);
     // (Zi, Zr) halves from (Zr, -Zi): swap the two fp16 halves of each word
     // and negate the new low one (rounding to fp16 is odd-symmetric, so this
     // is exactly the split of (Zi, Zr))
@@ -795,7 +808,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
       u4v o;
 #pragma unroll
       for (int k = 0; k < 4; ++k) o[k] = ((w[k] >> 16) | (w[k] << 16)) ^ 0x8000u;
-      *reinterpret_cast<u4v *>(dst + rec_bytes + (a.nZ + row) * kTabRow + q * 16) = o;
+      *reinterpret_cast<u4v *>(dst + rec_bytes + zpad + (a.nZ + row) * kTabRow + q * 16) = o;
     }
     for (int e = lane; e < a.nX * S; e += 64) {
       const int c = e / S, j = e % S;
@@ -1536,7 +1549,7 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
 static size_t split_sk_lds(const EwaldKPlan &p)
 {
   const int zrows = p.nZ + 8 * (roundup(2 * p.nZ, 16) / 16);
-  return 2 * ((size_t)(p.nX + p.nY + zrows) * kTabRow + kTabAtoms * sizeof(float));
+  return 2 * ((size_t)(p.nX + p.nY + zrows) * kTabRow + split_zpad(p.nZ) + kTabAtoms * sizeof(float));
 }
 
 template <int RT>

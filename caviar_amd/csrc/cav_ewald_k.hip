@@ -672,8 +672,8 @@ __device__ __forceinline__ void split_phase(int e, int nX, int nY, int ky, int k
 // One thread per (block, entry, atom of the block): consecutive threads write
 // consecutive atoms of one row.
 __global__ void k_tables_h(long npts, int kx, int ky, int kz, double ilx, double ily, double ilz,
-                           const double *x, const double *y, const double *z, unsigned char *xy,
-                           unsigned char *zt)
+                           const double *x, const double *y, const double *z, const double *q,
+                           float sa, unsigned char *xy, unsigned char *zt)
 {
   const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
   const long nblk = (npts + kTabAtoms - 1) / kTabAtoms;
@@ -684,7 +684,14 @@ __global__ void k_tables_h(long npts, int kx, int ky, int kz, double ilx, double
   const int e = (int)(be - blk * TS);
   const long a = blk * kTabAtoms + j;
   float cs = 0.f, sn = 0.f;
-  if (a < npts) split_phase(e, nX, nY, ky, kz, x[a] * ilx, y[a] * ily, z[a] * ilz, cs, sn);
+  if (a < npts) {
+    split_phase(e, nX, nY, ky, kz, x[a] * ilx, y[a] * ily, z[a] * ilz, cs, sn);
+    if (e < nX) {  // X carries the atom's scaled charge: q X Y in one complex product
+      const float qs = (float)q[a] * sa;
+      cs *= qs;
+      sn *= qs;
+    }
+  }
   put_split(xy, zt, blk, e, j, nX + nY, nZ, cs, sn);
 }
 
@@ -696,13 +703,10 @@ __device__ __forceinline__ f4 mfma_h(u4v a, u4v b, f4 c)
 
 struct SArgsH {
   int n, TS, nX, nY, nZ, Rtiles, chunk, stage, tiles_per_group, Rpad, SC;
-  float sa;
-  const double *q;
   const void *tab;   // blocked FP32 records [X | Y] (TS = nX + nY rows per block)
   const void *ztab;  // blocked split (Zr, -Zi) rows (nZ per block)
   float *part;       // [chunk][Rpad][SC]: FP32 chunk sums, summed over chunks in FP64
   const int4 *items; // batched: (first, end) table position, chunk slot, replica
-  const RepDev *rep; // batched: charges of replica r at q[obase + position - tbase]
   int ncols, groups; // logical grid, as SArgs
 };
 
@@ -720,15 +724,15 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   constexpr int S = kTabAtoms;
   const unsigned rec_bytes = (unsigned)a.TS * kTabRow;  // [X | Y] rows of a block
   const unsigned zrow_bytes = (unsigned)a.nZ * kTabRow;  // (Zr, -Zi) rows of a block
-  // per buffer: records; nZ (Zr, -Zi) rows then nZ (Zi, Zr) rows, padded to
-  // nZ + 8 SCT rows (rows past them are never written: their columns are
-  // masked on output); the block's scaled charges
+  // per buffer: records (X already carries each atom's scaled charge); nZ
+  // (Zr, -Zi) rows then nZ (Zi, Zr) rows, padded to nZ + 8 SCT rows (rows past
+  // them are never written: their columns are masked on output)
   const unsigned zrows = (unsigned)(a.nZ + 8 * SCT);
   // the (Zi, Zr) rows start zpad bytes later, so that they sit 32 banks away
   // from the (Zr, -Zi) rows: a column tile's even and odd lanes then read
   // disjoint banks
   const unsigned zpad = split_zpad(a.nZ);
-  const unsigned buf_bytes = rec_bytes + zrows * kTabRow + zpad + S * 4;
+  const unsigned buf_bytes = rec_bytes + zrows * kTabRow + zpad;
   const unsigned char *tab = reinterpret_cast<const unsigned char *>(a.tab);
   const unsigned char *ztab = reinterpret_cast<const unsigned char *>(a.ztab);
 
@@ -749,13 +753,11 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   long a0 = (long)chunk * a.chunk;
   long a1 = a0 + a.chunk;
   if (a1 > a.n) a1 = a.n;
-  long qoff = 0;
   if (a.items) {
     const int4 it = a.items[col];
     a0 = it.x;
     a1 = it.y;
     chunk = it.z;
-    qoff = (long)a.rep[it.w].obase - a.rep[it.w].tbase;
   }
 
   f4 acc[RT][SCT];
@@ -781,43 +783,43 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   const bool active = rt0 < rt_end;
 
   // The last wave of the workgroup is the producer: it streams each block's
-  // record and Z rows into the other LDS buffer while the compute waves
-  // multiply the current one, then folds each atom's scaled charge into its X
-  // entries (zero for atoms outside the chunk), so a compute wave forms q X Y
-  // with one complex product.  The compute waves issue no asynchronous LDS
-  // writes themselves, so their LDS reads never wait on them.
+  // record and (Zr, -Zi) rows into the other LDS buffer while the compute
+  // waves multiply the current one, then derives the (Zi, Zr) rows.  A chunk
+  // starts on a block and ends on one or at the end of the atoms, whose block
+  // is padded with zero entries, so no atom outside the chunk is staged.  The
+  // compute waves issue no asynchronous LDS writes themselves, so their LDS
+  // reads never wait on them.
   const bool producer = wave == nwaves - 1;
   auto issue = [&](long base, int b) {
     unsigned char *dst = smem_raw + b * buf_bytes;
     const long blk = base / S;
     copy_to_lds_async(tab + blk * rec_bytes, dst, rec_bytes, 0, 1, lane);
     copy_to_lds_async(ztab + blk * zrow_bytes, dst + rec_bytes, zrow_bytes, 0, 1, lane);
-    if (lane < S)
-      reinterpret_cast<float *>(dst + rec_bytes + zrows * kTabRow + zpad)[lane] =
-          base + lane < a1 ? (float)a.q[base + lane + qoff] * a.sa : 0.f;
   };
+  // (Zi, Zr) halves from (Zr, -Zi): swap the two fp16 halves of each word and
+  // negate the new low one (rounding to fp16 is odd-symmetric, so this is
+  // exactly the split of (Zi, Zr)).  Four 16-byte pieces per lane are read
+  // before any is written, so the reads overlap.
   auto finish = [&](int b) {
-    unsigned char *dst = smem_raw + b * buf_bytes;
-    const float *sq = reinterpret_cast<const float *>(dst + rec_bytes + zrows * kTabRow + zpad);
-    // (Zi, Zr) halves from (Zr, -Zi): swap the two fp16 halves of each word
-    // and negate the new low one (rounding to fp16 is odd-symmetric, so this
-    // is exactly the split of (Zi, Zr))
-    for (int e = lane; e < a.nZ * 16; e += 64) {
-      const int row = e >> 4, q = e & 15;
-      const u4v w = *reinterpret_cast<const u4v *>(dst + rec_bytes + row * kTabRow + q * 16);
-      u4v o;
+    unsigned char *zd = smem_raw + b * buf_bytes + rec_bytes;
+    const int total = a.nZ * 16;
+    for (int e0 = lane; e0 < total; e0 += 256) {
+      u4v w[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) o[k] = ((w[k] >> 16) | (w[k] << 16)) ^ 0x8000u;
-      *reinterpret_cast<u4v *>(dst + rec_bytes + zpad + (a.nZ + row) * kTabRow + q * 16) = o;
-    }
-    for (int e = lane; e < a.nX * S; e += 64) {
-      const int c = e / S, j = e % S;
-      Cx<float> *x = reinterpret_cast<Cx<float> *>(dst + c * kTabRow + j * 8);
-      const float qj = sq[j];
-      Cx<float> v = *x;
-      v.re = qj != 0.f ? v.re * qj : 0.f;
-      v.im = qj != 0.f ? v.im * qj : 0.f;
-      *x = v;
+      for (int k = 0; k < 4; ++k) {
+        const int e = e0 + 64 * k;
+        if (e < total) w[k] = *reinterpret_cast<const u4v *>(zd + (e >> 4) * kTabRow + (e & 15) * 16);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int e = e0 + 64 * k;
+        if (e < total) {
+          u4v o;
+#pragma unroll
+          for (int m = 0; m < 4; ++m) o[m] = ((w[k][m] >> 16) | (w[k][m] << 16)) ^ 0x8000u;
+          *reinterpret_cast<u4v *>(zd + zpad + (a.nZ + (e >> 4)) * kTabRow + (e & 15) * 16) = o;
+        }
+      }
     }
   };
 
@@ -902,14 +904,14 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
 struct FArgsH {
   int n, TS, nX, nY, nZ, ky, kz, R, Rpad, nkh;  // TS = nX + nY: FP32 [X | Y] record length
   const void *tab, *ztab, *bmh;
-  const double *q, *minv;
+  const double *minv;
   const int *type;
-  double c0, kxs, kys, kzs;  // c0 includes 1/sB
+  double c0, kxs, kys, kzs;  // c0 includes 1/(sA sB): X carries q sA, the operand sB
   double *ax, *ay, *az;
   const int4 *items;  // batched, as FArgs
   const RepDev *rep;
   long bm_stride;
-  double inv_sb;      // batched: c0 of a replica is multiplied by it
+  double inv_sb;      // batched: c0 of a replica is multiplied by it (1/(sA sB))
 };
 
 // Force pass of the split form: A = the atoms' Z (kappa = 2 nz + re/im, from
@@ -1118,10 +1120,11 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
       const long atom = base + tau * 16 + 4 * lk + e;
       if (lr == 0 && atom < a.n) {
         const double gx = a.c0 * a.kxs * vx, gy = a.c0 * a.kys * vy, gz = a.c0 * a.kzs * vz;
-        const double qm = a.q[atom] * a.minv[a.type[atom]];
-        a.ax[atom] += gx * qm;
-        a.ay[atom] += gy * qm;
-        a.az[atom] += gz * qm;
+        // the phase factors carry the atom's charge (times sA, removed in c0)
+        const double m = a.minv[a.type[atom]];
+        a.ax[atom] += gx * m;
+        a.ay[atom] += gy * m;
+        a.az[atom] += gz * m;
       }
     }
 }
@@ -1544,12 +1547,13 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
 }
 
 // LDS of a split S-pass workgroup: two buffers of one table block each
-// ([X | Y] rows, nZ + 8 SCT Z rows, 32 scaled charges).  (A third buffer, to
-// request blocks two ahead, measured slower: it halves the workgroups per CU.)
+// ([X | Y] rows, nZ + 8 SCT Z rows and the bank padding between the two Z
+// row sets).  (A third buffer, to request blocks two ahead, measured slower:
+// it halves the workgroups per CU.)
 static size_t split_sk_lds(const EwaldKPlan &p)
 {
   const int zrows = p.nZ + 8 * (roundup(2 * p.nZ, 16) / 16);
-  return 2 * ((size_t)(p.nX + p.nY + zrows) * kTabRow + split_zpad(p.nZ) + kTabAtoms * sizeof(float));
+  return 2 * ((size_t)(p.nX + p.nY + zrows) * kTabRow + split_zpad(p.nZ));
 }
 
 template <int RT>
@@ -1582,8 +1586,8 @@ static int sk_pipeline_h(cav_ctx *c)
   const long nblk = (c->n + kTabAtoms - 1) / kTabAtoms;
   timer_begin(c, c->t_tab);
   k_tables_h<<<grid_for(nblk * TS * kTabAtoms), kBlock, 0, c->stream>>>(
-      c->n, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z,
-      (unsigned char *)c->tab, (unsigned char *)c->ztab);
+      c->n, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z, c->q,
+      (float)c->sA, (unsigned char *)c->tab, (unsigned char *)c->ztab);
   timer_end(c, c->t_tab);
   SArgsH a;
   a.n = c->n;
@@ -1595,13 +1599,10 @@ static int sk_pipeline_h(cav_ctx *c)
   a.chunk = (c->n + c->sk_chunks - 1) / c->sk_chunks;
   a.chunk = (a.chunk + kTabAtoms - 1) / kTabAtoms * kTabAtoms;
   a.stage = kTabAtoms;
-  a.sa = (float)c->sA;
-  a.q = c->q;
   a.tab = c->tab;
   a.ztab = c->ztab;
   a.part = reinterpret_cast<float *>(c->sk_part);
   a.items = nullptr;
-  a.rep = nullptr;
   a.Rpad = p.Rpad;
   a.SC = roundup(2 * p.nZ, 16);
   const int RT = c->tune_rt;
@@ -1672,10 +1673,9 @@ static int kforce_h(cav_ctx *c)
   f.tab = c->tab;
   f.ztab = c->ztab;
   f.bmh = c->bmat_h;
-  f.q = c->q;
   f.minv = c->d_minv;
   f.type = c->type;
-  f.c0 = CAV_4PI * c->ek_ke * p.lxyz_inv / c->sB;
+  f.c0 = CAV_4PI * c->ek_ke * p.lxyz_inv / c->sB / c->sA;  // both powers of two
   f.kxs = CAV_2PI / p.lx;
   f.kys = CAV_2PI / p.ly;
   f.kzs = CAV_2PI / p.lz;
@@ -1928,7 +1928,8 @@ __global__ void kb_tables(const RepDev *rep, int kx, int ky, int kz, const doubl
 
 // Batched split tables: replica r's atoms fill the blocks from r.tbase.
 __global__ void kb_tables_h(const RepDev *rep, int kx, int ky, int kz, const double *x,
-                            const double *y, const double *z, unsigned char *xy, unsigned char *zt)
+                            const double *y, const double *z, const double *q, float sa,
+                            unsigned char *xy, unsigned char *zt)
 {
   const RepDev &r = rep[blockIdx.y];
   const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
@@ -1940,8 +1941,14 @@ __global__ void kb_tables_h(const RepDev *rep, int kx, int ky, int kz, const dou
   const int e = (int)(be - blk * TS);
   const long a = blk * kTabAtoms + j, s = r.base + a;
   float cs = 0.f, sn = 0.f;
-  if (a < r.n)
+  if (a < r.n) {
     split_phase(e, nX, nY, ky, kz, x[s] * r.il[0], y[s] * r.il[1], z[s] * r.il[2], cs, sn);
+    if (e < nX) {
+      const float qs = (float)q[r.obase + a] * sa;
+      cs *= qs;
+      sn *= qs;
+    }
+  }
   put_split(xy, zt, r.tbase / kTabAtoms + blk, e, j, nX + nY, nZ, cs, sn);
 }
 
@@ -2119,7 +2126,8 @@ static int batch_ewald_k_t(cav_batch *b)
   } else {
     const dim3 hgrid(grid_for((long)(b->maxn + kTabAtoms - 1) / kTabAtoms * kTabAtoms * TS), b->B);
     kb_tables_h<<<hgrid, kBlock, 0, b->stream>>>(b->d_rep, p.kx, p.ky, p.kz, b->x, b->y, b->z,
-                                                 (unsigned char *)b->tab, (unsigned char *)b->ztab);
+                                                 b->q_o, (float)b->sA, (unsigned char *)b->tab,
+                                                 (unsigned char *)b->ztab);
     SArgsH a;
     a.n = 0;
     a.TS = p.nX + p.nY;
@@ -2130,15 +2138,12 @@ static int batch_ewald_k_t(cav_batch *b)
     a.chunk = b->s_chunk;
     a.stage = kTabAtoms;
     a.tiles_per_group = 0;
-    a.sa = (float)b->sA;
-    a.q = b->q_o;
     a.tab = b->tab;
     a.ztab = b->ztab;
     a.part = reinterpret_cast<float *>(b->sk_part);
     a.Rpad = p.Rpad;
     a.SC = SC;
     a.items = b->s_items;
-    a.rep = b->d_rep;
     const int RT = b->s_rt;
     s_layout_h(p.Rpad, RT, &groups, &tpg, &waves);
     a.tiles_per_group = tpg;
@@ -2197,7 +2202,6 @@ static int batch_ewald_k_t(cav_batch *b)
     f.tab = b->tab;
     f.ztab = b->ztab;
     f.bmh = b->bmat_h;
-    f.q = b->q_o;
     f.minv = b->d_minv;
     f.type = b->type_o;
     f.c0 = f.kxs = f.kys = f.kzs = 0;
@@ -2207,7 +2211,7 @@ static int batch_ewald_k_t(cav_batch *b)
     f.items = b->f_items;
     f.rep = b->d_rep;
     f.bm_stride = b->bmh_stride * (long)sizeof(_Float16);
-    f.inv_sb = 1.0 / b->sB;
+    f.inv_sb = 1.0 / b->sB / b->sA;
     if (b->f_nt == 1)
       launch_kforce_h<1, kWavesF>(b->stream, f, b->nkh, b->n_fitems, split_force_lds_p(p, b->nkh, 1));
     else

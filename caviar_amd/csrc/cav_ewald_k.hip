@@ -2384,17 +2384,21 @@ int ewald_k_autotune(cav_ctx *c)
   }
   std::sort(timed.begin(), timed.end());
   Cand pick = {c->tune_rt, c->tune_stage, c->tune_chunk, c->tune_fnt, c->tune_fw};
+  std::vector<std::pair<float, int>> finals;
   float best = 1e30f;
   for (int q = 0; q < (int)timed.size() && q < 4; ++q) {
-    const Cand &k = cands[timed[q].second];
-    CAV_TRY(apply(k));
+    CAV_TRY(apply(cands[timed[q].second]));
     float ms = 0;
     CAV_TRY(time_it(10, &ms));
-    if (ms < best) {
-      best = ms;
-      pick = k;
-    }
+    finals.push_back({ms, timed[q].second});
+    best = std::min(best, ms);
   }
+  // within 0.5% of the fastest the smallest chunk wins (fewer atoms per FP32
+  // accumulation); a tie is below the run-to-run spread of these timings
+  int pq = -1;
+  for (const auto &f : finals)
+    if (f.first <= best * 1.005f && (pq < 0 || cands[f.second].chunk < cands[pq].chunk)) pq = f.second;
+  if (pq >= 0) pick = cands[pq];
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->timing = timing;

@@ -2339,8 +2339,11 @@ int ewald_k_autotune(cav_ctx *c)
   // error grows as sqrt(chunk)), FP64 across chunks; fewer chunks, fewer partials
   const int chunks[5] = {256, 512, 1024, 2048, 4096};
   if (c->precision == CAV_MIXED) {
-    for (int rt : {1, 2, 3, 4})  // the split S pass stages one 32-atom table block
-      for (int ch : chunks)
+    // the split S pass stages one 32-atom table block; 4096-atom chunks only
+    // pay beside the concurrent list build, where they tie with 2048 (within
+    // the timing spread) while running 2x slower in order
+    for (int rt : {1, 2, 3, 4})
+      for (int ch : {256, 512, 1024, 2048})
         for (int nt : {1, 2})
           for (int fw : {kWavesF, 16}) cands.push_back({rt, kTabAtoms, ch, nt, fw});
   } else {

@@ -41,7 +41,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_engine(system, precision, warmup, steps, dt, pair_kernel="auto"):
+def run_engine(system, precision, warmup, steps, dt, pair_kernel="auto", pin=None):
     from caviar_amd import Engine
     e = Engine(system, precision=precision, timing=False)
     e.set_pair_kernel(pair_kernel)
@@ -49,8 +49,12 @@ def run_engine(system, precision, warmup, steps, dt, pair_kernel="auto"):
     e.setup()
     log(f"{precision}: setup {time.perf_counter() - t0:.2f}s")
     t0 = time.perf_counter()
-    tiles = e.autotune()
-    log(f"{precision}: autotuned Ewald_k tiles {tiles} in {time.perf_counter() - t0:.2f}s")
+    if pin:
+        tiles = e.set_tiles(*pin)
+        log(f"{precision}: pinned Ewald_k tiles {tiles}")
+    else:
+        tiles = e.autotune()
+        log(f"{precision}: autotuned Ewald_k tiles {tiles} in {time.perf_counter() - t0:.2f}s")
     e.step(dt, warmup)
     t0 = time.perf_counter()
     e.step(dt, steps)          # every step ends with a stream synchronisation
@@ -285,10 +289,16 @@ def main():
                     help="unsampled steps from the random start before the NVE measurement")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched-replica workload and print its JSON")
+    ap.add_argument("--tiles", type=lambda v: [int(t) for t in v.split(",")], default=None,
+                    metavar="S_ROWS,S_CHUNK,F_TILES,F_WAVES",
+                    help="with --precision mixed|fp64: pin the Ewald_k tiles instead of "
+                         "tuning them (profiling runs that must see the same launches)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device with the HIP runtime (a kernel of "
                          "known traffic, to calibrate byte counters) and exit")
     args = ap.parse_args()
+    if args.tiles is not None and (len(args.tiles) != 4 or args.precision == "both"):
+        ap.error("--tiles takes four integers and needs --precision mixed or fp64")
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not part of this build)", file=sys.stderr)
@@ -306,7 +316,8 @@ def main():
     log(f"workload built in {time.perf_counter() - t0:.1f}s: N={system.n} L={system.size[0]:.3f}")
 
     if args.precision != "both":
-        run_engine(system, args.precision, args.warmup, args.steps, args.dt, args.pair_kernel)
+        run_engine(system, args.precision, args.warmup, args.steps, args.dt, args.pair_kernel,
+                   args.tiles)
         return
     # one engine at a time: each context's two HIP streams (short-range passes,
     # concurrent Ewald_k) then get hardware queues of their own

@@ -2397,10 +2397,15 @@ int ewald_k_autotune(cav_ctx *c)
     best = std::min(best, ms);
   }
   // within 0.5% of the fastest the smallest chunk wins (fewer atoms per FP32
-  // accumulation); a tie is below the run-to-run spread of these timings
+  // accumulation), then the fewest S-pass row tiles (the S pass runs faster
+  // in order with fewer); a tie is below the run-to-run spread of these timings
+  auto preferred = [&](int a, int b) {
+    return cands[a].chunk < cands[b].chunk ||
+           (cands[a].chunk == cands[b].chunk && cands[a].rt < cands[b].rt);
+  };
   int pq = -1;
   for (const auto &f : finals)
-    if (f.first <= best * 1.005f && (pq < 0 || cands[f.second].chunk < cands[pq].chunk)) pq = f.second;
+    if (f.first <= best * 1.005f && (pq < 0 || preferred(f.second, pq))) pq = f.second;
   if (pq >= 0) pick = cands[pq];
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

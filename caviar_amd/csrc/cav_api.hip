@@ -1035,8 +1035,8 @@ int cav_set_tiles(cav_ctx *c, int s_rows, int s_chunk, int f_tiles, int f_waves)
 {
   if (!c) return CAV_ERR_ARG;
   if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
-  if (s_rows < 1 || s_rows > 4 || s_chunk <= 0 || s_chunk % 32 || f_tiles < 1 || f_tiles > 2 ||
-      (f_waves != 8 && f_waves != 16))
+  if (s_rows < 1 || s_rows > 4 || s_chunk <= 0 || s_chunk > 4096 || s_chunk % 32 || f_tiles < 1 ||
+      f_tiles > 2 || (f_waves != 8 && f_waves != 16))
     return fail(c, CAV_ERR_ARG, "tiles out of range");
   if (!c->ek_on) return CAV_OK;
   (void)hipSetDevice(c->device);

@@ -218,7 +218,8 @@ int cav_autotune(cav_ctx *ctx);
 int cav_tiles(const cav_ctx *ctx, int *s_rows, int *s_stage, int *s_chunk, int *f_tiles,
               int *f_waves);
 /* Pins the Ewald_k tiles instead of tuning them (reproducible timings): S-pass
- * row tiles per wave 1..4, atoms per chunk (a multiple of 32, > 0); split form:
+ * row tiles per wave 1..4, atoms per chunk (a multiple of 32, 32..4096: the
+ * mixed form accumulates a chunk in FP32); split form:
  * force-pass atom tiles per wave 1..2 and waves per workgroup 8 or 16.  After
  * cav_setup; the forces of the current state are recomputed.  FP64 uses the
  * chunk only. */

@@ -305,6 +305,8 @@ def test_pinned_tiles_keep_results(gpu):
         e.set_tiles(5, 1024, 1, 16)
     with pytest.raises(CaviarError):
         e.set_tiles(3, 1000, 1, 16)
+    with pytest.raises(CaviarError):
+        e.set_tiles(3, 8192, 1, 16)
     e = Engine(s, precision="fp64")
     e.setup()
     e.set_tiles(3, 256, 1, 8)

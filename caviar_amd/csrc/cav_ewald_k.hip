@@ -669,23 +669,25 @@ __device__ __forceinline__ void split_phase(int e, int nX, int nY, int ky, int k
   sincospif((float)(2.0 * f), &sn, &cs);
 }
 
-// One thread per (block, entry, atom of the block): consecutive threads write
-// consecutive atoms of one row.
+// One thread per (atom, table entry); the grid's y index is the entry, so its
+// kind (X, Y or Z row) and coordinate are uniform per workgroup and no thread
+// divides by the entry count.  Consecutive threads write consecutive atoms of
+// one row.
 __global__ void k_tables_h(long npts, int kx, int ky, int kz, double ilx, double ily, double ilz,
                            const double *x, const double *y, const double *z, const double *q,
                            float sa, unsigned char *xy, unsigned char *zt)
 {
-  const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
-  const long nblk = (npts + kTabAtoms - 1) / kTabAtoms;
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nblk * TS * kTabAtoms) return;
-  const int j = (int)(t & (kTabAtoms - 1));
-  const long be = t / kTabAtoms, blk = be / TS;
-  const int e = (int)(be - blk * TS);
-  const long a = blk * kTabAtoms + j;
+  const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1;
+  const int e = (int)blockIdx.y;
+  const long nat = (npts + kTabAtoms - 1) / kTabAtoms * kTabAtoms;
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= nat) return;
+  const long blk = a / kTabAtoms;
+  const int j = (int)(a & (kTabAtoms - 1));
   float cs = 0.f, sn = 0.f;
   if (a < npts) {
-    split_phase(e, nX, nY, ky, kz, x[a] * ilx, y[a] * ily, z[a] * ilz, cs, sn);
+    const double u = e < nX ? x[a] * ilx : (e < nX + nY ? y[a] * ily : z[a] * ilz);
+    split_phase(e, nX, nY, ky, kz, u, u, u, cs, sn);
     if (e < nX) {  // X carries the atom's scaled charge: q X Y in one complex product
       const float qs = (float)q[a] * sa;
       cs *= qs;
@@ -1585,7 +1587,7 @@ static int sk_pipeline_h(cav_ctx *c)
   const int TS = p.nX + p.nY + p.nZ;
   const long nblk = (c->n + kTabAtoms - 1) / kTabAtoms;
   timer_begin(c, c->t_tab);
-  k_tables_h<<<grid_for(nblk * TS * kTabAtoms), kBlock, 0, c->stream>>>(
+  k_tables_h<<<dim3(grid_for(nblk * kTabAtoms), TS), kBlock, 0, c->stream>>>(
       c->n, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z, c->q,
       (float)c->sA, (unsigned char *)c->tab, (unsigned char *)c->ztab);
   timer_end(c, c->t_tab);

@@ -1928,23 +1928,25 @@ __global__ void kb_tables(const RepDev *rep, int kx, int ky, int kz, const doubl
   tab[(r.obase + a) * TS + e] = v;
 }
 
-// Batched split tables: replica r's atoms fill the blocks from r.tbase.
+// Batched split tables: replica r's atoms fill the blocks from r.tbase.  The
+// grid's y index is the replica and its z index the table entry, as in
+// k_tables_h.
 __global__ void kb_tables_h(const RepDev *rep, int kx, int ky, int kz, const double *x,
                             const double *y, const double *z, const double *q, float sa,
                             unsigned char *xy, unsigned char *zt)
 {
   const RepDev &r = rep[blockIdx.y];
-  const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1, TS = nX + nY + nZ;
-  const long nblk = (r.n + kTabAtoms - 1) / kTabAtoms;
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nblk * TS * kTabAtoms) return;
-  const int j = (int)(t & (kTabAtoms - 1));
-  const long be = t / kTabAtoms, blk = be / TS;
-  const int e = (int)(be - blk * TS);
-  const long a = blk * kTabAtoms + j, s = r.base + a;
+  const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1;
+  const int e = (int)blockIdx.z;
+  const long nat = (r.n + kTabAtoms - 1) / kTabAtoms * kTabAtoms;
+  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= nat) return;
+  const long blk = a / kTabAtoms, s = r.base + a;
+  const int j = (int)(a & (kTabAtoms - 1));
   float cs = 0.f, sn = 0.f;
   if (a < r.n) {
-    split_phase(e, nX, nY, ky, kz, x[s] * r.il[0], y[s] * r.il[1], z[s] * r.il[2], cs, sn);
+    const double u = e < nX ? x[s] * r.il[0] : (e < nX + nY ? y[s] * r.il[1] : z[s] * r.il[2]);
+    split_phase(e, nX, nY, ky, kz, u, u, u, cs, sn);
     if (e < nX) {
       const float qs = (float)q[r.obase + a] * sa;
       cs *= qs;
@@ -2126,7 +2128,7 @@ static int batch_ewald_k_t(cav_batch *b)
     const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
     launch_sk_rt<T, kRT>(b->stream, a, dim3(b->n_sitems, groups), smem, SC / 16, waves);
   } else {
-    const dim3 hgrid(grid_for((long)(b->maxn + kTabAtoms - 1) / kTabAtoms * kTabAtoms * TS), b->B);
+    const dim3 hgrid(grid_for((long)(b->maxn + kTabAtoms - 1) / kTabAtoms * kTabAtoms), b->B, TS);
     kb_tables_h<<<hgrid, kBlock, 0, b->stream>>>(b->d_rep, p.kx, p.ky, p.kz, b->x, b->y, b->z,
                                                  b->q_o, (float)b->sA, (unsigned char *)b->tab,
                                                  (unsigned char *)b->ztab);

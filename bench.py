@@ -35,6 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # per Ewald_r list pair 43, LJ pair +12, integration 20 per atom.
 FLOP_SK, FLOP_KF, FLOP_PAIR, FLOP_LJ, FLOP_INT = 16, 22, 43, 12, 20
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_ions_N262144_kmax16.json")
+TRAFFIC_SOURCE = ("read from profiles/pmc_traffic_ions_N262144_kmax16.json (rocprofv3 FETCH_SIZE and "
+                  "WRITE_SIZE of the same launch at the same tiles, scaled by a copy of known size); "
+                  "not measured in this run")
 
 
 def log(msg):
@@ -80,12 +83,27 @@ def force_rms(a, b):
     return float(np.sqrt(((a - b) ** 2).sum(axis=1).mean() / (b ** 2).sum(axis=1).mean()))
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(system, threads):
     """Oracle (FP64 C restatement of the reference path) on the host cores, on a
-    bounded sample of the same workload, extrapolated to one full step."""
+    bounded sample of the same workload, extrapolated to one full step, on
+    ``threads`` threads and on one.  The LJ + Ewald_r part (about 0.1% of the
+    step) is timed over the whole system on one thread; Ewald_k, parallel over
+    k in the S pass and over atoms in the force pass (no atomics), on samples:
+    S(k) for ks k-vectors over all atoms and the k-space force of na atoms over
+    all k.  On one core the oracle runs 1.43x faster than the reference itself
+    (profiles/cpu_anchor_ions8000_kmax8.md)."""
     from oracle import Oracle
     n = system.n
-    # pair part: LJ + Ewald_r over the full system (serial, like the reference)
     pair_sys = system.copy()
     pair_sys.ewald_k = None
     o = Oracle(pair_sys, threads=1)
@@ -95,18 +113,27 @@ def cpu_baseline(system, threads):
     t0 = time.perf_counter()
     o.forces()
     t_pair = time.perf_counter() - t0
-    # Ewald_k sample: S(k) for ks k-vectors over all atoms, force pass for na atoms over all k
-    ko = Oracle(system, threads=threads)
     nk = (2 * system.ewald_k.kx_max) * (2 * system.ewald_k.ky_max) * (2 * system.ewald_k.kz_max)
-    ks, na = 256, 1024
-    ts, tf = ko.ewald_k_sample(0, ks, 0, na)
-    t_step = t_pair + ts * nk / ks + tf * n / na
-    log(f"cpu: pair {t_pair:.2f}s (setup {t_setup:.2f}s), S(k) {ts:.2f}s/{ks}k, force {tf:.2f}s/{na} atoms "
-        f"-> {t_step:.1f}s per step")
-    return {"value": n / t_step, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": (f"one LJ+Ewald_r force evaluation over all {n} atoms (1 thread) + Ewald_k "
-                       f"S(k) for {ks} of {nk} k-vectors over all atoms and the k-space force for "
-                       f"{na} of {n} atoms over all k ({threads} threads), scaled to one full step")}
+    res = {}
+    for th, ks, na in ((threads, 16 * 64, 16 * 512), (1, 64, 512)):
+        ko = Oracle(system, threads=th)
+        ts, tf = ko.ewald_k_sample(0, ks, 0, na)
+        t_step = t_pair + ts * nk / ks + tf * n / na
+        res[th] = (t_step, ts, tf, ks, na)
+        log(f"cpu {th} threads: pair {t_pair:.2f}s (setup {t_setup:.2f}s), S(k) {ts:.2f}s/{ks}k, "
+            f"force {tf:.2f}s/{na} atoms -> {t_step:.1f}s per step")
+    t_all, ts, tf, ks, na = res[threads]
+    t_one, ts1, tf1, ks1, na1 = res[1]
+    return {"value": n / t_all, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "value_1_thread": n / t_one,
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "seconds_per_step": {"all_threads": round(t_all, 2), "one_thread": round(t_one, 2)},
+            "sample": (f"one LJ+Ewald_r force evaluation over all {n} atoms (1 thread, "
+                       f"{t_pair:.2f}s) + Ewald_k S(k) for {ks} of {nk} k-vectors over all atoms "
+                       f"({ts:.2f}s) and the k-space force of {na} of {n} atoms over all k "
+                       f"({tf:.2f}s) on {threads} threads; on 1 thread {ks1} k-vectors "
+                       f"({ts1:.2f}s) and {na1} atoms ({tf1:.2f}s); each scaled to one full step")}
 
 
 
@@ -140,13 +167,30 @@ def lattice(kmax):
     return rpad, sc, kz, nkh
 
 
+def separable_flop(kmax):
+    """FLOP per atom of the separable-lattice algorithm the Ewald_k kernels run
+    (exp(ik.r) = X(nx) Y(ny) Z(nz) over the half lattice nx >= 0): S pass, one
+    complex product q X Y per row (6) and one complex multiply-add per (row,
+    column) (8); force pass, two complex multiply-adds per (row, column) (U and
+    nz U, 16) and an epilogue per row (P = X Y and Im(P U) times the k
+    components, 22)."""
+    rows, cols = (kmax + 1) * (2 * kmax + 1), 2 * kmax + 1
+    return 6 * rows + 8 * rows * cols, 16 * rows * cols + 22 * rows
+
+
 def kernel_rooflines(n, kmax, ppa, lj_ppa, tm32, tm64):
-    """Per kernel family: launch time, algorithmic work per launch, the bound and
-    the achieved fraction of the MI355X unit it runs on, and the MFMA FLOP the
-    kernel actually issues (counted from its tile shapes; rocprofv3's
+    """Per kernel family: launch time, the bound, and three rates of the MI355X
+    unit it runs on: ``algorithmic`` (FLOP of the separable-lattice algorithm
+    the Ewald_k kernels run, separable_flop; the pair kernel's per-pair count)
+    with ``frac`` of the unit's peak; ``survey_rate`` (the per-(k-vector, atom)
+    convention of 16 + 22 FLOP, which counts a sincos per term the kernels never
+    evaluate and can therefore exceed the peak; no fraction); ``executed`` (MFMA
+    FLOP the kernel issues, from its tile shapes: the split form's three
+    products per term and the row padding; rocprofv3's
     SQ_INSTS_VALU_MFMA_MOPS_* x 512 gives the same totals)."""
     nk = (2 * kmax) ** 3
     rpad, sc, kz, nkh = lattice(kmax)
+    sep_sk, sep_kf = separable_flop(kmax)
     traffic = {}
     if os.path.exists(TRAFFIC_FILE):
         traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch", {})
@@ -157,17 +201,19 @@ def kernel_rooflines(n, kmax, ppa, lj_ppa, tm32, tm64):
         exec_kf = (4 * rpad * 32 * nkh * 2 * 3 if split else 2 * 4 * rpad * kz) * n
         exec_sk = (2 * rpad * sc * 2 * 3 if split else 2 * rpad * sc * 2) * n
         rows = {
-            "ewald_k_force": ("mfma", mfma, FLOP_KF * nk * n, exec_kf),
-            "ewald_k_sk": ("mfma", mfma, FLOP_SK * nk * n, exec_sk),
+            "ewald_k_force": ("mfma", mfma, sep_kf * n, FLOP_KF * nk * n, exec_kf),
+            "ewald_k_sk": ("mfma", mfma, sep_sk * n, FLOP_SK * nk * n, exec_sk),
             "pair": ("gather", "fp32_vector" if split else "fp64_vector",
-                     n * (FLOP_PAIR * ppa + FLOP_LJ * lj_ppa), None),
+                     n * (FLOP_PAIR * ppa + FLOP_LJ * lj_ppa), None, None),
         }
         fo = {}
-        for k, (bound, unit, alg, ex) in rows.items():
+        for k, (bound, unit, alg, survey, ex) in rows.items():
             ms = tm[k][0]
             ach = alg / (ms * 1e-3) / 1e12
             d = {"ms": round(ms, 4), "bound": bound, "unit": "TFLOP/s", "peak": PEAK[unit],
                  "peak_unit": unit, "achieved": round(ach, 2), "frac": round(ach / PEAK[unit], 4)}
+            if survey is not None:
+                d["survey_rate"] = round(survey / (ms * 1e-3) / 1e12, 2)
             if ex is not None:
                 d["executed"] = round(ex / (ms * 1e-3) / 1e12, 2)
                 d["executed_frac"] = round(ex / (ms * 1e-3) / 1e12 / PEAK[unit], 4)
@@ -175,6 +221,7 @@ def kernel_rooflines(n, kmax, ppa, lj_ppa, tm32, tm64):
             if tb:
                 d["traffic"] = tb
                 d["traffic_GBs"] = round(tb / (ms * 1e-3) / 1e9, 1)
+                d["traffic_source"] = TRAFFIC_SOURCE
             fo[k] = d
         for k in ("list", "neighbor", "ewald_k_tables", "integrate"):
             fo[k] = {"ms": round(tm[k][0], 4)}
@@ -356,12 +403,18 @@ def main():
     # dominant kernel of the mixed step
     dom = max(("ewald_k_force", "ewald_k_sk", "pair"), key=lambda k: tm32[k][0])
     kd = kernels["mixed"][dom]
-    roof = {"bound": kd["bound"], "achieved": kd["achieved"], "peak": kd["peak"], "unit": kd["unit"],
-            "frac": kd["frac"], "traffic": kd.get("traffic"), "kernel": dom,
-            "ms_per_launch": kd["ms"], "executed": kd.get("executed"),
-            "executed_frac": kd.get("executed_frac"),
-            "note": "achieved = algorithmic FLOP per launch / measured launch time; peak = the "
-                    "MI355X unit the kernel runs on; executed = MFMA FLOP actually issued"}
+    ms = kd["ms"]
+    alg = (FLOP_KF if dom == "ewald_k_force" else FLOP_SK) * nk * n if dom != "pair" else None
+    ach = round(alg / (ms * 1e-3) / 1e12, 2) if alg else kd["achieved"]
+    roof = {"bound": kd["bound"], "achieved": ach, "peak": kd["peak"], "unit": kd["unit"],
+            "frac": round(ach / kd["peak"], 4), "traffic": kd.get("traffic"), "kernel": dom,
+            "ms_per_launch": ms, "separable_achieved": kd["achieved"],
+            "separable_frac": kd["frac"], "executed": kd.get("executed"),
+            "executed_frac": kd.get("executed_frac"), "traffic_source": TRAFFIC_SOURCE,
+            "note": "achieved = 22 FLOP per (k-vector, atom) x K x N per launch (the per-term "
+                    "convention) / the measured launch time; separable_achieved = FLOP of the "
+                    "separable-lattice algorithm the kernel runs; executed = MFMA FLOP issued (three "
+                    "fp16 products per term, row padding); peak = the fp16 matrix peak it runs on"}
     step_flop = n * (FLOP_PAIR * ppa + FLOP_LJ * lj_ppa + (FLOP_SK + FLOP_KF) * nk + FLOP_INT)
 
     out = {
@@ -375,7 +428,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32/fp64 mixed",
+        "dtype": "f16 MFMA x3 hi/lo split (k-space) + f32 (pair, k tables) + f64 (sums, integration)",
         "data": "synthetic",
         "config": {"workload": "charged LJ + Ewald ions, N=262144, rho=0.3, kmax=16, mixed precision",
                    "n_atoms": n, "kmax": args.kmax, "k_vectors": nk, "dt": args.dt,
@@ -403,8 +456,9 @@ def main():
     if args.replicas:
         out["batched"] = run_batched(args)
     if not args.no_cpu_baseline:
+        # every host core the box gives this GPU (16), as OMP_NUM_THREADS states there
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        threads = max(1, min(threads, 16))
+        threads = max(1, min(threads, os.cpu_count() or 1, 16))
         out["cpu_baseline"] = cpu_baseline(system, threads)
     print(json.dumps(out))
 

@@ -18,15 +18,16 @@ import os
 
 import numpy as np
 
-from .system import (EwaldKSettings, EwaldRSettings, LJSettings, NeighborSettings,  # noqa: F401
-                     System)
+from .system import (ELCSettings, EwaldKSettings, EwaldRSettings, LJListSettings,  # noqa: F401
+                     LJSettings, NeighborSettings, ShortRangeSettings, System, lj_list_args,
+                     short_range_args)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcaviar_amd.so")
 
 CAV_FP64 = 0
 CAV_MIXED = 1
-FORCE_FIELDS = {"lj": 1, "ewald_r": 2, "ewald_k": 4}
+FORCE_FIELDS = {"lj": 1, "ewald_r": 2, "ewald_k": 4, "lj_list": 8, "short_range": 16, "elc": 32}
 _ARRAYS = {"pos": 0, "vel": 1, "acc": 2, "acc_old": 3}
 _ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "call out of order",
            4: "NaN acceleration", 5: "capacity"}
@@ -69,6 +70,10 @@ def load_library():
         "cav_set_neighbor": [P, D, D, I, I, I, I],
         "cav_set_lj": [P, D, dp, dp],
         "cav_set_ewald_r": [P, D, D, dp],
+        "cav_set_lj_list": [P, D, dp, dp, dp, dp, dp, I, dp, dp, I, I, D],
+        "cav_set_short_range": [P, D, D, D, dp, dp],
+        "cav_set_elc": [P, I, I, I, D, D],
+        "cav_energy_elc": [P, dp],
         "cav_set_ewald_k": [P, D, D, I, I, I, I, D, I, D],
         "cav_upload_atoms": [P, I, ip, ip, ip, dp, dp],
         "cav_download": [P, I, dp],
@@ -181,6 +186,13 @@ class Engine:
             eps = np.ascontiguousarray(s.lj.epsilon, dtype=np.float64).reshape(nt * nt)
             sig = np.ascontiguousarray(s.lj.sigma, dtype=np.float64).reshape(nt * nt)
             self._check(self.lib.cav_set_lj(self.h, s.lj.cutoff, _dp(eps), _dp(sig)))
+        if s.lj_list is not None:
+            a = lj_list_args(s.lj_list, nt)
+            self._check(self.lib.cav_set_lj_list(self.h, a[0], *[_dp(x) for x in a[1:6]], a[6],
+                                                 _dp(a[7]), _dp(a[8]), a[9], a[10], a[11]))
+        if s.short_range is not None:
+            a = short_range_args(s.short_range, nt)
+            self._check(self.lib.cav_set_short_range(self.h, a[0], a[1], a[2], _dp(a[3]), _dp(a[4])))
         if s.ewald_r is not None:
             lam = s.ewald_r.lambda_
             lam = None if lam is None else np.ascontiguousarray(lam, dtype=np.float64).reshape(nt * nt)
@@ -192,6 +204,10 @@ class Engine:
                 self.h, e.alpha, e.k_electrostatic, e.kx_max, e.ky_max, e.kz_max,
                 int(e.epsilon_dipole is not None), float(e.epsilon_dipole or 0.0),
                 -1 if e.slab_normal_axis is None else int(e.slab_normal_axis), float(e.slab_lz)))
+        if s.elc is not None:
+            c = s.elc
+            self._check(self.lib.cav_set_elc(self.h, int(c.slab_normal_axis), int(c.kx_max),
+                                             int(c.ky_max), float(c.lz), float(c.k_electrostatic)))
         ids = np.ascontiguousarray(s.ids, dtype=np.int32)
         typ = np.ascontiguousarray(s.types, dtype=np.int32)
         mol = None if s.mol is None else np.ascontiguousarray(s.mol, dtype=np.int32)
@@ -336,9 +352,15 @@ class Engine:
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
         f = np.empty_like(pts)
         p = np.empty(len(pts))
-        self._check(self.lib.cav_point_field(self.h, {"r": 0, "k": 1}[which], len(pts), _dp(pts),
-                                             _dp(f), _dp(p)))
+        self._check(self.lib.cav_point_field(self.h, {"r": 0, "k": 1, "elc": 2}[which], len(pts),
+                                             _dp(pts), _dp(f), _dp(p)))
         return f, p
+
+    def energy_elc(self) -> float:
+        """Electrostatic_ewald_slab_correction::energy of the last force evaluation."""
+        v = ctypes.c_double()
+        self._check(self.lib.cav_energy_elc(self.h, ctypes.byref(v)))
+        return v.value
 
     def set_parameters(self, epsilon=None, sigma=None, charge=None, mass=None):
         """Change force-field parameter values after setup (same structure);

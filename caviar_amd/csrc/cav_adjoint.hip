@@ -340,25 +340,57 @@ __global__ void kb_fill_weights(long nown, int mode, int s, const int *type_o, c
 
 using namespace cav;
 
-extern "C" int cav_adjoint(cav_ctx *c, const double *w_host, double *g_eps, double *g_sigma,
-                           double *g_charge)
+// Device buffers of one adjoint call, in doubles: residual weights, per-block
+// partials, reduced outputs, and for Ewald_k the per-atom weights, the species
+// and residual-weighted structure factors and the per-atom k-space field.
+struct AdjWork {
+  double *w = nullptr, *part = nullptr, *out = nullptr, *u = nullptr, *ss = nullptr,
+         *sw = nullptr, *f4 = nullptr;
+};
+
+struct AdjSizes {
+  size_t w = 0, part = 0, out = 0, u = 0, ss = 0, sw = 0, f4 = 0;
+};
+
+static AdjSizes adjoint_sizes(const cav_ctx *c)
 {
-  if (!c || !w_host) return CAV_ERR_ARG;
+  const int n = c->n, nt = c->nt, nv = 2 * nt * nt + nt;
+  const int nblk = grid_for((long)n * kLPAa);
+  AdjSizes z;
+  z.w = 3 * (size_t)n + 1;
+  z.part = (size_t)nblk * nv + 4 * (size_t)grid_for(n) * nt +
+           (size_t)grid_for((long)c->ek.R * c->ek.nZ + 1) * nt + 64;
+  z.out = nv + 2 * (size_t)nt;
+  if (c->ek_on) {
+    const size_t plane = (size_t)c->ek.Rpad * 2 * c->ek.nZ;
+    z.u = (size_t)n + 1;
+    z.ss = plane * nt;
+    z.sw = plane * 3;
+    z.f4 = 4 * (size_t)(n + 1);
+  }
+  return z;
+}
+
+static int adjoint_checks(cav_ctx *c)
+{
   if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run before cav_adjoint");
   if (c->lj_on && c->lj_cellwalk)
     return fail(c, CAV_ERR_STATE, "the adjoint needs the LJ cutoff within the neighbour-list cutoff");
   if (c->dipole) return fail(c, CAV_ERR_STATE, "the adjoint does not cover the epsilon_dipole term");
+  if (c->ljl_on || c->sr_on)
+    return fail(c, CAV_ERR_STATE, "the adjoint covers lj_cell_list, ewald_r and ewald_k only");
   if (c->nt > 4) return fail(c, CAV_ERR_ARG, "the adjoint supports up to 4 atom types");
-  (void)hipSetDevice(c->device);
+  return CAV_OK;
+}
+
+// The adjoint on a given workspace (every buffer written before it is read,
+// so its previous contents do not matter).  hout has nv + 2 nt doubles.
+static int adjoint_run(cav_ctx *c, const double *w_host, const AdjWork &k, double *hout)
+{
   const int n = c->n, nt = c->nt, nt2 = nt * nt, nv = 2 * nt2 + nt;
-  double *dw = nullptr, *dpart = nullptr, *dout = nullptr;
   const int nblk = grid_for((long)n * kLPAa);
-  CAV_HIP(hipMalloc(&dw, sizeof(double) * (3 * (size_t)n + 1)));
-  CAV_HIP(hipMalloc(&dpart, sizeof(double) * ((size_t)nblk * nv + 4 * (size_t)grid_for(n) * nt +
-                                              (size_t)grid_for((long)c->ek.R * c->ek.nZ + 1) * nt + 64)));
-  CAV_HIP(hipMalloc(&dout, sizeof(double) * (nv + 2 * nt)));
-  CAV_HIP(hipMemcpyAsync(dw, w_host, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-  std::vector<double> hout(nv + 2 * (size_t)nt, 0.0);
+  CAV_HIP(hipMemcpyAsync(k.w, w_host, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  for (int v = 0; v < nv + 2 * nt; ++v) hout[v] = 0.0;
   if (c->lj_on || c->er_on) {
     AdjArgs a;
     a.n = n;
@@ -369,7 +401,7 @@ extern "C" int cav_adjoint(cav_ctx *c, const double *w_host, double *g_eps, doub
     a.mol = c->mol;
     a.nl = c->nl;
     a.nn = c->nn;
-    a.w = dw;
+    a.w = k.w;
     a.eps = c->d_raw_eps;
     a.sig = c->d_raw_sig;
     a.lj_rc = c->lj_rc;
@@ -383,62 +415,93 @@ extern "C" int cav_adjoint(cav_ctx *c, const double *w_host, double *g_eps, doub
     a.two_a_rpi = 2 * c->er_alpha * CAV_PIS_INV;
     a.lambda = c->d_lambda;
     a.charge = c->d_charge;
-    a.part = dpart;
+    a.part = k.part;
     a.rep = nullptr;
     const size_t sh = sizeof(double) * (kBlock / kLPAa) * nv;
     launch_adjoint_pairs(c->nt, dim3(nblk), sh, c->stream, a);
-    k_reduce_cols<<<1, 64, 0, c->stream>>>(nblk, nv, dpart, dout);
+    k_reduce_cols<<<1, 64, 0, c->stream>>>(nblk, nv, k.part, k.out);
     CAV_HIP(hipGetLastError());
-    CAV_HIP(hipMemcpyAsync(hout.data(), dout, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
+    CAV_HIP(hipMemcpyAsync(hout, k.out, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
     CAV_HIP(hipStreamSynchronize(c->stream));
   }
   if (c->ek_on) {
     const EwaldKPlan &p = c->ek;
     const size_t plane = (size_t)p.Rpad * 2 * p.nZ;
-    double *u = nullptr, *ss = nullptr, *sw = nullptr, *f4 = nullptr;
-    CAV_HIP(hipMalloc(&u, sizeof(double) * (size_t)(n + 1)));
-    CAV_HIP(hipMalloc(&ss, sizeof(double) * plane * nt));
-    CAV_HIP(hipMalloc(&sw, sizeof(double) * plane * 3));
-    CAV_HIP(hipMalloc(&f4, sizeof(double) * 4 * (size_t)(n + 1)));
     // per-atom k-space field G_i (uses the operand of the last force evaluation)
-    CAV_TRY(ewald_k_field(c, f4));
+    CAV_TRY(ewald_k_field(c, k.f4));
     for (int s = 0; s < nt; ++s) {
-      k_fill_weights<<<grid_for(n), kBlock, 0, c->stream>>>(n, -1, s, c->type, c->q, dw, u);
-      CAV_TRY(ewald_k_weighted_sk(c, u, ss + s * plane));
+      k_fill_weights<<<grid_for(n), kBlock, 0, c->stream>>>(n, -1, s, c->type, c->q, k.w, k.u);
+      CAV_TRY(ewald_k_weighted_sk(c, k.u, k.ss + s * plane));
     }
     for (int d = 0; d < 3; ++d) {
-      k_fill_weights<<<grid_for(n), kBlock, 0, c->stream>>>(n, d, 0, c->type, c->q, dw, u);
-      CAV_TRY(ewald_k_weighted_sk(c, u, sw + d * plane));
+      k_fill_weights<<<grid_for(n), kBlock, 0, c->stream>>>(n, d, 0, c->type, c->q, k.w, k.u);
+      CAV_TRY(ewald_k_weighted_sk(c, k.u, k.sw + d * plane));
     }
     const int nb = grid_for((long)p.R * p.nZ);
     k_adjoint_kspace<<<nb, kBlock, 0, c->stream>>>(p.R, p.Rpad, p.nY, p.nZ, p.ky, p.kz, nt,
                                                    CAV_2PI / p.lx, CAV_2PI / p.ly, CAV_2PI / p.lz,
-                                                   c->d_W, ss, sw, dpart);
-    k_reduce_cols<<<1, 64, 0, c->stream>>>(nb, nt, dpart, dout + nv);
+                                                   c->d_W, k.ss, k.sw, k.part);
+    k_reduce_cols<<<1, 64, 0, c->stream>>>(nb, nt, k.part, k.out + nv);
     const int nb2 = grid_for(n);
-    double *dpart2 = dpart + (size_t)nb * nt + 16;
-    k_adjoint_field<<<nb2, kBlock, 0, c->stream>>>(n, nt, dw, f4, c->type, dpart2);
-    k_reduce_cols<<<1, 64, 0, c->stream>>>(nb2, nt, dpart2, dout + nv + nt);
+    double *part2 = k.part + (size_t)nb * nt + 16;
+    k_adjoint_field<<<nb2, kBlock, 0, c->stream>>>(n, nt, k.w, k.f4, c->type, part2);
+    k_reduce_cols<<<1, 64, 0, c->stream>>>(nb2, nt, part2, k.out + nv + nt);
     CAV_HIP(hipGetLastError());
-    CAV_HIP(hipMemcpyAsync(hout.data() + nv, dout + nv, sizeof(double) * 2 * nt,
+    CAV_HIP(hipMemcpyAsync(hout + nv, k.out + nv, sizeof(double) * 2 * nt,
                            hipMemcpyDeviceToHost, c->stream));
     CAV_HIP(hipStreamSynchronize(c->stream));
     const double C = CAV_4PI * c->ek_ke * p.lxyz_inv;
     for (int s = 0; s < nt; ++s) hout[2 * nt2 + s] += C * hout[nv + s] + hout[nv + nt + s];
-    (void)hipFree(u);
-    (void)hipFree(ss);
-    (void)hipFree(sw);
-    (void)hipFree(f4);
   }
-  (void)hipFree(dw);
-  (void)hipFree(dpart);
-  (void)hipFree(dout);
+  return CAV_OK;
+}
+
+static void adjoint_outputs(const cav_ctx *c, const double *hout, double *g_eps, double *g_sigma,
+                            double *g_charge)
+{
+  const int nt = c->nt, nt2 = nt * nt;
   if (g_eps)
     for (int p = 0; p < nt2; ++p) g_eps[p] = hout[p];
   if (g_sigma)
     for (int p = 0; p < nt2; ++p) g_sigma[p] = hout[nt2 + p];
   if (g_charge)
     for (int s = 0; s < nt; ++s) g_charge[s] = hout[2 * nt2 + s];
+}
+
+extern "C" int cav_adjoint(cav_ctx *c, const double *w_host, double *g_eps, double *g_sigma,
+                           double *g_charge)
+{
+  if (!c || !w_host) return CAV_ERR_ARG;
+  CAV_TRY(adjoint_checks(c));
+  (void)hipSetDevice(c->device);
+  // parameter values changed since the last force call: charges (owned and
+  // ghosts), their packed copies and the k-space weights follow them
+  const bool dirty = c->params_dirty;
+  CAV_TRY(refresh_params(c));
+  if (dirty && (c->lj_on || c->er_on)) CAV_TRY(launch_pack(c));
+  // the workspace of this call, allocated here and released on every exit
+  struct Owned {
+    AdjWork k;
+    ~Owned()
+    {
+      double *ps[] = {k.w, k.part, k.out, k.u, k.ss, k.sw, k.f4};
+      for (double *q : ps)
+        if (q) (void)hipFree(q);
+    }
+  } own;
+  const AdjSizes z = adjoint_sizes(c);
+  CAV_HIP(hipMalloc(&own.k.w, sizeof(double) * z.w));
+  CAV_HIP(hipMalloc(&own.k.part, sizeof(double) * z.part));
+  CAV_HIP(hipMalloc(&own.k.out, sizeof(double) * z.out));
+  if (c->ek_on) {
+    CAV_HIP(hipMalloc(&own.k.u, sizeof(double) * z.u));
+    CAV_HIP(hipMalloc(&own.k.ss, sizeof(double) * z.ss));
+    CAV_HIP(hipMalloc(&own.k.sw, sizeof(double) * z.sw));
+    CAV_HIP(hipMalloc(&own.k.f4, sizeof(double) * z.f4));
+  }
+  std::vector<double> hout(z.out, 0.0);
+  CAV_TRY(adjoint_run(c, w_host, own.k, hout.data()));
+  adjoint_outputs(c, hout.data(), g_eps, g_sigma, g_charge);
   return CAV_OK;
 }
 

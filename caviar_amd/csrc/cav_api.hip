@@ -147,6 +147,17 @@ __global__ void k_kick_join(int n, double hdt, double *vx, double *vy, double *v
   vz[i] = xadd(vz[i], xmul(hdt, xadd(a[2], aoz[i])));
 }
 
+// a += e per component, one rounded addition (the concurrent Ewald_k's join).
+__global__ void k_add3(int n, double *ax, double *ay, double *az, const double *ex,
+                       const double *ey, const double *ez)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  ax[i] = xadd(ax[i], ex[i]);
+  ay[i] = xadd(ay[i], ey[i]);
+  az[i] = xadd(az[i], ez[i]);
+}
+
 __global__ void k_begin_step(int *flags)
 {
   flags[F_WRAPPED] = 0;
@@ -171,6 +182,16 @@ __global__ void k_charge_of_type(int n, const int *type, const double *charge, d
 {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) q[i] = charge[type[i]];
+}
+
+// The same over the ghost slots n .. n + flags[F_NGHOST) of the last exchange
+// (their types are copies of their sources'), bounded by the slot capacity.
+__global__ void k_ghost_charge_of_type(int n, int gcap, const int *flags, const int *type,
+                                       const double *charge, double *q)
+{
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  const int ng = min(flags[F_NGHOST], gcap);
+  if (g < ng) q[n + g] = charge[type[n + g]];
 }
 
 }  // namespace cav
@@ -198,6 +219,7 @@ static void free_all(cav_ctx *c)
                   c->mol,     c->q,      c->gsrc,      c->flags,    c->blk,      c->bin_of,
                   c->bin_cnt, c->bin_start, c->bin_items, c->d_st_off, c->d_st,  c->scan_tmp,
                   c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
+                  c->d_ljl,   c->d_sr_lambda, c->d_elc_k, c->d_elc_chi, c->d_elc_dip,
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
                   c->d_W,     c->kacc,      c->order,    c->ord_blk,  c->xq,
                   c->tid,     c->d_raw_eps, c->d_raw_sig, c->ztab, c->bmat_h, c->cpos, c->sk_sum,
@@ -251,6 +273,33 @@ static int upload_params(cav_ctx *c)
   if (c->lambda_set) lam = c->lambda;
   CAV_TRY(dmalloc(c, &c->d_lambda, lam.size()));
   CAV_HIP(hipMemcpyAsync(c->d_lambda, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice, c->stream));
+  // force_field lj: coef(r) = -c12 r^-14 + c6 r^-8 + cshift with the shift at
+  // the pair's own cutoff c (lj.cpp:421-434); rc2 = c, or -1 where eps = 0
+  // (lj.cpp:416-418 skips those pairs)
+  std::vector<double> ljl(4 * (size_t)std::max(nt2, 1), 0.0);
+  if (c->ljl_on) {
+    for (int p = 0; p < nt2; ++p) {
+      const double e = c->ljl_eps[p], s = c->ljl_sig[p];
+      const double cc = c->ljl_cut_on ? c->ljl_cut[p] * c->ljl_cut[p] : c->ljl_rc * c->ljl_rc;
+      if (e == 0) {
+        ljl[3 * nt2 + p] = -1.0;
+        continue;
+      }
+      const double s2 = s * s, s6 = s2 * s2 * s2, s12 = s6 * s6;
+      const double rcsi = 1 / cc;
+      const double rhoc2 = s * s * rcsi, rhoc6 = rhoc2 * rhoc2 * rhoc2, rhoc12 = rhoc6 * rhoc6;
+      ljl[p] = 48 * e * s12;
+      ljl[nt2 + p] = 24 * e * s6;
+      ljl[2 * nt2 + p] = 4 * e * (12 * rhoc12 * rcsi - 6 * rhoc6 * rcsi);
+      ljl[3 * nt2 + p] = cc;
+    }
+  }
+  CAV_TRY(dmalloc(c, &c->d_ljl, ljl.size()));
+  CAV_HIP(hipMemcpyAsync(c->d_ljl, ljl.data(), sizeof(double) * ljl.size(), hipMemcpyHostToDevice, c->stream));
+  std::vector<double> srl(std::max(nt2, 1), 1.0);
+  if (c->sr_lambda_set) srl = c->sr_lambda;
+  CAV_TRY(dmalloc(c, &c->d_sr_lambda, srl.size()));
+  CAV_HIP(hipMemcpyAsync(c->d_sr_lambda, srl.data(), sizeof(double) * srl.size(), hipMemcpyHostToDevice, c->stream));
   return CAV_OK;
 }
 
@@ -344,13 +393,17 @@ static int box_dev(cav_ctx *c, const double **lo, const double **hi, const doubl
 }
 
 // New parameter values (same structure) set after setup: upload them, refresh
-// the per-atom charges and the k-space weights.  Ghost charges follow from the
-// next ghost exchange.
-static int refresh_params(cav_ctx *c)
+// the per-atom charges of owned atoms and of the current ghosts (a force call
+// without a new exchange reads the ghosts' charges: the reference reads
+// charge[type_j] for ghosts too), and the k-space weights.
+int cav::refresh_params(cav_ctx *c)
 {
   if (!c->params_dirty) return CAV_OK;
   CAV_TRY(upload_params(c));
   k_charge_of_type<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->type, c->d_charge, c->q);
+  if (c->gcap > 0)
+    k_ghost_charge_of_type<<<grid_for(c->gcap), kBlock, 0, c->stream>>>(c->n, c->gcap, c->flags,
+                                                                          c->type, c->d_charge, c->q);
   CAV_HIP(hipGetLastError());
   if (c->ek_on) {
     CAV_TRY(ewald_k_plan(c));
@@ -384,7 +437,10 @@ static int ewald_k_fork(cav_ctx *c)
   } side(c);
   k_zero3<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->ax, c->ay, c->az);
   CAV_HIP(hipGetLastError());
-  CAV_TRY(launch_ewald_k(c));
+  c->defer_dipole = true;
+  const int st = launch_ewald_k(c);
+  c->defer_dipole = false;
+  CAV_TRY(st);
   CAV_HIP(hipEventRecord(c->ev_join, c->stream));
   return CAV_OK;
 }
@@ -424,7 +480,9 @@ static int forces(cav_ctx *c, bool kspace)
   else
     CAV_TRY(launch_zero_acc(c));
   if (c->lj_on && c->lj_cellwalk) CAV_TRY(launch_lj_cellwalk(c));
+  if (c->ljl_on || c->sr_on) CAV_TRY(launch_pairs_list(c, true, true, true, !pair));
   if (c->ek_on && kspace) CAV_TRY(launch_ewald_k(c));
+  if (c->elc_on && kspace) CAV_TRY(launch_elc(c));
   return CAV_OK;
 }
 
@@ -632,6 +690,110 @@ int cav_set_lj(cav_ctx *c, double rc, const double *eps, const double *sigma)
   return CAV_OK;
 }
 
+int cav_set_lj_list(cav_ctx *c, double cutoff, const double *eps, const double *sig,
+                    const double *eps_atom, const double *sig_atom, const double *cutoff_list,
+                    int wca, const double *lambda_s, const double *lambda_e,
+                    int ignore_intra_molecule, int jump_fix, double jump_tol)
+{
+  if (!c || c->nt <= 0) return fail(c, CAV_ERR_ARG, "set types before lj");
+  if (cutoff < 0) return fail(c, CAV_ERR_ARG, "Force field cutoff have to non-negative.");
+  const int nt = c->nt, nt2 = nt * nt;
+  const bool by_array = eps || sig, by_atom = eps_atom || sig_atom;
+  if (by_array && by_atom) return fail(c, CAV_ERR_ARG, "lj: (input_by_atom && input_by_array)");
+  if (!by_array && !by_atom) return fail(c, CAV_ERR_ARG, "lj: no epsilon/sigma given");
+  for (int p = 0; p < nt2; ++p) {
+    if ((eps && eps[p] < 0) || (sig && sig[p] < 0) || (cutoff_list && cutoff_list[p] < 0) ||
+        (lambda_s && lambda_s[p] < 0) || (lambda_e && lambda_e[p] < 0))
+      return fail(c, CAV_ERR_ARG, "lj: epsilon, sigma, cutoff_list and lambdas have to be non-negative.");
+  }
+  // Lj::verify_settings (lj.cpp:162-339): type matrices by array, or by atom
+  // with Lorentz-Berthelot off the diagonal (lj.cpp:207-222); WCA cutoffs from
+  // the unscaled sigma (lj.cpp:278-294); then lambda_s scales sigma and
+  // lambda_e epsilon (lj.cpp:310-338)
+  std::vector<double> e(nt2, 0.0), s(nt2, 0.0), cut(nt2, 0.0);
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j) {
+      const int p = i * nt + j;
+      if (by_atom) {
+        const double ea = eps_atom ? eps_atom[i] : 0, eb = eps_atom ? eps_atom[j] : 0;
+        const double sa = sig_atom ? sig_atom[i] : 0, sb = sig_atom ? sig_atom[j] : 0;
+        s[p] = i == j ? sa : 0.5 * (sa + sb);
+        e[p] = i == j ? ea : std::sqrt(ea * eb);
+      } else {
+        e[p] = eps ? eps[p] : 0;
+        s[p] = sig ? sig[p] : 0;
+      }
+    }
+  if (wca) {
+    const double cut_coef = std::pow(2.0, 1.0 / 6.0);
+    for (int p = 0; p < nt2; ++p) cut[p] = cut_coef * s[p];
+  } else if (cutoff_list) {
+    cut.assign(cutoff_list, cutoff_list + nt2);
+  }
+  for (int p = 0; p < nt2; ++p) {
+    if (lambda_s) s[p] *= lambda_s[p];
+    if (lambda_e) e[p] *= lambda_e[p];
+  }
+  const bool cut_on = wca || cutoff_list;
+  const bool keep = c->setup_done && c->ljl_on;
+  c->ljl_on = true;
+  c->ljl_rc = cutoff;
+  c->ljl_eps = e;
+  c->ljl_sig = s;
+  c->ljl_cut = cut;
+  c->ljl_cut_on = cut_on;
+  c->ljl_ignore_intra = ignore_intra_molecule != 0;
+  c->ljl_jump = jump_fix != 0;
+  c->ljl_jump_tol = jump_tol;
+  if (keep)
+    c->params_dirty = true;
+  else
+    c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_set_short_range(cav_ctx *c, double cutoff, double ke, double beta, const double ext[3],
+                        const double *lambda)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (cutoff < 0) return fail(c, CAV_ERR_ARG, "Force field cutoff have to non-negative.");
+  if (ke < 0) return fail(c, CAV_ERR_ARG, "k_electrostatic has to be non-negative.");
+  if (lambda && c->nt <= 0) return fail(c, CAV_ERR_ARG, "set types before lambda");
+  if (lambda)
+    for (int p = 0; p < c->nt * c->nt; ++p)
+      if (lambda[p] < 0) return fail(c, CAV_ERR_ARG, "lambda has to be non-negative.");
+  const bool keep = c->setup_done && c->sr_on && (lambda != nullptr) == c->sr_lambda_set;
+  c->sr_on = true;
+  c->sr_rc = cutoff;
+  c->sr_ke = ke;
+  c->sr_beta = beta;
+  for (int d = 0; d < 3; ++d) c->sr_ext[d] = ext ? ext[d] : 0.0;
+  c->sr_lambda_set = lambda != nullptr;
+  if (lambda) c->sr_lambda.assign(lambda, lambda + c->nt * c->nt);
+  if (keep)
+    c->params_dirty = true;
+  else
+    c->setup_done = false;
+  return CAV_OK;
+}
+
+int cav_set_elc(cav_ctx *c, int axis, int kx, int ky, double lz, double ke)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (axis < 0 || axis > 2) return fail(c, CAV_ERR_ARG, "expected 'x', 'y' or 'z'.");
+  if (kx < 0 || ky < 0) return fail(c, CAV_ERR_ARG, "'k_max' have to non-negative.");
+  if (lz < 0) return fail(c, CAV_ERR_ARG, "'lz' has to be non-negative.");
+  if (ke < 0) return fail(c, CAV_ERR_ARG, "k_electrostatic has to be non-negative.");
+  c->elc_on = true;
+  c->elc_axis = axis;
+  c->elc_kx = kx;
+  c->elc_ky = ky;
+  c->elc_lz = lz;
+  c->elc_ke = ke;
+  c->setup_done = false;
+  return CAV_OK;
+}
+
 int cav_set_ewald_r(cav_ctx *c, double alpha, double ke, const double *lambda)
 {
   if (!c) return CAV_ERR_ARG;
@@ -802,7 +964,8 @@ int cav_setup(cav_ctx *c)
   if (!c) return CAV_ERR_ARG;
   if (!c->box_set) return fail(c, CAV_ERR_STATE, "set the box before setup");
   if (c->nt <= 0) return fail(c, CAV_ERR_STATE, "set types before setup");
-  if ((c->lj_on || c->er_on) && !c->nl_on) return fail(c, CAV_ERR_STATE, "lj_cell_list / ewald_r need a neighborlist");
+  if ((c->lj_on || c->er_on || c->ljl_on || c->sr_on) && !c->nl_on)
+    return fail(c, CAV_ERR_STATE, "lj_cell_list / lj / ewald_r / short_range need a neighborlist");
   if (c->lj_on && !c->build_cells) return fail(c, CAV_ERR_STATE, "lj_cell_list needs build_cell_list");
   (void)hipSetDevice(c->device);
   // box on the device (lo, hi, L)
@@ -847,6 +1010,7 @@ int cav_setup(cav_ctx *c)
       CAV_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     }
   }
+  if (c->elc_on) CAV_TRY(elc_plan(c));
   // initial list capacity from the mean density, refined by the first build
   if (c->nl_on) {
     const double vol = c->L[0] * c->L[1] * c->L[2];
@@ -858,6 +1022,7 @@ int cav_setup(cav_ctx *c)
   }
   k_set_flag<<<1, 1, 0, c->stream>>>(c->flags, F_NGHOST_OLD, -1);
   for (int attempt = 0; attempt < 4; ++attempt) {
+    if (attempt) CAV_TRY(elc_rewind(c));  // the retried evaluation is the same one
     k_begin_step<<<1, 1, 0, c->stream>>>(c->flags);
     CAV_TRY(wrap_and_ghosts(c));
     CAV_TRY(neighbor_phase(c, true));
@@ -898,7 +1063,18 @@ int cav_step(cav_ctx *c, double dt, int nsteps)
       CAV_TRY(launch_ghosts(c));
       CAV_TRY(neighbor_phase(c, false));
       CAV_TRY(forces(c, !side));
-      if (side) {
+      if (side && (c->dipole || c->elc_on)) {
+        // a = ((a_sr + a_k) + a_dip) + a_elc, as in order: add, dipole, slab
+        // correction, then the kick
+        CAV_TRY(ewald_k_join(c));
+        k_add3<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->ax, c->ay, c->az, c->ekx,
+                                                         c->eky, c->ekz);
+        if (c->dipole) CAV_TRY(launch_dipole(c));
+        if (c->elc_on) CAV_TRY(launch_elc(c));
+        k_kick<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, 0.5 * dt, c->vx, c->vy, c->vz,
+                                                         c->ax, c->ay, c->az, c->aox, c->aoy,
+                                                         c->aoz);
+      } else if (side) {
         CAV_TRY(ewald_k_join(c));
         k_kick_join<<<grid_for(c->n), kBlock, 0, c->stream>>>(
             c->n, 0.5 * dt, c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->ekx, c->eky, c->ekz,
@@ -913,12 +1089,14 @@ int cav_step(cav_ctx *c, double dt, int nsteps)
       harvest_timers(c);
       if (c->h_flags[F_NAN]) {
         CAV_TRY(snapshot(c, false));
+        CAV_TRY(elc_rewind(c));
         CAV_HIP(hipStreamSynchronize(c->stream));
         return fail(c, CAV_ERR_NAN, "NaN acceleration before the drift");
       }
       CAV_TRY(check_capacity(c, &redo));
       if (redo) {
         CAV_TRY(snapshot(c, false));
+        CAV_TRY(elc_rewind(c));
         k_set_flag<<<1, 1, 0, c->stream>>>(c->flags, F_FORCE, 1);
       }
     }
@@ -942,6 +1120,7 @@ int cav_recalculate(cav_ctx *c, int force_rebuild)
   CAV_TRY(check_capacity(c, &redo));
   if (redo) {
     CAV_TRY(neighbor_phase(c, true));
+    CAV_TRY(elc_rewind(c));
     CAV_TRY(forces(c));
     CAV_TRY(sync_flags(c));
   }
@@ -994,14 +1173,17 @@ int cav_forces(cav_ctx *c, int mask)
 {
   if (!c) return CAV_ERR_ARG;
   if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
-  if (mask & ~(CAV_FF_LJ | CAV_FF_EWALD_R | CAV_FF_EWALD_K))
+  if (mask & ~(CAV_FF_LJ | CAV_FF_EWALD_R | CAV_FF_EWALD_K | CAV_FF_LJ_LIST | CAV_FF_SHORT_RANGE | CAV_FF_ELC))
     return fail(c, CAV_ERR_ARG, "unknown force-field bit in mask");
   (void)hipSetDevice(c->device);
   CAV_TRY(refresh_params(c));
   const bool lj = (mask & CAV_FF_LJ) && c->lj_on, er = (mask & CAV_FF_EWALD_R) && c->er_on;
+  const bool ljl = (mask & CAV_FF_LJ_LIST) && c->ljl_on, sr = (mask & CAV_FF_SHORT_RANGE) && c->sr_on;
   if ((lj && !c->lj_cellwalk) || er) CAV_TRY(launch_pairs(c, lj, er, true));
   if (lj && c->lj_cellwalk) CAV_TRY(launch_lj_cellwalk(c));
+  if (ljl || sr) CAV_TRY(launch_pairs_list(c, ljl, sr, true, true));
   if ((mask & CAV_FF_EWALD_K) && c->ek_on) CAV_TRY(launch_ewald_k(c));
+  if ((mask & CAV_FF_ELC) && c->elc_on) CAV_TRY(launch_elc(c));
   CAV_HIP(hipStreamSynchronize(c->stream));
   return CAV_OK;
 }
@@ -1014,6 +1196,7 @@ int cav_autotune(cav_ctx *c)
   (void)hipSetDevice(c->device);
   CAV_TRY(ewald_k_autotune(c));
   // the tuning launches added k-space terms: recompute the forces of the current state
+  CAV_TRY(elc_rewind(c));
   CAV_TRY(forces(c));
   CAV_TRY(sync_flags(c));
   return CAV_OK;
@@ -1049,6 +1232,7 @@ int cav_set_tiles(cav_ctx *c, int s_rows, int s_chunk, int f_tiles, int f_waves)
   c->tune_chunk = s_chunk;
   c->tuned = true;
   CAV_TRY(ewald_k_alloc(c));
+  CAV_TRY(elc_rewind(c));
   CAV_TRY(forces(c));
   CAV_TRY(sync_flags(c));
   return CAV_OK;
@@ -1214,6 +1398,7 @@ int cav_energy(cav_ctx *c, double *e_lj, double *e_r, double *e_k)
   if (!c) return CAV_ERR_ARG;
   if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
   (void)hipSetDevice(c->device);
+  CAV_TRY(refresh_params(c));
   if (e_k) {
     if (!c->ek_on) return fail(c, CAV_ERR_STATE, "no Ewald_k");
     CAV_TRY(launch_energy_k(c, e_k));
@@ -1235,11 +1420,21 @@ int cav_energy(cav_ctx *c, double *e_lj, double *e_r, double *e_k)
   return CAV_OK;
 }
 
+int cav_energy_elc(cav_ctx *c, double *e)
+{
+  if (!c || !e) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  if (!c->elc_on) return fail(c, CAV_ERR_STATE, "no electrostatic_ewald_slab_correction");
+  (void)hipSetDevice(c->device);
+  return energy_elc(c, e);
+}
+
 int cav_point_field(cav_ctx *c, int which, int np, const double *pts, double *field, double *pot)
 {
   if (!c || np < 0 || (np && !pts)) return CAV_ERR_ARG;
   if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
   (void)hipSetDevice(c->device);
+  CAV_TRY(refresh_params(c));
   if (which == 0) {
     if (!c->er_on || !c->build_cells) return fail(c, CAV_ERR_STATE, "Ewald_r point queries need build_cell_list");
     CAV_TRY(launch_bins(c, false));
@@ -1249,7 +1444,11 @@ int cav_point_field(cav_ctx *c, int which, int np, const double *pts, double *fi
     if (!c->ek_on) return fail(c, CAV_ERR_STATE, "no Ewald_k");
     return point_field_k(c, np, pts, field, pot);
   }
-  return fail(c, CAV_ERR_ARG, "which must be 0 (ewald_r) or 1 (ewald_k)");
+  if (which == 2) {
+    if (!c->elc_on) return fail(c, CAV_ERR_STATE, "no electrostatic_ewald_slab_correction");
+    return point_field_elc(c, np, pts, field, pot);
+  }
+  return fail(c, CAV_ERR_ARG, "which must be 0 (ewald_r), 1 (ewald_k) or 2 (slab correction)");
 }
 
 int cav_enable_timing(cav_ctx *c, int on)

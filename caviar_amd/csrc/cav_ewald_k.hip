@@ -1720,14 +1720,20 @@ int launch_ewald_k(cav_ctx *c)
     CAV_TRY(sk_pipeline<float>(c));
     CAV_TRY(kforce<float>(c, c->n, c->tab, nullptr));
   }
-  if (c->dipole) {
-    const int nb = grid_for(c->n);
-    k_dipole_part<<<nb, kBlock, 0, c->stream>>>(c->n, c->q, c->x, c->y, c->z, c->part);
-    k_dipole_apply<<<nb, kBlock, 0, c->stream>>>(c->n, nb, c->part, c->ek.dip_coef,
-                                                 (double)c->ek.nk_ref, c->q, c->d_minv, c->type,
-                                                 c->ax, c->ay, c->az, c->scal + 8);
-    CAV_HIP(hipGetLastError());
-  }
+  if (c->dipole && !c->defer_dipole) CAV_TRY(launch_dipole(c));
+  return CAV_OK;
+}
+
+// The epsilon_dipole term, added into the accelerations after the k-space sum
+// (the reference adds it inside the same per-atom loop, after the k terms).
+int launch_dipole(cav_ctx *c)
+{
+  const int nb = grid_for(c->n);
+  k_dipole_part<<<nb, kBlock, 0, c->stream>>>(c->n, c->q, c->x, c->y, c->z, c->part);
+  k_dipole_apply<<<nb, kBlock, 0, c->stream>>>(c->n, nb, c->part, c->ek.dip_coef,
+                                               (double)c->ek.nk_ref, c->q, c->d_minv, c->type,
+                                               c->ax, c->ay, c->az, c->scal + 8);
+  CAV_HIP(hipGetLastError());
   return CAV_OK;
 }
 

@@ -126,6 +126,23 @@ struct cav_ctx {
   double er_alpha = 1, er_ke = 1;
   bool lambda_set = false;
   std::vector<double> lambda;
+  // force_field lj (Verlet list): type matrices resolved as Lj::verify_settings
+  // resolves them, per-pair cutoffs (cutoff_list or wca), flags
+  bool ljl_on = false, ljl_cut_on = false;
+  double ljl_rc = 0, ljl_jump_tol = 0;
+  int ljl_ignore_intra = 0, ljl_jump = 0;
+  std::vector<double> ljl_eps, ljl_sig, ljl_cut;
+  // Electrostatic_short_range
+  bool sr_on = false, sr_lambda_set = false;
+  double sr_rc = 0, sr_ke = 1, sr_beta = 1, sr_ext[3] = {0, 0, 0};
+  std::vector<double> sr_lambda;
+  // Electrostatic_ewald_slab_correction (ELC)
+  bool elc_on = false;
+  int elc_axis = 2, elc_kx = 1, elc_ky = 1;
+  double elc_lz = 1, elc_ke = 1, elc_sum_e_coef = 0, elc_dip_coef = 0;
+  double *d_elc_k = nullptr;    // k grids and coefficients
+  double *d_elc_chi = nullptr;  // chi sums of the last evaluation
+  double *d_elc_dip = nullptr;  // previous dipole sum, dipole field; saved copy
   bool ek_on = false;
   double ek_alpha = 0, ek_ke = 1, eps_dipole = 1, slab_lz = 1;
   int kx = 1, ky = 1, kz = 1, dipole = 0, slab_axis = -1;
@@ -173,6 +190,8 @@ struct cav_ctx {
   double *d_minv = nullptr, *d_charge = nullptr, *d_eps = nullptr, *d_sig = nullptr,
          *d_lambda = nullptr;
   double *d_raw_eps = nullptr, *d_raw_sig = nullptr;  // epsilon/sigma matrices as given
+  double *d_ljl = nullptr;        // force_field lj: [c12 | c6 | cshift | rc2] per type pair
+  double *d_sr_lambda = nullptr;  // Electrostatic_short_range lambda per type pair
   bool params_dirty = false;  // parameter values changed after setup
 
   // reductions: per-block partials and results
@@ -184,6 +203,9 @@ struct cav_ctx {
   hipStream_t kstream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double *ekx = nullptr, *eky = nullptr, *ekz = nullptr, *part_k = nullptr;
+  // the side stream's Ewald_k leaves the dipole term to the main stream, which
+  // adds it after the k-space sum (the in-order rounding: (a_sr + a_k) + a_dip)
+  bool defer_dipole = false;
   double *scal = nullptr;  // [0] virial, [1] e_lj, [2] e_r, [3] e_k, [4..] scratch
   double *h_scal = nullptr;
 
@@ -314,6 +336,7 @@ int fail(cav_ctx *c, int code, const std::string &msg);
 int hip_check(cav_ctx *c, hipError_t e, const char *what);
 void timer_begin(cav_ctx *c, Timer &t);
 int tuning_phase(cav_ctx *c);
+int refresh_params(cav_ctx *c);  // parameter values set after setup -> device
 void timer_end(cav_ctx *c, Timer &t);
 
 // launchers (defined in the .hip files)
@@ -324,9 +347,17 @@ int launch_list(cav_ctx *c, bool conditional);
 int launch_save_old(cav_ctx *c, bool conditional);
 bool cell_staged(const cav_ctx *c);
 int launch_pairs(cav_ctx *c, bool lj, bool er, bool accumulate);
+int launch_pack(cav_ctx *c);
+int launch_pairs_list(cav_ctx *c, bool ljl, bool sr, bool accumulate, bool pack);  // packed (x, y, z, q), (type, id) of owned + ghost slots
 int launch_lj_cellwalk(cav_ctx *c);
 int launch_zero_acc(cav_ctx *c);
 int launch_ewald_k(cav_ctx *c);
+int launch_dipole(cav_ctx *c);  // epsilon_dipole term into the accelerations
+int elc_plan(cav_ctx *c);        // ELC k grids and buffers (setup)
+int launch_elc(cav_ctx *c);      // ELC chi sums + forces into the accelerations
+int elc_rewind(cav_ctx *c);       // ELC dipole state back before a re-evaluation
+int energy_elc(cav_ctx *c, double *e);
+int point_field_elc(cav_ctx *c, int np, const double *pts, double *field, double *pot);
 int forces_all(cav_ctx *c);
 int launch_drift(cav_ctx *c, double dt);
 int launch_kick(cav_ctx *c, double dt);

@@ -259,6 +259,164 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
   block_virial(vir, a.part + blockIdx.x);
 }
 
+// ------------------------------------- force_field lj and short-range kernel --
+//
+// The Verlet-list force fields that are not fused into k_pairs: the reference's
+// force_field lj (lj.cpp:341-483) and Electrostatic_short_range
+// (electrostatic_short_range.cpp:133-258).  Each adds its own pass over the
+// same full rows (the reference runs each field as its own loop), with the
+// row-gather layout of k_pairs: kLPA lanes per atom, partial forces summed by
+// a fixed shuffle tree, one virial partial per block.
+struct ListArgs {
+  // force_field lj per type pair ([row type][partner type]): coef(r) =
+  // -c12 r^-14 + c6 r^-8 + cshift, pair skipped when r^2 > rc2 (rc2 < 0 for
+  // eps = 0, which the reference skips)
+  const double *ljl;  // [c12 | c6 | cshift | rc2] blocks of nt^2
+  int ljl_ignore_intra, ljl_jump;
+  double ljl_jump_tol;
+  // Electrostatic_short_range
+  double sr_rc, sr_rc2, sr_ke, sr_beta1, sr_ext[3];
+  const double *sr_lambda;  // nt^2, or nullptr
+};
+
+// Force coefficient of one listed pair for the list-based fields.  The
+// virial of each field counts the reference's list entries (owned j > i or a
+// ghost) with id_i < id_j, from the field's own dr^2 (lj clamps it with
+// jump_tol).
+template <typename T, bool LJL, bool SR>
+__device__ __forceinline__ T list_coef(const PairArgs &a, const ListArgs &b, int nt2, int i, int j,
+                                       int ti, int idi, int moli, int molj, const int2 &tj, T qi,
+                                       const double4 &pi4, const double4 &pj, T &dx, T &dy, T &dz,
+                                       double &vir)
+{
+  const double dxd = pj.x - pi4.x, dyd = pj.y - pi4.y, dzd = pj.z - pi4.z;
+  dx = (T)dxd;
+  dy = (T)dyd;
+  dz = (T)dzd;
+  T r2;
+  if constexpr (sizeof(T) == 8)
+    r2 = dot_rn_p(dxd, dyd, dzd);
+  else
+    r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  const int p = (j < a.n && j < i) ? tj.x * a.nt + ti : ti * a.nt + tj.x;
+  const bool counted = (j >= a.n || j > i) && idi < tj.y;
+  T coef = 0;
+  if constexpr (LJL) {
+    if (!(b.ljl_ignore_intra && moli == molj) && !(r2 > (T)b.ljl[3 * nt2 + p])) {
+      T d2 = r2;
+      if (b.ljl_jump && d2 < (T)b.ljl_jump_tol) d2 = (T)b.ljl_jump_tol;
+      const T ri2 = recip(d2);
+      const T ri4 = ri2 * ri2, ri8 = ri4 * ri4;
+      const T cl = -(T)b.ljl[p] * (ri8 * ri4 * ri2) + (T)b.ljl[nt2 + p] * ri8 + (T)b.ljl[2 * nt2 + p];
+      coef += cl;
+      if (counted) vir += -(double)cl * (double)d2;
+    }
+  }
+  if constexpr (SR) {
+    T lam = 1;
+    bool skip = false;
+    if (b.sr_lambda && moli == molj) {
+      const T l = (T)b.sr_lambda[p];
+      if (l == 0) skip = true;
+      lam = l;
+    }
+    if (!skip && r2 < (T)b.sr_rc2) {
+      const T qj = (T)pj.w;
+      T cs;
+      if constexpr (sizeof(T) == 8) {
+        const double rn = sqrt(r2);
+        cs = -lam * b.sr_ke * (double)qi * qj / (r2 * rn) * (1.0 - pow(rn / b.sr_rc, b.sr_beta1));
+      } else {
+        const float rinv = __frsqrt_rn(r2);
+        const float rn = r2 * rinv;
+        const float sh = 1.0f - powf(rn * (float)(1.0 / b.sr_rc), (float)b.sr_beta1);
+        cs = -lam * (float)b.sr_ke * qi * qj * (rinv * rinv * rinv) * sh;
+      }
+      coef += cs;
+      if (counted) vir += -(double)cs * (double)r2;
+    }
+  }
+  return coef;
+}
+
+template <typename T, bool LJL, bool SR>
+__global__ __launch_bounds__(kBlock) void k_pairs_list(PairArgs a, ListArgs b)
+{
+  const int nt2 = a.nt * a.nt;
+  const long per = gridDim.x >> 3;  // XCD-aware order, as k_pairs
+  const long blk = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const long gid = blk * kBlock + threadIdx.x;
+  const int t = (int)(gid / kLPA), sub = (int)(gid % kLPA);
+  double vir = 0.0;
+  T fx = 0, fy = 0, fz = 0;
+  int i = 0, ti = 0;
+  T qi = 0;
+  if (t < a.n) {
+    i = a.order ? a.order[t] : t;
+    const double4 pi4 = a.xq[i];
+    const int2 tii = a.tid[i];
+    ti = tii.x;
+    const int moli = a.mol[i];
+    qi = (T)pi4.w;
+    const int cnt = a.nn[i];
+    const int *row = a.nl + (long)i * a.cap;
+    constexpr int U = 2;
+    for (int k0 = sub; k0 < cnt; k0 += U * kLPA) {
+      int jj[U];
+      double4 pj[U];
+      int2 tj[U];
+      int mj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u * kLPA;
+        jj[u] = k < cnt ? row[k] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = jj[u] < 0 ? i : jj[u];
+        pj[u] = a.xq[j];
+        tj[u] = a.tid[j];
+        mj[u] = a.mol[j];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (jj[u] < 0) break;
+        T dx, dy, dz;
+        const T coef = list_coef<T, LJL, SR>(a, b, nt2, i, jj[u], ti, tii.y, moli, mj[u], tj[u], qi,
+                                             pi4, pj[u], dx, dy, dz, vir);
+        fx = fma(coef, dx, fx);
+        fy = fma(coef, dy, fy);
+        fz = fma(coef, dz, fz);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < kLPA; o <<= 1) {
+    fx += __shfl_xor(fx, o, 64);
+    fy += __shfl_xor(fy, o, 64);
+    fz += __shfl_xor(fz, o, 64);
+  }
+  if (t < a.n && sub == 0) {
+    const double mi = a.minv[ti];
+    double ex = 0, ey = 0, ez = 0;
+    if constexpr (SR) {  // the external field, once per atom after its pairs
+      ex = b.sr_ext[0] * (double)qi * mi;
+      ey = b.sr_ext[1] * (double)qi * mi;
+      ez = b.sr_ext[2] * (double)qi * mi;
+    }
+    if (a.accumulate) {
+      a.ax[i] += (double)fx * mi + ex;
+      a.ay[i] += (double)fy * mi + ey;
+      a.az[i] += (double)fz * mi + ez;
+    } else {
+      a.ax[i] = (double)fx * mi + ex;
+      a.ay[i] = (double)fy * mi + ey;
+      a.az[i] = (double)fz * mi + ez;
+    }
+  }
+  block_virial(vir, a.part + blockIdx.x);
+}
+
 // ----------------------------------------------------- cell-staged pair kernel --
 //
 // One workgroup per cell: the cell's de-duplicated 27-stencil is staged in LDS
@@ -586,6 +744,64 @@ static void launch_pairs_cells_t(cav_ctx *c, const PairArgs &a, bool lj, bool er
     k_pairs_cells<T, true, false><<<g, kBlock, lds, c->stream>>>(a, ca);
   else
     k_pairs_cells<T, false, true><<<g, kBlock, lds, c->stream>>>(a, ca);
+}
+
+// The list-based fields (force_field lj, Electrostatic_short_range) on the
+// current list, one pass for the fields in use; `accumulate` false overwrites
+// the accelerations and the virial (this pass is then the first force of the
+// step); `pack` first refreshes the packed copies (when no fused pair pass has
+// packed them this step).
+int launch_pairs_list(cav_ctx *c, bool want_ljl, bool want_sr, bool accumulate, bool pack)
+{
+  const bool ljl = want_ljl && c->ljl_on, sr = want_sr && c->sr_on;
+  if (!ljl && !sr) return CAV_OK;
+  if (pack) CAV_TRY(launch_pack(c));
+  PairArgs a = pair_args(c);
+  a.accumulate = accumulate ? 1 : 0;
+  ListArgs b;
+  b.ljl = c->d_ljl;
+  b.ljl_ignore_intra = c->ljl_ignore_intra;
+  b.ljl_jump = c->ljl_jump;
+  b.ljl_jump_tol = c->ljl_jump_tol;
+  b.sr_rc = c->sr_rc;
+  b.sr_rc2 = c->sr_rc * c->sr_rc;
+  b.sr_ke = c->sr_ke;
+  b.sr_beta1 = c->sr_beta + 1.0;
+  for (int d = 0; d < 3; ++d) b.sr_ext[d] = c->sr_ext[d];
+  b.sr_lambda = c->sr_lambda_set ? c->d_sr_lambda : nullptr;
+  const int g = pair_grid(c);
+  if (g > c->npart) return fail(c, CAV_ERR_STATE, "partial-sum buffer smaller than the pair grid");
+  timer_begin(c, c->t_pair);
+  if (c->precision == CAV_FP64) {
+    if (ljl && sr)
+      k_pairs_list<double, true, true><<<g, kBlock, 0, c->stream>>>(a, b);
+    else if (ljl)
+      k_pairs_list<double, true, false><<<g, kBlock, 0, c->stream>>>(a, b);
+    else
+      k_pairs_list<double, false, true><<<g, kBlock, 0, c->stream>>>(a, b);
+  } else {
+    if (ljl && sr)
+      k_pairs_list<float, true, true><<<g, kBlock, 0, c->stream>>>(a, b);
+    else if (ljl)
+      k_pairs_list<float, true, false><<<g, kBlock, 0, c->stream>>>(a, b);
+    else
+      k_pairs_list<float, false, true><<<g, kBlock, 0, c->stream>>>(a, b);
+  }
+  timer_end(c, c->t_pair);
+  CAV_HIP(hipGetLastError());
+  return reduce_partials(c, g, 0, accumulate);
+}
+
+// Repacks (x, y, z, q) and (type, id) of the current owned atoms and ghosts
+// for readers of the packed copies other than the pair kernels (the adjoint).
+int launch_pack(cav_ctx *c)
+{
+  const long tot = (long)c->n + c->gcap;
+  k_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->x, c->y, c->z, c->q, c->type,
+                                                 c->id, c->flags, c->xq, c->tid, nullptr, nullptr,
+                                                 nullptr);
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
 }
 
 int launch_pairs(cav_ctx *c, bool want_lj, bool want_er, bool accumulate)

@@ -1,19 +1,24 @@
 // caviar_amd_run: run a CASL script of the LJ + Ewald path on the GPU.
 //
-//   caviar_amd_run [--precision fp64|mixed] [--device N] script.casl [MODE OUT]
+//   caviar_amd_run [--precision fp64|mixed] [--device N] [--molecules FILE] script.casl [MODE OUT]
 //   caviar_amd_run --parse-only script.casl
 //
 // The script uses the reference's own commands (domain box, atom_data basic,
-// neighborlist basic, force_field lj_cell_list / electrostatic_ewald_r /
-// electrostatic_ewald_k, md_simulator basic ... run).  With MODE and OUT the
+// neighborlist basic, force_field lj_cell_list / lj / electrostatic_ewald_r /
+// electrostatic_short_range / electrostatic_ewald_k, md_simulator basic ... run).  With MODE and OUT the
 // state is written to OUT as text:
-//   MODE "setup[+er_energy][+ek_energy][+sk][+field]": after the script,
+//   MODE "setup[+er_energy][+ek_energy][+elc_energy][+sk][+field]": after the script,
 //        Md_simulator::setup (wrap, ghosts, list, one force evaluation) unless
 //        a run already set up, then owned positions / velocities /
 //        accelerations, ghosts, virial, the Verlet list and the requested
 //        energies, S(k) and point fields (at the owned positions shifted by
 //        (0.1, -0.2, 0.3)).
 //   MODE "run": the same dump after the script's own run.
+//   MODE "mrun": the script has no run: the simulator's run is called after
+//        the molecule indices are set (--molecules), then the dump.
+// --molecules FILE sets atom_struct_owned.molecule_index from FILE (one integer
+// per owned atom, in input order) after the script is read, as the
+// reference's add_molecule commands would have set them.
 // Numbers are printed with 17 significant digits.  --parse-only reads the
 // script without touching the GPU (run / step are counted, not executed) and
 // prints the settings it collected as JSON.
@@ -95,6 +100,38 @@ static void print_settings(Interpreter &in, Md_simulator *sim)
       std::fprintf(f, ", \"sigma\": ");
       json_matrix(f, lj->sigma);
       std::fprintf(f, "}, ");
+    } else if (auto *l = dynamic_cast<force_field::Lj *>(ff)) {
+      std::fprintf(f, "\"lj_list\": {\"cutoff\": %.17g, \"epsilon\": ", l->cutoff);
+      json_matrix(f, l->epsilon);
+      std::fprintf(f, ", \"sigma\": ");
+      json_matrix(f, l->sigma);
+      std::fprintf(f, ", \"epsilon_atom\": ");
+      json_vec(f, l->epsilon_atom);
+      std::fprintf(f, ", \"sigma_atom\": ");
+      json_vec(f, l->sigma_atom);
+      std::fprintf(f, ", \"cutoff_list\": ");
+      json_matrix(f, l->cutoff_list);
+      std::fprintf(f, ", \"lambda_s\": ");
+      json_matrix(f, l->lambda_s);
+      std::fprintf(f, ", \"lambda_e\": ");
+      json_matrix(f, l->lambda_e);
+      std::fprintf(f, ", \"wca\": %s, \"ignore_intra_molecule\": %s, \"jump_tol\": ", l->wca ? "true" : "false",
+                   l->ignore_intra_molecule ? "true" : "false");
+      if (l->jump_fix)
+        std::fprintf(f, "%.17g}, ", l->jump_tol);
+      else
+        std::fprintf(f, "null}, ");
+    } else if (auto *sr = dynamic_cast<force_field::Electrostatic_short_range *>(ff)) {
+      std::fprintf(f,
+                   "\"short_range\": {\"cutoff\": %.17g, \"k_electrostatic\": %.17g, \"beta\": %.17g, "
+                   "\"external_field\": [%.17g, %.17g, %.17g], \"lambda_\": ",
+                   sr->cutoff, sr->k_electrostatic, sr->beta, sr->external_field.x, sr->external_field.y,
+                   sr->external_field.z);
+      if (sr->lambda_is_set)
+        json_matrix(f, sr->lambda);
+      else
+        std::fprintf(f, "null");
+      std::fprintf(f, "}, ");
     } else if (auto *er = dynamic_cast<force_field::Electrostatic_ewald_r *>(ff)) {
       std::fprintf(f, "\"ewald_r\": {\"alpha\": %.17g, \"k_electrostatic\": %.17g, \"lambda_\": ", er->alpha,
                    er->k_electrostatic);
@@ -103,6 +140,11 @@ static void print_settings(Interpreter &in, Md_simulator *sim)
       else
         std::fprintf(f, "null");
       std::fprintf(f, "}, ");
+    } else if (auto *el = dynamic_cast<force_field::Electrostatic_ewald_slab_correction *>(ff)) {
+      std::fprintf(f,
+                   "\"elc\": {\"slab_normal_axis\": %d, \"kx_max\": %d, \"ky_max\": %d, \"lz\": %.17g, "
+                   "\"k_electrostatic\": %.17g}, ",
+                   el->slab_normal_axis, el->kx_max, el->ky_max, el->lz, el->k_electrostatic);
     } else if (auto *ek = dynamic_cast<force_field::Electrostatic_ewald_k *>(ff)) {
       std::fprintf(f,
                    "\"ewald_k\": {\"alpha\": %.17g, \"k_electrostatic\": %.17g, \"kx_max\": %d, "
@@ -153,6 +195,17 @@ static void dump(Md_simulator *sim, const std::string &mode, const char *out)
   for (Force_field *ff : sim->force_field) {
     auto *er = dynamic_cast<force_field::Electrostatic_ewald_r *>(ff);
     auto *ek = dynamic_cast<force_field::Electrostatic_ewald_k *>(ff);
+    auto *elc = dynamic_cast<force_field::Electrostatic_ewald_slab_correction *>(ff);
+    if (elc && mode.find("+elc_energy") != std::string::npos) std::fprintf(f, "energy_elc %.17g\n", elc->energy());
+    if (elc && mode.find("+field") != std::string::npos) {
+      std::fprintf(f, "field_elc %zu\n", own.position.size());
+      for (const auto &p : own.position) {
+        const Vector r{p.x + 0.1, p.y - 0.2, p.z + 0.3};
+        const Vector e = elc->field(r);
+        const double phi = elc->potential(r);
+        std::fprintf(f, "%.17g %.17g %.17g %.17g\n", e.x, e.y, e.z, phi);
+      }
+    }
     if (er && mode.find("+er_energy") != std::string::npos) std::fprintf(f, "energy_r %.17g\n", er->energy());
     if (ek && mode.find("+ek_energy") != std::string::npos) {
       std::fprintf(f, "energy_k %.17g\n", ek->energy());
@@ -180,8 +233,13 @@ int main(int argc, char **argv)
 {
   int precision = CAV_FP64, device = 0;
   bool parse_only = false;
+  std::string molecules;
   std::vector<std::string> pos;
   for (int a = 1; a < argc; ++a) {
+    if (!std::strcmp(argv[a], "--molecules") && a + 1 < argc) {
+      molecules = argv[++a];
+      continue;
+    }
     if (!std::strcmp(argv[a], "--precision") && a + 1 < argc) {
       const std::string p = argv[++a];
       if (p == "fp64")
@@ -200,7 +258,9 @@ int main(int argc, char **argv)
       pos.push_back(argv[a]);
   }
   if (pos.size() != 1 && pos.size() != 3) {
-    std::fprintf(stderr, "usage: %s [--precision fp64|mixed] [--device N] [--parse-only] script.casl [MODE OUT]\n",
+    std::fprintf(stderr,
+                 "usage: %s [--precision fp64|mixed] [--device N] [--molecules FILE] [--parse-only] script.casl "
+                 "[MODE OUT]\n",
                  argv[0]);
     return 2;
   }
@@ -211,6 +271,18 @@ int main(int argc, char **argv)
     if (in.objects.md_simulator.empty()) throw Error("script must create an md_simulator");
     Md_simulator *sim = in.objects.md_simulator.begin()->second.get();
     if (in.objects.md_simulator.size() > 1) throw Error("one md_simulator per script on this path");
+    if (!molecules.empty()) {
+      if (!sim->atom_data) throw Error("--molecules: the simulator has no atom_data");
+      auto &mol = sim->atom_data->atom_struct_owned.molecule_index;
+      FILE *mf = std::fopen(molecules.c_str(), "r");
+      if (!mf) throw Error("cannot open " + molecules);
+      for (auto &m : mol)
+        if (std::fscanf(mf, "%d", &m) != 1) {
+          std::fclose(mf);
+          throw Error(molecules + ": too few molecule indices");
+        }
+      std::fclose(mf);
+    }
     if (parse_only) {
       print_settings(in, sim);
       return 0;
@@ -219,8 +291,10 @@ int main(int argc, char **argv)
       const std::string mode = pos[1];
       if (mode.rfind("setup", 0) == 0) {
         if (!sim->initialized) sim->setup();
+      } else if (mode == "mrun") {
+        sim->run();
       } else if (mode != "run")
-        throw Error("MODE is setup[+...] or run");
+        throw Error("MODE is setup[+...], run or mrun");
       dump(sim, mode, pos[2].c_str());
     }
   } catch (const std::exception &e) {

@@ -124,10 +124,16 @@ void Interpreter::create(casl::Parser &p, const std::string &base)
     std::unique_ptr<Force_field> o;
     if (ty == "lj_cell_list")
       o = std::make_unique<force_field::Lj_cell_list>();
+    else if (ty == "lj")
+      o = std::make_unique<force_field::Lj>();
+    else if (ty == "electrostatic_short_range")
+      o = std::make_unique<force_field::Electrostatic_short_range>();
     else if (ty == "electrostatic_ewald_r")
       o = std::make_unique<force_field::Electrostatic_ewald_r>();
     else if (ty == "electrostatic_ewald_k")
       o = std::make_unique<force_field::Electrostatic_ewald_k>();
+    else if (ty == "electrostatic_ewald_slab_correction")
+      o = std::make_unique<force_field::Electrostatic_ewald_slab_correction>();
     else
       p.fail(undefined);
     o->objects = &objects;

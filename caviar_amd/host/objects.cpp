@@ -494,6 +494,142 @@ double Lj_cell_list::energy()
   return e;
 }
 
+bool Lj::read(casl::Parser *p)
+{
+  return read_commands(p, [&](const std::string &t, bool &done) {
+    if (t == "cutoff") {
+      cutoff = p->real("cutoff");
+      if (cutoff < 0.0) p->fail("Force field cutoff have to non-negative.");
+    } else if (t == "cutoff_list") {
+      if (read_matrix_element(p, cutoff_list, 0.0, "cutoff_list") < 0) p->fail("Epsilon have to be non-negative.");
+      cutoff_list_activated = true;
+    } else if (t == "epsilon") {
+      if (read_matrix_element(p, epsilon, 0.0, "epsilon") < 0) p->fail("Epsilon have to be non-negative.");
+      input_by_array = true;
+    } else if (t == "sigma") {
+      if (read_matrix_element(p, sigma, 0.0, "sigma") < 0) p->fail("Sigma have to be non-negative.");
+      input_by_array = true;
+    } else if (t == "lambda_s") {
+      if (read_matrix_element(p, lambda_s, 1.0, "lambda_s") < 0) p->fail("Sigma have to be non-negative.");
+      lambda_s_is_set = true;
+    } else if (t == "lambda_e") {
+      if (read_matrix_element(p, lambda_e, 1.0, "lambda_e") < 0) p->fail("Sigma have to be non-negative.");
+      lambda_e_is_set = true;
+    } else if (t == "epsilon_atom" || t == "sigma_atom") {
+      // GET_A_STDVECTOR_REAL_ELEMENT (preprocessors.h:592-600): grows with 0
+      auto &v = t == "epsilon_atom" ? epsilon_atom : sigma_atom;
+      const long i = p->integer(t + " index");
+      const double x = p->real(t);
+      if (i < 0) p->fail(t + ": negative type index");
+      if ((long)v.size() < i + 1) v.resize(i + 1, 0.0);
+      v[i] = x;
+      input_by_atom = true;
+    } else if (t == "set_neighborlist" || t == "neighborlist")
+      neighborlist = objects->find_neighborlist(p->name("neighborlist"), p);
+    else if (t == "set_atom_data" || t == "atom_data")
+      atom_data = objects->find_atom_data(p->name("atom_data"), p);
+    else if (t == "set_domain" || t == "domain")
+      domain = objects->find_domain(p->name("domain"), p);
+    else if (t == "jump_tol") {
+      jump_fix = true;
+      jump_tol = p->real("jump_tol");
+    } else if (t == "monitor_jump")
+      monitor_jump = true;
+    else if (t == "ignore_intra_molecule")
+      ignore_intra_molecule = true;
+    else if (t == "wca") {
+      wca = true;
+      cutoff_list_activated = true;
+    } else if (t == "make_off_diagonal_vectors")
+      make_off_diagonal_vectors = true;
+    else
+      return false;
+    return true;
+  });
+}
+
+void Lj::verify_settings()
+{
+  if (!atom_data) throw Error("lj: atom_data = nullptr");
+  if (!neighborlist) throw Error("lj: neighborlist = nullptr");
+  if (input_by_atom && input_by_array) throw Error("lj: (input_by_atom && input_by_array)");
+  if (!input_by_atom && !input_by_array) throw Error("lj: no epsilon / sigma given");
+}
+
+void Lj::init()
+{
+  verify_settings();
+  cav_ctx *c = atom_data->device();
+  const int nt = atom_data->num_types();
+  std::vector<double> e, s, ea, sa;
+  if (input_by_array) {
+    e = dense(epsilon, nt, 0.0);
+    s = dense(sigma, nt, 0.0);
+  } else {
+    ea.assign(nt, 0.0);
+    sa.assign(nt, 0.0);
+    for (int i = 0; i < nt && i < (int)epsilon_atom.size(); ++i) ea[i] = epsilon_atom[i];
+    for (int i = 0; i < nt && i < (int)sigma_atom.size(); ++i) sa[i] = sigma_atom[i];
+  }
+  const auto cl = dense(cutoff_list, nt, 0.0);
+  const auto ls = dense(lambda_s, nt, 1.0), le = dense(lambda_e, nt, 1.0);
+  check(c,
+        cav_set_lj_list(c, cutoff, input_by_array ? e.data() : nullptr, input_by_array ? s.data() : nullptr,
+                        input_by_atom ? ea.data() : nullptr, input_by_atom ? sa.data() : nullptr,
+                        cutoff_list_activated && !wca ? cl.data() : nullptr, wca ? 1 : 0,
+                        lambda_s_is_set ? ls.data() : nullptr, lambda_e_is_set ? le.data() : nullptr,
+                        ignore_intra_molecule ? 1 : 0, jump_fix ? 1 : 0, jump_tol),
+        "cav_set_lj_list");
+}
+
+bool Electrostatic_short_range::read(casl::Parser *p)
+{
+  return read_commands(p, [&](const std::string &t, bool &done) {
+    if (t == "cutoff") {
+      cutoff = p->real("cutoff");
+      if (cutoff < 0.0) p->fail("Force field cutoff have to non-negative.");
+    } else if (t == "external_field") {
+      external_field.x = p->real("external_field");
+      external_field.y = p->real("external_field");
+      external_field.z = p->real("external_field");
+    } else if (t == "k_electrostatic") {
+      k_electrostatic = p->real("k_electrostatic");
+      if (k_electrostatic < 0) p->fail("k_electrostatic has to be non-negative.");
+    } else if (t == "lambda") {
+      if (read_matrix_element(p, lambda, 1.0, "lambda") < 0) p->fail("Sigma have to be non-negative.");
+      lambda_is_set = true;
+    } else if (t == "beta")
+      beta = p->real("beta");
+    else if (t == "set_neighborlist" || t == "neighborlist")
+      neighborlist = objects->find_neighborlist(p->name("neighborlist"), p);
+    else if (t == "set_atom_data" || t == "atom_data")
+      atom_data = objects->find_atom_data(p->name("atom_data"), p);
+    else if (t == "set_domain" || t == "domain")
+      domain = objects->find_domain(p->name("domain"), p);
+    else
+      return false;
+    return true;
+  });
+}
+
+void Electrostatic_short_range::verify_settings()
+{
+  if (!atom_data) throw Error("electrostatic_short_range: atom_data = nullptr");
+  if (!neighborlist) throw Error("electrostatic_short_range: neighborlist = nullptr");
+  if (cutoff < 0.0) throw Error("electrostatic_short_range: Force field cutoff have to non-negative.");
+}
+
+void Electrostatic_short_range::init()
+{
+  verify_settings();
+  cav_ctx *c = atom_data->device();
+  const int nt = atom_data->num_types();
+  const auto lam = dense(lambda, nt, 1.0);
+  const double ext[3] = {external_field.x, external_field.y, external_field.z};
+  check(c, cav_set_short_range(c, cutoff, k_electrostatic, beta, ext, lambda_is_set ? lam.data() : nullptr),
+        "cav_set_short_range");
+}
+
 bool Electrostatic_ewald_r::read(casl::Parser *p)
 {
   return read_commands(p, [&](const std::string &t, bool &done) {
@@ -659,6 +795,76 @@ std::vector<double> Electrostatic_ewald_k::potential_k_coef()
   std::vector<double> sk(2 * (size_t)nk);
   if (nk) check(c, cav_get_sk(c, sk.data()), "cav_get_sk");
   return sk;
+}
+
+bool Electrostatic_ewald_slab_correction::read(casl::Parser *p)
+{
+  return read_commands(p, [&](const std::string &t, bool &done) {
+    if (t == "k_max") {
+      const long k = p->integer("k_max");
+      if (k < 0) p->fail("'k_max' have to non-negative.");
+      kx_max = ky_max = (int)k;
+    } else if (t == "kx_max" || t == "ky_max") {
+      const long k = p->integer(t);
+      if (k < 0) p->fail("'" + t + "' have to non-negative.");
+      (t == "kx_max" ? kx_max : ky_max) = (int)k;
+    } else if (t == "lz") {
+      lz = p->real("lz");
+      if (lz < 0) p->fail("'lz' has to be non-negative.");
+    } else if (t == "slab_normal_axis") {
+      casl::Token a = p->value();
+      if (a.kind != casl::Kind::identifier) p->fail("expected 'x', 'y' or 'z'.");
+      // an identifier other than x, y, z leaves the axis as it was
+      // (electrostatic_ewald_slab_correction.cpp:80-92)
+      if (a.text == "x") slab_normal_axis = 0;
+      if (a.text == "y") slab_normal_axis = 1;
+      if (a.text == "z") slab_normal_axis = 2;
+    } else if (t == "k_electrostatic") {
+      k_electrostatic = p->real("k_electrostatic");
+      if (k_electrostatic < 0) p->fail("k_electrostatic has to be non-negative.");
+    } else if (t == "set_atom_data" || t == "atom_data")
+      atom_data = objects->find_atom_data(p->name("atom_data"), p);
+    else if (t == "set_domain" || t == "domain")
+      domain = objects->find_domain(p->name("domain"), p);
+    else
+      return false;
+    return true;
+  });
+}
+
+void Electrostatic_ewald_slab_correction::verify_settings()
+{
+  if (!atom_data) throw Error("electrostatic_ewald_slab_correction: atom_data = nullptr");
+  if (!domain) throw Error("electrostatic_ewald_slab_correction: domain = nullptr");
+}
+
+void Electrostatic_ewald_slab_correction::init()
+{
+  verify_settings();
+  cav_ctx *c = atom_data->device();
+  check(c, cav_set_elc(c, slab_normal_axis, kx_max, ky_max, lz, k_electrostatic), "cav_set_elc");
+}
+
+double Electrostatic_ewald_slab_correction::energy()
+{
+  cav_ctx *c = ff_ctx(this);
+  double e = 0.0;
+  check(c, cav_energy_elc(c, &e), "cav_energy_elc");
+  return e;
+}
+
+double Electrostatic_ewald_slab_correction::potential(const Vector &r)
+{
+  double phi = 0.0;
+  point_query(this, 2, r, nullptr, &phi);
+  return phi;
+}
+
+Vector Electrostatic_ewald_slab_correction::field(const Vector &r)
+{
+  Vector e;
+  point_query(this, 2, r, &e, nullptr);
+  return e;
 }
 
 }  // namespace force_field

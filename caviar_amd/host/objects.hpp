@@ -8,8 +8,12 @@
 //                               objects/neighborlist.cpp:66-88,193-251
 //   Force_field                 include/caviar/objects/force_field.h:33-61
 //   force_field::Lj_cell_list   objects/force_field/lj_cell_list.cpp:33-83
+//   force_field::Lj             objects/force_field/lj.cpp:42-339
+//   force_field::Electrostatic_short_range  objects/force_field/electrostatic_short_range.cpp:39-131
 //   force_field::Electrostatic_ewald_r  objects/force_field/electrostatic_ewald_r.cpp:31-110
 //   force_field::Electrostatic_ewald_k  objects/force_field/electrostatic_ewald_k.cpp:31-159
+//   force_field::Electrostatic_ewald_slab_correction
+//                               objects/force_field/electrostatic_ewald_slab_correction.cpp:30-131
 //   Md_simulator (basic)        objects/md_simulator.cpp:58-233,382-617
 //
 // The particle arrays belong to Atom_data, as in the reference; here they live
@@ -159,6 +163,44 @@ class Lj_cell_list : public Force_field {
   std::vector<std::vector<double>> epsilon, sigma;
 };
 
+// force_field lj: the LJ over the Verlet list (lj.cpp:42-483).  The matrices
+// are kept as read (row-ragged, unset entries 0; lambda_s / lambda_e unset 1);
+// the C ABI resolves them as Lj::verify_settings does (by-atom mixing, WCA
+// cutoffs, lambda scaling).
+class Lj : public Force_field {
+ public:
+  bool read(casl::Parser *parser) override;
+  void verify_settings() override;
+  void init() override;
+  int mask() const override { return CAV_FF_LJ_LIST; }
+  const char *type_name() const override { return "lj"; }
+  std::vector<std::vector<double>> epsilon, sigma, cutoff_list, lambda_s, lambda_e;
+  std::vector<double> epsilon_atom, sigma_atom;
+  bool input_by_array = false, input_by_atom = false;
+  bool cutoff_list_activated = false, wca = false;
+  bool lambda_s_is_set = false, lambda_e_is_set = false;
+  bool ignore_intra_molecule = false;
+  bool jump_fix = false, monitor_jump = false, make_off_diagonal_vectors = false;
+  double jump_tol = 1e-6;  // lj.cpp:33
+};
+
+// force_field electrostatic_short_range: shifted Coulomb within the cutoff
+// over the Verlet list plus a uniform external field
+// (electrostatic_short_range.cpp:39-258).
+class Electrostatic_short_range : public Force_field {
+ public:
+  Electrostatic_short_range() { cutoff = -1.0; }  // electrostatic_short_range.cpp:34
+  bool read(casl::Parser *parser) override;
+  void verify_settings() override;
+  void init() override;
+  int mask() const override { return CAV_FF_SHORT_RANGE; }
+  const char *type_name() const override { return "electrostatic_short_range"; }
+  double k_electrostatic = 1.0, beta = 1.0;
+  Vector external_field;
+  bool lambda_is_set = false;
+  std::vector<std::vector<double>> lambda;  // missing entries read as 1
+};
+
 class Electrostatic_ewald_r : public Force_field {
  public:
   Electrostatic_ewald_r() { cutoff = 0.0; }
@@ -200,6 +242,25 @@ class Electrostatic_ewald_k : public Force_field {
   bool slab_geometry = false;
   int slab_normal_axis = 2;
   double slab_lz = 1.0;
+};
+
+// force_field electrostatic_ewald_slab_correction: the ELC term beside Ewald
+// in slab geometry (electrostatic_ewald_slab_correction.cpp:30-181).
+class Electrostatic_ewald_slab_correction : public Force_field {
+ public:
+  bool read(casl::Parser *parser) override;
+  void verify_settings() override;
+  void init() override;
+  double energy() override;
+  double potential(const Vector &r) override;
+  Vector field(const Vector &r) override;
+  int mask() const override { return CAV_FF_ELC; }
+  const char *type_name() const override { return "electrostatic_ewald_slab_correction"; }
+  // electrostatic_ewald_slab_correction.cpp:31-42
+  int slab_normal_axis = 2;
+  double k_electrostatic = 1.0;
+  int kx_max = 1, ky_max = 1;
+  double lz = 1.0;
 };
 
 }  // namespace force_field

@@ -8,8 +8,12 @@ the short-range + Ewald path, with the same names and defaults:
   charge, owned atoms (id, type, molecule index, position, velocity);
 * ``neighborlist basic`` (neighborlist.cpp:193-251, neighborlist.h:134);
 * ``force_field lj_cell_list`` (lj_cell_list.cpp:33-76);
+* ``force_field lj`` (lj.cpp:42-147) and ``force_field
+  electrostatic_short_range`` (electrostatic_short_range.cpp:39-99);
 * ``force_field electrostatic_ewald_r`` (electrostatic_ewald_r.cpp:31-84);
-* ``force_field electrostatic_ewald_k`` (electrostatic_ewald_k.cpp:31-139).
+* ``force_field electrostatic_ewald_k`` (electrostatic_ewald_k.cpp:31-139);
+* ``force_field electrostatic_ewald_slab_correction``
+  (electrostatic_ewald_slab_correction.cpp:44-126).
 
 A :class:`System` is consumed by :class:`caviar_amd.Engine` (GPU path).
 """
@@ -45,6 +49,53 @@ class LJSettings:
     cutoff: float
     epsilon: np.ndarray
     sigma: np.ndarray
+
+
+@dataclass
+class LJListSettings:
+    """``force_field lj``, the Verlet-list LJ (lj.cpp:42-147): type matrices
+    ``epsilon``/``sigma`` (nt x nt, unset entries 0) or per-type
+    ``epsilon_atom``/``sigma_atom`` (Lorentz-Berthelot off the diagonal,
+    lj.cpp:207-222); per-pair ``cutoff_list`` or ``wca``; ``lambda_s`` /
+    ``lambda_e`` scale sigma / epsilon (unset entries 1);
+    ``ignore_intra_molecule``; ``jump_tol`` (None: off)."""
+    cutoff: float
+    epsilon: Optional[np.ndarray] = None
+    sigma: Optional[np.ndarray] = None
+    epsilon_atom: Optional[np.ndarray] = None
+    sigma_atom: Optional[np.ndarray] = None
+    cutoff_list: Optional[np.ndarray] = None
+    wca: bool = False
+    lambda_s: Optional[np.ndarray] = None
+    lambda_e: Optional[np.ndarray] = None
+    ignore_intra_molecule: bool = False
+    jump_tol: Optional[float] = None
+
+
+@dataclass
+class ShortRangeSettings:
+    """``force_field electrostatic_short_range`` (electrostatic_short_range.cpp:
+    39-99): shifted Coulomb within ``cutoff`` over the Verlet list, plus a
+    uniform ``external_field``; ``lambda_`` per type pair for same-molecule
+    pairs."""
+    cutoff: float
+    k_electrostatic: float = 1.0
+    beta: float = 1.0
+    external_field: tuple = (0.0, 0.0, 0.0)
+    lambda_: Optional[np.ndarray] = None
+
+
+@dataclass
+class ELCSettings:
+    """``force_field electrostatic_ewald_slab_correction``, the ELC term of
+    Ewald sums in slab geometry (electrostatic_ewald_slab_correction.cpp:
+    44-126): the slab normal axis (0, 1, 2), the in-plane k counts, the slab
+    thickness ``lz`` (at least the box height along the normal)."""
+    slab_normal_axis: int = 2
+    kx_max: int = 1
+    ky_max: int = 1
+    lz: float = 1.0
+    k_electrostatic: float = 1.0
 
 
 @dataclass
@@ -88,8 +139,11 @@ class System:
     mol: Optional[np.ndarray] = None
     neighbor: Optional[NeighborSettings] = None
     lj: Optional[LJSettings] = None
+    lj_list: Optional[LJListSettings] = None
+    short_range: Optional[ShortRangeSettings] = None
     ewald_r: Optional[EwaldRSettings] = None
     ewald_k: Optional[EwaldKSettings] = None
+    elc: Optional[ELCSettings] = None
     meta: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -126,3 +180,27 @@ class System:
     def copy(self) -> "System":
         import copy
         return copy.deepcopy(self)
+
+
+def lj_list_args(lj: "LJListSettings", nt: int) -> tuple:
+    """The ``force_field lj`` settings as the C-ABI argument tuple (after the
+    handle): cutoff, eps, sig, eps_atom, sig_atom, cutoff_list, wca, lambda_s,
+    lambda_e, ignore_intra_molecule, jump_fix, jump_tol; arrays as contiguous
+    FP64 or None."""
+    def mat(a):
+        return None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(nt * nt)
+
+    def vec(a):
+        return None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(nt)
+    return (float(lj.cutoff), mat(lj.epsilon), mat(lj.sigma), vec(lj.epsilon_atom),
+            vec(lj.sigma_atom), mat(lj.cutoff_list), int(bool(lj.wca)), mat(lj.lambda_s),
+            mat(lj.lambda_e), int(bool(lj.ignore_intra_molecule)), int(lj.jump_tol is not None),
+            float(lj.jump_tol or 0.0))
+
+
+def short_range_args(sr: "ShortRangeSettings", nt: int) -> tuple:
+    """The ``force_field electrostatic_short_range`` settings as the C-ABI
+    argument tuple (after the handle)."""
+    lam = None if sr.lambda_ is None else np.ascontiguousarray(sr.lambda_, dtype=np.float64).reshape(nt * nt)
+    ext = np.ascontiguousarray(sr.external_field, dtype=np.float64).reshape(3)
+    return float(sr.cutoff), float(sr.k_electrostatic), float(sr.beta), ext, lam

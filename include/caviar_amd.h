@@ -37,7 +37,14 @@ enum cav_precision {
 enum cav_array { CAV_POS = 0, CAV_VEL = 1, CAV_ACC = 2, CAV_ACC_OLD = 3 };
 
 /* Force fields of one context, as bits of the cav_forces mask. */
-enum cav_force_field { CAV_FF_LJ = 1, CAV_FF_EWALD_R = 2, CAV_FF_EWALD_K = 4 };
+enum cav_force_field {
+  CAV_FF_LJ = 1,           /* force_field lj_cell_list */
+  CAV_FF_EWALD_R = 2,      /* force_field electrostatic_ewald_r */
+  CAV_FF_EWALD_K = 4,      /* force_field electrostatic_ewald_k */
+  CAV_FF_LJ_LIST = 8,      /* force_field lj (Verlet list) */
+  CAV_FF_SHORT_RANGE = 16, /* force_field electrostatic_short_range */
+  CAV_FF_ELC = 32          /* force_field electrostatic_ewald_slab_correction */
+};
 
 /* ---- lifetime ---------------------------------------------------------- */
 /* Replaces the objects the interpreter creates (object_creator/objects_types.cpp:25-51). */
@@ -60,6 +67,33 @@ int cav_set_neighbor(cav_ctx *ctx, double cutoff, double cutoff_extra_coef, int 
 /* force_field lj_cell_list: cutoff, epsilon i j, sigma i j (lj_cell_list.cpp:33-76);
  * eps/sigma are ntypes*ntypes row-major [ti][tj]. */
 int cav_set_lj(cav_ctx *ctx, double cutoff, const double *epsilon, const double *sigma);
+/* force_field lj (Verlet-list LJ): cutoff, epsilon i j / sigma i j (by array:
+ * eps and sig, ntypes*ntypes, unset entries 0) or epsilon_atom i / sigma_atom
+ * i (by atom: eps_atom and sig_atom, ntypes, Lorentz-Berthelot off the
+ * diagonal), cutoff_list i j (ntypes*ntypes, unset 0) or wca (cutoff
+ * 2^(1/6) sigma_ij), lambda_s / lambda_e i j (ntypes*ntypes, unset 1; scale
+ * sigma / epsilon), ignore_intra_molecule, jump_tol (jump_fix != 0).  NULL
+ * for what is not given.  Resolved as Lj::read / verify_settings resolve them
+ * (lj.cpp:42-147,162-339); forces as Lj::calculate_acceleration
+ * (lj.cpp:341-483). */
+int cav_set_lj_list(cav_ctx *ctx, double cutoff, const double *eps, const double *sig,
+                    const double *eps_atom, const double *sig_atom, const double *cutoff_list,
+                    int wca, const double *lambda_s, const double *lambda_e,
+                    int ignore_intra_molecule, int jump_fix, double jump_tol);
+/* force_field electrostatic_short_range: cutoff, k_electrostatic, beta,
+ * external_field (3 doubles or NULL), lambda i j (ntypes*ntypes or NULL)
+ * (electrostatic_short_range.cpp:39-99); forces as calculate_acceleration
+ * (electrostatic_short_range.cpp:133-258). */
+int cav_set_short_range(cav_ctx *ctx, double cutoff, double k_electrostatic, double beta,
+                        const double external_field[3], const double *lambda);
+/* force_field electrostatic_ewald_slab_correction (ELC): slab_normal_axis
+ * (0 x, 1 y, 2 z), kx_max, ky_max, lz (slab thickness, at least the box
+ * height along the normal), k_electrostatic
+ * (electrostatic_ewald_slab_correction.cpp:44-126); forces as its
+ * calculate_acceleration (electrostatic_ewald_slab_correction.cpp:134-181,
+ * electrostatic_ewald_slab_correction/initialize.cpp, field.cpp). */
+int cav_set_elc(cav_ctx *ctx, int slab_normal_axis, int kx_max, int ky_max, double lz,
+                double k_electrostatic);
 /* force_field electrostatic_ewald_r: alpha, k_electrostatic, lambda i j
  * (electrostatic_ewald_r.cpp:31-110); lambda may be NULL. */
 int cav_set_ewald_r(cav_ctx *ctx, double alpha, double k_electrostatic, const double *lambda);
@@ -136,10 +170,16 @@ int cav_get_sk(cav_ctx *ctx, double *sk);
  * the shifted-force LJ energy matching Lj_cell_list's force (no reference
  * counterpart; the reference's Lj_cell_list has no energy). NULL outputs skipped. */
 int cav_energy(cav_ctx *ctx, double *e_lj, double *e_r, double *e_k);
+/* Electrostatic_ewald_slab_correction::energy (electrostatic_ewald_slab_correction/
+ * energy.cpp:32-146) from the last force evaluation. */
+int cav_energy_elc(cav_ctx *ctx, double *e);
 /* Batched Force_field::field(const Vector&) / potential(const Vector&):
  * which 0 = Electrostatic_ewald_r (electrostatic_ewald_r/field.cpp:31-96,
  * potential.cpp:30-86), 1 = Electrostatic_ewald_k (electrostatic_ewald_k/
- * field.cpp:80-132, potential.cpp:31-129).  np points (AoS 3np doubles). */
+ * field.cpp:80-132, potential.cpp:31-129), 2 = Electrostatic_ewald_slab_
+ * correction (electrostatic_ewald_slab_correction/field.cpp:29-136,
+ * potential.cpp:30-79; state of the last force evaluation).  np points (AoS
+ * 3np doubles). */
 int cav_point_field(cav_ctx *ctx, int which, int np, const double *pts, double *field,
                     double *potential);
 
@@ -152,7 +192,8 @@ int cav_point_field(cav_ctx *ctx, int which, int np, const double *pts, double *
  * gradient of the reference's force formulas (lj_cell_list.cpp:143-155,
  * electrostatic_ewald_r.cpp:183-211, electrostatic_ewald_k.cpp:197-226).
  * Parameter values may be changed with the cav_set_* calls after setup; they
- * take effect at the next cav_recalculate / cav_step.  NULL outputs skipped. */
+ * take effect at the next call that evaluates forces, energies, point fields
+ * or this adjoint (owned and ghost charges alike).  NULL outputs skipped. */
 int cav_adjoint(cav_ctx *ctx, const double *weights, double *g_epsilon, double *g_sigma,
                 double *g_charge);
 

@@ -47,10 +47,20 @@ def _load():
         "orc_set_neighbor": (None, [_p, _d, _d, _i, _i, _i, _i]),
         "orc_set_lj": (None, [_p, _d, _dp, _dp]),
         "orc_set_ewald_r": (None, [_p, _d, _d, _dp]),
+        "orc_set_lj_list": (_i, [_p, _d, _dp, _dp, _dp, _dp, _dp, _i, _dp, _dp, _i, _i, _d]),
+        "orc_set_short_range": (None, [_p, _d, _d, _d, _dp, _dp]),
+        "orc_lj_list_acc": (None, [_p]),
+        "orc_set_elc": (None, [_p, _i, _i, _i, _d, _d]),
+        "orc_elc_acc": (None, [_p]),
+        "orc_energy_elc": (_d, [_p]),
+        "orc_point_elc": (None, [_p, _i, _dp, _dp, _dp]),
+        "orc_short_range_acc": (None, [_p]),
         "orc_set_ewald_k": (None, [_p, _d, _d, _i, _i, _i, _i, _d, _i, _d]),
         "orc_setup": (_i, [_p]),
         "orc_step": (_i, [_p, _d]),
         "orc_forces": (None, [_p]),
+        "orc_forces_sampled": (None, [_p, _i, _ip]),
+        "orc_setup_sampled": (_i, [_p, _i, _ip]),
         "orc_wrap": (_i, [_p]),
         "orc_ghosts": (None, [_p]),
         "orc_build": (_i, [_p, _i]),
@@ -135,10 +145,26 @@ class Oracle:
             eps = np.ascontiguousarray(cfg.lj.epsilon, dtype=np.float64).reshape(nt * nt)
             sig = np.ascontiguousarray(cfg.lj.sigma, dtype=np.float64).reshape(nt * nt)
             L.orc_set_lj(self.h, cfg.lj.cutoff, _dptr(eps), _dptr(sig))
+        if cfg.lj_list is not None:
+            from caviar_amd.system import lj_list_args
+            a = lj_list_args(cfg.lj_list, nt)
+            self._keep.append(a)
+            if L.orc_set_lj_list(self.h, a[0], *[_dptr(x) for x in a[1:6]], a[6], _dptr(a[7]),
+                                 _dptr(a[8]), a[9], a[10], a[11]):
+                raise ValueError("force_field lj: give epsilon/sigma by array or by atom, not both")
+        if cfg.short_range is not None:
+            from caviar_amd.system import short_range_args
+            a = short_range_args(cfg.short_range, nt)
+            self._keep.append(a)
+            L.orc_set_short_range(self.h, a[0], a[1], a[2], _dptr(a[3]), _dptr(a[4]))
         if cfg.ewald_r is not None:
             lam = cfg.ewald_r.lambda_
             lam = None if lam is None else np.ascontiguousarray(lam, dtype=np.float64).reshape(nt * nt)
             L.orc_set_ewald_r(self.h, cfg.ewald_r.alpha, cfg.ewald_r.k_electrostatic, _dptr(lam))
+        if cfg.elc is not None:
+            c = cfg.elc
+            L.orc_set_elc(self.h, int(c.slab_normal_axis), int(c.kx_max), int(c.ky_max), float(c.lz),
+                          float(c.k_electrostatic))
         if cfg.ewald_k is not None:
             e = cfg.ewald_k
             L.orc_set_ewald_k(self.h, e.alpha, e.k_electrostatic, e.kx_max, e.ky_max, e.kz_max,
@@ -156,6 +182,12 @@ class Oracle:
     # driver
     def setup(self):
         return self.lib.orc_setup(self.h)
+
+    def setup_sampled(self, idx):
+        """setup() with the Ewald_k force pass over the atoms ``idx`` only
+        (after the full S(k)); other atoms lack their k-space term."""
+        self._idx = np.ascontiguousarray(idx, dtype=np.int32)
+        return self.lib.orc_setup_sampled(self.h, len(self._idx), _iptr(self._idx))
 
     def step(self, dt: float):
         return self.lib.orc_step(self.h, float(dt))
@@ -253,6 +285,16 @@ class Oracle:
         f = np.empty_like(pts)
         p = np.empty(len(pts))
         self.lib.orc_point_k(self.h, len(pts), _dptr(pts), _dptr(f), _dptr(p))
+        return f, p
+
+    def energy_elc(self) -> float:
+        return self.lib.orc_energy_elc(self.h)
+
+    def point_elc(self, pts):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        f = np.empty_like(pts)
+        p = np.empty(len(pts))
+        self.lib.orc_point_elc(self.h, len(pts), _dptr(pts), _dptr(f), _dptr(p))
         return f, p
 
     def adjoint(self, weights):

@@ -52,6 +52,20 @@ struct orc_sys {
   int lj_on;
   double lj_rc;
   double *eps, *sig;
+  /* force_field lj (Verlet-list LJ): resolved type matrices */
+  int ljl_on, ljl_cut_on, ljl_ignore_intra, ljl_jump;
+  double ljl_rc, ljl_jump_tol;
+  double *ljl_eps, *ljl_sig, *ljl_cut;
+  /* Electrostatic_short_range */
+  int sr_on, sr_lambda_set;
+  double sr_rc, sr_ke, sr_beta, sr_ext[3];
+  double *sr_lambda;
+  /* Electrostatic_ewald_slab_correction (ELC) */
+  int elc_on, elc_init, elc_axis, elc_kx, elc_ky;
+  double elc_ke, elc_lz, elc_lx, elc_ly, elc_hz, elc_sum_e_coef, elc_dip_coef;
+  double elc_dip_sum[3], elc_dip_field[3];
+  double *elc_kxv, *elc_kyv, *elc_kpv, *elc_kxc, *elc_kyc, *elc_kpc;
+  double *elc_chi_x, *elc_chi_y, *elc_chi_p; /* [kx][2][2], [ky][2][2], [kx*ky][2][2][2] */
   /* Electrostatic_ewald_r */
   int er_on, lambda_set;
   double er_alpha, er_ke;
@@ -82,7 +96,9 @@ void orc_destroy(orc_sys *s)
   void *ptrs[] = {s->mass, s->minv, s->charge, s->id, s->type, s->mol, s->pos, s->vel,
                   s->acc, s->acc_old, s->gpos, s->gsrc, s->st_off, s->st, s->bin_off,
                   s->bin_items, s->nl_off, s->nl_items, s->pos_old, s->gpos_old, s->eps,
-                  s->sig, s->lambda, s->kvec, s->fk, s->sk};
+                  s->sig, s->lambda, s->kvec, s->fk, s->sk, s->ljl_eps, s->ljl_sig,
+                  s->ljl_cut, s->sr_lambda, s->elc_kxv, s->elc_kyv, s->elc_kpv, s->elc_kxc,
+                  s->elc_kyc, s->elc_kpc, s->elc_chi_x, s->elc_chi_y, s->elc_chi_p};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
     free(ptrs[i]);
   free(s);
@@ -166,6 +182,100 @@ void orc_set_lj(orc_sys *s, double rc, const double *eps, const double *sigma)
   free(s->sig);
   s->eps = (double *)dup_arr(eps, sizeof(double) * s->nt * s->nt);
   s->sig = (double *)dup_arr(sigma, sizeof(double) * s->nt * s->nt);
+}
+
+/* force_field lj: Lj::read + Lj::verify_settings (lj.cpp:42-147,162-339) for
+ * nt types: matrices by array (eps/sig, nt*nt, unset entries 0) or by atom
+ * (eps_atom/sig_atom, nt, Lorentz-Berthelot off the diagonal, lj.cpp:207-222);
+ * cutoff_list (nt*nt, unset 0) or wca (2^(1/6) sigma_ij, before the lambda_s
+ * scaling, lj.cpp:278-294); lambda_s scales sigma, lambda_e epsilon (nt*nt,
+ * unset 1; lj.cpp:310-338).  Returns 1 when both input forms are given
+ * (lj.cpp:167-170) or neither. */
+int orc_set_lj_list(orc_sys *s, double cutoff, const double *eps, const double *sig,
+                    const double *eps_atom, const double *sig_atom, const double *cutoff_list,
+                    int wca, const double *lambda_s, const double *lambda_e,
+                    int ignore_intra_molecule, int jump_fix, double jump_tol)
+{
+  const int nt = s->nt, nt2 = nt * nt;
+  const int by_array = eps || sig, by_atom = eps_atom || sig_atom;
+  if ((by_array && by_atom) || (!by_array && !by_atom))
+    return 1;
+  free(s->ljl_eps);
+  free(s->ljl_sig);
+  free(s->ljl_cut);
+  s->ljl_eps = (double *)calloc(nt2 ? nt2 : 1, sizeof(double));
+  s->ljl_sig = (double *)calloc(nt2 ? nt2 : 1, sizeof(double));
+  s->ljl_cut = (double *)calloc(nt2 ? nt2 : 1, sizeof(double));
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j) {
+      const int p = i * nt + j;
+      if (by_atom) {
+        const double ea = eps_atom ? eps_atom[i] : 0, eb = eps_atom ? eps_atom[j] : 0;
+        const double sa = sig_atom ? sig_atom[i] : 0, sb = sig_atom ? sig_atom[j] : 0;
+        if (i == j) {
+          s->ljl_sig[p] = sa;
+          s->ljl_eps[p] = ea;
+        } else {
+          s->ljl_sig[p] = 0.5 * (sa + sb);
+          s->ljl_eps[p] = sqrt(ea * eb);
+        }
+      } else {
+        s->ljl_eps[p] = eps ? eps[p] : 0;
+        s->ljl_sig[p] = sig ? sig[p] : 0;
+      }
+    }
+  s->ljl_cut_on = wca || cutoff_list != NULL;
+  if (wca) {
+    const double cut_coef = pow(2.0, 1.0 / 6.0);
+    for (int p = 0; p < nt2; ++p)
+      s->ljl_cut[p] = cut_coef * s->ljl_sig[p];
+  } else if (cutoff_list) {
+    memcpy(s->ljl_cut, cutoff_list, sizeof(double) * nt2);
+  }
+  for (int p = 0; p < nt2; ++p) {
+    if (lambda_s)
+      s->ljl_sig[p] *= lambda_s[p];
+    if (lambda_e)
+      s->ljl_eps[p] *= lambda_e[p];
+  }
+  s->ljl_rc = cutoff;
+  s->ljl_ignore_intra = ignore_intra_molecule;
+  s->ljl_jump = jump_fix;
+  s->ljl_jump_tol = jump_tol;
+  s->ljl_on = 1;
+  return 0;
+}
+
+/* Electrostatic_short_range::read (electrostatic_short_range.cpp:39-99):
+ * cutoff, k_electrostatic, beta, external_field, lambda (nt*nt, unset 1). */
+void orc_set_short_range(orc_sys *s, double cutoff, double ke, double beta, const double ext[3],
+                         const double *lambda)
+{
+  const int nt2 = s->nt * s->nt;
+  s->sr_rc = cutoff;
+  s->sr_ke = ke;
+  s->sr_beta = beta;
+  for (int d = 0; d < 3; ++d)
+    s->sr_ext[d] = ext ? ext[d] : 0.0;
+  free(s->sr_lambda);
+  s->sr_lambda = (double *)dup_arr(lambda, sizeof(double) * (nt2 ? nt2 : 1));
+  s->sr_lambda_set = lambda != NULL;
+  s->sr_on = 1;
+}
+
+/* Electrostatic_ewald_slab_correction::read (electrostatic_ewald_slab_correction.cpp:
+ * 44-126): slab_normal_axis (default z), kx_max / ky_max, lz, k_electrostatic. */
+void orc_set_elc(orc_sys *s, int axis, int kx, int ky, double lz, double ke)
+{
+  s->elc_axis = axis;
+  s->elc_kx = kx;
+  s->elc_ky = ky;
+  s->elc_lz = lz;
+  s->elc_ke = ke;
+  s->elc_init = 0;
+  for (int d = 0; d < 3; ++d)
+    s->elc_dip_sum[d] = s->elc_dip_field[d] = 0.0; /* Vector<double>() is zero */
+  s->elc_on = 1;
 }
 
 void orc_set_ewald_r(orc_sys *s, double alpha, double ke, const double *lambda)
@@ -593,6 +703,426 @@ void orc_lj_acc(orc_sys *s)
   }
 }
 
+/* Lj::calculate_acceleration (lj.cpp:341-483): every Verlet-list entry;
+ * same-molecule pairs skipped with ignore_intra_molecule; c = cutoff^2 or
+ * cutoff_list[ti][tj]^2; skip dr^2 > c; jump_fix clamps dr^2 below jump_tol;
+ * eps_ij = 0 skipped; the force shift at c; virial -coef*dr^2 when
+ * id_i < id_j; Newton-3 on owned partners. */
+void orc_lj_list_acc(orc_sys *s)
+{
+  const int nt = s->nt;
+  const double cutoff_sq = s->ljl_rc * s->ljl_rc;
+  double vir = 0;
+  for (int i = 0; i < s->n; ++i) {
+    const double *pi = s->pos + 3 * i;
+    const int ti = s->type[i];
+    const double mi = s->minv[ti];
+    const int idi = s->id[i], moli = s->mol[i];
+    for (long t = s->nl_off[i]; t < s->nl_off[i + 1]; ++t) {
+      const int j = s->nl_items[t];
+      const int ghost = j >= s->n;
+      const double *pj = apos(s, j);
+      const int tj = atype(s, j);
+      if (s->ljl_ignore_intra && moli == amol(s, j))
+        continue;
+      double dr[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+      double dr2 = dot3(dr);
+      double c = cutoff_sq;
+      if (s->ljl_cut_on)
+        c = s->ljl_cut[ti * nt + tj] * s->ljl_cut[ti * nt + tj];
+      if (dr2 > c)
+        continue;
+      if (s->ljl_jump && dr2 < s->ljl_jump_tol)
+        dr2 = s->ljl_jump_tol;
+      const double e = s->ljl_eps[ti * nt + tj];
+      if (e == 0)
+        continue;
+      const double sg = s->ljl_sig[ti * nt + tj];
+      const double rcsi = 1 / c;
+      const double rhoc2 = sg * sg * rcsi;
+      const double rhoc6 = rhoc2 * rhoc2 * rhoc2;
+      const double rhoc12 = rhoc6 * rhoc6;
+      const double dsi = 1 / dr2;
+      const double rho2 = sg * sg * dsi;
+      const double rho6 = rho2 * rho2 * rho2;
+      const double rho12 = rho6 * rho6;
+      const double coef = 4 * e * (-12 * rho12 * dsi + 6 * rho6 * dsi + 12 * rhoc12 * rcsi -
+                                   6 * rhoc6 * rcsi);
+      if (idi < aid(s, j))
+        vir += -coef * dr2;
+      double f[3] = {coef * dr[0], coef * dr[1], coef * dr[2]};
+      for (int d = 0; d < 3; ++d)
+        s->acc[3 * i + d] += f[d] * mi;
+      if (!ghost) {
+        const double mj = s->minv[tj];
+        for (int d = 0; d < 3; ++d)
+          s->acc[3 * j + d] -= f[d] * mj;
+      }
+    }
+  }
+  s->virial += vir;
+}
+
+/* Electrostatic_short_range::calculate_acceleration (electrostatic_short_range.cpp:
+ * 133-258): every Verlet-list entry; lambda of same-molecule pairs (0 skips);
+ * skip dr^2 >= cutoff^2; coef = -lambda ke qi qj / (r^2 r) (1 - (r/rc)^(beta+1));
+ * virial -coef*dr^2 when id_i < id_j; then the external field q_i E per atom. */
+void orc_short_range_acc(orc_sys *s)
+{
+  const int nt = s->nt;
+  const double cutoff = s->sr_rc, cutoff_sq = cutoff * cutoff, ke = s->sr_ke;
+  double vir = 0;
+  for (int i = 0; i < s->n; ++i) {
+    const double *pi = s->pos + 3 * i;
+    const int ti = s->type[i];
+    const double qi = s->charge[ti], mi = s->minv[ti];
+    const int idi = s->id[i], moli = s->mol[i];
+    for (long t = s->nl_off[i]; t < s->nl_off[i + 1]; ++t) {
+      const int j = s->nl_items[t];
+      const int ghost = j >= s->n;
+      const double *pj = apos(s, j);
+      const int tj = atype(s, j);
+      double lam = 1.0;
+      if (s->sr_lambda_set && moli == amol(s, j)) {
+        if (s->sr_lambda[ti * nt + tj] == 0.0)
+          continue;
+        lam *= s->sr_lambda[ti * nt + tj];
+      }
+      double dr[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+      const double dr2 = dot3(dr);
+      if (dr2 >= cutoff_sq)
+        continue;
+      const double qj = s->charge[tj];
+      const double rn = sqrt(dr2);
+      const double coef = -lam * ke * qi * qj / (dr2 * rn) * (1.0 - pow(rn / cutoff, s->sr_beta + 1));
+      if (idi < aid(s, j))
+        vir += -coef * dr2;
+      double f[3] = {coef * dr[0], coef * dr[1], coef * dr[2]};
+      for (int d = 0; d < 3; ++d)
+        s->acc[3 * i + d] += f[d] * mi;
+      if (!ghost) {
+        const double mj = s->minv[tj];
+        for (int d = 0; d < 3; ++d)
+          s->acc[3 * j + d] -= f[d] * mj;
+      }
+    }
+    for (int d = 0; d < 3; ++d)
+      s->acc[3 * i + d] += s->sr_ext[d] * qi * mi;
+  }
+  s->virial += vir;
+}
+
+/* ---------------------------------------------------- slab correction (ELC) -- */
+
+/* give_slab_local_coordinates (electrostatic_ewald_slab_correction.h:55-77). */
+static void elc_local(const orc_sys *s, const double *g, double *l)
+{
+  if (s->elc_axis == 0) {
+    l[0] = g[1];
+    l[1] = g[2];
+    l[2] = g[0];
+  } else if (s->elc_axis == 1) {
+    l[0] = g[2];
+    l[1] = g[0];
+    l[2] = g[1];
+  } else {
+    l[0] = g[0];
+    l[1] = g[1];
+    l[2] = g[2];
+  }
+}
+
+/* give_slab_global_coordinates (electrostatic_ewald_slab_correction.h:79-101),
+ * as the reference maps it: for axis x it applies axis y's inverse and for
+ * axis y axis x's (the two are not each other's local maps' inverses). */
+static void elc_global(const orc_sys *s, const double *l, double *g)
+{
+  if (s->elc_axis == 0) {
+    g[0] = l[1];
+    g[1] = l[2];
+    g[2] = l[0];
+  } else if (s->elc_axis == 1) {
+    g[0] = l[2];
+    g[1] = l[0];
+    g[2] = l[1];
+  } else {
+    g[0] = l[0];
+    g[1] = l[1];
+    g[2] = l[2];
+  }
+}
+
+/* initialize (electrostatic_ewald_slab_correction/initialize.cpp:31-228): box
+ * lengths in slab coordinates and the k grids once; then per call the dipole
+ * sum and the chi sums.  The dipole field of a call comes from the previous
+ * call's dipole sum (initialize.cpp:86: dipole_sum is read before it is
+ * updated), zero on the first call. */
+static void elc_initialize(orc_sys *s)
+{
+  const int kx = s->elc_kx, ky = s->elc_ky;
+  if (!s->elc_init) {
+    s->elc_init = 1;
+    const double glx = s->L[0], gly = s->L[1], glz = s->L[2];
+    if (s->elc_axis == 0) {
+      s->elc_lx = gly;
+      s->elc_ly = glz;
+      s->elc_hz = glx;
+    } else if (s->elc_axis == 1) {
+      s->elc_lx = glx;
+      s->elc_ly = glz;
+      s->elc_hz = gly;
+    } else {
+      s->elc_lx = glx;
+      s->elc_ly = gly;
+      s->elc_hz = glz;
+    }
+    const double lx_ly_inv = 1.0 / (s->elc_lx * s->elc_ly);
+    s->elc_sum_e_coef = -ORC_4PI * lx_ly_inv;
+    /* make_slab_k_vectors (initialize.cpp:119-160) */
+    free(s->elc_kxv);
+    free(s->elc_kyv);
+    free(s->elc_kpv);
+    free(s->elc_kxc);
+    free(s->elc_kyc);
+    free(s->elc_kpc);
+    s->elc_kxv = (double *)calloc(kx + 1, sizeof(double));
+    s->elc_kxc = (double *)calloc(kx + 1, sizeof(double));
+    s->elc_kyv = (double *)calloc(ky + 1, sizeof(double));
+    s->elc_kyc = (double *)calloc(ky + 1, sizeof(double));
+    s->elc_kpv = (double *)calloc(kx * ky + 1, sizeof(double));
+    s->elc_kpc = (double *)calloc(kx * ky + 1, sizeof(double));
+    const double lx_inv = 1.0 / s->elc_lx, ly_inv = 1.0 / s->elc_ly, lz = s->elc_lz;
+    for (int ix = 0; ix < kx; ++ix) {
+      s->elc_kxv[ix] = ORC_2PI * (ix + 1) * lx_inv;
+      s->elc_kxc[ix] = 1.0 / (s->elc_kxv[ix] * (exp(s->elc_kxv[ix] * lz) - 1.0));
+    }
+    for (int iy = 0; iy < ky; ++iy) {
+      s->elc_kyv[iy] = ORC_2PI * (iy + 1) * ly_inv;
+      s->elc_kyc[iy] = 1.0 / (s->elc_kyv[iy] * (exp(s->elc_kyv[iy] * lz) - 1.0));
+    }
+    int ip = 0;
+    for (int ix = 0; ix < kx; ++ix)
+      for (int iy = 0; iy < ky; ++iy) {
+        s->elc_kpv[ip] = sqrt((s->elc_kxv[ix] * s->elc_kxv[ix]) + (s->elc_kyv[iy] * s->elc_kyv[iy]));
+        s->elc_kpc[ip] = 1.0 / (s->elc_kpv[ip] * (exp(s->elc_kpv[ip] * lz) - 1.0));
+        ++ip;
+      }
+    const double l_xyz_inv = lx_ly_inv / lz;
+    s->elc_dip_coef = s->elc_ke * ORC_4PI * l_xyz_inv;
+    free(s->elc_chi_x);
+    free(s->elc_chi_y);
+    free(s->elc_chi_p);
+    s->elc_chi_x = (double *)calloc(4 * kx + 1, sizeof(double));
+    s->elc_chi_y = (double *)calloc(4 * ky + 1, sizeof(double));
+    s->elc_chi_p = (double *)calloc(8 * kx * ky + 1, sizeof(double));
+  }
+  /* calculate_dipole_sum (initialize.cpp:75-117) */
+  double ds[3] = {0, 0, 0};
+  for (int j = 0; j < s->n; ++j) {
+    const double q = s->charge[s->type[j]];
+    for (int d = 0; d < 3; ++d)
+      ds[d] += q * s->pos[3 * j + d];
+  }
+  double dfv[3];
+  for (int d = 0; d < 3; ++d)
+    dfv[d] = -s->elc_dip_sum[d] * s->elc_dip_coef;
+  for (int d = 0; d < 3; ++d) {
+    const int on = d == s->elc_axis;
+    s->elc_dip_field[d] = on ? dfv[d] : 0.0;
+    s->elc_dip_sum[d] = on ? ds[d] : 0.0;
+  }
+  /* make_slab_chi_vectors (initialize.cpp:162-268) */
+  memset(s->elc_chi_x, 0, sizeof(double) * 4 * kx);
+  memset(s->elc_chi_y, 0, sizeof(double) * 4 * ky);
+  memset(s->elc_chi_p, 0, sizeof(double) * 8 * kx * ky);
+  for (int j = 0; j < s->n; ++j) {
+    const double q = s->charge[s->type[j]];
+    double p[3];
+    elc_local(s, s->pos + 3 * j, p);
+    int ip = 0;
+    for (int ix = 0; ix < kx; ++ix) {
+      const double kxv = s->elc_kxv[ix];
+      const double sinhkxz = sinh(kxv * p[2]), coshkxz = cosh(kxv * p[2]);
+      const double sinkxx = sin(kxv * p[0]), coskxx = cos(kxv * p[0]);
+      double *cx = s->elc_chi_x + 4 * ix;
+      cx[0] += q * sinhkxz * sinkxx;
+      cx[1] += q * sinhkxz * coskxx;
+      cx[2] += q * coshkxz * sinkxx;
+      cx[3] += q * coshkxz * coskxx;
+      for (int iy = 0; iy < ky; ++iy) {
+        const double kyv = s->elc_kyv[iy], kpv = s->elc_kpv[ip];
+        const double sinhkpz = sinh(kpv * p[2]), coshkpz = cosh(kpv * p[2]);
+        const double sinkyy = sin(kyv * p[1]), coskyy = cos(kyv * p[1]);
+        double *cp = s->elc_chi_p + 8 * ip; /* [a][b][c] at 4a + 2b + c */
+        cp[0] += q * sinhkpz * sinkxx * sinkyy;
+        cp[1] += q * sinhkpz * sinkxx * coskyy;
+        cp[2] += q * sinhkpz * coskxx * sinkyy;
+        cp[4] += q * coshkpz * sinkxx * sinkyy;
+        cp[3] += q * sinhkpz * coskxx * coskyy;
+        cp[5] += q * coshkpz * sinkxx * coskyy;
+        cp[6] += q * coshkpz * coskxx * sinkyy;
+        cp[7] += q * coshkpz * coskxx * coskyy;
+        if (ix == 0) {
+          const double sinhkyz = sinh(kyv * p[2]), coshkyz = cosh(kyv * p[2]);
+          double *cy = s->elc_chi_y + 4 * iy;
+          cy[0] += q * sinhkyz * sinkyy;
+          cy[1] += q * sinhkyz * coskyy;
+          cy[2] += q * coshkyz * sinkyy;
+          cy[3] += q * coshkyz * coskyy;
+        }
+        ++ip;
+      }
+    }
+  }
+}
+
+/* Electrostatic_ewald_slab_correction::field (electrostatic_ewald_slab_correction/
+ * field.cpp:29-136): the order-N scheme over the chi sums, in slab coordinates,
+ * plus the dipole field. */
+static void elc_field(const orc_sys *s, const double *r, double *out)
+{
+  const int kx = s->elc_kx, ky = s->elc_ky;
+  double f[3] = {0, 0, 0};
+  double p[3];
+  elc_local(s, r, p);
+  int ip = 0;
+  for (int ix = 0; ix < kx; ++ix) {
+    const double kxv = s->elc_kxv[ix];
+    const double sinhkxz = sinh(kxv * p[2]), coshkxz = cosh(kxv * p[2]);
+    const double sinkxx = sin(kxv * p[0]), coskxx = cos(kxv * p[0]);
+    const double *c = s->elc_chi_x + 4 * ix;
+    const double sx_chi = -kxv * (-(+c[0] * coskxx * sinhkxz) - (-c[1] * sinkxx * sinhkxz) +
+                                  (+c[2] * coskxx * coshkxz) + (-c[3] * sinkxx * coshkxz));
+    const double sz_chi = -kxv * (-(+c[0] * coshkxz * sinkxx) - (+c[1] * coshkxz * coskxx) +
+                                  (+c[2] * sinhkxz * sinkxx) + (+c[3] * sinhkxz * coskxx));
+    f[0] += s->elc_kxc[ix] * sx_chi;
+    f[2] += s->elc_kxc[ix] * sz_chi;
+    for (int iy = 0; iy < ky; ++iy) {
+      const double kyv = s->elc_kyv[iy], kpv = s->elc_kpv[ip];
+      const double sinhkpz = sinh(kpv * p[2]), coshkpz = cosh(kpv * p[2]);
+      const double sinkyy = sin(kyv * p[1]), coskyy = cos(kyv * p[1]);
+      const double *q = s->elc_chi_p + 8 * ip; /* 000 001 010 011 100 101 110 111 */
+      const double sx = -kxv * (-(+q[0] * coskxx * sinhkpz * sinkyy) - (+q[1] * coskxx * sinhkpz * coskyy) -
+                                (-q[2] * sinkxx * sinhkpz * sinkyy) + (+q[4] * coskxx * coshkpz * sinkyy) -
+                                (-q[3] * sinkxx * sinhkpz * coskyy) + (+q[5] * coskxx * coshkpz * coskyy) +
+                                (-q[6] * sinkxx * coshkpz * sinkyy) + (-q[7] * sinkxx * coshkpz * coskyy));
+      const double sy = -kyv * (-(+q[0] * coskyy * sinhkpz * sinkxx) - (-q[1] * sinkyy * sinhkpz * sinkxx) -
+                                (+q[2] * coskyy * sinhkpz * coskxx) + (+q[4] * coskyy * coshkpz * sinkxx) -
+                                (-q[3] * sinkyy * sinhkpz * coskxx) + (-q[5] * sinkyy * coshkpz * sinkxx) +
+                                (+q[6] * coskyy * coshkpz * coskxx) + (-q[7] * sinkyy * coshkpz * coskxx));
+      const double sz = -kpv * (-(+q[0] * coshkpz * sinkxx * sinkyy) - (+q[1] * coshkpz * sinkxx * coskyy) -
+                                (+q[2] * coshkpz * coskxx * sinkyy) + (+q[4] * sinhkpz * sinkxx * sinkyy) -
+                                (+q[3] * coshkpz * coskxx * coskyy) + (+q[5] * sinhkpz * sinkxx * coskyy) +
+                                (+q[6] * sinhkpz * coskxx * sinkyy) + (+q[7] * sinhkpz * coskxx * coskyy));
+      f[0] += 2.0 * s->elc_kpc[ip] * sx;
+      f[1] += 2.0 * s->elc_kpc[ip] * sy;
+      f[2] += 2.0 * s->elc_kpc[ip] * sz;
+      if (ix == 0) {
+        const double sinhkyz = sinh(kyv * p[2]), coshkyz = cosh(kyv * p[2]);
+        const double *cy = s->elc_chi_y + 4 * iy;
+        const double syy = -kyv * (-(+cy[0] * coskyy * sinhkyz) - (-cy[1] * sinkyy * sinhkyz) +
+                                   (+cy[2] * coskyy * coshkyz) + (-cy[3] * sinkyy * coshkyz));
+        const double syz = -kyv * (-(+cy[0] * sinkyy * coshkyz) - (+cy[1] * coskyy * coshkyz) +
+                                   (+cy[2] * sinkyy * sinhkyz) + (+cy[3] * coskyy * sinhkyz));
+        f[1] += s->elc_kyc[iy] * syy;
+        f[2] += s->elc_kyc[iy] * syz;
+      }
+      ++ip;
+    }
+  }
+  const double sc = 2.0 * s->elc_sum_e_coef * s->elc_ke;
+  for (int d = 0; d < 3; ++d)
+    f[d] *= sc;
+  double g[3];
+  elc_global(s, f, g);
+  for (int d = 0; d < 3; ++d)
+    out[d] = g[d] + s->elc_dip_field[d];
+}
+
+/* Electrostatic_ewald_slab_correction::calculate_acceleration
+ * (electrostatic_ewald_slab_correction.cpp:134-181): initialize, then
+ * force = q_i field(r_i) per owned atom. */
+void orc_elc_acc(orc_sys *s)
+{
+  elc_initialize(s);
+  for (int i = 0; i < s->n; ++i) {
+    const int ti = s->type[i];
+    const double qi = s->charge[ti], mi = s->minv[ti];
+    double e[3];
+    elc_field(s, s->pos + 3 * i, e);
+    for (int d = 0; d < 3; ++d) {
+      const double force = qi * e[d];
+      s->acc[3 * i + d] += force * mi;
+    }
+  }
+}
+
+/* energy (electrostatic_ewald_slab_correction/energy.cpp:32-138): the chi sums
+ * of the last force evaluation, plus 0.5 dipole_coef |dipole_sum|^2. */
+double orc_energy_elc(orc_sys *s)
+{
+  const int kx = s->elc_kx, ky = s->elc_ky;
+  double sum_e = 0.0;
+  for (int ip = 0; ip < kx * ky; ++ip) {
+    const double *q = s->elc_chi_p + 8 * ip;
+    const double sum_chi = -(q[0] * q[0]) - (q[1] * q[1]) - (q[2] * q[2]) + (q[4] * q[4]) - (q[3] * q[3]) +
+                           (q[5] * q[5]) + (q[6] * q[6]) + (q[7] * q[7]);
+    sum_e += 2.0 * s->elc_kpc[ip] * sum_chi;
+  }
+  for (int ix = 0; ix < kx; ++ix) {
+    const double *c = s->elc_chi_x + 4 * ix;
+    const double sum_chi = -(c[0] * c[0]) - (c[1] * c[1]) + (c[2] * c[2]) + (c[3] * c[3]);
+    sum_e += s->elc_kxc[ix] * sum_chi;
+  }
+  for (int iy = 0; iy < ky; ++iy) {
+    const double *c = s->elc_chi_y + 4 * iy;
+    const double sum_chi = -(c[0] * c[0]) - (c[1] * c[1]) + (c[2] * c[2]) + (c[3] * c[3]);
+    sum_e += s->elc_kyc[iy] * sum_chi;
+  }
+  sum_e *= s->elc_ke * s->elc_sum_e_coef;
+  const double *d = s->elc_dip_sum;
+  return sum_e + 0.5 * s->elc_dip_coef * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+
+/* potential(const Vector&) (electrostatic_ewald_slab_correction/potential.cpp:
+ * 30-79) and field(const Vector&) at np points (state of the last force
+ * evaluation). */
+void orc_point_elc(orc_sys *s, int np, const double *pts, double *field, double *pot)
+{
+  const int kx = s->elc_kx, ky = s->elc_ky;
+  for (int k = 0; k < np; ++k) {
+    const double *r = pts + 3 * k;
+    if (field)
+      elc_field(s, r, field + 3 * k);
+    if (!pot)
+      continue;
+    double sum_p = 0;
+    for (int j = 0; j < s->n; ++j) {
+      double dg[3] = {r[0] - s->pos[3 * j], r[1] - s->pos[3 * j + 1], r[2] - s->pos[3 * j + 2]};
+      double p[3];
+      elc_local(s, dg, p);
+      const double qj = s->charge[s->type[j]];
+      double sum_kx = 0, sum_ky = 0, sum_kp = 0;
+      int ip = 0;
+      for (int ix = 0; ix < kx; ++ix) {
+        const double coshkxz = cosh(s->elc_kxv[ix] * p[2]);
+        const double coskxx = cos(s->elc_kxv[ix] * p[0]);
+        sum_kx += s->elc_kxc[ix] * coshkxz * coskxx;
+        for (int iy = 0; iy < ky; ++iy) {
+          const double coshkpz = cosh(s->elc_kpv[ip] * p[2]);
+          const double coskyy = cos(s->elc_kyv[iy] * p[1]);
+          sum_kp += s->elc_kpc[ip] * coshkpz * coskxx * coskyy;
+          if (ix == 0)
+            sum_ky += s->elc_kyc[iy] * cosh(s->elc_kyv[iy] * p[2]) * coskyy;
+          ++ip;
+        }
+      }
+      sum_p += qj * ((2.0 * sum_kp) + sum_kx + sum_ky);
+    }
+    pot[k] = sum_p * s->elc_sum_e_coef * s->elc_ke;
+  }
+}
+
 /* Electrostatic_ewald_r::calculate_acceleration (electrostatic_ewald_r.cpp:115-227):
  * every Verlet-list entry (no cutoff test), exact-zero separation skipped,
  * virial -coef*r^2 when id_i < id_j. */
@@ -713,10 +1243,15 @@ static void ek_structure_factor(orc_sys *s, int k0, int k1)
  * The reference loops k outer, i inner; per atom the k terms are added in the
  * same k order here.  With epsilon_dipole the dipole force is added once per
  * k-vector, as the reference's loop does (electrostatic_ewald_k.cpp:216-219). */
-static void ek_force_pass(orc_sys *s, int i0, int i1)
+static void ek_force_pass_idx(orc_sys *s, int i0, int i1, const int *idx);
+static void ek_force_pass(orc_sys *s, int i0, int i1) { ek_force_pass_idx(s, i0, i1, NULL); }
+
+/* The force pass over atoms idx[i0..i1) (idx NULL: atoms i0..i1). */
+static void ek_force_pass_idx(orc_sys *s, int i0, int i1, const int *idx)
 {
 #pragma omp parallel for schedule(static) num_threads(g_threads)
-  for (int i = i0; i < i1; ++i) {
+  for (int ii = i0; ii < i1; ++ii) {
+    const int i = idx ? idx[ii] : ii;
     const int ti = s->type[i];
     const double qi = s->charge[ti], mi = s->minv[ti];
     const double *p = s->pos + 3 * i;
@@ -771,15 +1306,56 @@ void orc_ewald_k_sample(orc_sys *s, int k0, int k1, int i0, int i1, double *t_s,
   *t_f = t2 - t1;
 }
 
+void orc_forces_sampled(orc_sys *s, int m, const int *idx)
+{
+  orc_reset_acc(s);
+  if (s->lj_on)
+    orc_lj_acc(s);
+  if (s->ljl_on)
+    orc_lj_list_acc(s);
+  if (s->er_on)
+    orc_ewald_r_acc(s);
+  if (s->sr_on)
+    orc_short_range_acc(s);
+  if (s->ek_on) {
+    if (!s->ek_init)
+      ek_init(s);
+    ek_structure_factor(s, 0, s->nk);
+    ek_force_pass_idx(s, 0, m, idx);
+  }
+  if (s->elc_on)
+    orc_elc_acc(s);
+}
+
+int orc_setup_sampled(orc_sys *s, int m, const int *idx)
+{
+  s->virial = 0;
+  if (s->nl_on)
+    nl_init(s);
+  orc_wrap(s);
+  orc_ghosts(s);
+  orc_build(s, 1);
+  orc_forces_sampled(s, m, idx);
+  return 0;
+}
+
+/* The force fields in the order the fixtures' scripts add them to the
+ * simulator (md_simulator.cpp:370-373 evaluates them in that order). */
 void orc_forces(orc_sys *s)
 {
   orc_reset_acc(s);
   if (s->lj_on)
     orc_lj_acc(s);
+  if (s->ljl_on)
+    orc_lj_list_acc(s);
   if (s->er_on)
     orc_ewald_r_acc(s);
+  if (s->sr_on)
+    orc_short_range_acc(s);
   if (s->ek_on)
     orc_ewald_k_acc(s);
+  if (s->elc_on)
+    orc_elc_acc(s);
 }
 
 /* ---------------------------------------------------------------- driver -- */

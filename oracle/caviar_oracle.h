@@ -39,6 +39,21 @@ void orc_set_neighbor(orc_sys *s, double cutoff, double coef, int from_cells, in
                       int all_atom_test, int rebuild_test);
 /* force_field lj_cell_list (lj_cell_list.cpp:33-76); eps/sigma are nt*nt row-major [ti][tj]. */
 void orc_set_lj(orc_sys *s, double rc, const double *eps, const double *sigma);
+/* force_field lj (lj.cpp:42-147 read, 162-339 verify_settings): eps/sig nt*nt
+ * (by array) or eps_atom/sig_atom nt (by atom, Lorentz-Berthelot), cutoff_list
+ * nt*nt or NULL, wca, lambda_s/lambda_e nt*nt or NULL, ignore_intra_molecule,
+ * jump_fix + jump_tol.  Returns 1 if both or neither input forms are given. */
+int orc_set_lj_list(orc_sys *s, double cutoff, const double *eps, const double *sig,
+                    const double *eps_atom, const double *sig_atom, const double *cutoff_list,
+                    int wca, const double *lambda_s, const double *lambda_e,
+                    int ignore_intra_molecule, int jump_fix, double jump_tol);
+/* force_field electrostatic_short_range (electrostatic_short_range.cpp:39-99):
+ * cutoff, k_electrostatic, beta, external_field (3 or NULL), lambda nt*nt or NULL. */
+void orc_set_short_range(orc_sys *s, double cutoff, double ke, double beta, const double ext[3],
+                         const double *lambda);
+/* force_field electrostatic_ewald_slab_correction (electrostatic_ewald_slab_correction.cpp:
+ * 44-126): slab_normal_axis 0/1/2, kx_max, ky_max, lz, k_electrostatic. */
+void orc_set_elc(orc_sys *s, int axis, int kx, int ky, double lz, double ke);
 /* force_field electrostatic_ewald_r (electrostatic_ewald_r.cpp:31-110); lambda nt*nt or NULL. */
 void orc_set_ewald_r(orc_sys *s, double alpha, double ke, const double *lambda);
 /* force_field electrostatic_ewald_k (electrostatic_ewald_k.cpp:31-139).
@@ -61,10 +76,20 @@ int orc_build(orc_sys *s, int update); /* Neighborlist::build */
 void orc_reset_acc(orc_sys *s);
 void orc_lj_acc(orc_sys *s);
 void orc_ewald_r_acc(orc_sys *s);
+void orc_lj_list_acc(orc_sys *s);
+void orc_short_range_acc(orc_sys *s);
+void orc_elc_acc(orc_sys *s);
 void orc_ewald_k_acc(orc_sys *s);
 /* Bounded-sample timing of Ewald_k for the CPU baseline: S(k) over all atoms
  * for k in [k0,k1), and the force pass for atoms [i0,i1) over all k. */
 void orc_ewald_k_sample(orc_sys *s, int k0, int k1, int i0, int i1, double *t_s, double *t_f);
+
+/* orc_forces / orc_setup with the Ewald_k force pass restricted to the m
+ * atoms idx[0..m) (after the full structure factor), for comparisons at sizes
+ * where the whole O(N K) force pass would take too long on the CPU; the other
+ * atoms' accelerations lack their k-space term.  Same arithmetic per atom. */
+void orc_forces_sampled(orc_sys *s, int m, const int *idx);
+int orc_setup_sampled(orc_sys *s, int m, const int *idx);
 
 /* Getters. what: 0 pos, 1 vel, 2 acc, 3 acc_old (3n doubles). */
 int orc_n(const orc_sys *s);
@@ -86,6 +111,8 @@ double orc_energy_r(orc_sys *s);      /* Electrostatic_ewald_r::energy, P7 cell 
 double orc_energy_k(orc_sys *s);      /* Electrostatic_ewald_k::energy (k + self [+ dipole]) */
 double orc_self_energy(orc_sys *s);
 double orc_energy_lj(orc_sys *s);     /* shifted-force LJ energy (no reference counterpart) */
+double orc_energy_elc(orc_sys *s);    /* Electrostatic_ewald_slab_correction::energy */
+void orc_point_elc(orc_sys *s, int np, const double *pts, double *field, double *pot);
 void orc_point_r(orc_sys *s, int np, const double *pts, double *field, double *pot);
 void orc_point_k(orc_sys *s, int np, const double *pts, double *field, double *pot);
 

@@ -11,6 +11,14 @@
 //          ghosts, the Verlet list, accelerations, virial and energies.
 //   mode "run":   the script ends with 'sim run'; the harness dumps the final
 //          positions, velocities, accelerations and the last Verlet list.
+//   mode "mrun":  as "run", but the script has no 'sim run': the harness sets
+//          the molecule indices first (see below), then calls Md_simulator::run.
+// An optional fourth argument names a file of molecule indices, one integer
+// per owned atom in input order, written into atom_struct_owned.molecule_index
+// before setup or run (what the reference's add_molecule commands produce;
+// atoms read from an XYZ file otherwise all carry -1).
+// "+elc_energy" calls Electrostatic_ewald_slab_correction::energy(); "+field"
+// also dumps the slab correction's field/potential as field_elc.
 // Flags appended to the mode: "+er_energy" also calls Electrostatic_ewald_r::
 // energy(), "+ek_energy" calls Electrostatic_ewald_k::energy(), "+sk" dumps
 // S(k) (potential_k_coef_cmplx), "+field" dumps field/potential at the owned
@@ -22,6 +30,7 @@
 #include "caviar/objects/force_field.h"
 #include "caviar/objects/force_field/electrostatic_ewald_k.h"
 #include "caviar/objects/force_field/electrostatic_ewald_r.h"
+#include "caviar/objects/force_field/electrostatic_ewald_slab_correction.h"
 #include "caviar/objects/md_simulator.h"
 #include "caviar/objects/neighborlist.h"
 
@@ -60,11 +69,44 @@ int main(int argc, char **argv)
   auto *sim = oc->md_simulator[0];
   auto *ad = oc->atom_data[0];
 
+  if (argc > 4)
+  {
+    FILE *mf = std::fopen(argv[4], "r");
+    if (!mf)
+    {
+      std::fprintf(stderr, "cannot open %s\n", argv[4]);
+      return 4;
+    }
+    auto &mol = ad->atom_struct_owned.molecule_index;
+    int nmol = 0;
+    for (size_t i = 0; i < mol.size(); ++i)
+    {
+      if (std::fscanf(mf, "%d", &mol[i]) != 1)
+      {
+        std::fprintf(stderr, "%s: too few molecule indices\n", argv[4]);
+        return 4;
+      }
+      if (mol[i] + 1 > nmol)
+        nmol = mol[i] + 1;
+    }
+    std::fclose(mf);
+    // one molecule record per index, with its atoms' ids, as add_molecule
+    // leaves them (atom_data.cpp:706-721 looks them up by index)
+    ad->molecule_struct_owned.resize(nmol);
+    for (size_t i = 0; i < mol.size(); ++i)
+      if (mol[i] > -1)
+        ad->molecule_struct_owned[mol[i]].atom_list.push_back(ad->atom_struct_owned.id[i]);
+  }
+
   if (mode.rfind("setup", 0) == 0)
   {
     sim->initialize();
     ad->reset_virial();
     sim->setup();
+  }
+  else if (mode.rfind("mrun", 0) == 0)
+  {
+    sim->run();
   }
 
   FILE *f = std::fopen(argv[3], "w");
@@ -97,6 +139,20 @@ int main(int argc, char **argv)
   {
     auto *er = dynamic_cast<caviar::force_field::Electrostatic_ewald_r *>(ff);
     auto *ek = dynamic_cast<caviar::force_field::Electrostatic_ewald_k *>(ff);
+    auto *elc = dynamic_cast<caviar::force_field::Electrostatic_ewald_slab_correction *>(ff);
+    if (elc && mode.find("+elc_energy") != std::string::npos)
+      std::fprintf(f, "energy_elc %.17g\n", elc->energy());
+    if (elc && mode.find("+field") != std::string::npos)
+    {
+      std::fprintf(f, "field_elc %zu\n", own.position.size());
+      for (const auto &p : own.position)
+      {
+        Vector<double> r{p.x + 0.1, p.y - 0.2, p.z + 0.3};
+        auto e = elc->field(r);
+        double phi = elc->potential(r);
+        std::fprintf(f, "%.17g %.17g %.17g %.17g\n", e.x, e.y, e.z, phi);
+      }
+    }
     if (er && mode.find("+er_energy") != std::string::npos)
       std::fprintf(f, "energy_r %.17g\n", er->energy());
     if (ek && mode.find("+ek_energy") != std::string::npos)

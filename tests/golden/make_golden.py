@@ -3,7 +3,7 @@
 Run in the development container only (it needs /root/reference):
 
     make -f oracle/ref.mk -j8          # builds oracle/_ref/ref_harness
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [original] [c2] [slab] [ljsr] [elc]   # default: all
 
 For every case it writes the input as an XYZ file plus a CASL script in the
 reference's own syntax (neighborlist basic / force_field ... / md_simulator
@@ -29,8 +29,9 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from caviar_amd import configs  # noqa: E402
-from caviar_amd.system import (EwaldKSettings, EwaldRSettings, LJSettings,  # noqa: E402
-                               NeighborSettings, System)
+from caviar_amd.system import (ELCSettings, EwaldKSettings, EwaldRSettings,  # noqa: E402
+                               LJListSettings, LJSettings, NeighborSettings, ShortRangeSettings,
+                               System)
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 B2 = "/root/reference/benchmark/b2-ewald"
@@ -84,6 +85,41 @@ def casl_for(sysm: System, xyz: str, run_steps: int = 0, dt: float = 0.001) -> s
         L.append("f_lj set_neighborlist neigh")
         L.append("f_lj set_atom_data adata")
         ffs.append("f_lj")
+    if sysm.lj_list is not None:
+        lj = sysm.lj_list
+        L.append("force_field lj f_ljl")
+        L.append(f"f_ljl cutoff {R(lj.cutoff)}")
+
+        def mat(name, a, skip=None):
+            for i in range(nt):
+                for j in range(nt):
+                    v = float(np.asarray(a)[i][j])
+                    if skip is not None and v == skip:
+                        continue  # left unset: the reference's default fills it
+                    L.append(f"f_ljl {name} {i} {j} {R(v)}")
+        if lj.epsilon is not None:
+            mat("epsilon", lj.epsilon, skip=0.0)
+        if lj.sigma is not None:
+            mat("sigma", lj.sigma, skip=0.0)
+        for name, a in (("epsilon_atom", lj.epsilon_atom), ("sigma_atom", lj.sigma_atom)):
+            if a is not None:
+                for i in range(nt):
+                    L.append(f"f_ljl {name} {i} {R(float(a[i]))}")
+        if lj.cutoff_list is not None:
+            mat("cutoff_list", lj.cutoff_list)
+        if lj.wca:
+            L.append("f_ljl wca")
+        if lj.lambda_s is not None:
+            mat("lambda_s", lj.lambda_s, skip=1.0)
+        if lj.lambda_e is not None:
+            mat("lambda_e", lj.lambda_e, skip=1.0)
+        if lj.ignore_intra_molecule:
+            L.append("f_ljl ignore_intra_molecule")
+        if lj.jump_tol is not None:
+            L.append(f"f_ljl jump_tol {R(lj.jump_tol)}")
+        L.append("f_ljl set_neighborlist neigh")
+        L.append("f_ljl set_atom_data adata")
+        ffs.append("f_ljl")
     if sysm.ewald_r is not None:
         e = sysm.ewald_r
         L.append("force_field electrostatic_ewald_r f_er")
@@ -96,6 +132,21 @@ def casl_for(sysm: System, xyz: str, run_steps: int = 0, dt: float = 0.001) -> s
                 for j in range(nt):
                     L.append(f"f_er lambda {i} {j} {R(float(e.lambda_[i][j]))}")
         ffs.append("f_er")
+    if sysm.short_range is not None:
+        sr = sysm.short_range
+        L.append("force_field electrostatic_short_range f_sr")
+        L.append(f"f_sr cutoff {R(sr.cutoff)}")
+        L.append(f"f_sr k_electrostatic {R(sr.k_electrostatic)}")
+        L.append(f"f_sr beta {R(sr.beta)}")
+        ex = [R(float(v)) for v in sr.external_field]
+        L.append(f"f_sr external_field {ex[0]} {ex[1]} {ex[2]}")
+        if sr.lambda_ is not None:
+            for i in range(nt):
+                for j in range(nt):
+                    L.append(f"f_sr lambda {i} {j} {R(float(sr.lambda_[i][j]))}")
+        L.append("f_sr set_neighborlist neigh")
+        L.append("f_sr set_atom_data adata")
+        ffs.append("f_sr")
     if sysm.ewald_k is not None:
         e = sysm.ewald_k
         L.append("force_field electrostatic_ewald_k f_ek")
@@ -108,7 +159,21 @@ def casl_for(sysm: System, xyz: str, run_steps: int = 0, dt: float = 0.001) -> s
         L.append(f"f_ek kz_max {e.kz_max}")
         if e.epsilon_dipole is not None:
             L.append(f"f_ek epsilon_dipole {R(e.epsilon_dipole)}")
+        if e.slab_normal_axis is not None:
+            L.append(f"f_ek slab_normal_axis {'xyz'[e.slab_normal_axis]}")
+            L.append(f"f_ek slab_lz {R(e.slab_lz)}")
         ffs.append("f_ek")
+    if sysm.elc is not None:
+        c = sysm.elc
+        L.append("force_field electrostatic_ewald_slab_correction f_elc")
+        L.append("f_elc set_atom_data adata")
+        L.append("f_elc set_domain dom")
+        L.append(f"f_elc k_electrostatic {R(c.k_electrostatic)}")
+        L.append(f"f_elc kx_max {c.kx_max}")
+        L.append(f"f_elc ky_max {c.ky_max}")
+        L.append(f"f_elc lz {R(c.lz)}")
+        L.append(f"f_elc slab_normal_axis {'xyz'[c.slab_normal_axis]}")
+        ffs.append("f_elc")
     L.append("md_simulator basic sim")
     L.append("sim set_atom_data adata")
     for f in ffs:
@@ -118,8 +183,8 @@ def casl_for(sysm: System, xyz: str, run_steps: int = 0, dt: float = 0.001) -> s
     L.append("sim initial_step 0")
     L.append(f"sim final_step {run_steps}")
     L.append(f"sim dt {R(dt)}")
-    if run_steps:
-        L.append("sim run")
+    if run_steps and sysm.mol is None:
+        L.append("sim run")  # with molecule indices the harness runs it after setting them
     return "\n".join(L) + "\n"
 
 
@@ -159,7 +224,7 @@ def parse(out: str) -> dict:
             a = np.array([[float(x) for x in toks[k + i].split()] for i in range(n)]).reshape(n, 2)
             res[tag] = a
             k += n
-        elif tag in ("field_r", "field_k"):
+        elif tag in ("field_r", "field_k", "field_elc"):
             n = int(head[1])
             res[tag] = np.array([[float(x) for x in toks[k + i].split()] for i in range(n)]).reshape(n, 4)
             k += n
@@ -186,10 +251,20 @@ def settings_json(s: System) -> str:
                         np.asarray(s.ewald_r.lambda_).tolist()}
     if s.ewald_k:
         d["ewald_k"] = vars(s.ewald_k)
+
+    def plain(v):
+        return np.asarray(v).tolist() if isinstance(v, (np.ndarray, list, tuple)) else v
+    if s.lj_list:
+        d["lj_list"] = {k: plain(v) for k, v in vars(s.lj_list).items()}
+    if s.short_range:
+        d["short_range"] = {k: plain(v) for k, v in vars(s.short_range).items()}
+    if s.elc:
+        d["elc"] = vars(s.elc)
     return json.dumps(d)
 
 
-def run_case(name: str, sysm: System, mode: str, run_steps: int = 0, dt: float = 0.001) -> None:
+def run_case(name: str, sysm: System, mode: str, run_steps: int = 0, dt: float = 0.001,
+             sort_pairs: bool = False) -> None:
     with tempfile.TemporaryDirectory() as td:
         xyz = os.path.join(td, "in.xyz")
         write_xyz(xyz, sysm)
@@ -197,19 +272,34 @@ def run_case(name: str, sysm: System, mode: str, run_steps: int = 0, dt: float =
         with open(casl, "w") as f:
             f.write(casl_for(sysm, "in.xyz", run_steps, dt))
         out = os.path.join(td, "out.txt")
-        r = subprocess.run([HARNESS, casl, mode, out], cwd=td, capture_output=True, text=True)
+        cmd = [HARNESS, casl, mode, out]
+        if sysm.mol is not None:
+            molf = os.path.join(td, "mol.txt")
+            np.savetxt(molf, sysm.mol, fmt="%d")
+            cmd.append(molf)
+            if mode == "run":
+                cmd[2] = "mrun"
+        r = subprocess.run(cmd, cwd=td, capture_output=True, text=True)
         if r.returncode != 0 or not os.path.exists(out):
             print(r.stdout[-2000:], r.stderr[-2000:])
             raise RuntimeError(f"harness failed for {name}")
         res = parse(out)
     arrays = {f"in_{k}": v for k, v in (("pos", sysm.pos), ("vel", sysm.vel),
                                          ("types", sysm.types))}
+    if sysm.mol is not None:
+        arrays["in_mol"] = sysm.mol
     arrays["settings"] = np.array(settings_json(sysm))
     arrays["mode"] = np.array(mode)
     arrays["run_steps"] = np.array(run_steps)
     arrays["dt"] = np.array(dt)
+    arrays["pairs_sorted"] = np.array(int(sort_pairs))
     for k, v in res.items():
-        arrays[f"ref_{k}"] = np.asarray(v)
+        v = np.asarray(v)
+        if k == "pairs" and sort_pairs and len(v):
+            # large lists are stored sorted (tests compare sorted sets), which
+            # compresses to a fraction of the reference's stencil order
+            v = v[np.lexsort((v[:, 1], v[:, 0]))]
+        arrays[f"ref_{k}"] = v
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, **arrays)
     print(f"{name}: n={sysm.n} ghosts={len(res.get('ghost_pos', []))} "
@@ -228,9 +318,7 @@ def b2_input():
     return pos, q, energy, forces
 
 
-def main():
-    if not os.path.exists(HARNESS):
-        raise SystemExit("build oracle/_ref/ref_harness first: make -f oracle/ref.mk -j8")
+def original_cases():
     # 1. The reference's own known-answer fixture (benchmark/b2-ewald), kept as
     #    published data next to the reference's run on the same input.
     pos, q, e_pub, f_pub = b2_input()
@@ -285,5 +373,132 @@ def main():
     run_case("rebuild300_run10", rbt, "run", run_steps=10, dt=0.002)
 
 
+
+def c2_cases():
+    # 9. The 8000-ion LJ + Ewald workload itself (rho 0.3, kmax 8): setup, and
+    #    the 200-step NVE run at dt 0.002.
+    c2 = configs.ions(n=8000, kmax=8, seed=1)
+    run_case("ions8000_setup", c2, "setup+er_energy+ek_energy+sk", sort_pairs=True)
+    run_case("ions8000_run200", c2, "run", run_steps=200, dt=0.002, sort_pairs=True)
+
+
+def slab_cases():
+    # 10. Ewald_k slab geometry: a non-periodic axis, the reference's
+    #     slab_normal_axis / slab_lz override of that axis' length
+    #     (electrostatic_ewald_k/initialize.cpp:53-73), along z and along x.
+    sz = configs.ions(n=500, kmax=4, seed=71)
+    sz.bc = np.array([1, 1, 0], dtype=np.int32)
+    sz.ewald_k.kz_max = 9
+    sz.ewald_k.slab_normal_axis = 2
+    sz.ewald_k.slab_lz = 3.0 * float(sz.size[2])
+    run_case("slabz500_setup", sz, "setup+er_energy+ek_energy+sk+field")
+    run_case("slabz500_run15", sz, "run", run_steps=15, dt=0.002)
+    sx = configs.ions(n=400, kmax=3, seed=73)
+    sx.bc = np.array([0, 1, 1], dtype=np.int32)
+    sx.ewald_k.kx_max = 7
+    sx.ewald_k.slab_normal_axis = 0
+    sx.ewald_k.slab_lz = 2.5 * float(sx.size[0])
+    run_case("slabx400_setup", sx, "setup+ek_energy+sk")
+
+
+def molecular_system(nmol: int, rho_mol: float, seed: int) -> System:
+    """nmol three-site molecules (types 0, 1, 2; molecule index = molecule
+    number) in a periodic cube: centres with minimum separation 2.4, sites on a
+    randomly rotated triangle of side 1.0 around each centre."""
+    box = (nmol / rho_mol) ** (1.0 / 3.0)
+    centres = configs.random_nonoverlap(nmol, box, 2.4, seed)
+    rng = np.random.default_rng(seed + 7)
+    tri = np.array([[0.0, 0.57735, 0.0], [-0.5, -0.288675, 0.0], [0.5, -0.288675, 0.0]])
+    pos = []
+    for c in centres:
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        pos.append(c + tri @ q.T)
+    pos = np.mod(np.concatenate(pos), box)
+    n = 3 * nmol
+    types = np.tile(np.arange(3, dtype=np.int32), nmol)
+    mol = np.repeat(np.arange(nmol, dtype=np.int32), 3)
+    vel = configs.maxwell_boltzmann(n, 0.8, np.ones(n), rng)
+    return System(lower=[0, 0, 0], upper=[box] * 3, bc=[1, 1, 1], ghost_cutoff=3.5 * 1.12347 + 0.1,
+                  mass=[1.0, 1.5, 0.8], charge=[0.8, -0.5, -0.3], pos=pos, vel=vel, types=types,
+                  mol=mol, neighbor=NeighborSettings(cutoff=3.5))
+
+
+def ljsr_cases():
+    # 11. force_field lj and electrostatic_short_range (lj.cpp, electrostatic_
+    #     short_range.cpp) over the Verlet list.
+    # (a) epsilon_atom / sigma_atom (Lorentz-Berthelot), short-range Coulomb
+    #     with an external field; no molecules.
+    a = configs.ions(n=400, kmax=1, seed=81, dmin=1.1)
+    a.lj = a.ewald_r = a.ewald_k = None
+    a.lj_list = LJListSettings(cutoff=2.5, epsilon_atom=np.array([1.0, 0.8]),
+                               sigma_atom=np.array([1.0, 1.2]))
+    a.short_range = ShortRangeSettings(cutoff=3.0, k_electrostatic=2.0, beta=1.0,
+                                       external_field=(0.1, -0.05, 0.2))
+    run_case("ljatom_sr400_setup", a, "setup")
+    run_case("ljatom_sr400_run15", a, "run", run_steps=15, dt=0.002)
+    # (b) three-site molecules: an asymmetric epsilon matrix with unset
+    #     entries, WCA cutoffs, lambda_s / lambda_e, ignore_intra_molecule;
+    #     short range with beta 2 and a per-type lambda of same-molecule pairs
+    #     (one zero entry, which skips those pairs).
+    b = molecular_system(120, 0.03, seed=83)
+    b.lj_list = LJListSettings(
+        cutoff=2.5, epsilon=np.array([[1.0, 0.7, 0.0], [0.9, 0.8, 0.5], [0.4, 0.6, 1.2]]),
+        sigma=np.array([[1.0, 1.1, 0.9], [1.1, 1.2, 1.0], [0.9, 1.0, 0.8]]), wca=True,
+        lambda_s=np.array([[1.0, 0.9, 1.0], [0.95, 1.0, 1.0], [1.0, 1.0, 1.1]]),
+        lambda_e=np.array([[0.5, 1.0, 1.0], [1.0, 1.0, 0.8], [1.0, 1.0, 1.0]]),
+        ignore_intra_molecule=True)
+    b.short_range = ShortRangeSettings(
+        cutoff=3.2, k_electrostatic=1.5, beta=2.0, external_field=(0.0, 0.3, -0.1),
+        lambda_=np.array([[0.0, 0.5, 0.5], [0.5, 1.0, 0.25], [0.5, 0.25, 1.0]]))
+    run_case("ljwca_mol360_setup", b, "setup")
+    run_case("ljwca_mol360_run10", b, "run", run_steps=10, dt=0.002)
+    # (c) every field together (lj_cell_list, lj with a cutoff_list and
+    #     jump_tol, Ewald real space, short range, Ewald k-space), added in
+    #     that order.
+    c = configs.ions(n=500, kmax=4, seed=85)
+    c.lj_list = LJListSettings(cutoff=2.0, epsilon=np.array([[0.5, 0.4], [0.4, 0.3]]),
+                               sigma=np.array([[0.9, 1.0], [1.0, 1.1]]),
+                               cutoff_list=np.array([[2.2, 2.6], [2.6, 3.0]]), jump_tol=1.0)
+    c.short_range = ShortRangeSettings(cutoff=2.8, k_electrostatic=0.5, beta=1.5)
+    run_case("allfields500_setup", c, "setup+er_energy+ek_energy")
+    run_case("allfields500_run10", c, "run", run_steps=10, dt=0.002)
+
+
+def elc_cases():
+    # 12. Electrostatic_ewald_slab_correction (ELC) beside Ewald in slab mode:
+    #     ions in a box non-periodic along the slab normal, Ewald_k with the
+    #     normal's length replaced by the slab thickness, ELC over the same
+    #     thickness; along z (setup, energy, fields and a run: the ELC dipole
+    #     field lags one evaluation, which only a run exercises) and along y.
+    z = configs.ions(n=400, kmax=4, seed=91)
+    z.bc = np.array([1, 1, 0], dtype=np.int32)
+    lz = 3.0 * float(z.size[2])
+    z.ewald_k.kz_max = 10
+    z.ewald_k.slab_normal_axis = 2
+    z.ewald_k.slab_lz = lz
+    z.elc = ELCSettings(slab_normal_axis=2, kx_max=4, ky_max=3, lz=lz, k_electrostatic=2.0)
+    run_case("elcz400_setup", z, "setup+elc_energy+ek_energy+field")
+    run_case("elcz400_run12", z, "run", run_steps=12, dt=0.002)
+    y = configs.ions(n=300, kmax=3, seed=93)
+    y.bc = np.array([1, 0, 1], dtype=np.int32)
+    ly = 2.5 * float(y.size[1])
+    y.ewald_k.ky_max = 7
+    y.ewald_k.slab_normal_axis = 1
+    y.ewald_k.slab_lz = ly
+    y.elc = ELCSettings(slab_normal_axis=1, kx_max=3, ky_max=2, lz=ly, k_electrostatic=2.0)
+    run_case("elcy300_run8", y, "run", run_steps=8, dt=0.002)
+
+
+CASES = {"original": original_cases, "c2": c2_cases, "slab": slab_cases, "ljsr": ljsr_cases,
+         "elc": elc_cases}
+
+
+def main(groups):
+    if not os.path.exists(HARNESS):
+        raise SystemExit("build oracle/_ref/ref_harness first: make -f oracle/ref.mk -j8")
+    for g in groups or list(CASES):
+        CASES[g]()
+
+
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

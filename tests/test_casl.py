@@ -42,6 +42,16 @@ def write_case(tmp_path, name):
     return s, ref, meta, str(script)
 
 
+def molecule_args(tmp_path, s, mode):
+    """With molecule indices (set by the reference harness from a file before
+    its run), the driver gets the same file and runs after setting them."""
+    if s.mol is None:
+        return [], mode
+    f = tmp_path / "mol.txt"
+    np.savetxt(f, s.mol, fmt="%d")
+    return ["--molecules", str(f)], ("mrun" if mode == "run" else mode)
+
+
 def run_driver(*args, cwd=None):
     return subprocess.run([DRIVER, *args], capture_output=True, text=True, cwd=cwd, timeout=600)
 
@@ -75,6 +85,35 @@ def test_parse_only_collects_the_fixture_settings(tmp_path, name):
         assert np.array_equal(np.array(d["lj"]["sigma"]), np.asarray(s.lj.sigma))
     else:
         assert "lj" not in d
+    if s.lj_list is not None:
+        g, w = d["lj_list"], s.lj_list
+        assert g["cutoff"] == w.cutoff and g["wca"] == w.wca
+        assert g["ignore_intra_molecule"] == w.ignore_intra_molecule and g["jump_tol"] == w.jump_tol
+        nt = s.ntypes
+
+        def dense(m, fill):
+            out = np.full((nt, nt), fill)
+            for i, row in enumerate(m):
+                out[i, :len(row)] = row
+            return out
+        for key, fill in (("epsilon", 0.0), ("sigma", 0.0), ("cutoff_list", 0.0), ("lambda_s", 1.0),
+                          ("lambda_e", 1.0)):
+            want = getattr(w, key)
+            if want is None:
+                assert g[key] == [], key
+            else:
+                assert np.array_equal(dense(g[key], fill), np.asarray(want)), key
+        for key in ("epsilon_atom", "sigma_atom"):
+            want = getattr(w, key)
+            assert (g[key] == []) if want is None else np.array_equal(g[key], want), key
+    if s.short_range is not None:
+        g, w = d["short_range"], s.short_range
+        assert (g["cutoff"], g["k_electrostatic"], g["beta"]) == (w.cutoff, w.k_electrostatic, w.beta)
+        assert g["external_field"] == list(map(float, w.external_field))
+        if w.lambda_ is None:
+            assert g["lambda_"] is None
+        else:
+            assert np.array_equal(np.array(g["lambda_"]), np.asarray(w.lambda_))
     if s.ewald_r is not None:
         e = d["ewald_r"]
         assert e["alpha"] == s.ewald_r.alpha and e["k_electrostatic"] == s.ewald_r.k_electrostatic
@@ -87,6 +126,8 @@ def test_parse_only_collects_the_fixture_settings(tmp_path, name):
         assert (e["alpha"], e["k_electrostatic"]) == (k.alpha, k.k_electrostatic)
         assert (e["kx_max"], e["ky_max"], e["kz_max"]) == (k.kx_max, k.ky_max, k.kz_max)
         assert e["epsilon_dipole"] == k.epsilon_dipole
+    if s.elc is not None:
+        assert d["elc"] == vars(s.elc)
     assert d["dt"] == meta["dt"] and d["final_step"] == meta["run_steps"]
 
 
@@ -222,10 +263,10 @@ def check_dump(out, s, ref, meta, exact_tol=1e-12):
         assert np.array_equal(got["ghost_id"], ref["ghost_id"])
         v = float(ref["virial"])
         assert abs(got["virial"] - v) <= 1e-12 * max(abs(v), 1e-300) or v == got["virial"]
-    for key in ("energy_r", "energy_k", "energy_k_self"):
+    for key in ("energy_r", "energy_k", "energy_k_self", "energy_elc"):
         if key in ref:
             assert abs(got[key] - ref[key]) <= 1e-12 * abs(ref[key]), key
-    for key in ("sk", "field_r", "field_k"):
+    for key in ("sk", "field_r", "field_k", "field_elc"):
         if key in ref:
             assert F.rel_err(got[key], ref[key]) <= 1e-12, key
 
@@ -237,7 +278,8 @@ def test_driver_reproduces_the_reference_run(gpu, tmp_path, name):
     s, ref, meta, script = write_case(tmp_path, name)
     print(f"[case] caviar_amd_run {name} mode={meta['mode']}", flush=True)
     out = str(tmp_path / "out.txt")
-    r = run_driver(script, meta["mode"], out)
+    extra, mode = molecule_args(tmp_path, s, meta["mode"])
+    r = run_driver(*extra, script, mode, out)
     assert r.returncode == 0, r.stderr
     check_dump(out, s, ref, meta)
 

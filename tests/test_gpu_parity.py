@@ -53,7 +53,13 @@ def test_setup_parity_fp64(gpu, name):
     if "sk" in ref:
         sk = e.sk()
         assert F.rel_err(np.c_[sk.real, sk.imag], ref["sk"]) <= 1e-12
+    if "energy_elc" in ref:
+        assert abs(e.energy_elc() - ref["energy_elc"]) <= 1e-12 * abs(ref["energy_elc"])
     shifted = e.get("pos") + np.array([0.1, -0.2, 0.3])
+    if "field_elc" in ref:
+        f, p = e.point_field("elc", shifted)
+        assert F.rel_err(f, ref["field_elc"][:, :3]) <= 1e-12
+        assert F.rel_err(p, ref["field_elc"][:, 3]) <= 1e-12
     if "field_k" in ref:
         f, p = e.point_field("k", shifted)
         assert F.rel_err(f, ref["field_k"][:, :3]) <= 1e-12
@@ -84,7 +90,9 @@ def test_b2_published_known_answer(gpu):
 
 
 @pytest.mark.parametrize("name", ["ions600_setup", "ions2000_setup", "aniso400_setup",
-                                  "lambda_dipole300_setup", "small200_setup", "lj4000_setup"])
+                                  "lambda_dipole300_setup", "small200_setup", "lj4000_setup",
+                                  "ions8000_setup", "slabz500_setup", "ljatom_sr400_setup",
+                                  "ljwca_mol360_setup", "allfields500_setup", "elcz400_setup"])
 def test_mixed_precision_force_error(gpu, name):
     e64, s, ref, meta = engine_for(name)
     e32, _, _, _ = engine_for(name, "mixed")
@@ -134,8 +142,9 @@ def test_fp64_is_bitwise_reproducible(gpu):
 
 
 def test_against_oracle_generated_inputs(gpu):
-    """Inputs made here (not fixtures): unequal kmax per axis with a slab
-    length, and a box with the lower corner away from the origin."""
+    """Inputs made here (not fixtures): unequal kmax per axis and a box with
+    the lower corner away from the origin (slab geometry has its own reference
+    fixture)."""
     s = configs.ions(n=700, kmax=4, seed=77)
     shift = np.array([-3.0, 5.5, 1.25])
     s.lower = s.lower + shift
@@ -212,11 +221,15 @@ def test_list_capacity_growth_is_transparent(gpu):
     assert F.rel_err(e.get("acc"), o.get("acc")) <= 1e-12
 
 
-@pytest.mark.parametrize("name", ["ions600_setup", "lj256_setup", "lambda_dipole300_setup"])
+@pytest.mark.parametrize("name", ["ions600_setup", "lj256_setup", "lambda_dipole300_setup",
+                                  "allfields500_setup", "ljwca_mol360_setup", "elcz400_setup"])
 def test_per_field_calculate_acceleration(gpu, name):
     """Force_field::calculate_acceleration one field at a time (cav_forces):
-    each field's acceleration against the oracle's same field, and the three
-    added together against the fused force phase (1e-12 relative)."""
+    each field's acceleration against the oracle's same field, and all of them
+    added together against the fused force phase (1e-12 relative).  The slab
+    correction is stateful: its dipole field comes from the previous
+    evaluation's dipole sum (zero at setup), so with it the fields are compared
+    with one more evaluation at the same positions instead of with setup's."""
     s, ref, meta = F.load(name)
     print(f"[case] per-field {name}", flush=True)
     e = Engine(s)
@@ -224,8 +237,12 @@ def test_per_field_calculate_acceleration(gpu, name):
     fused = e.get("acc")
     o = Oracle(s)
     o.setup()
-    fields = [f for f, on in (("lj", s.lj), ("ewald_r", s.ewald_r), ("ewald_k", s.ewald_k)) if on is not None]
-    orc = {"lj": o.lib.orc_lj_acc, "ewald_r": o.lib.orc_ewald_r_acc, "ewald_k": o.lib.orc_ewald_k_acc}
+    fields = [f for f, on in (("lj", s.lj), ("lj_list", s.lj_list), ("ewald_r", s.ewald_r),
+                              ("short_range", s.short_range), ("ewald_k", s.ewald_k),
+                              ("elc", s.elc)) if on is not None]
+    orc = {"lj": o.lib.orc_lj_acc, "ewald_r": o.lib.orc_ewald_r_acc, "ewald_k": o.lib.orc_ewald_k_acc,
+           "lj_list": o.lib.orc_lj_list_acc, "short_range": o.lib.orc_short_range_acc,
+           "elc": o.lib.orc_elc_acc}
     total = np.zeros_like(fused)
     for f in fields:
         e.reset_acceleration()
@@ -238,8 +255,11 @@ def test_per_field_calculate_acceleration(gpu, name):
         total += mine
     e.reset_acceleration()
     e.forces(*fields)
-    assert F.rel_err(e.get("acc"), fused) <= 1e-12
-    assert F.rel_err(total, fused) <= 1e-12
+    if s.elc is None:
+        assert F.rel_err(e.get("acc"), fused) <= 1e-12
+        assert F.rel_err(total, fused) <= 1e-12
+    else:
+        assert F.rel_err(e.get("acc"), total) <= 1e-12
     # the exchange + build pieces reproduce the fused path's ghosts and list
     e.exchange()
     e.neighbor_build(False)
@@ -265,13 +285,16 @@ def test_autotune_keeps_results(gpu):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_concurrent_ewald_k_same_trajectory(gpu, precision):
+@pytest.mark.parametrize("dipole", [False, True])
+def test_concurrent_ewald_k_same_trajectory(gpu, precision, dipole):
     """Ewald_k on its own stream, concurrent with the ghost exchange, list
     build and pair kernel, gives the in-order trajectory bit for bit (the side
-    accelerations are added to the short-range ones once, as in order), over
-    steps that rebuild the list."""
+    accelerations are added to the short-range ones once, then the
+    epsilon_dipole term, as in order), over steps that rebuild the list."""
     s = configs.ions(n=3000, kmax=6, seed=4)
-    print(f"[case] concurrent Ewald_k {precision}", flush=True)
+    if dipole:
+        s.ewald_k.epsilon_dipole = 1.5
+    print(f"[case] concurrent Ewald_k {precision} dipole={dipole}", flush=True)
     out = []
     for overlap in (True, False):
         e = Engine(s.copy(), precision=precision)
@@ -371,3 +394,37 @@ def test_pair_kernel_switch_mid_run(gpu):
     assert e.pair_kernel() == "cells"
     assert F.rel_err(e.get("acc"), o.get("acc")) <= 1e-9
     assert F.rel_err(e.get("pos"), o.get("pos")) <= 1e-9
+
+
+def test_charges_changed_after_setup(gpu):
+    """New type charges set after setup reach the owned atoms and the current
+    ghosts before the next force, energy and point-field call (cav_forces on
+    the existing ghosts and list, no new exchange), as a freshly set-up engine
+    with those charges computes them."""
+    s = configs.ions(n=900, kmax=4, seed=21)
+    print("[case] charges changed after setup", flush=True)
+    e = Engine(s.copy())
+    e.setup()
+    newq = np.array([0.7, -0.7])
+    e.set_parameters(charge=newq)
+    e.reset_acceleration()
+    e.reset_virial()
+    e.forces("ewald_r", "ewald_k")
+    t = s.copy()
+    t.charge = newq
+    f = Engine(t)
+    f.setup()
+    f.reset_acceleration()
+    f.reset_virial()
+    f.forces("ewald_r", "ewald_k")
+    assert F.rel_err(e.get("acc"), f.get("acc")) <= 1e-12
+    assert abs(e.virial() - f.virial()) <= 1e-12 * abs(f.virial())
+    _, er0, ek0 = e.energy(lj=False)
+    _, er1, ek1 = f.energy(lj=False)
+    assert abs(er0 - er1) <= 1e-12 * abs(er1)
+    assert abs(ek0 - ek1) <= 1e-12 * abs(ek1)
+    pts = np.random.default_rng(3).random((16, 3)) * s.size + s.lower
+    for which in ("r", "k"):
+        a, pa = e.point_field(which, pts)
+        b, pb = f.point_field(which, pts)
+        assert F.rel_err(a, b) <= 1e-12 and F.rel_err(pa, pb) <= 1e-12

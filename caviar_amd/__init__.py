@@ -96,6 +96,7 @@ def load_library():
         "cav_kernel_time": [P, ctypes.c_char_p, dp, ip],
         "cav_enable_timing": [P, I],
         "cav_adjoint": [P, dp, dp, dp, dp],
+        "cav_adjoint_noalloc": [P, dp, dp, dp, dp],
         "cav_exchange": [P, ip],
         "cav_batch_create": [I, I, ctypes.POINTER(P)],
         "cav_batch_destroy": [P],
@@ -131,6 +132,8 @@ def load_library():
         fn.restype = I
     lib.cav_last_error.argtypes = [P]
     lib.cav_last_error.restype = ctypes.c_char_p
+    lib.cav_adjoint_noalloc_scratch_bytes.argtypes = []
+    lib.cav_adjoint_noalloc_scratch_bytes.restype = ctypes.c_size_t
     lib.cav_batch_last_error.argtypes = [P]
     lib.cav_batch_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -391,6 +394,18 @@ class Engine:
         gs = np.zeros(nt * nt)
         gq = np.zeros(nt)
         self._check(self.lib.cav_adjoint(self.h, _dp(w), _dp(ge), _dp(gs), _dp(gq)))
+        return ge.reshape(nt, nt), gs.reshape(nt, nt), gq
+
+    def adjoint_noalloc(self, weights):
+        """adjoint() through cav_adjoint_noalloc: the same result, with the
+        workspace in the library's fixed per-GPU scratch array (no device
+        allocation in the call)."""
+        nt = self.system.ntypes
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(self.n, 3)
+        ge = np.zeros(nt * nt)
+        gs = np.zeros(nt * nt)
+        gq = np.zeros(nt)
+        self._check(self.lib.cav_adjoint_noalloc(self.h, _dp(w), _dp(ge), _dp(gs), _dp(gq)))
         return ge.reshape(nt, nt), gs.reshape(nt, nt), gq
 
     def list_stats(self):

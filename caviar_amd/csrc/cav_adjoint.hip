@@ -12,6 +12,8 @@
 // electrostatic_ewald_k.cpp:197-226).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "cav_internal.hpp"
 
@@ -502,6 +504,82 @@ extern "C" int cav_adjoint(cav_ctx *c, const double *w_host, double *g_eps, doub
   std::vector<double> hout(z.out, 0.0);
   CAV_TRY(adjoint_run(c, w_host, own.k, hout.data()));
   adjoint_outputs(c, hout.data(), g_eps, g_sigma, g_charge);
+  return CAV_OK;
+}
+
+// ---- the adjoint without allocation ---------------------------------------
+
+// Workspace of cav_adjoint_noalloc: one array per GPU, held for the life of
+// the process, shared by every call on that GPU.  Its host-side name is not a
+// device address: host code uses only the address hipGetSymbolAddress gives
+// for the current device (kept in adj_scratch_dev).
+constexpr size_t kAdjScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[kAdjScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex adj_scratch_mu;
+static double *adj_scratch_dev[kMaxDevices];
+
+// The device address of this GPU's scratch, looked up once per device with
+// that device current.
+static int adj_scratch(cav_ctx *c, double **out)
+{
+  if (c->device < 0 || c->device >= kMaxDevices) return fail(c, CAV_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(adj_scratch_mu);
+  if (!adj_scratch_dev[c->device]) {
+    double *ptr = nullptr;
+    CAV_HIP(hipSetDevice(c->device));
+    CAV_HIP(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(c, CAV_ERR_HIP, "no device address for the adjoint scratch");
+    adj_scratch_dev[c->device] = ptr;
+  }
+  *out = adj_scratch_dev[c->device];
+  return CAV_OK;
+}
+
+extern "C" size_t cav_adjoint_noalloc_scratch_bytes(void) { return kAdjScratchBytes; }
+
+extern "C" int cav_adjoint_noalloc(cav_ctx *c, const double *w_host, double *g_eps,
+                                   double *g_sigma, double *g_charge)
+{
+  if (!c || !w_host) return CAV_ERR_ARG;
+  CAV_TRY(adjoint_checks(c));
+  if (c->params_dirty)
+    return fail(c, CAV_ERR_STATE,
+                "parameter values changed since the last force evaluation: cav_adjoint_noalloc "
+                "allocates nothing, so call cav_recalculate, cav_forces or cav_adjoint first");
+  const AdjSizes z = adjoint_sizes(c);
+  // each buffer starts on a 256-byte boundary (32 doubles)
+  auto up = [](size_t v) { return (v + 31) / 32 * 32; };
+  const size_t need = up(z.w) + up(z.part) + up(z.out) + up(z.u) + up(z.ss) + up(z.sw) + up(z.f4);
+  if (need * sizeof(double) > kAdjScratchBytes)
+    return fail(c, CAV_ERR_CAPACITY,
+                "cav_adjoint_noalloc needs " + std::to_string(need * sizeof(double)) +
+                    " bytes of workspace, above its fixed " + std::to_string(kAdjScratchBytes) +
+                    "-byte scratch: use cav_adjoint");
+  (void)hipSetDevice(c->device);
+  double *base = nullptr;
+  CAV_TRY(adj_scratch(c, &base));
+  AdjWork k;
+  double *q = base;
+  k.w = q;
+  q += up(z.w);
+  k.part = q;
+  q += up(z.part);
+  k.out = q;
+  q += up(z.out);
+  if (c->ek_on) {
+    k.u = q;
+    q += up(z.u);
+    k.ss = q;
+    q += up(z.ss);
+    k.sw = q;
+    q += up(z.sw);
+    k.f4 = q;
+  }
+  double hout[2 * 16 + 4 + 8];  // nv + 2 nt for nt <= 4
+  CAV_TRY(adjoint_run(c, w_host, k, hout));
+  adjoint_outputs(c, hout, g_eps, g_sigma, g_charge);
   return CAV_OK;
 }
 

@@ -13,6 +13,8 @@
 #ifndef CAVIAR_AMD_H
 #define CAVIAR_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -196,6 +198,24 @@ int cav_point_field(cav_ctx *ctx, int which, int np, const double *pts, double *
  * or this adjoint (owned and ghost charges alike).  NULL outputs skipped. */
 int cav_adjoint(cav_ctx *ctx, const double *weights, double *g_epsilon, double *g_sigma,
                 double *g_charge);
+
+/* ---- the adjoint without allocation -------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight fitting loops, harnesses that
+ * forbid hipMalloc): cav_adjoint's kernels and results, with the call's
+ * workspace (residual weights, partial sums, weighted structure factors,
+ * per-atom k-space field) carved from one fixed device array per GPU of
+ * cav_adjoint_noalloc_scratch_bytes() bytes (32 MiB, held until the process
+ * ends) instead of per-call allocations; the call allocates nothing.
+ * CAV_ERR_CAPACITY when this system's workspace exceeds the array (about
+ * 400k atoms with two types at kmax 16: use cav_adjoint), CAV_ERR_STATE when
+ * parameter values were changed after the last force evaluation (applying
+ * them allocates: call cav_recalculate or cav_forces first).  Every call on
+ * one GPU shares the array and leaves its contents behind: calls that
+ * overlap on one GPU (from several host threads) race. */
+int cav_adjoint_noalloc(cav_ctx *ctx, const double *weights, double *g_epsilon, double *g_sigma,
+                        double *g_charge);
+size_t cav_adjoint_noalloc_scratch_bytes(void);
 
 /* ---- batched replicas (no counterpart in the reference) ------------------ */
 /* Many independent systems of one force field, each with its own atoms and

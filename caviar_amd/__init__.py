@@ -18,9 +18,9 @@ import os
 
 import numpy as np
 
-from .system import (ELCSettings, EwaldKSettings, EwaldRSettings, LJListSettings,  # noqa: F401
-                     LJSettings, NeighborSettings, ShortRangeSettings, System, lj_list_args,
-                     short_range_args)
+from .system import (BerendsenSettings, ELCSettings, EwaldKSettings,  # noqa: F401
+                     EwaldRSettings, LJListSettings, LJSettings, NeighborSettings, NVESettings,
+                     ShortRangeSettings, System, ThermoSettings, lj_list_args, short_range_args)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcaviar_amd.so")
@@ -73,6 +73,11 @@ def load_library():
         "cav_set_lj_list": [P, D, dp, dp, dp, dp, dp, I, dp, dp, I, I, D],
         "cav_set_short_range": [P, D, D, D, dp, dp],
         "cav_set_elc": [P, I, I, I, D, D],
+        "cav_set_thermo": [P, D, I, I, I],
+        "cav_set_berendsen": [P, D, D, D, I, D, D],
+        "cav_set_nve": [P, D, D, D, D, D, I],
+        "cav_set_step_index": [P, L],
+        "cav_thermo": [P, dp, dp, dp],
         "cav_energy_elc": [P, dp],
         "cav_set_ewald_k": [P, D, D, I, I, I, I, D, I, D],
         "cav_upload_atoms": [P, I, ip, ip, ip, dp, dp],
@@ -211,6 +216,18 @@ class Engine:
             c = s.elc
             self._check(self.lib.cav_set_elc(self.h, int(c.slab_normal_axis), int(c.kx_max),
                                              int(c.ky_max), float(c.lz), float(c.k_electrostatic)))
+        if s.thermo is not None:
+            t = s.thermo
+            self._check(self.lib.cav_set_thermo(self.h, float(t.k_b), int(t.n_r_df),
+                                                int(t.temperature_process), int(t.pressure_process)))
+        if s.berendsen is not None:
+            b = s.berendsen
+            self._check(self.lib.cav_set_berendsen(self.h, b.temperature, b.coupling, b.dt, int(b.step),
+                                                   b.lambda_min, b.lambda_max))
+        if s.nve is not None:
+            v = s.nve
+            self._check(self.lib.cav_set_nve(self.h, v.energy_per_dof, v.energy_tot, v.kbt, v.kb,
+                                             v.temperature, int(v.step)))
         ids = np.ascontiguousarray(s.ids, dtype=np.int32)
         typ = np.ascontiguousarray(s.types, dtype=np.int32)
         mol = None if s.mol is None else np.ascontiguousarray(s.mol, dtype=np.int32)
@@ -358,6 +375,15 @@ class Engine:
         self._check(self.lib.cav_point_field(self.h, {"r": 0, "k": 1, "elc": 2}[which], len(pts),
                                              _dp(pts), _dp(f), _dp(p)))
         return f, p
+
+    def thermo(self):
+        """(kinetic energy now, temperature and pressure as last finalized)."""
+        v = [ctypes.c_double() for _ in range(3)]
+        self._check(self.lib.cav_thermo(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_step_index(self, step: int):
+        self._check(self.lib.cav_set_step_index(self.h, int(step)))
 
     def energy_elc(self) -> float:
         """Electrostatic_ewald_slab_correction::energy of the last force evaluation."""

@@ -220,6 +220,7 @@ static void free_all(cav_ctx *c)
                   c->bin_cnt, c->bin_start, c->bin_items, c->d_st_off, c->d_st,  c->scan_tmp,
                   c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
                   c->d_ljl,   c->d_sr_lambda, c->d_elc_k, c->d_elc_chi, c->d_elc_dip,
+                  c->d_thermo, c->d_mass,
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
                   c->d_W,     c->kacc,      c->order,    c->ord_blk,  c->xq,
                   c->tid,     c->d_raw_eps, c->d_raw_sig, c->ztab, c->bmat_h, c->cpos, c->sk_sum,
@@ -238,7 +239,9 @@ static int upload_params(cav_ctx *c)
   const int nt = c->nt, nt2 = nt * nt;
   CAV_TRY(dmalloc(c, &c->d_minv, nt));
   CAV_TRY(dmalloc(c, &c->d_charge, nt));
+  CAV_TRY(dmalloc(c, &c->d_mass, nt));
   CAV_HIP(hipMemcpyAsync(c->d_minv, c->minv.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->stream));
+  CAV_HIP(hipMemcpyAsync(c->d_mass, c->mass.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->stream));
   CAV_HIP(hipMemcpyAsync(c->d_charge, c->charge.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->stream));
   // LJ table blocks: [c12 | c6 | cshift | u12 | u6 | uc], see the pair kernels
   std::vector<double> lj(6 * (size_t)std::max(nt2, 1), 0.0);
@@ -794,6 +797,75 @@ int cav_set_elc(cav_ctx *c, int axis, int kx, int ky, double lz, double ke)
   return CAV_OK;
 }
 
+int cav_set_thermo(cav_ctx *c, double k_b, int n_r_df, int temperature_process, int pressure_process)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (n_r_df < -1 || n_r_df > 6) return fail(c, CAV_ERR_ARG, "n_r_df must be -1 or 0..6");
+  c->k_b = k_b;
+  c->n_r_df = n_r_df;
+  c->temperature_process = temperature_process != 0;
+  c->pressure_process = pressure_process != 0;
+  c->thermo_on = c->temperature_process || c->pressure_process || c->ber_on || c->nve_on;
+  return CAV_OK;
+}
+
+int cav_set_berendsen(cav_ctx *c, double temperature, double coupling, double dt, int step,
+                      double lambda_min, double lambda_max)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (temperature <= 0 || coupling <= 0 || dt <= 0 || step <= 0 || lambda_min <= 0 || lambda_max <= 0)
+    return fail(c, CAV_ERR_ARG, "berendsen: temperature, coupling, dt, step and lambdas have to be positive");
+  c->ber_on = true;
+  c->ber_t0 = temperature;
+  c->ber_coupling = coupling;
+  c->ber_dt = dt;
+  c->ber_step = step;
+  c->ber_lmin = lambda_min;
+  c->ber_lmax = lambda_max;
+  c->thermo_on = true;
+  return CAV_OK;
+}
+
+int cav_set_nve(cav_ctx *c, double energy_per_dof, double energy_tot, double kbt, double kb,
+                double temperature, int step)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (step <= 0) return fail(c, CAV_ERR_ARG, "step have to non-negative.");
+  // Nve::verify_settings (constraint/nve.cpp:97-132)
+  const bool kbt_set = kbt > 0, t_set = temperature > 0, kb_set = kb > 0;
+  if ((t_set && kb_set) != (t_set || kb_set)) return fail(c, CAV_ERR_ARG, "kb and temperature should be set together.");
+  const bool e_set = energy_per_dof > 0 || energy_tot > 0, kt_set = (t_set && kb_set) || kbt_set;
+  if (!(e_set || kt_set)) return fail(c, CAV_ERR_ARG, "nve: an energy or a temperature has to be set");
+  if (e_set && kt_set) return fail(c, CAV_ERR_ARG, "Only one of 'energy' and 'temperature' can be set.");
+  if (energy_per_dof > 0 && energy_tot > 0)
+    return fail(c, CAV_ERR_ARG, "Only one of 'energy_per_dof' and 'energy_tot' can be set.");
+  if (kbt_set && t_set && kb_set) return fail(c, CAV_ERR_ARG, "Only one of 'kbt' and 'temperatue kb' can be set.");
+  c->nve_on = true;
+  c->nve_e_dof = energy_per_dof;
+  c->nve_e_tot = energy_tot;
+  c->nve_kbt = kbt;
+  c->nve_kb = kb;
+  c->nve_t = temperature;
+  c->nve_step = step;
+  c->thermo_on = true;
+  return CAV_OK;
+}
+
+int cav_set_step_index(cav_ctx *c, long step)
+{
+  if (!c) return CAV_ERR_ARG;
+  c->step_index = step;
+  return CAV_OK;
+}
+
+int cav_thermo(cav_ctx *c, double *kinetic_energy, double *temperature, double *pressure)
+{
+  if (!c) return CAV_ERR_ARG;
+  if (!c->setup_done) return fail(c, CAV_ERR_STATE, "cav_setup must run first");
+  (void)hipSetDevice(c->device);
+  return thermo_read(c, kinetic_energy, temperature, pressure);
+}
+
 int cav_set_ewald_r(cav_ctx *c, double alpha, double ke, const double *lambda)
 {
   if (!c) return CAV_ERR_ARG;
@@ -1011,6 +1083,7 @@ int cav_setup(cav_ctx *c)
     }
   }
   if (c->elc_on) CAV_TRY(elc_plan(c));
+  CAV_TRY(thermo_alloc(c));
   // initial list capacity from the mean density, refined by the first build
   if (c->nl_on) {
     const double vol = c->L[0] * c->L[1] * c->L[2];
@@ -1033,6 +1106,7 @@ int cav_setup(cav_ctx *c)
     if (!redo) break;
   }
   if (c->h_flags[F_OVERFLOW]) return fail(c, CAV_ERR_CAPACITY, "Verlet list capacity could not be met");
+  CAV_TRY(thermo_temperature(c));  // Md_simulator::setup (md_simulator.cpp:543)
   harvest_timers(c);
   c->setup_done = true;
   c->list_valid = c->nl_on;
@@ -1085,6 +1159,7 @@ int cav_step(cav_ctx *c, double dt, int nsteps)
                                                          c->aoz);
       }
       CAV_HIP(hipGetLastError());
+      CAV_TRY(thermo_end_of_step(c, c->step_index));
       CAV_TRY(sync_flags(c));
       harvest_timers(c);
       if (c->h_flags[F_NAN]) {
@@ -1101,6 +1176,7 @@ int cav_step(cav_ctx *c, double dt, int nsteps)
       }
     }
     if (redo) return fail(c, CAV_ERR_CAPACITY, "Verlet list capacity could not be met");
+    ++c->step_index;
   }
   return CAV_OK;
 }

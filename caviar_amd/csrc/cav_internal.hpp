@@ -136,6 +136,18 @@ struct cav_ctx {
   bool sr_on = false, sr_lambda_set = false;
   double sr_rc = 0, sr_ke = 1, sr_beta = 1, sr_ext[3] = {0, 0, 0};
   std::vector<double> sr_lambda;
+  // temperature / pressure (atom_data k_b, n_r_df, temperature_process,
+  // pressure_process) and the velocity-rescaling constraints of a step
+  double k_b = -1.0;
+  int n_r_df = -1;
+  bool temperature_process = false, pressure_process = false, thermo_on = false;
+  bool ber_on = false, nve_on = false;
+  int ber_step = 1, nve_step = 1;
+  double ber_t0 = 0, ber_coupling = 1, ber_dt = 1, ber_lmin = 0.8, ber_lmax = 1.25;
+  double nve_e_dof = -1, nve_e_tot = -1, nve_kbt = -1, nve_kb = -1, nve_t = -1;
+  long step_index = 0;          // Md_simulator::current_step of the next step
+  double *d_thermo = nullptr;   // scalars and per-block partials
+  double *d_mass = nullptr;     // per-type mass
   // Electrostatic_ewald_slab_correction (ELC)
   bool elc_on = false;
   int elc_axis = 2, elc_kx = 1, elc_ky = 1;
@@ -353,6 +365,10 @@ int launch_lj_cellwalk(cav_ctx *c);
 int launch_zero_acc(cav_ctx *c);
 int launch_ewald_k(cav_ctx *c);
 int launch_dipole(cav_ctx *c);  // epsilon_dipole term into the accelerations
+int thermo_alloc(cav_ctx *c);     // thermo scalars and partials (setup)
+int thermo_temperature(cav_ctx *c);  // finalize_temperature (with temperature_process)
+int thermo_end_of_step(cav_ctx *c, long ts);  // temperature, thermostats, pressure
+int thermo_read(cav_ctx *c, double *ke, double *temp, double *press);
 int elc_plan(cav_ctx *c);        // ELC k grids and buffers (setup)
 int launch_elc(cav_ctx *c);      // ELC chi sums + forces into the accelerations
 int elc_rewind(cav_ctx *c);       // ELC dipole state back before a re-evaluation

@@ -7,13 +7,14 @@
 // neighborlist basic, force_field lj_cell_list / lj / electrostatic_ewald_r /
 // electrostatic_short_range / electrostatic_ewald_k, md_simulator basic ... run).  With MODE and OUT the
 // state is written to OUT as text:
-//   MODE "setup[+er_energy][+ek_energy][+elc_energy][+sk][+field]": after the script,
+//   MODE "setup[+er_energy][+ek_energy][+elc_energy][+sk][+field][+thermo]": after the script,
 //        Md_simulator::setup (wrap, ghosts, list, one force evaluation) unless
 //        a run already set up, then owned positions / velocities /
 //        accelerations, ghosts, virial, the Verlet list and the requested
 //        energies, S(k) and point fields (at the owned positions shifted by
 //        (0.1, -0.2, 0.3)).
-//   MODE "run": the same dump after the script's own run.
+//   MODE "run[+thermo]": the same dump after the script's own run ("+thermo":
+//        Atom_data::kinetic_energy, temperature and pressure as well).
 //   MODE "mrun": the script has no run: the simulator's run is called after
 //        the molecule indices are set (--molecules), then the dump.
 // --molecules FILE sets atom_struct_owned.molecule_index from FILE (one integer
@@ -192,6 +193,11 @@ static void dump(Md_simulator *sim, const std::string &mode, const char *out)
     for (size_t i = 0; i < nl.size(); ++i)
       for (unsigned j : nl[i]) std::fprintf(f, "%zu %u\n", i, j);
   }
+  if (mode.find("+thermo") != std::string::npos) {
+    std::fprintf(f, "kinetic_energy %.17g\n", ad->kinetic_energy());
+    std::fprintf(f, "temperature %.17g\n", ad->temperature());
+    std::fprintf(f, "pressure %.17g\n", ad->pressure());
+  }
   for (Force_field *ff : sim->force_field) {
     auto *er = dynamic_cast<force_field::Electrostatic_ewald_r *>(ff);
     auto *ek = dynamic_cast<force_field::Electrostatic_ewald_k *>(ff);
@@ -291,10 +297,10 @@ int main(int argc, char **argv)
       const std::string mode = pos[1];
       if (mode.rfind("setup", 0) == 0) {
         if (!sim->initialized) sim->setup();
-      } else if (mode == "mrun") {
+      } else if (mode.rfind("mrun", 0) == 0) {
         sim->run();
-      } else if (mode != "run")
-        throw Error("MODE is setup[+...], run or mrun");
+      } else if (mode.rfind("run", 0) != 0)
+        throw Error("MODE is setup[+...], run[+...] or mrun[+...]");
       dump(sim, mode, pos[2].c_str());
     }
   } catch (const std::exception &e) {

@@ -43,7 +43,8 @@ bool Interpreter::command(casl::Parser &p)
     define_variable(p, c == "int");
     return true;
   }
-  if (c == "domain" || c == "atom_data" || c == "neighborlist" || c == "force_field" || c == "md_simulator") {
+  if (c == "domain" || c == "atom_data" || c == "neighborlist" || c == "force_field" || c == "md_simulator" ||
+      c == "constraint" || c == "writer") {
     create(p, c);
     return true;
   }
@@ -58,6 +59,10 @@ bool Interpreter::command(casl::Parser &p)
   else if (auto it = objects.force_field.find(c); it != objects.force_field.end())
     in_file = it->second->read(&p);
   else if (auto it = objects.md_simulator.find(c); it != objects.md_simulator.end())
+    in_file = it->second->read(&p);
+  else if (auto it = objects.constraint.find(c); it != objects.constraint.end())
+    in_file = it->second->read(&p);
+  else if (auto it = objects.writer.find(c); it != objects.writer.end())
     in_file = it->second->read(&p);
   else
     p.fail("Invalid command or object name: " + c +
@@ -138,6 +143,21 @@ void Interpreter::create(casl::Parser &p, const std::string &base)
       p.fail(undefined);
     o->objects = &objects;
     objects.force_field[nm] = std::move(o);
+  } else if (base == "constraint") {
+    std::unique_ptr<Constraint> o;
+    if (ty == "berendsen")
+      o = std::make_unique<constraint::Berendsen>();
+    else if (ty == "nve")
+      o = std::make_unique<constraint::Nve>();
+    else
+      p.fail(undefined);
+    o->objects = &objects;
+    objects.constraint[nm] = std::move(o);
+  } else if (base == "writer") {
+    if (ty != "atom_data") p.fail(undefined);
+    auto o = std::make_unique<writer::Atom_data>();
+    o->objects = &objects;
+    objects.writer[nm] = std::move(o);
   } else {
     if (ty != "basic") p.fail(undefined);
     auto o = std::make_unique<Md_simulator>();

@@ -6,8 +6,9 @@
 //
 // Accepted creations: domain box, atom_data basic, neighborlist basic (also as
 // cell_list = cells + Verlet list from cells, verlet_list = brute force),
-// force_field lj_cell_list | electrostatic_ewald_r | electrostatic_ewald_k,
-// md_simulator basic.  Anything else is reported as an error naming the
+// force_field lj_cell_list | lj | electrostatic_ewald_r | electrostatic_short_range |
+// electrostatic_ewald_k | electrostatic_ewald_slab_correction, constraint berendsen | nve,
+// writer atom_data, md_simulator basic.  Anything else is reported as an error naming the
 // command, the way the reference reports an unknown command.
 #pragma once
 

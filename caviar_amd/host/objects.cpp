@@ -2,6 +2,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <random>
 
 namespace caviar_amd {
 
@@ -45,6 +48,14 @@ Neighborlist *Objects::find_neighborlist(const std::string &n, casl::Parser *p)
 Force_field *Objects::find_force_field(const std::string &n, casl::Parser *p)
 {
   return find_in(force_field, n, "force_field", p);
+}
+Constraint *Objects::find_constraint(const std::string &n, casl::Parser *p)
+{
+  return find_in(constraint, n, "constraint", p);
+}
+writer::Atom_data *Objects::find_writer(const std::string &n, casl::Parser *p)
+{
+  return find_in(writer, n, "writer", p);
 }
 
 // Reads "keyword value ..." pairs until the end of the line, as every
@@ -152,6 +163,21 @@ bool Atom_data::read(casl::Parser *p)
     } else if (t == "k_b") {
       k_b = p->real("k_b");
       if (k_b < 0.0) p->fail("k_b have to non-negative.");
+    } else if (t == "n_r_df") {
+      n_r_df = (int)p->integer("n_r_df");
+    } else if (t == "temperature_process") {
+      temperature_process = p->integer("temperature_process") != 0;
+    } else if (t == "pressure_process") {
+      pressure_process = p->integer("pressure_process") != 0;
+    } else if (t == "add_random_velocity") {
+      // atom_data.cpp:280-288: seed, amplitude (its absolute value), then the
+      // rest of the line is not read
+      const long seed = p->integer("seed");
+      double amp = p->real("amplitude");
+      if (amp < 0) amp = -amp;
+      add_random_velocity((unsigned)seed, amp);
+      p->end_of_line();
+      done = true;
     } else if (t == "add_type_mass") {
       const long i = p->integer("type");
       const double m = p->real("mass");
@@ -248,6 +274,56 @@ void Atom_data::add_xyz_data_file(casl::Parser *p)
   }
 }
 
+void Atom_data::add_random_velocity(unsigned seed, double amplitude)
+{
+  std::mt19937 mt(seed);
+  std::uniform_real_distribution<double> dist(-amplitude, amplitude);
+  auto &o = atom_struct_owned;
+  for (auto &v : o.velocity) {
+    v.x = dist(mt);
+    v.y = dist(mt);
+    v.z = dist(mt);
+  }
+  // owned_velocity_cm (kinetic_energy.cpp:413-462): sum v m / sum m
+  double f[3] = {0, 0, 0}, msum = 0;
+  const auto &mass = atom_type_params.mass;
+  for (size_t i = 0; i < o.velocity.size(); ++i) {
+    const double m = o.type[i] < mass.size() ? mass[o.type[i]] : 0.0;
+    msum += m;
+    f[0] += o.velocity[i].x * m;
+    f[1] += o.velocity[i].y * m;
+    f[2] += o.velocity[i].z * m;
+  }
+  const Vector cm{f[0] / msum, f[1] / msum, f[2] / msum};
+  for (size_t i = 0; i < o.velocity.size(); ++i)
+    if (o.type[i] < mass.size() && mass[o.type[i]] != 0.0) {
+      o.velocity[i].x -= cm.x;
+      o.velocity[i].y -= cm.y;
+      o.velocity[i].z -= cm.z;
+    }
+}
+
+double Atom_data::kinetic_energy()
+{
+  double ke = 0.0;
+  check(device(), cav_thermo(device(), &ke, nullptr, nullptr), "cav_thermo");
+  return ke;
+}
+
+double Atom_data::temperature()
+{
+  double t = 0.0;
+  check(device(), cav_thermo(device(), nullptr, &t, nullptr), "cav_thermo");
+  return t;
+}
+
+double Atom_data::pressure()
+{
+  double p = 0.0;
+  check(device(), cav_thermo(device(), nullptr, nullptr, &p), "cav_thermo");
+  return p;
+}
+
 void Atom_data::verify_settings()
 {
   if (!domain) throw Error("atom_data: domain = nullptr");
@@ -280,6 +356,9 @@ void Atom_data::upload()
     if (t < (int)atom_type_params.charge.size()) charge[t] = atom_type_params.charge[t];
   }
   check(c, cav_set_types(c, nt, mass.data(), charge.data()), "cav_set_types");
+  if (temperature_process && k_b < 0) throw Error("atom_data: k_b is not set.");
+  check(c, cav_set_thermo(c, k_b, n_r_df, temperature_process ? 1 : 0, pressure_process ? 1 : 0),
+        "cav_set_thermo");
   const auto &o = atom_struct_owned;
   const int n = (int)o.position.size();
   std::vector<int> id(n), type(n), mol(n);
@@ -869,6 +948,200 @@ Vector Electrostatic_ewald_slab_correction::field(const Vector &r)
 
 }  // namespace force_field
 
+// -------------------------------------------------------------- constraints --
+
+namespace constraint {
+
+bool Berendsen::read(casl::Parser *p)
+{
+  return read_commands(p, [&](const std::string &t, bool &done) {
+    if (t == "temperature") {
+      temperature = p->real("temperature");
+      if (temperature <= 0.0) p->fail("Temperature have to non-negative.");
+    } else if (t == "coupling") {
+      coupling = p->real("coupling");
+      if (coupling <= 0.0) p->fail("coupling have to non-negative.");
+    } else if (t == "dt") {
+      dt = p->real("dt");
+      if (dt <= 0.0) p->fail("dt have to non-negative.");
+    } else if (t == "step") {
+      step = (int)p->integer("step");
+      if (step <= 0) p->fail("step have to non-negative.");
+    } else if (t == "lambda_min") {
+      lambda_min = p->real("lambda_min");
+      if (lambda_min <= 0.0) p->fail("lambda_min have to non-negative.");
+    } else if (t == "lambda_max") {
+      lambda_max = p->real("lambda_max");
+      if (lambda_max <= 0.0) p->fail("lambda_max have to non-negative.");
+    } else if (t == "set_atom_data" || t == "atom_data")
+      atom_data = objects->find_atom_data(p->name("atom_data"), p);
+    else
+      return false;
+    return true;
+  });
+}
+
+void Berendsen::verify_settings()
+{
+  if (!atom_data) throw Error("berendsen: atom_data = nullptr");
+  if (dt < 0.0) throw Error("berendsen: dt is not set.");
+  if (coupling < 0.0) throw Error("berendsen: coupling is not set.");
+}
+
+void Berendsen::init()
+{
+  verify_settings();
+  cav_ctx *c = atom_data->device();
+  check(c, cav_set_berendsen(c, temperature, coupling, dt, step, lambda_min, lambda_max), "cav_set_berendsen");
+}
+
+bool Nve::read(casl::Parser *p)
+{
+  return read_commands(p, [&](const std::string &t, bool &done) {
+    auto positive = [&](double &v, const char *what) {
+      v = p->real(what);
+      if (v <= 0.0) p->fail(std::string(what) + " have to non-negative.");
+    };
+    if (t == "energy_per_dof")
+      positive(energy_per_dof, "energy_per_dof");
+    else if (t == "energy_tot")
+      positive(energy_tot, "energy_tot");
+    else if (t == "kb")
+      positive(kb, "kb");
+    else if (t == "kbt")
+      positive(kbt, "kbt");
+    else if (t == "temperature")
+      positive(temperature, "temperature");
+    else if (t == "step") {
+      step = (int)p->integer("step");
+      if (step <= 0) p->fail("step have to non-negative.");
+    } else if (t == "set_atom_data" || t == "atom_data")
+      atom_data = objects->find_atom_data(p->name("atom_data"), p);
+    else
+      return false;
+    return true;
+  });
+}
+
+void Nve::verify_settings()
+{
+  if (!atom_data) throw Error("nve: atom_data = nullptr");
+}
+
+void Nve::init()
+{
+  verify_settings();
+  cav_ctx *c = atom_data->device();
+  check(c, cav_set_nve(c, energy_per_dof, energy_tot, kbt, kb, temperature, step), "cav_set_nve");
+}
+
+}  // namespace constraint
+
+// ------------------------------------------------------------------- writer --
+
+namespace writer {
+
+struct Atom_data::Files {
+  std::ofstream xyz, energy, temperature, pressure;
+};
+
+Atom_data::~Atom_data() { delete files_; }
+
+bool Atom_data::read(casl::Parser *p)
+{
+  return read_commands(p, [&](const std::string &t, bool &done) {
+    auto steps = [&](long &v, bool &on) {
+      v = p->integer(t);
+      on = v > 0;
+    };
+    if (t == "set_atom_data" || t == "atom_data")
+      atom_data = objects->find_atom_data(p->name("atom_data"), p);
+    else if (t == "set_domain" || t == "domain")
+      (void)objects->find_domain(p->name("domain"), p);
+    else if (t == "xyz_step")
+      steps(xyz_step, output_xyz);
+    else if (t == "energy_step")
+      steps(energy_step, output_energy);
+    else if (t == "temperature_step")
+      steps(temperature_step, output_temperature);
+    else if (t == "pressure_step")
+      steps(pressure_step, output_pressure);
+    else if (t == "file_name_xyz")
+      file_name_xyz = p->value().text;
+    else if (t == "file_name_energy")
+      file_name_energy = p->value().text;
+    else if (t == "file_name_temperature")
+      file_name_temperature = p->value().text;
+    else if (t == "file_name_pressure")
+      file_name_pressure = p->value().text;
+    else if (t == "xyz_output_id")
+      xyz_output_id = p->integer(t) != 0;
+    else if (t == "xyz_output_velocity")
+      xyz_output_velocity = p->integer(t) != 0;
+    else if (t == "xyz_output_acceleration")
+      xyz_output_acceleration = p->integer(t) != 0;
+    else if (t == "xyz_precision") {
+      xyz_precision = (int)p->integer(t);
+      if (xyz_precision < 1 || xyz_precision > 17) p->fail("xyz_precision: 1..17");
+    } else
+      return false;
+    return true;
+  });
+}
+
+void Atom_data::open()
+{
+  if (!atom_data) throw Error("writer atom_data: atom_data = nullptr");
+  files_ = new Files;
+  auto path = [&](const std::string &f, const char *ext) { return (dir.empty() ? f : dir + "/" + f) + ext; };
+  if (output_xyz) files_->xyz.open(path(file_name_xyz, ".xyz"));
+  if (output_energy) files_->energy.open(path(file_name_energy, ".txt"));
+  if (output_temperature) files_->temperature.open(path(file_name_temperature, ".txt"));
+  if (output_pressure) files_->pressure.open(path(file_name_pressure, ".txt"));
+}
+
+bool Atom_data::wants(long i) const
+{
+  return (output_xyz && i % xyz_step == 0) || (output_energy && i % energy_step == 0) ||
+         (output_temperature && i % temperature_step == 0) || (output_pressure && i % pressure_step == 0);
+}
+
+// write_serial (writer/atom_data.cpp:678-720) for the outputs above, with the
+// owned arrays as the last step left them.
+void Atom_data::write(long i, double t)
+{
+  if (!files_) open();
+  if (output_xyz && i % xyz_step == 0) {
+    // dump_xyz_serial (writer/atom_data/dump_xyz_serial.cpp:33-71)
+    atom_data->sync_from_device();
+    const auto &o = atom_data->atom_struct_owned;
+    std::ostream &f = files_->xyz;
+    f << std::setprecision(xyz_precision);
+    f << o.position.size() << "\nAtom\n";
+    for (size_t k = 0; k < o.position.size(); ++k) {
+      f << o.type[k];
+      if (xyz_output_id) f << " " << o.id[k];
+      f << " " << o.position[k].x << " " << o.position[k].y << " " << o.position[k].z;
+      if (xyz_output_velocity) f << " " << o.velocity[k].x << " " << o.velocity[k].y << " " << o.velocity[k].z;
+      if (xyz_output_acceleration)
+        f << " " << o.acceleration[k].x << " " << o.acceleration[k].y << " " << o.acceleration[k].z;
+      f << "\n";
+    }
+    f << std::flush;
+    f << std::setprecision(6);
+  }
+  // dump_energy_serial, dump_temperature_serial, dump_pressure_serial
+  // (writer/atom_data/dump_energy.cpp:44-139)
+  if (output_energy && i % energy_step == 0)
+    files_->energy << i << " " << t << " " << atom_data->kinetic_energy() << std::endl;
+  if (output_temperature && i % temperature_step == 0)
+    files_->temperature << i << " " << t << " " << atom_data->temperature() << std::endl;
+  if (output_pressure && i % pressure_step == 0)
+    files_->pressure << i << " " << t << " " << atom_data->pressure() << std::endl;
+}
+
+}  // namespace writer
+
 // ------------------------------------------------------------ Md_simulator --
 
 bool Md_simulator::read(casl::Parser *p)
@@ -880,6 +1153,10 @@ bool Md_simulator::read(casl::Parser *p)
       force_field.push_back(objects->find_force_field(p->name("force_field"), p));
     else if (t == "add_neighborlist" || t == "neighborlist")
       neighborlist.push_back(objects->find_neighborlist(p->name("neighborlist"), p));
+    else if (t == "add_constraint" || t == "constraint")
+      constraint.push_back(objects->find_constraint(p->name("constraint"), p));
+    else if (t == "add_writer" || t == "writer")
+      writer.push_back(objects->find_writer(p->name("writer"), p));
     else if (t == "integrator_type") {
       casl::Token it = p->value();
       if (it.kind != casl::Kind::identifier || it.text != "velocity_verlet")
@@ -895,10 +1172,9 @@ bool Md_simulator::read(casl::Parser *p)
       initial_step = current_step;
     } else if (t == "run")
       run();
-    else if (t == "step") {
-      ++current_step;
+    else if (t == "step")
       step();
-    } else
+    else
       return false;
     return true;
   });
@@ -920,6 +1196,17 @@ void Md_simulator::verify_settings()
   }
   for (Neighborlist *n : neighborlist)
     if (n->atom_data != atom_data) throw Error("neighborlist: must use the simulator's atom_data");
+  int ber = 0, nve = 0;
+  for (Constraint *c : constraint) {
+    if (c->atom_data != atom_data) throw Error(std::string(c->type_name()) + ": must use the simulator's atom_data");
+    ber += dynamic_cast<constraint::Berendsen *>(c) != nullptr;
+    nve += dynamic_cast<constraint::Nve *>(c) != nullptr;
+  }
+  if (ber > 1 || nve > 1) throw Error("md_simulator: one berendsen and one nve constraint at most on this path");
+  if (ber && nve)
+    for (size_t k = 0; k < constraint.size(); ++k)
+      if (dynamic_cast<constraint::Nve *>(constraint[k]))
+        if (k == 0) throw Error("md_simulator: with both, berendsen must come before nve on this path");
 }
 
 bool Md_simulator::dry_run = false;
@@ -937,10 +1224,13 @@ void Md_simulator::setup()
   atom_data->upload();
   for (Neighborlist *n : neighborlist) n->init();
   for (Force_field *f : force_field) f->init();
+  for (Constraint *k : constraint) k->init();
   cav_ctx *c = atom_data->device();
   check(c, cav_setup(c), "cav_setup");
   initialized = true;
   atom_data->sync_from_device();
+  time = dt * initial_step;  // md_simulator.cpp:397
+  for (writer::Atom_data *w : writer) w->write(0, 0.0);  // md_simulator.cpp:545-546
 }
 
 void Md_simulator::advance(long nsteps)
@@ -950,7 +1240,23 @@ void Md_simulator::advance(long nsteps)
   steps_requested += nsteps;
   if (dry_run) return;
   cav_ctx *c = atom_data->device();
-  check(c, cav_step(c, dt, (int)nsteps), "cav_step");
+  if (writer.empty()) {
+    check(c, cav_set_step_index(c, current_step), "cav_set_step_index");
+    check(c, cav_step(c, dt, (int)nsteps), "cav_step");
+    current_step += nsteps;
+    time += dt * (double)nsteps;
+  } else {
+    // one step at a time: each writer sees the state at the end of its step
+    // (md_simulator.cpp:672-673), time before the step's own advance
+    for (long k = 0; k < nsteps; ++k) {
+      check(c, cav_set_step_index(c, current_step), "cav_set_step_index");
+      check(c, cav_step(c, dt, 1), "cav_step");
+      for (writer::Atom_data *w : writer)
+        if (w->wants(current_step)) w->write(current_step, time);
+      time += dt;
+      ++current_step;
+    }
+  }
   atom_data->sync_from_device();
 }
 
@@ -959,6 +1265,7 @@ void Md_simulator::step() { advance(1); }
 void Md_simulator::run()
 {
   verify_settings();
+  current_step = initial_step;  // md_simulator.cpp:247-248
   advance(final_step - initial_step);
   if (final_step > initial_step) current_step = final_step;
 }

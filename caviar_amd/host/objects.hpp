@@ -70,6 +70,14 @@ class Atom_data : public Pointers {
   void verify_settings();
   bool add_atom(unsigned id, unsigned type, const Vector &pos, const Vector &vel);
   void add_xyz_data_file(casl::Parser *parser);
+  // add_random_velocity SEED AMP (atom_data.cpp:597-615): libstdc++'s mt19937
+  // and uniform_real_distribution, then the centre-of-mass velocity removed.
+  void add_random_velocity(unsigned seed, double amplitude);
+  // kinetic_energy() now; temperature() and pressure() as last finalized
+  // (kinetic_energy.cpp:140-292).
+  double kinetic_energy();
+  double temperature();
+  double pressure();
   void reset_owned_acceleration();
   void reset_virial();
   double virial();
@@ -94,7 +102,10 @@ class Atom_data : public Pointers {
     std::vector<double> mass, mass_inv, charge, radius;
   } atom_type_params;
   double ghost_cutoff = 0.0;
-  double k_b = 1.0;
+  double k_b = -1.0;               // atom_data.cpp:54
+  int n_r_df = -1;                 // from the boundary conditions (atom_data.cpp:322-350)
+  bool temperature_process = true; // atom_data.h:735
+  bool pressure_process = false;
   Domain *domain = nullptr;
   int device_index = 0;
   int precision = CAV_FP64;
@@ -265,6 +276,77 @@ class Electrostatic_ewald_slab_correction : public Force_field {
 
 }  // namespace force_field
 
+// Velocity-rescaling constraints of the velocity-Verlet step
+// (md_simulator.cpp:619-633), the reference's constraint objects.
+class Constraint : public Pointers {
+ public:
+  virtual ~Constraint() = default;
+  virtual bool read(casl::Parser *parser) = 0;
+  virtual void verify_settings() = 0;
+  // Push the settings into the context (before setup).
+  virtual void init() = 0;
+  virtual const char *type_name() const = 0;
+  Atom_data *atom_data = nullptr;
+};
+
+namespace constraint {
+
+// constraint berendsen (constraint/berendsen.cpp:25-146; berendsen.h:50-60)
+class Berendsen : public Constraint {
+ public:
+  bool read(casl::Parser *parser) override;
+  void verify_settings() override;
+  void init() override;
+  const char *type_name() const override { return "berendsen"; }
+  double temperature = 0.0, coupling = -1.0, dt = -1.0;
+  int step = 1;
+  double lambda_min = 0.8, lambda_max = 1.25;
+};
+
+// constraint nve (constraint/nve.cpp:25-178)
+class Nve : public Constraint {
+ public:
+  bool read(casl::Parser *parser) override;
+  void verify_settings() override;
+  void init() override;
+  const char *type_name() const override { return "nve"; }
+  double energy_per_dof = -1.0, energy_tot = -1.0, temperature = -1.0, kb = -1.0, kbt = -1.0;
+  int step = 1;
+};
+
+}  // namespace constraint
+
+namespace writer {
+
+// writer atom_data, serial output (writer/atom_data.cpp, writer/atom_data/
+// dump_xyz_serial.cpp:33-71, dump_energy.cpp): xyz frames, kinetic energy,
+// temperature and pressure, every *_step steps, as "<name>.xyz" / ".txt" with
+// the stream's default formatting; xyz_precision (an addition, default the
+// stream's 6 digits) writes the xyz frames at up to 17 significant digits.
+class Atom_data : public Pointers {
+ public:
+  ~Atom_data();
+  bool read(casl::Parser *parser);
+  void write(long step, double time);
+  // whether write() has anything to write at this step
+  bool wants(long step) const;
+  caviar_amd::Atom_data *atom_data = nullptr;
+  long xyz_step = 100, energy_step = 100, temperature_step = 100, pressure_step = 100;
+  bool output_xyz = false, output_energy = false, output_temperature = false, output_pressure = false;
+  std::string file_name_xyz = "o_xyz", file_name_energy = "o_energy",
+              file_name_temperature = "o_temperature", file_name_pressure = "o_pressure";
+  bool xyz_output_id = false, xyz_output_velocity = false, xyz_output_acceleration = false;
+  int xyz_precision = 6;
+  std::string dir;  // where relative file names go (the script's folder is not used: the run's)
+
+ private:
+  struct Files;
+  Files *files_ = nullptr;
+  void open();
+};
+
+}  // namespace writer
+
 class Md_simulator : public Pointers {
  public:
   bool read(casl::Parser *parser);
@@ -279,7 +361,10 @@ class Md_simulator : public Pointers {
   Atom_data *atom_data = nullptr;
   std::vector<Force_field *> force_field;
   std::vector<Neighborlist *> neighborlist;
+  std::vector<Constraint *> constraint;
+  std::vector<writer::Atom_data *> writer;
   double dt = 0.001;
+  double time = 0.0;  // dt * initial_step at setup, then += dt per step
   long initial_step = 0, final_step = 0, current_step = 0;
   bool initialized = false;
   // Read scripts without the GPU: run / step only count their steps.
@@ -299,6 +384,8 @@ class Objects {
   std::map<std::string, std::unique_ptr<Neighborlist>> neighborlist;
   std::map<std::string, std::unique_ptr<Force_field>> force_field;
   std::map<std::string, std::unique_ptr<Md_simulator>> md_simulator;
+  std::map<std::string, std::unique_ptr<Constraint>> constraint;
+  std::map<std::string, std::unique_ptr<writer::Atom_data>> writer;
   std::vector<std::string> order;  // creation order of every name
 
   bool has(const std::string &name) const;
@@ -306,6 +393,8 @@ class Objects {
   Atom_data *find_atom_data(const std::string &name, casl::Parser *p);
   Neighborlist *find_neighborlist(const std::string &name, casl::Parser *p);
   Force_field *find_force_field(const std::string &name, casl::Parser *p);
+  Constraint *find_constraint(const std::string &name, casl::Parser *p);
+  writer::Atom_data *find_writer(const std::string &name, casl::Parser *p);
 };
 
 }  // namespace caviar_amd

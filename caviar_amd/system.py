@@ -99,6 +99,40 @@ class ELCSettings:
 
 
 @dataclass
+class ThermoSettings:
+    """``atom_data basic`` k_b, n_r_df (-1: from the boundary conditions,
+    atom_data.cpp:322-350), temperature_process (on by default,
+    atom_data.h:735) and pressure_process (atom_data.cpp:121-151)."""
+    k_b: float = -1.0
+    n_r_df: int = -1
+    temperature_process: bool = True
+    pressure_process: bool = False
+
+
+@dataclass
+class BerendsenSettings:
+    """``constraint berendsen`` (constraint/berendsen.cpp:33-146)."""
+    temperature: float
+    coupling: float
+    dt: float
+    step: int = 1
+    lambda_min: float = 0.8
+    lambda_max: float = 1.25
+
+
+@dataclass
+class NVESettings:
+    """``constraint nve`` (constraint/nve.cpp:33-178); one of energy_per_dof,
+    energy_tot, kbt or kb + temperature (unset values <= 0)."""
+    energy_per_dof: float = -1.0
+    energy_tot: float = -1.0
+    kbt: float = -1.0
+    kb: float = -1.0
+    temperature: float = -1.0
+    step: int = 1
+
+
+@dataclass
 class EwaldRSettings:
     """``force_field electrostatic_ewald_r`` (electrostatic_ewald_r.cpp:31-84)."""
     alpha: float = 1.0
@@ -144,6 +178,12 @@ class System:
     ewald_r: Optional[EwaldRSettings] = None
     ewald_k: Optional[EwaldKSettings] = None
     elc: Optional[ELCSettings] = None
+    thermo: Optional[ThermoSettings] = None
+    berendsen: Optional[BerendsenSettings] = None
+    nve: Optional[NVESettings] = None
+    # atom_data add_random_velocity SEED AMP (atom_data.cpp:280-288), applied
+    # by the CASL layer; ``vel`` then holds the velocities it produced
+    random_velocity: Optional[tuple] = None
     meta: dict = field(default_factory=dict)
 
     def __post_init__(self):

@@ -106,6 +106,31 @@ int cav_set_ewald_k(cav_ctx *ctx, double alpha, double k_electrostatic, int kx_m
                     int kz_max, int dipole, double epsilon_dipole, int slab_normal_axis,
                     double slab_lz);
 
+/* atom_data basic: k_b, n_r_df (-1: from the boundary conditions,
+ * atom_data.cpp:322-350), temperature_process, pressure_process
+ * (atom_data.cpp:121-151).  With temperature_process the temperature is
+ * finalized after setup and at the end of every step
+ * (kinetic_energy.cpp:162-202), with pressure_process the pressure at the end
+ * of every step (kinetic_energy.cpp:210-292); n_r_df above 3 (angular momentum
+ * removed) is not on this path. */
+int cav_set_thermo(cav_ctx *ctx, double k_b, int n_r_df, int temperature_process,
+                   int pressure_process);
+/* constraint berendsen: temperature, coupling, dt, step, lambda_min, lambda_max
+ * (constraint/berendsen.cpp:33-146; defaults 1, 0.8, 1.25), applied at the end
+ * of each step whose index is a multiple of step. */
+int cav_set_berendsen(cav_ctx *ctx, double temperature, double coupling, double dt, int step,
+                      double lambda_min, double lambda_max);
+/* constraint nve: energy_per_dof, energy_tot, kbt, kb, temperature (unset:
+ * <= 0), step (constraint/nve.cpp:33-178). */
+int cav_set_nve(cav_ctx *ctx, double energy_per_dof, double energy_tot, double kbt, double kb,
+                double temperature, int step);
+/* Md_simulator::current_step of the next step (thermostats act at multiples
+ * of their step); cav_step advances it by one per step. */
+int cav_set_step_index(cav_ctx *ctx, long step);
+/* Atom_data::kinetic_energy now (kinetic_energy.cpp:140-154), and temperature()
+ * and pressure() as last finalized (0 before; NULL outputs skipped). */
+int cav_thermo(cav_ctx *ctx, double *kinetic_energy, double *temperature, double *pressure);
+
 /* ---- atoms --------------------------------------------------------------- */
 /* Atom_data::add_atom for n atoms at once (atom_data.cpp:487-521); mol may be
  * NULL (molecule index -1, as add_atom sets).  Host AoS input. */

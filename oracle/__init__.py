@@ -51,6 +51,13 @@ def _load():
         "orc_set_short_range": (None, [_p, _d, _d, _d, _dp, _dp]),
         "orc_lj_list_acc": (None, [_p]),
         "orc_set_elc": (None, [_p, _i, _i, _i, _d, _d]),
+        "orc_set_thermo": (None, [_p, _d, _i, _i, _i]),
+        "orc_set_step_index": (None, [_p, ctypes.c_long]),
+        "orc_kinetic_energy": (_d, [_p]),
+        "orc_temperature": (_d, [_p]),
+        "orc_pressure": (_d, [_p]),
+        "orc_set_berendsen": (None, [_p, _d, _d, _d, _i, _d, _d]),
+        "orc_set_nve": (None, [_p, _d, _d, _d, _d, _d, _i]),
         "orc_elc_acc": (None, [_p]),
         "orc_energy_elc": (_d, [_p]),
         "orc_point_elc": (None, [_p, _i, _dp, _dp, _dp]),
@@ -161,6 +168,17 @@ class Oracle:
             lam = cfg.ewald_r.lambda_
             lam = None if lam is None else np.ascontiguousarray(lam, dtype=np.float64).reshape(nt * nt)
             L.orc_set_ewald_r(self.h, cfg.ewald_r.alpha, cfg.ewald_r.k_electrostatic, _dptr(lam))
+        if cfg.thermo is not None:
+            t = cfg.thermo
+            L.orc_set_thermo(self.h, float(t.k_b), int(t.n_r_df), int(t.temperature_process),
+                             int(t.pressure_process))
+        if cfg.berendsen is not None:
+            b = cfg.berendsen
+            L.orc_set_berendsen(self.h, b.temperature, b.coupling, b.dt, int(b.step), b.lambda_min,
+                                b.lambda_max)
+        if cfg.nve is not None:
+            v = cfg.nve
+            L.orc_set_nve(self.h, v.energy_per_dof, v.energy_tot, v.kbt, v.kb, v.temperature, int(v.step))
         if cfg.elc is not None:
             c = cfg.elc
             L.orc_set_elc(self.h, int(c.slab_normal_axis), int(c.kx_max), int(c.ky_max), float(c.lz),
@@ -286,6 +304,11 @@ class Oracle:
         p = np.empty(len(pts))
         self.lib.orc_point_k(self.h, len(pts), _dptr(pts), _dptr(f), _dptr(p))
         return f, p
+
+    def thermo(self):
+        """(kinetic energy now, temperature and pressure as last finalized)."""
+        return (self.lib.orc_kinetic_energy(self.h), self.lib.orc_temperature(self.h),
+                self.lib.orc_pressure(self.h))
 
     def energy_elc(self) -> float:
         return self.lib.orc_energy_elc(self.h)

@@ -76,6 +76,16 @@ struct orc_sys {
   double dip_sum[3], dip_field[3];
   double *kvec, *fk, *sk;
   double virial;
+  /* temperature / pressure (atom_data.cpp:121-151, kinetic_energy.cpp) */
+  double k_b;
+  int n_r_df, temperature_process, pressure_process;
+  double temperature, pressure;
+  long step_index; /* Md_simulator::current_step of the next orc_step */
+  /* constraint berendsen (constraint/berendsen.cpp) and nve (constraint/nve.cpp) */
+  int ber_on, ber_step;
+  double ber_t0, ber_coupling, ber_dt, ber_lmin, ber_lmax;
+  int nve_on, nve_step;
+  double nve_e_dof, nve_e_tot, nve_kbt, nve_kb, nve_t;
 };
 
 static int g_threads = 1;
@@ -86,6 +96,8 @@ orc_sys *orc_create(void)
   orc_sys *s = (orc_sys *)calloc(1, sizeof(orc_sys));
   s->nl_coef = 1.12347; /* neighborlist.h:134 */
   s->slab_axis = -1;
+  s->k_b = -1.0;   /* atom_data.cpp:54 */
+  s->n_r_df = -1;  /* from the boundary conditions (atom_data.cpp:322-350) */
   return s;
 }
 
@@ -1358,6 +1370,152 @@ void orc_forces(orc_sys *s)
     orc_elc_acc(s);
 }
 
+/* ------------------------------------------- temperature, pressure, thermostats -- */
+
+/* atom_data k_b, n_r_df (-1: from the boundary conditions), temperature_process,
+ * pressure_process (atom_data.cpp:121-151). */
+void orc_set_thermo(orc_sys *s, double k_b, int n_r_df, int temperature_process, int pressure_process)
+{
+  s->k_b = k_b;
+  s->n_r_df = n_r_df;
+  s->temperature_process = temperature_process;
+  s->pressure_process = pressure_process;
+  s->temperature = s->pressure = 0.0; /* reset_temperature / reset_pressure */
+}
+
+void orc_set_step_index(orc_sys *s, long step) { s->step_index = step; }
+
+/* get_n_r_df (atom_data.cpp:322-350), without stochastic forces. */
+static int n_r_df(const orc_sys *s)
+{
+  if (s->n_r_df >= 0)
+    return s->n_r_df;
+  const int bc_sum = s->bc[0] + s->bc[1] + s->bc[2];
+  return bc_sum == 0 ? 6 : (bc_sum == 3 ? 3 : 6 - bc_sum);
+}
+
+/* kinetic_energy_mpi_domain (kinetic_energy.cpp:30-138) for n_r_df 0..3: the
+ * centre-of-mass velocity (owned_velocity_cm, :413-462) removed for 1..3.
+ * Returns NAN for n_r_df 4..6 (the angular-momentum correction is not on
+ * this path). */
+double orc_kinetic_energy(const orc_sys *s)
+{
+  const int nr = n_r_df(s);
+  if (nr > 3)
+    return NAN;
+  double vcm[3] = {0, 0, 0};
+  if (nr >= 1) {
+    double f[3] = {0, 0, 0}, msum = 0.0;
+    for (int i = 0; i < s->n; ++i) {
+      const double m = s->mass[s->type[i]];
+      msum += m;
+      for (int d = 0; d < 3; ++d)
+        f[d] += s->vel[3 * i + d] * m;
+    }
+    for (int d = 0; d < 3; ++d)
+      vcm[d] = f[d] / msum;
+  }
+  double e = 0.0;
+  for (int i = 0; i < s->n; ++i) {
+    const double m = s->mass[s->type[i]];
+    double v[3];
+    for (int d = 0; d < 3; ++d)
+      v[d] = nr >= 1 ? s->vel[3 * i + d] - vcm[d] : s->vel[3 * i + d];
+    e += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  }
+  return e * 0.5;
+}
+
+/* degree_of_freedoms_mpi_domain (kinetic_energy.cpp:681-718) without molecules' bonds. */
+static int dof(const orc_sys *s) { return 3 * s->n - n_r_df(s); }
+
+/* finalize_temperature (kinetic_energy.cpp:162-202): 2 KE / (k_b dof). */
+static void finalize_temperature(orc_sys *s)
+{
+  if (!s->temperature_process)
+    return;
+  s->temperature = 2.0 * orc_kinetic_energy(s) / (s->k_b * dof(s));
+}
+
+/* finalize_pressure (kinetic_energy.cpp:210-292): n k_b T / V + virial / (3V). */
+static void finalize_pressure(orc_sys *s)
+{
+  if (!s->pressure_process)
+    return;
+  const double volume = s->L[0] * s->L[1] * s->L[2];
+  const double p2 = (0.0 + 0 + s->virial) / (3.0 * volume);
+  const double p1 = ((double)s->n * s->k_b * s->temperature) / volume;
+  s->pressure = p1 + p2;
+}
+
+double orc_temperature(const orc_sys *s) { return s->temperature; }
+double orc_pressure(const orc_sys *s) { return s->pressure; }
+
+/* constraint berendsen: temperature, coupling, dt, step, lambda_min, lambda_max
+ * (berendsen.cpp:33-118, defaults berendsen.h:50-60). */
+void orc_set_berendsen(orc_sys *s, double t0, double coupling, double dt, int step, double lmin,
+                       double lmax)
+{
+  s->ber_on = 1;
+  s->ber_t0 = t0;
+  s->ber_coupling = coupling;
+  s->ber_dt = dt;
+  s->ber_step = step;
+  s->ber_lmin = lmin;
+  s->ber_lmax = lmax;
+}
+
+/* constraint nve: energy_per_dof, energy_tot, kbt, kb + temperature, step
+ * (nve.cpp:33-132); unset values <= 0. */
+void orc_set_nve(orc_sys *s, double e_dof, double e_tot, double kbt, double kb, double t, int step)
+{
+  s->nve_on = 1;
+  s->nve_e_dof = e_dof;
+  s->nve_e_tot = e_tot;
+  s->nve_kbt = kbt;
+  s->nve_kb = kb;
+  s->nve_t = t;
+  s->nve_step = step;
+}
+
+static void scale_velocities(orc_sys *s, double lambda)
+{
+  for (int q = 0; q < 3 * s->n; ++q)
+    s->vel[q] *= lambda;
+}
+
+/* apply_thermostat of berendsen (berendsen.cpp:120-146) and nve (nve.cpp:134-178)
+ * at timestep `ts`, in that order; returns whether any scaled. */
+static int thermostats(orc_sys *s, long ts)
+{
+  int scaled = 0;
+  if (s->ber_on && ts % s->ber_step == 0) {
+    scaled = 1;
+    const double t = s->temperature;
+    double lambda = sqrt(1.0 + (s->ber_dt / s->ber_coupling) * ((s->ber_t0 / t) - 1.0));
+    if (lambda < s->ber_lmin)
+      lambda = s->ber_lmin;
+    if (lambda > s->ber_lmax)
+      lambda = s->ber_lmax;
+    scale_velocities(s, lambda);
+  }
+  if (s->nve_on && ts % s->nve_step == 0) {
+    scaled = 1;
+    const double e_now = orc_kinetic_energy(s);
+    double lambda = 1.0;
+    if (s->nve_e_dof > 0.0)
+      lambda = sqrt(s->nve_e_dof * dof(s) / e_now);
+    else if (s->nve_e_tot > 0.0)
+      lambda = sqrt(s->nve_e_tot / e_now);
+    else if (s->nve_kbt > 0.0)
+      lambda = sqrt(dof(s) * s->nve_kbt * 0.5 / e_now);
+    else if (s->nve_t > 0.0)
+      lambda = sqrt(dof(s) * s->nve_kb * s->nve_t * 0.5 / e_now);
+    scale_velocities(s, lambda);
+  }
+  return scaled;
+}
+
 /* ---------------------------------------------------------------- driver -- */
 
 int orc_setup(orc_sys *s)
@@ -1369,6 +1527,7 @@ int orc_setup(orc_sys *s)
   orc_ghosts(s);
   orc_build(s, 1);
   orc_forces(s);
+  finalize_temperature(s); /* md_simulator.cpp:543 */
   return 0;
 }
 
@@ -1392,6 +1551,13 @@ int orc_step(orc_sys *s, double dt)
   orc_forces(s);
   for (int q = 0; q < 3 * s->n; ++q)
     s->vel[q] += 0.5 * dt * (s->acc[q] + s->acc_old[q]);
+  /* md_simulator.cpp:619-633: temperature, thermostats (and the temperature
+   * again after they scaled), pressure */
+  finalize_temperature(s);
+  if (thermostats(s, s->step_index))
+    finalize_temperature(s);
+  finalize_pressure(s);
+  s->step_index++;
   return 0;
 }
 

@@ -91,6 +91,20 @@ void orc_ewald_k_sample(orc_sys *s, int k0, int k1, int i0, int i1, double *t_s,
 void orc_forces_sampled(orc_sys *s, int m, const int *idx);
 int orc_setup_sampled(orc_sys *s, int m, const int *idx);
 
+/* atom_data k_b, n_r_df (-1: from the boundary conditions), temperature_process,
+ * pressure_process (atom_data.cpp:121-151; kinetic_energy.cpp:162-292). */
+void orc_set_thermo(orc_sys *s, double k_b, int n_r_df, int temperature_process, int pressure_process);
+/* Md_simulator::current_step of the next orc_step (thermostats act when it is
+ * a multiple of their step). */
+void orc_set_step_index(orc_sys *s, long step);
+double orc_kinetic_energy(const orc_sys *s);   /* NAN for n_r_df 4..6 */
+double orc_temperature(const orc_sys *s);      /* last finalized (0 until then) */
+double orc_pressure(const orc_sys *s);
+/* constraint berendsen (berendsen.cpp:33-146) and nve (nve.cpp:33-178). */
+void orc_set_berendsen(orc_sys *s, double t0, double coupling, double dt, int step, double lmin,
+                       double lmax);
+void orc_set_nve(orc_sys *s, double e_dof, double e_tot, double kbt, double kb, double t, int step);
+
 /* Getters. what: 0 pos, 1 vel, 2 acc, 3 acc_old (3n doubles). */
 int orc_n(const orc_sys *s);
 void orc_get(const orc_sys *s, int what, double *out);

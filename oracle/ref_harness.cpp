@@ -17,6 +17,7 @@
 // per owned atom in input order, written into atom_struct_owned.molecule_index
 // before setup or run (what the reference's add_molecule commands produce;
 // atoms read from an XYZ file otherwise all carry -1).
+// "+thermo" dumps Atom_data::kinetic_energy(), temperature() and pressure().
 // "+elc_energy" calls Electrostatic_ewald_slab_correction::energy(); "+field"
 // also dumps the slab correction's field/potential as field_elc.
 // Flags appended to the mode: "+er_energy" also calls Electrostatic_ewald_r::
@@ -133,6 +134,13 @@ int main(int argc, char **argv)
       for (auto j : nl->neighlist[i])
         std::fprintf(f, "%zu %u\n", i, j);
     std::fprintf(f, "no_bins %d %d %d\n", nl->no_bins.x, nl->no_bins.y, nl->no_bins.z);
+  }
+
+  if (mode.find("+thermo") != std::string::npos)
+  {
+    std::fprintf(f, "kinetic_energy %.17g\n", ad->kinetic_energy());
+    std::fprintf(f, "temperature %.17g\n", ad->temperature());
+    std::fprintf(f, "pressure %.17g\n", ad->pressure());
   }
 
   for (auto *ff : sim->force_field)

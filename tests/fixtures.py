@@ -7,8 +7,9 @@ import os
 
 import numpy as np
 
-from caviar_amd.system import (ELCSettings, EwaldKSettings, EwaldRSettings, LJListSettings,
-                               LJSettings, NeighborSettings, ShortRangeSettings, System)
+from caviar_amd.system import (BerendsenSettings, ELCSettings, EwaldKSettings, EwaldRSettings,
+                               LJListSettings, LJSettings, NeighborSettings, NVESettings,
+                               ShortRangeSettings, System, ThermoSettings)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -45,12 +46,23 @@ def load(name: str):
         sd = arrs(d["short_range"])
         sd["external_field"] = tuple(np.asarray(sd["external_field"]).tolist())
         sr = ShortRangeSettings(**sd)
+    ref = {k[4:]: z[k] for k in z.files if k.startswith("ref_")}
+    vel = z["in_vel"]
+    if "random_velocity" in d:
+        # the run starts from the velocities the reference's add_random_velocity
+        # made (dumped by a setup of the same script; a setup fixture's own)
+        vel = ref["vel_rv"] if "vel_rv" in ref else ref["vel"]
     s = System(lower=d["lower"], upper=d["upper"], bc=d["bc"], ghost_cutoff=d["ghost_cutoff"],
-               mass=d["mass"], charge=d["charge"], pos=z["in_pos"], vel=z["in_vel"],
+               mass=d["mass"], charge=d["charge"], pos=z["in_pos"], vel=vel,
                types=z["in_types"], mol=z["in_mol"] if "in_mol" in z.files else None,
                neighbor=nl, lj=lj, lj_list=ljl, short_range=sr, ewald_r=er, ewald_k=ek,
-               elc=ELCSettings(**d["elc"]) if "elc" in d else None)
-    ref = {k[4:]: z[k] for k in z.files if k.startswith("ref_")}
+               elc=ELCSettings(**d["elc"]) if "elc" in d else None,
+               thermo=ThermoSettings(**d["thermo"]) if "thermo" in d else None,
+               berendsen=BerendsenSettings(**d["berendsen"]) if "berendsen" in d else None,
+               nve=NVESettings(**d["nve"]) if "nve" in d else None,
+               random_velocity=tuple(d["random_velocity"]) if "random_velocity" in d else None)
+    if "writer" in d:
+        s.meta["writer"] = d["writer"]
     meta = {"mode": str(z["mode"]), "run_steps": int(z["run_steps"]), "dt": float(z["dt"]),
             # large fixtures store the reference's list sorted, not in its order
             "pairs_sorted": bool(int(z["pairs_sorted"])) if "pairs_sorted" in z.files else False}

@@ -3,7 +3,7 @@
 Run in the development container only (it needs /root/reference):
 
     make -f oracle/ref.mk -j8          # builds oracle/_ref/ref_harness
-    python tests/golden/make_golden.py [original] [c2] [slab] [ljsr] [elc]   # default: all
+    python tests/golden/make_golden.py [original] [c2] [slab] [ljsr] [elc] [thermo]   # default: all
 
 For every case it writes the input as an XYZ file plus a CASL script in the
 reference's own syntax (neighborlist basic / force_field ... / md_simulator
@@ -29,9 +29,9 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from caviar_amd import configs  # noqa: E402
-from caviar_amd.system import (ELCSettings, EwaldKSettings, EwaldRSettings,  # noqa: E402
-                               LJListSettings, LJSettings, NeighborSettings, ShortRangeSettings,
-                               System)
+from caviar_amd.system import (BerendsenSettings, ELCSettings, EwaldKSettings,  # noqa: E402
+                               EwaldRSettings, LJListSettings, LJSettings, NeighborSettings,
+                               NVESettings, ShortRangeSettings, System, ThermoSettings)
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 B2 = "/root/reference/benchmark/b2-ewald"
@@ -58,6 +58,16 @@ def casl_for(sysm: System, xyz: str, run_steps: int = 0, dt: float = 0.001) -> s
         L.append(f"adata add_type_mass {t} {R(sysm.mass[t])}")
         L.append(f"adata add_type_charge {t} {R(sysm.charge[t])}")
     L.append(f'adata add_xyz_data_file file_name "{xyz}" read_velocity')
+    if sysm.thermo is not None:
+        t = sysm.thermo
+        L.append(f"adata k_b {R(t.k_b)}")
+        if t.n_r_df >= 0:
+            L.append(f"adata n_r_df {t.n_r_df}")
+        L.append(f"adata temperature_process {int(t.temperature_process)}")
+        if t.pressure_process:
+            L.append("adata pressure_process 1")
+    if sysm.random_velocity is not None:
+        L.append(f"adata add_random_velocity {int(sysm.random_velocity[0])} {R(sysm.random_velocity[1])}")
     nl = sysm.neighbor
     ffs = []
     if nl is not None:
@@ -174,7 +184,34 @@ def casl_for(sysm: System, xyz: str, run_steps: int = 0, dt: float = 0.001) -> s
         L.append(f"f_elc lz {R(c.lz)}")
         L.append(f"f_elc slab_normal_axis {'xyz'[c.slab_normal_axis]}")
         ffs.append("f_elc")
+    cons = []
+    if sysm.berendsen is not None:
+        b = sysm.berendsen
+        L.append("constraint berendsen c_ber")
+        L.append(f"c_ber temperature {R(b.temperature)} coupling {R(b.coupling)} dt {R(b.dt)}")
+        L.append(f"c_ber step {b.step} lambda_min {R(b.lambda_min)} lambda_max {R(b.lambda_max)}")
+        L.append("c_ber set_atom_data adata")
+        cons.append("c_ber")
+    if sysm.nve is not None:
+        v = sysm.nve
+        L.append("constraint nve c_nve")
+        for k in ("energy_per_dof", "energy_tot", "kbt", "kb", "temperature"):
+            if getattr(v, k) > 0:
+                L.append(f"c_nve {k} {R(getattr(v, k))}")
+        L.append(f"c_nve step {v.step}")
+        L.append("c_nve set_atom_data adata")
+        cons.append("c_nve")
+    writer = sysm.meta.get("writer")
+    if writer:
+        L.append("writer atom_data w_ad")
+        L.append("w_ad set_atom_data adata")
+        for k, v in writer.items():
+            L.append(f"w_ad {k} {v}")
     L.append("md_simulator basic sim")
+    for c in cons:
+        L.append(f"sim add_constraint {c}")
+    if writer:
+        L.append("sim add_writer w_ad")
     L.append("sim set_atom_data adata")
     for f in ffs:
         L.append(f"sim add_force_field {f}")
@@ -260,6 +297,13 @@ def settings_json(s: System) -> str:
         d["short_range"] = {k: plain(v) for k, v in vars(s.short_range).items()}
     if s.elc:
         d["elc"] = vars(s.elc)
+    for k in ("thermo", "berendsen", "nve"):
+        if getattr(s, k) is not None:
+            d[k] = vars(getattr(s, k))
+    if s.random_velocity is not None:
+        d["random_velocity"] = [int(s.random_velocity[0]), float(s.random_velocity[1])]
+    if s.meta.get("writer"):
+        d["writer"] = s.meta["writer"]
     return json.dumps(d)
 
 
@@ -271,6 +315,9 @@ def run_case(name: str, sysm: System, mode: str, run_steps: int = 0, dt: float =
         casl = os.path.join(td, "in.casl")
         with open(casl, "w") as f:
             f.write(casl_for(sysm, "in.xyz", run_steps, dt))
+        casl_setup = os.path.join(td, "in_setup.casl")
+        with open(casl_setup, "w") as f:
+            f.write(casl_for(sysm, "in.xyz", 0, dt))
         out = os.path.join(td, "out.txt")
         cmd = [HARNESS, casl, mode, out]
         if sysm.mol is not None:
@@ -284,6 +331,17 @@ def run_case(name: str, sysm: System, mode: str, run_steps: int = 0, dt: float =
             print(r.stdout[-2000:], r.stderr[-2000:])
             raise RuntimeError(f"harness failed for {name}")
         res = parse(out)
+        # files the script's writer wrote (o_*.xyz / o_*.txt), kept as text
+        for fn in sorted(os.listdir(td)):
+            if fn.startswith("o_"):
+                res["file_" + fn.replace(".", "_")] = np.array(open(os.path.join(td, fn)).read())
+        if sysm.random_velocity is not None and run_steps:
+            # the velocities add_random_velocity produced: the run's start
+            r0 = subprocess.run([HARNESS, casl_setup, "setup", out], cwd=td, capture_output=True,
+                                text=True)
+            if r0.returncode != 0:
+                raise RuntimeError(f"harness setup failed for {name}")
+            res["vel_rv"] = parse(out)["vel"]
     arrays = {f"in_{k}": v for k, v in (("pos", sysm.pos), ("vel", sysm.vel),
                                          ("types", sysm.types))}
     if sysm.mol is not None:
@@ -489,8 +547,35 @@ def elc_cases():
     run_case("elcy300_run8", y, "run", run_steps=8, dt=0.002)
 
 
+def thermo_cases():
+    # 13. Temperature and pressure (atom_data k_b / temperature_process /
+    #     pressure_process), random velocities (add_random_velocity: libstdc++
+    #     mt19937 + uniform_real_distribution, centre-of-mass motion removed),
+    #     the berendsen and nve constraints, and writer atom_data output
+    #     (xyz with velocities, temperature, pressure and kinetic energy files).
+    a = configs.ions(n=300, kmax=3, seed=101)
+    a.thermo = ThermoSettings(k_b=1.0, temperature_process=True, pressure_process=True)
+    a.random_velocity = (7, 0.5)
+    a.berendsen = BerendsenSettings(temperature=1.5, coupling=0.01, dt=0.002, step=2)
+    a.meta["writer"] = {"xyz_step": 5, "file_name_xyz": "o_xyz", "xyz_output_velocity": 1,
+                        "xyz_output_id": 1, "temperature_step": 2, "pressure_step": 2,
+                        "energy_step": 3}
+    run_case("thermo_ber300_setup", a, "setup+thermo")
+    run_case("thermo_ber300_run20", a, "run+thermo", run_steps=20, dt=0.002)
+    b = configs.ions(n=250, kmax=3, seed=103)
+    b.thermo = ThermoSettings(k_b=0.5, n_r_df=0, temperature_process=True)
+    b.nve = NVESettings(energy_per_dof=0.6, step=3)
+    run_case("thermo_nve250_run15", b, "run+thermo", run_steps=15, dt=0.002)
+    # berendsen without temperature_process: the temperature it reads stays 0
+    # and lambda is clamped to lambda_max (berendsen.cpp:131-140)
+    c = configs.ions(n=200, kmax=2, seed=105)
+    c.thermo = ThermoSettings(k_b=1.0, temperature_process=False)
+    c.berendsen = BerendsenSettings(temperature=1.0, coupling=0.1, dt=0.002, step=1, lambda_max=1.01)
+    run_case("thermo_berclamp200_run6", c, "run+thermo", run_steps=6, dt=0.002)
+
+
 CASES = {"original": original_cases, "c2": c2_cases, "slab": slab_cases, "ljsr": ljsr_cases,
-         "elc": elc_cases}
+         "elc": elc_cases, "thermo": thermo_cases}
 
 
 def main(groups):

@@ -129,6 +129,10 @@ def test_parse_only_collects_the_fixture_settings(tmp_path, name):
     if s.elc is not None:
         assert d["elc"] == vars(s.elc)
     assert d["dt"] == meta["dt"] and d["final_step"] == meta["run_steps"]
+    if s.random_velocity is not None:
+        # add_random_velocity ran in the script: the driver's velocities are the
+        # reference's bit for bit (checksum above, formed from s.vel = those)
+        assert d["vel_checksum"] == vs
 
 
 EXAMPLE = """
@@ -263,7 +267,8 @@ def check_dump(out, s, ref, meta, exact_tol=1e-12):
         assert np.array_equal(got["ghost_id"], ref["ghost_id"])
         v = float(ref["virial"])
         assert abs(got["virial"] - v) <= 1e-12 * max(abs(v), 1e-300) or v == got["virial"]
-    for key in ("energy_r", "energy_k", "energy_k_self", "energy_elc"):
+    for key in ("energy_r", "energy_k", "energy_k_self", "energy_elc", "kinetic_energy", "temperature",
+                "pressure"):
         if key in ref:
             assert abs(got[key] - ref[key]) <= 1e-12 * abs(ref[key]), key
     for key in ("sk", "field_r", "field_k", "field_elc"):
@@ -279,9 +284,33 @@ def test_driver_reproduces_the_reference_run(gpu, tmp_path, name):
     print(f"[case] caviar_amd_run {name} mode={meta['mode']}", flush=True)
     out = str(tmp_path / "out.txt")
     extra, mode = molecule_args(tmp_path, s, meta["mode"])
-    r = run_driver(*extra, script, mode, out)
+    r = run_driver(*extra, script, mode, out, cwd=str(tmp_path))
     assert r.returncode == 0, r.stderr
     check_dump(out, s, ref, meta)
+    check_writer_files(tmp_path, ref, meta)
+
+
+def check_writer_files(tmp_path, ref, meta):
+    """Files the script's writer wrote, against the reference's: the same
+    lines and tokens, numbers (printed with 6 significant digits) within one
+    unit in their last printed digit."""
+    for key in ref:
+        if not key.startswith("file_"):
+            continue
+        name = key[5:].rsplit("_", 1)
+        path = tmp_path / (name[0] + "." + name[1])
+        assert path.exists(), path
+        got = path.read_text().split("\n")
+        want = str(ref[key]).split("\n")
+        assert len(got) == len(want), key
+        for g, w in zip(got, want):
+            gt, wt = g.split(), w.split()
+            assert len(gt) == len(wt), (key, g, w)
+            for a, b in zip(gt, wt):
+                if a == b:
+                    continue
+                x, y = float(a), float(b)
+                assert abs(x - y) <= 1e-5 * max(abs(y), 1e-300) + 1e-300, (key, g, w)
 
 
 @pytest.mark.gpu

@@ -53,6 +53,7 @@ def test_setup_parity_fp64(gpu, name):
     if "sk" in ref:
         sk = e.sk()
         assert F.rel_err(np.c_[sk.real, sk.imag], ref["sk"]) <= 1e-12
+    check_thermo(e, ref, 1e-12)
     if "energy_elc" in ref:
         assert abs(e.energy_elc() - ref["energy_elc"]) <= 1e-12 * abs(ref["energy_elc"])
     shifted = e.get("pos") + np.array([0.1, -0.2, 0.3])
@@ -70,6 +71,17 @@ def test_setup_parity_fp64(gpu, name):
         assert F.rel_err(p, ref["field_r"][:, 3]) <= 1e-12
 
 
+def check_thermo(e, ref, tol):
+    """Kinetic energy, temperature and pressure (the last two as finalized at
+    the end of the last step, or after setup) against the reference's."""
+    if "temperature" not in ref:
+        return
+    ke, t, p = e.thermo()
+    for got, key in ((ke, "kinetic_energy"), (t, "temperature"), (p, "pressure")):
+        want = float(ref[key])
+        assert abs(got - want) <= tol * max(abs(want), 1e-300) or got == want, key
+
+
 @pytest.mark.parametrize("name", RUNS)
 def test_trajectory_parity_fp64(gpu, name):
     e, s, ref, meta = engine_for(name)
@@ -77,6 +89,7 @@ def test_trajectory_parity_fp64(gpu, name):
     assert F.rel_err(e.get("vel"), ref["vel"]) <= 1e-9
     assert F.rel_err(e.get("acc"), ref["acc"]) <= 1e-9
     assert np.array_equal(F.sorted_pairs(e.pairs()), F.sorted_pairs(ref["pairs"]))
+    check_thermo(e, ref, 1e-9)
 
 
 def test_b2_published_known_answer(gpu):

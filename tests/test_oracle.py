@@ -70,6 +70,10 @@ def test_oracle_matches_reference_bitwise(name):
         f, p = o.point_k(shifted)
         assert F.rel_err(f, ref["field_k"][:, :3]) < 1e-15
         assert np.array_equal(p, ref["field_k"][:, 3])
+    if "temperature" in ref:
+        ke, t, p = o.thermo()
+        assert t == float(ref["temperature"]) and p == float(ref["pressure"])
+        assert ke == float(ref["kinetic_energy"])
     if "energy_elc" in ref:
         assert o.energy_elc() == float(ref["energy_elc"])
     if "field_elc" in ref:

@@ -144,6 +144,7 @@ struct SArgs {
   int Rpad, SC;
   const int4 *items;  // batched: per workgroup column (first atom, end atom, chunk slot, replica)
   int ncols, groups;  // logical grid: atom chunks (or items) x row-tile groups
+  int ct0;            // first column tile of this launch (launches cover 8 tiles each)
 };
 
 // Issue the asynchronous copy of `bytes` contiguous bytes (a multiple of 16,
@@ -160,7 +161,8 @@ __device__ __forceinline__ void copy_to_lds_async(const unsigned char *src, unsi
 }
 
 // One workgroup: an atom chunk x a group of row tiles; wave w owns RT row
-// tiles and all SCT column tiles.  A (rows x (atom,re/im)) is generated from
+// tiles and the SCT column tiles from ct0 (wider nz axes take one launch per
+// 8 column tiles).  A (rows x (atom,re/im)) is generated from
 // the staged X, Y tables and charges, B ((atom,re/im) x (nz,re/im)) from Z.
 // Stages of atom records are double-buffered in LDS: the next stage streams
 // in from HBM (direct-to-LDS loads) while the current one feeds the MFMAs.
@@ -223,7 +225,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
   bool zvalid[SCT];
 #pragma unroll
   for (int s = 0; s < SCT; ++s) {
-    const int col = s * 16 + lr;
+    const int col = (a.ct0 + s) * 16 + lr;
     zc[s] = col >> 1;
     zt[s] = col & 1;
     zvalid[s] = zc[s] < a.nZ;
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = (rt0 + r) * 16 + mfma_row<T>(lk, e);
-        const int col = s * 16 + lr;
+        const int col = (a.ct0 + s) * 16 + lr;
         const double v = (double)acc[r][s][e];
         a.part[((long)chunk * a.Rpad + row) * a.SC + col] = v;
       }
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk(SArgs a)
 __host__ __device__ constexpr int split_blk_bytes(int nkh) { return 4 * nkh * 2 * 64 * 16 + 128; }
 
 template <typename T, typename P>
-__global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz, int KSP,
+__global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz, int KSS, int KSP,
                             const P *part, const double *W, double *sk, T *bm,
                             double *epart, double inv_sa, _Float16 *bmh, int nkh, double sb,
                             const int2 *rchunks, long bm_stride, long bmh_stride)
@@ -354,7 +356,9 @@ __global__ void k_sk_reduce(int nchunks, int Rpad, int SC, int R, int nZ, int kz
       for (int sidx = 0; sidx < 2; ++sidx) {
         const int kap = 2 * c + sidx;
         const int lanef = (kap & 3) * 16 + rr;
-        bm[(((long)rb * 4 + comp) * 64 + lanef) * KSP + (kap >> 2)] = (T)v[comp][sidx];
+        const int ks = kap >> 2, sl = ks / KSS;  // K-step and its slab
+        bm[((((long)sl * (Rpad >> 4) + rb) * 4 + comp) * 64 + lanef) * KSP + (ks - sl * KSS)] =
+            (T)v[comp][sidx];
         if (bmh) {
           // split form: K-step kap'/32, lane (kap'%32/8)*16 + rr, element
           // kap'%8; 64 lanes x 8 hi halves, then 64 lanes x 8 lo halves
@@ -442,8 +446,10 @@ constexpr int kWavesF = 8;  // waves per force-pass workgroup (share each operan
 
 struct FArgs {
   int n, TS, nX, nY, nZ, ky, R, Rpad, KZ, KSP;
+  int s0, nks;     // this launch's K slab: first K-step and K-steps (<= KS)
+  int out4_add;    // add into out4 (K slabs after the first) instead of writing it
   const void *tab;
-  const void *bm;
+  const void *bm;  // this slab's operand blocks
   const double *q, *minv;
   const int *type;
   double c0, kxs, kys, kzs;  // C = 4 pi ke / V; 2 pi / l per axis
@@ -483,16 +489,16 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
   }
   const int offY = a.nX, offZ = a.nX + a.nY;
 
-  // A fragments: A[atom = l%16][kappa = 4s + l/16] = (Zr, Zi)(c = kappa/2)
+  // A fragments: A[atom = l%16][kappa = 4 (s0 + s) + l/16] = (Zr, Zi)(c = kappa/2)
   T am[NT][KS];
 #pragma unroll
   for (int tau = 0; tau < NT; ++tau) {
     const long atom = base + tau * 16 + lr;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      const int kap = 4 * s + lk, c = kap >> 1;
+      const int kap = 4 * (a.s0 + s) + lk, c = kap >> 1;
       T v = 0;
-      if (atom < a.n && c < a.nZ && s * 4 < a.KZ) {
+      if (atom < a.n && c < a.nZ && s < a.nks) {
         const Cx<T> Z = tab[atom * a.TS + offZ + c];
         v = (kap & 1) ? Z.im : Z.re;
       }
@@ -508,7 +514,7 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
   constexpr int VW = 16 / (int)sizeof(T);  // values per 16-byte vector
   typedef T Tvw __attribute__((ext_vector_type(VW)));
   const int nrb = a.Rpad >> 4;
-  const int nks = a.KZ >> 2;  // K-steps in use (<= KS)
+  const int nks = a.nks;  // K-steps in use (<= KS)
   const int blk_bytes = blk_elems * (int)sizeof(T);
   // the operand of row block rb streams into LDS while rb-1 is multiplied
   copy_to_lds_async(bm, smem_raw, blk_bytes, wave, nwaves, lane);
@@ -587,7 +593,12 @@ __global__ __launch_bounds__(64 * kWavesF) void k_kforce(FArgs a)
       const long atom = base + tau * 16 + mfma_row<T>(lk, e);
       if (lr == 0 && atom < a.n) {
         const double gx = a.c0 * a.kxs * vx, gy = a.c0 * a.kys * vy, gz = a.c0 * a.kzs * vz;
-        if (a.out4) {
+        if (a.out4 && a.out4_add) {
+          a.out4[4 * atom + 0] += gx;
+          a.out4[4 * atom + 1] += gy;
+          a.out4[4 * atom + 2] += gz;
+          a.out4[4 * atom + 3] += a.c0 * vp;
+        } else if (a.out4) {
           a.out4[4 * atom + 0] = gx;
           a.out4[4 * atom + 1] = gy;
           a.out4[4 * atom + 2] = gz;
@@ -1182,6 +1193,19 @@ __global__ void k_dipole_apply(int n, int nblk, const double *part, double dip_c
 
 static int roundup(int v, int m) { return (v + m - 1) / m * m; }
 
+// Elements of one K slab of the FP32/FP64 force-pass operand (all row blocks).
+static size_t slab_elems(const EwaldKPlan &p) { return (size_t)(p.Rpad / 16) * 4 * 64 * p.KSP; }
+
+// Atoms per LDS stage of the FP32/FP64 S pass: the largest of 32, 16, ..., 2
+// whose two buffers of records (TS complex values each) and charges fit in
+// 150 KB, or 0 if none does.
+static int s_stage_for(int TS, size_t tsz)
+{
+  for (int st = 32; st >= 2; st /= 2)
+    if (2 * (size_t)st * TS * 2 * tsz + 2 * (size_t)st * tsz <= 150 * 1024) return st;
+  return 0;
+}
+
 // The k lattice of one box: sizes, W = m(k) f(k) over the half lattice and
 // the map from the reference's k order.  Returns an error message or nullptr.
 const char *plan_k_lattice(EwaldKPlan &p, int kx, int ky, int kz, int precision, double alpha,
@@ -1197,16 +1221,23 @@ const char *plan_k_lattice(EwaldKPlan &p, int kx, int ky, int kz, int precision,
   p.R = p.nX * p.nY;
   p.Rpad = roundup(p.R, 16);
   p.KZ = roundup(2 * p.nZ, 4);
+  // the force pass holds at most 32 K-steps (128 kappa) of A fragments in
+  // registers and of operand in LDS: longer nz axes are split into slabs of
+  // equal depth, one launch each, adding into the same outputs
+  p.nslab = (p.KZ / 4 + 31) / 32;
+  p.KSS = (p.KZ / 4 + p.nslab - 1) / p.nslab;
   // fragment stride: whole 16-byte vectors, an odd number of them per lane so
   // that the lanes' LDS reads of the force-pass operand hit distinct banks
   {
     const int vw = precision == CAV_FP64 ? 2 : 4;
-    int ksp = roundup(p.KZ / 4, vw);
+    int ksp = roundup(p.KSS, vw);
     if ((ksp / vw) % 2 == 0) ksp += vw;
     p.KSP = ksp;
   }
-  if (p.KZ / 4 > 32) return "kz_max above 31 is not supported";
-  if (roundup(2 * p.nZ, 16) / 16 > 8) return "kz_max above 31 is not supported";
+  // the S pass stages at least two atom records of nX + nY + nZ complex values
+  // per LDS buffer (double-buffered)
+  if (s_stage_for(p.nX + p.nY + p.nZ, precision == CAV_FP64 ? 8 : 4) == 0)
+    return "k lattice too large: one S-pass stage of two atom records exceeds LDS";
   p.nk_ref = (2 * p.kx) * (2 * p.ky) * (2 * p.kz);
   p.lx = lx;
   p.ly = ly;
@@ -1334,8 +1365,7 @@ static void s_layout_rt(int Rpad, int RT, int *groups, int *tiles_per_group, int
 
 static int s_stage(const cav_ctx *c, size_t tsz)
 {
-  const int TS = c->ek.nX + c->ek.nY + c->ek.nZ;
-  return 2 * 32 * (size_t)TS * 2 * tsz + 2 * 32 * tsz <= 150 * 1024 ? 32 : 16;
+  return s_stage_for(c->ek.nX + c->ek.nY + c->ek.nZ, tsz);
 }
 
 static int s_chunk(const cav_ctx *c, int ntarget)
@@ -1376,7 +1406,7 @@ int ewald_k_alloc(cav_ctx *c)
   if (c->sk_sum) (void)hipFree(c->sk_sum);
   CAV_HIP(hipMalloc(&c->sk_sum, sizeof(double) * (size_t)p.Rpad * SC));
   if (c->bmat) (void)hipFree(c->bmat);
-  const size_t bm_elems = (size_t)(p.Rpad / 16) * 4 * 64 * p.KSP;
+  const size_t bm_elems = p.nslab * slab_elems(p);
   CAV_HIP(hipMalloc(&c->bmat, bm_elems * tsz));
   if (c->precision == CAV_FP64)
     k_zero_t<double><<<grid_for(bm_elems), kBlock, 0, c->stream>>>(bm_elems, (double *)c->bmat);
@@ -1422,15 +1452,18 @@ static void launch_sk_rt(hipStream_t st, SArgs a, dim3 lgrid, size_t smem, int S
   allow_smem(k_sk<T, RT, 6>, smem);
   allow_smem(k_sk<T, RT, 7>, smem);
   allow_smem(k_sk<T, RT, 8>, smem);
-  switch (SCT) {
-  case 1: k_sk<T, RT, 1><<<grid, block, smem, st>>>(a); break;
-  case 2: k_sk<T, RT, 2><<<grid, block, smem, st>>>(a); break;
-  case 3: k_sk<T, RT, 3><<<grid, block, smem, st>>>(a); break;
-  case 4: k_sk<T, RT, 4><<<grid, block, smem, st>>>(a); break;
-  case 5: k_sk<T, RT, 5><<<grid, block, smem, st>>>(a); break;
-  case 6: k_sk<T, RT, 6><<<grid, block, smem, st>>>(a); break;
-  case 7: k_sk<T, RT, 7><<<grid, block, smem, st>>>(a); break;
-  default: k_sk<T, RT, 8><<<grid, block, smem, st>>>(a); break;
+  // one launch per 8 column tiles; each writes its own columns of the partials
+  for (a.ct0 = 0; a.ct0 < SCT; a.ct0 += 8) {
+    switch (std::min(8, SCT - a.ct0)) {
+    case 1: k_sk<T, RT, 1><<<grid, block, smem, st>>>(a); break;
+    case 2: k_sk<T, RT, 2><<<grid, block, smem, st>>>(a); break;
+    case 3: k_sk<T, RT, 3><<<grid, block, smem, st>>>(a); break;
+    case 4: k_sk<T, RT, 4><<<grid, block, smem, st>>>(a); break;
+    case 5: k_sk<T, RT, 5><<<grid, block, smem, st>>>(a); break;
+    case 6: k_sk<T, RT, 6><<<grid, block, smem, st>>>(a); break;
+    case 7: k_sk<T, RT, 7><<<grid, block, smem, st>>>(a); break;
+    default: k_sk<T, RT, 8><<<grid, block, smem, st>>>(a); break;
+    }
   }
 }
 
@@ -1438,26 +1471,36 @@ static void launch_sk_rt(hipStream_t st, SArgs a, dim3 lgrid, size_t smem, int S
 // (keeps the kernel at 2+ waves per SIMD).
 template <typename T> constexpr int kforce_nt() { return sizeof(T) == 8 ? 1 : 2; }
 
+// The force pass, one launch per K slab (one for nz axes of up to 128 kappa);
+// `a.bm` is the first slab's operand, the others follow it.
 template <typename T>
-static void launch_kforce_t(hipStream_t st, const FArgs &a, int KS, int g)
+static void launch_kforce_t(hipStream_t st, FArgs a, const EwaldKPlan &p, int g)
 {
   constexpr int NT = kforce_nt<T>();
   const size_t smem = 2 * (size_t)4 * 64 * a.KSP * sizeof(T);
-  if (KS <= 8) {
-    allow_smem(k_kforce<T, 8, NT>, smem);
-    k_kforce<T, 8, NT><<<g, 64 * kWavesF, smem, st>>>(a);
-  } else if (KS <= 12) {
-    allow_smem(k_kforce<T, 12, NT>, smem);
-    k_kforce<T, 12, NT><<<g, 64 * kWavesF, smem, st>>>(a);
-  } else if (KS <= 17) {
-    allow_smem(k_kforce<T, 17, NT>, smem);
-    k_kforce<T, 17, NT><<<g, 64 * kWavesF, smem, st>>>(a);
-  } else if (KS <= 24) {
-    allow_smem(k_kforce<T, 24, NT>, smem);
-    k_kforce<T, 24, NT><<<g, 64 * kWavesF, smem, st>>>(a);
-  } else {
-    allow_smem(k_kforce<T, 32, NT>, smem);
-    k_kforce<T, 32, NT><<<g, 64 * kWavesF, smem, st>>>(a);
+  const int KS = p.KSS, nks = p.KZ / 4;
+  const unsigned char *bm0 = reinterpret_cast<const unsigned char *>(a.bm);
+  for (int sl = 0; sl < p.nslab; ++sl) {
+    a.s0 = sl * p.KSS;
+    a.nks = std::min(p.KSS, nks - a.s0);
+    a.out4_add = sl > 0;
+    a.bm = bm0 + sl * slab_elems(p) * sizeof(T);
+    if (KS <= 8) {
+      allow_smem(k_kforce<T, 8, NT>, smem);
+      k_kforce<T, 8, NT><<<g, 64 * kWavesF, smem, st>>>(a);
+    } else if (KS <= 12) {
+      allow_smem(k_kforce<T, 12, NT>, smem);
+      k_kforce<T, 12, NT><<<g, 64 * kWavesF, smem, st>>>(a);
+    } else if (KS <= 17) {
+      allow_smem(k_kforce<T, 17, NT>, smem);
+      k_kforce<T, 17, NT><<<g, 64 * kWavesF, smem, st>>>(a);
+    } else if (KS <= 24) {
+      allow_smem(k_kforce<T, 24, NT>, smem);
+      k_kforce<T, 24, NT><<<g, 64 * kWavesF, smem, st>>>(a);
+    } else {
+      allow_smem(k_kforce<T, 32, NT>, smem);
+      k_kforce<T, 32, NT><<<g, 64 * kWavesF, smem, st>>>(a);
+    }
   }
 }
 
@@ -1503,7 +1546,7 @@ static int sk_pipeline(cav_ctx *c)
   const long plane = (long)p.Rpad * a.SC;
   k_chunk_sum<double><<<(plane + 63) / 64, kBlock, 0, c->stream>>>(c->sk_part, nch, plane, c->sk_sum);
   k_sk_reduce<T, double><<<grid_for(nthr), kBlock, 0, c->stream>>>(
-      1, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_sum, c->d_W, c->sk, (T *)c->bmat, c->part,
+      1, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSS, p.KSP, c->sk_sum, c->d_W, c->sk, (T *)c->bmat, c->part,
       1.0, nullptr, 0, 1.0, nullptr, 0, 0);
   CAV_HIP(hipGetLastError());
   return reduce_partials(c, grid_for(nthr), 3);
@@ -1542,7 +1585,7 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
   f.bm_stride = 0;
   timer_begin(c, c->t_force);
   constexpr int NT = kforce_nt<T>();
-  launch_kforce_t<T>(c->stream, f, p.KZ / 4, (npts + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF));
+  launch_kforce_t<T>(c->stream, f, p, (npts + 16 * NT * kWavesF - 1) / (16 * NT * kWavesF));
   timer_end(c, c->t_force);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
@@ -1628,7 +1671,7 @@ static int sk_pipeline_h(cav_ctx *c)
   k_chunk_sum4<<<(plane / 4 + 63) / 64, kBlock, 0, c->stream>>>(
       reinterpret_cast<const float *>(c->sk_part), nch, plane, c->sk_sum);
   k_sk_reduce<float, double><<<grid_for(nthr), kBlock, 0, c->stream>>>(
-      1, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSP, c->sk_sum, c->d_W, c->sk, (float *)c->bmat,
+      1, p.Rpad, a.SC, p.R, p.nZ, p.kz, p.KSS, p.KSP, c->sk_sum, c->d_W, c->sk, (float *)c->bmat,
       c->part, 1.0 / c->sA, (_Float16 *)c->bmat_h, c->nkh, c->sB, nullptr, 0, 0);
   CAV_HIP(hipGetLastError());
   return reduce_partials(c, grid_for(nthr), 3);
@@ -1645,6 +1688,15 @@ static size_t split_force_lds_p(const EwaldKPlan &p, int nkh, int nt = 2, int w 
 static size_t split_force_lds(const cav_ctx *c)
 {
   return split_force_lds_p(c->ek, c->nkh, c->tune_fnt, c->tune_fw);
+}
+
+// Whether the split form's fixed tiling covers this k lattice: at most 8 S-pass
+// column tiles and 4 force-pass K-steps (|nz| <= 31), and both passes' LDS
+// within a CU's 160 KB.  Wider lattices take the FP32 tiles.
+static bool split_fits(const EwaldKPlan &p, int nkh, int nt, int w)
+{
+  return roundup(2 * p.nZ, 16) / 16 <= 8 && nkh <= 4 &&
+         split_force_lds_p(p, nkh, nt, w) <= 160 * 1024 && split_sk_lds(p) <= 160 * 1024;
 }
 
 template <int NT, int W>
@@ -1713,7 +1765,7 @@ int launch_ewald_k(cav_ctx *c)
   if (c->precision == CAV_FP64) {
     CAV_TRY(sk_pipeline<double>(c));
     CAV_TRY(kforce<double>(c, c->n, c->tab, nullptr));
-  } else if (split_force_lds(c) <= 160 * 1024 && split_sk_lds(c->ek) <= 160 * 1024) {
+  } else if (split_fits(c->ek, c->nkh, c->tune_fnt, c->tune_fw)) {
     CAV_TRY(sk_pipeline_h(c));
     CAV_TRY(kforce_h(c));
   } else {  // k lattices too wide for the split form's LDS tiles: FP32 tiles
@@ -1965,7 +2017,7 @@ __global__ void kb_tables_h(const RepDev *rep, int kx, int ky, int kz, const dou
 
 static bool batch_split(const cav_batch *b)
 {
-  return b->precision == CAV_MIXED;
+  return b->precision == CAV_MIXED && split_fits(b->ek, b->nkh, b->f_nt, kWavesF);
 }
 
 int batch_ewald_k_plan(cav_batch *b)
@@ -2069,7 +2121,7 @@ int batch_ewald_k_alloc(cav_batch *b)
   BAT_HIP(hipMalloc(&b->sk, sizeof(double) * (size_t)b->B * p.Rpad * 2 * p.nZ));
   BAT_HIP(hipMalloc(&b->epart, sizeof(double) * (size_t)b->B * grid_for((long)p.Rpad * p.nZ)));
   // force-pass operands per replica (padding entries stay zero)
-  b->bm_stride = (long)(p.Rpad / 16) * 4 * 64 * p.KSP;
+  b->bm_stride = (long)(p.nslab * slab_elems(p));
   BAT_HIP(hipMalloc(&b->bmat, b->bm_stride * tsz * b->B));
   BAT_HIP(hipMemset(b->bmat, 0, b->bm_stride * tsz * b->B));
   if (split) {
@@ -2080,18 +2132,20 @@ int batch_ewald_k_alloc(cav_batch *b)
   BAT_HIP(hipMalloc(&b->d_W, sizeof(double) * b->W.size()));
   BAT_HIP(hipMemcpy(b->d_W, b->W.data(), sizeof(double) * b->W.size(), hipMemcpyHostToDevice));
   // force-pass items: groups of one workgroup's atoms per replica
-  const int wg = split ? 16 * b->f_nt * kWavesF : 16 * kforce_nt<double>() * kWavesF;
+  // (item ends are clipped, so a workgroup spanning more atoms than wg, as the
+  // field pass's FP32 tiles may, computes each atom once)
+  const int wg = split ? 16 * b->f_nt * kWavesF
+                       : 16 * (b->precision == CAV_FP64 ? kforce_nt<double>() : kforce_nt<float>()) *
+                             kWavesF;
   std::vector<int4> fitems;
   for (int r : order) {
     const RepDev &d = b->reps[r];
     for (int a0 = 0; a0 < d.n; a0 += wg)
-      fitems.push_back(make_int4(d.obase + a0, d.obase + d.n, 0, r));
+      fitems.push_back(make_int4(d.obase + a0, d.obase + std::min(d.n, a0 + wg), 0, r));
   }
   b->n_fitems = (int)fitems.size();
   BAT_HIP(hipMalloc(&b->f_items, sizeof(int4) * std::max(1, b->n_fitems)));
   BAT_HIP(hipMemcpy(b->f_items, fitems.data(), sizeof(int4) * fitems.size(), hipMemcpyHostToDevice));
-  if (split && (split_force_lds_p(p, b->nkh) > 160 * 1024 || split_sk_lds(p) > 160 * 1024))
-    return bfail(b, CAV_ERR_ARG, "k lattice too wide for the batched split form");
   return CAV_OK;
 }
 
@@ -2110,7 +2164,8 @@ static int batch_ewald_k_t(cav_batch *b)
   const dim3 tgrid(grid_for((long)b->maxn * TS), b->B);
   int groups, tpg, waves;
   s_layout_rt(p.Rpad, kRT, &groups, &tpg, &waves);
-  if (b->precision == CAV_FP64) {
+  const bool split = batch_split(b);
+  if (!split) {
     kb_tables<T><<<tgrid, kBlock, 0, b->stream>>>(b->d_rep, p.kx, p.ky, p.kz, b->x, b->y, b->z,
                                                   (Cx<T> *)b->tab);
     SArgs a;
@@ -2123,7 +2178,7 @@ static int batch_ewald_k_t(cav_batch *b)
     a.kz = p.kz;
     a.Rtiles = p.Rpad / 16;
     a.chunk = b->s_chunk;
-    a.stage = 2 * 32 * (size_t)TS * 2 * sizeof(T) + 2 * 32 * sizeof(T) <= 150 * 1024 ? 32 : 16;
+    a.stage = s_stage_for(TS, sizeof(T));
     a.tiles_per_group = tpg;
     a.q = b->q_o;
     a.tab = b->tab;
@@ -2166,12 +2221,17 @@ static int batch_ewald_k_t(cav_batch *b)
       batch_sk_h<3>(b, a, groups, smem, SC / 16, waves);
   }
   const int nthr = p.Rpad * p.nZ;
-  // partials: FP32 from the split S pass, FP64 from the FP64 one
-  k_sk_reduce<T, T><<<dim3(grid_for(nthr), b->B), kBlock, 0, b->stream>>>(
-      0, p.Rpad, SC, p.R, p.nZ, p.kz, p.KSP, reinterpret_cast<const T *>(b->sk_part), b->d_W, b->sk, (T *)b->bmat, b->epart,
-      1.0 / b->sA, batch_split(b) ? (_Float16 *)b->bmat_h : nullptr, b->nkh, b->sB, b->rchunks,
-      b->bm_stride, b->bmh_stride);
-  if (b->precision == CAV_FP64) {
+  // partials: FP32 from the split S pass, FP64 from the FP32 / FP64 tiles
+  if (split)
+    k_sk_reduce<T, float><<<dim3(grid_for(nthr), b->B), kBlock, 0, b->stream>>>(
+        0, p.Rpad, SC, p.R, p.nZ, p.kz, p.KSS, p.KSP, reinterpret_cast<const float *>(b->sk_part),
+        b->d_W, b->sk, (T *)b->bmat, b->epart, 1.0 / b->sA, (_Float16 *)b->bmat_h, b->nkh, b->sB,
+        b->rchunks, b->bm_stride, b->bmh_stride);
+  else
+    k_sk_reduce<T, double><<<dim3(grid_for(nthr), b->B), kBlock, 0, b->stream>>>(
+        0, p.Rpad, SC, p.R, p.nZ, p.kz, p.KSS, p.KSP, b->sk_part, b->d_W, b->sk, (T *)b->bmat,
+        b->epart, 1.0, nullptr, b->nkh, 1.0, b->rchunks, b->bm_stride, b->bmh_stride);
+  if (!split) {
     FArgs f;
     f.n = 0;
     f.TS = TS;
@@ -2196,7 +2256,7 @@ static int batch_ewald_k_t(cav_batch *b)
     f.items = b->f_items;
     f.rep = b->d_rep;
     f.bm_stride = b->bm_stride * (long)sizeof(T);
-    launch_kforce_t<T>(b->stream, f, p.KZ / 4, b->n_fitems);
+    launch_kforce_t<T>(b->stream, f, p, b->n_fitems);
   } else {
     FArgsH f;
     f.n = 0;
@@ -2280,7 +2340,7 @@ static int batch_field_t(cav_batch *b, double *out4)
   f.items = b->f_items;
   f.rep = b->d_rep;
   f.bm_stride = b->bm_stride * (long)sizeof(T);
-  launch_kforce_t<T>(b->stream, f, p.KZ / 4, b->n_fitems);
+  launch_kforce_t<T>(b->stream, f, p, b->n_fitems);
   BAT_HIP(hipGetLastError());
   return CAV_OK;
 }
@@ -2312,7 +2372,7 @@ static int batch_wsk_t(cav_batch *b, const double *u, double *out)
   a.kz = p.kz;
   a.Rtiles = p.Rpad / 16;
   a.chunk = b->s_chunk;
-  a.stage = 2 * 32 * (size_t)TS * 2 * sizeof(T) + 2 * 32 * sizeof(T) <= 150 * 1024 ? 32 : 16;
+  a.stage = s_stage_for(TS, sizeof(T));
   a.tiles_per_group = tpg;
   a.q = u;
   a.tab = b->tab;
@@ -2348,7 +2408,7 @@ int ewald_k_autotune(cav_ctx *c)
   // atoms per S-pass chunk: FP32 accumulation within a chunk (its rounding
   // error grows as sqrt(chunk)), FP64 across chunks; fewer chunks, fewer partials
   const int chunks[5] = {256, 512, 1024, 2048, 4096};
-  if (c->precision == CAV_MIXED) {
+  if (c->precision == CAV_MIXED && split_fits(c->ek, c->nkh, 1, kWavesF)) {
     // the split S pass stages one 32-atom table block; 4096-atom chunks only
     // pay beside the concurrent list build, where they tie with 2048 (within
     // the timing spread) while running 2x slower in order
@@ -2390,7 +2450,9 @@ int ewald_k_autotune(cav_ctx *c)
   std::vector<std::pair<float, int>> timed;
   for (int q = 0; q < (int)cands.size(); ++q) {
     CAV_TRY(apply(cands[q]));
-    if (c->precision == CAV_MIXED && split_force_lds(c) > 160 * 1024) continue;
+    if (c->precision == CAV_MIXED && cands[q].stage == kTabAtoms &&
+        !split_fits(c->ek, c->nkh, c->tune_fnt, c->tune_fw))
+      continue;
     float ms = 0;
     CAV_TRY(time_it(3, &ms));
     timed.push_back({ms, q});

@@ -72,7 +72,9 @@ struct EwaldKPlan {
   int R = 0;      // rows (nx,ny)
   int Rpad = 0;   // rows rounded up to 16
   int KZ = 0;     // 2*nZ rounded up to a multiple of 4 (real-embedded nz axis)
-  int KSP = 0;    // K-steps (KZ/4) rounded up to a multiple of 4: fragment stride
+  int nslab = 1;  // K slabs of the FP32/FP64 force pass (at most 32 K-steps each)
+  int KSS = 0;    // K-steps per slab (the last slab may hold fewer)
+  int KSP = 0;    // K-steps of a slab padded to whole, odd 16-byte vectors: fragment stride
   int nk_ref = 0; // (2kx)(2ky)(2kz), the reference's used count
   double lx = 1, ly = 1, lz = 1, lxyz_inv = 0, dip_coef = 0;
   std::vector<double> W;        // R*nZ: multiplicity(k) * f(k) over the half space

@@ -3,7 +3,12 @@
 // objects compute them at the end of a velocity-Verlet step
 // (md_simulator.cpp:619-633):
 //   finalize_temperature (kinetic_energy.cpp:162-202): T = 2 KE / (k_b dof),
-//     KE = 1/2 sum m (v - v_cm)^2 (v_cm = 0 for n_r_df = 0), dof = 3n - n_r_df;
+//     KE = 1/2 sum m v'^2 with v' = v (n_r_df 0) or v - v_cm (1..3), dof =
+//     3n - n_r_df.  For 4..6 the reference forms the inertia tensor I about
+//     the mass centre and tests matrix_inverse_3d(I, I^-1) != 0
+//     (kinetic_energy.cpp:92-105), whose 0 is success: an invertible I leaves
+//     KE = 0 (its warning branch), an exactly singular one a zero I^-1, so
+//     only v_cm is removed; the same holds here;
 //   constraint berendsen (constraint/berendsen.cpp:120-146) and nve
 //     (constraint/nve.cpp:134-178): v *= lambda every `step` steps;
 //   finalize_pressure (kinetic_energy.cpp:210-292): P = n k_b T / V +
@@ -24,8 +29,14 @@ enum ThermoSlot {
   T_KE = 8,     // kinetic energy
   T_TEMP = 9,   // last finalized temperature
   T_LAMBDA = 10,
-  T_PRESS = 11  // last finalized pressure
+  T_PRESS = 11, // last finalized pressure
+  T_MR = 12,    // sum m r (3)
+  T_PCM = 15,   // mass centre (3)
+  T_INERTIA = 18,  // inertia tensor about it: xx yy zz xy yz zx (6)
+  T_KEEP = 24,  // n_r_df 4..6: 1 when the sum is kept (I exactly singular)
+  T_HEAD = 32   // per-block partials follow
 };
+constexpr int kThermoPartWidth = 6;  // widest per-block partial (the inertia tensor)
 
 __global__ void k_mv_part(int n, const double *vx, const double *vy, const double *vz,
                           const int *type, const double *mass, double *part)
@@ -50,17 +61,71 @@ __global__ void k_mv_part(int n, const double *vx, const double *vy, const doubl
   if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// sum m r (owned_position_cm_mpi_domain, kinetic_energy.cpp:391-411)
+__global__ void k_mr_part(int n, const double *x, const double *y, const double *z,
+                          const int *type, const double *mass, double *part)
+{
+  __shared__ double red[3][kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v[3] = {0, 0, 0};
+  if (i < n) {
+    const double m = mass[type[i]];
+    v[0] = x[i] * m;
+    v[1] = y[i] * m;
+    v[2] = z[i] * m;
+  }
+  for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// The inertia tensor about the mass centre (owned_inertia_tensor_cm_mpi_domain,
+// kinetic_energy.cpp:611-640), p = r - p_cm: xx yy zz xy yz zx.
+__global__ void k_inertia_part(int n, const double *x, const double *y, const double *z,
+                               const int *type, const double *mass, const double *t, double *part)
+{
+  __shared__ double red[kThermoPartWidth][kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v[kThermoPartWidth] = {0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const double m = mass[type[i]];
+    const double px = x[i] - t[T_PCM], py = y[i] - t[T_PCM + 1], pz = z[i] - t[T_PCM + 2];
+    v[0] = m * (py * py + pz * pz);
+    v[1] = m * (px * px + pz * pz);
+    v[2] = m * (px * px + py * py);
+    v[3] = -(m * (px * py));
+    v[4] = -(m * (py * pz));
+    v[5] = -(m * (pz * px));
+  }
+  for (int k = 0; k < kThermoPartWidth; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < kThermoPartWidth; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < kThermoPartWidth)
+    part[kThermoPartWidth * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// remove: 0 nothing, 1 the centre-of-mass velocity, 2 the same where t[T_KEEP]
+// is set and an empty sum otherwise (n_r_df 4..6)
 __global__ void k_mv2_part(int n, const double *vx, const double *vy, const double *vz,
-                           const int *type, const double *mass, const double *t, int remove_cm,
+                           const int *type, const double *mass, const double *t, int remove,
                            double *part)
 {
   __shared__ double red[kBlock];
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double e = 0;
-  if (i < n) {
+  if (i < n && (remove < 2 || t[T_KEEP] != 0.0)) {
     const double m = mass[type[i]];
     double a = vx[i], b = vy[i], c = vz[i];
-    if (remove_cm) {
+    if (remove >= 1) {
       a -= t[T_VCM];
       b -= t[T_VCM + 1];
       c -= t[T_VCM + 2];
@@ -97,6 +162,29 @@ __global__ void k_thermo_sum(int nblk, int width, const double *part, double *t,
 __global__ void k_vcm(double *t)
 {
   for (int d = 0; d < 3; ++d) t[T_VCM + d] = t[T_MV + d] / t[T_MV + 3];
+}
+
+__global__ void k_pcm(double *t)
+{
+  for (int d = 0; d < 3; ++d) t[T_PCM + d] = t[T_MR + d] / t[T_MV + 3];
+}
+
+// matrix_inverse_3d's determinant (utility/common_template_functions.h:168-172);
+// the reference keeps the sum only when it is exactly zero (kinetic_energy.cpp:92-105).
+__global__ void k_inertia_keep(double *t)
+{
+  const double *q = t + T_INERTIA;
+  double A[3][3];
+  A[0][0] = q[0];
+  A[1][1] = q[1];
+  A[2][2] = q[2];
+  A[0][1] = A[1][0] = q[3];
+  A[1][2] = A[2][1] = q[4];
+  A[2][0] = A[0][2] = q[5];
+  const double det = (A[0][0] * (A[1][1] * A[2][2] - A[2][1] * A[1][2])) -
+                     (A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])) +
+                     (A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]));
+  t[T_KEEP] = det == 0.0 ? 1.0 : 0.0;
 }
 
 // What the scalar thread computes after the sums: the kinetic energy, then
@@ -180,7 +268,7 @@ int thermo_alloc(cav_ctx *c)
 {
   if (c->d_thermo) (void)hipFree(c->d_thermo);
   c->d_thermo = nullptr;
-  const size_t count = 16 + 4 * (size_t)grid_for(std::max(c->n, 1));
+  const size_t count = T_HEAD + kThermoPartWidth * (size_t)grid_for(std::max(c->n, 1));
   CAV_HIP(hipMalloc(&c->d_thermo, sizeof(double) * count));
   CAV_HIP(hipMemsetAsync(c->d_thermo, 0, sizeof(double) * count, c->stream));
   return CAV_OK;
@@ -189,19 +277,26 @@ int thermo_alloc(cav_ctx *c)
 // kinetic_energy_mpi_domain (kinetic_energy.cpp:30-138) into t[T_KE].
 static int kinetic_energy(cav_ctx *c)
 {
-  if (n_r_df(c) > 3)
-    return fail(c, CAV_ERR_STATE,
-                "kinetic energy with n_r_df > 3 (angular momentum removed) is not on this path");
-  double *t = c->d_thermo, *part = c->d_thermo + 16;
+  double *t = c->d_thermo, *part = c->d_thermo + T_HEAD;
   const int nb = grid_for(c->n);
-  const bool cm = n_r_df(c) >= 1;
-  if (cm) {
+  const int nr = n_r_df(c);
+  const int remove = nr == 0 ? 0 : (nr <= 3 ? 1 : 2);
+  if (remove >= 1) {
     k_mv_part<<<nb, kBlock, 0, c->stream>>>(c->n, c->vx, c->vy, c->vz, c->type, c->d_mass, part);
     k_thermo_sum<<<1, kBlock, 0, c->stream>>>(nb, 4, part, t, T_MV);
     k_vcm<<<1, 1, 0, c->stream>>>(t);
   }
+  if (remove == 2) {
+    k_mr_part<<<nb, kBlock, 0, c->stream>>>(c->n, c->x, c->y, c->z, c->type, c->d_mass, part);
+    k_thermo_sum<<<1, kBlock, 0, c->stream>>>(nb, 3, part, t, T_MR);
+    k_pcm<<<1, 1, 0, c->stream>>>(t);
+    k_inertia_part<<<nb, kBlock, 0, c->stream>>>(c->n, c->x, c->y, c->z, c->type, c->d_mass, t,
+                                                part);
+    k_thermo_sum<<<1, kBlock, 0, c->stream>>>(nb, kThermoPartWidth, part, t, T_INERTIA);
+    k_inertia_keep<<<1, 1, 0, c->stream>>>(t);
+  }
   k_mv2_part<<<nb, kBlock, 0, c->stream>>>(c->n, c->vx, c->vy, c->vz, c->type, c->d_mass, t,
-                                          cm ? 1 : 0, part);
+                                          remove, part);
   k_thermo_sum<<<1, kBlock, 0, c->stream>>>(nb, 1, part, t, T_MV2);
   k_thermo_scalar<<<1, 1, 0, c->stream>>>(t, OP_KE, scalars(c), c->scal);
   CAV_HIP(hipGetLastError());
@@ -248,7 +343,7 @@ int thermo_end_of_step(cav_ctx *c, long ts)
 int thermo_read(cav_ctx *c, double *ke, double *temp, double *press)
 {
   if (ke) CAV_TRY(kinetic_energy(c));
-  double t[16];
+  double t[T_HEAD];
   CAV_HIP(hipMemcpyAsync(t, c->d_thermo, sizeof(t), hipMemcpyDeviceToHost, c->stream));
   CAV_HIP(hipStreamSynchronize(c->stream));
   if (ke) *ke = t[T_KE];

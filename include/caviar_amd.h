@@ -111,8 +111,10 @@ int cav_set_ewald_k(cav_ctx *ctx, double alpha, double k_electrostatic, int kx_m
  * (atom_data.cpp:121-151).  With temperature_process the temperature is
  * finalized after setup and at the end of every step
  * (kinetic_energy.cpp:162-202), with pressure_process the pressure at the end
- * of every step (kinetic_energy.cpp:210-292); n_r_df above 3 (angular momentum
- * removed) is not on this path. */
+ * of every step (kinetic_energy.cpp:210-292).  The kinetic energy removes the
+ * centre-of-mass velocity for n_r_df 1..3; for 4..6 it is 0 unless the
+ * inertia tensor about the mass centre is exactly singular, as the reference's
+ * test of matrix_inverse_3d's return code makes it (kinetic_energy.cpp:92-105). */
 int cav_set_thermo(cav_ctx *ctx, double k_b, int n_r_df, int temperature_process,
                    int pressure_process);
 /* constraint berendsen: temperature, coupling, dt, step, lambda_min, lambda_max

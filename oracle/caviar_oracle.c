@@ -1394,26 +1394,66 @@ static int n_r_df(const orc_sys *s)
   return bc_sum == 0 ? 6 : (bc_sum == 3 ? 3 : 6 - bc_sum);
 }
 
-/* kinetic_energy_mpi_domain (kinetic_energy.cpp:30-138) for n_r_df 0..3: the
- * centre-of-mass velocity (owned_velocity_cm, :413-462) removed for 1..3.
- * Returns NAN for n_r_df 4..6 (the angular-momentum correction is not on
- * this path). */
+/* owned_velocity_cm (kinetic_energy.cpp:413-457): sum m v / sum m; also the
+ * mass sum. */
+static double velocity_cm(const orc_sys *s, double vcm[3])
+{
+  double f[3] = {0, 0, 0}, msum = 0.0;
+  for (int i = 0; i < s->n; ++i) {
+    const double m = s->mass[s->type[i]];
+    msum += m;
+    for (int d = 0; d < 3; ++d)
+      f[d] += s->vel[3 * i + d] * m;
+  }
+  for (int d = 0; d < 3; ++d)
+    vcm[d] = f[d] / msum;
+  return msum;
+}
+
+/* kinetic_energy_mpi_domain (kinetic_energy.cpp:30-138): 1/2 sum m v'^2 with
+ * v' = v for n_r_df 0 and v - v_cm for 1..3 (owned_velocity_cm, :413-457).
+ * For 4..6 the reference forms the inertia tensor I about the mass centre
+ * (:611-640, p_cm :391-411) and tests matrix_inverse_3d(I, I^-1) != 0
+ * (:92-105), whose 0 is success: an invertible I (det != 0, the usual case)
+ * takes the warning branch and leaves the kinetic energy 0; an exactly
+ * singular one leaves I^-1 zero, so only v_cm is removed.  The rigid-rotation
+ * term (I^-1 L) x (r - p_cm) is then zero in every case and L never matters. */
 double orc_kinetic_energy(const orc_sys *s)
 {
   const int nr = n_r_df(s);
-  if (nr > 3)
-    return NAN;
   double vcm[3] = {0, 0, 0};
   if (nr >= 1) {
-    double f[3] = {0, 0, 0}, msum = 0.0;
-    for (int i = 0; i < s->n; ++i) {
-      const double m = s->mass[s->type[i]];
-      msum += m;
+    const double msum = velocity_cm(s, vcm);
+    if (nr >= 4) {
+      double pcm[3] = {0, 0, 0}, A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int i = 0; i < s->n; ++i) {
+        const double m = s->mass[s->type[i]];
+        for (int d = 0; d < 3; ++d)
+          pcm[d] += s->pos[3 * i + d] * m;
+      }
       for (int d = 0; d < 3; ++d)
-        f[d] += s->vel[3 * i + d] * m;
+        pcm[d] = pcm[d] / msum;
+      for (int i = 0; i < s->n; ++i) {
+        const double m = s->mass[s->type[i]];
+        const double p[3] = {s->pos[3 * i] - pcm[0], s->pos[3 * i + 1] - pcm[1],
+                             s->pos[3 * i + 2] - pcm[2]};
+        A[0][0] += m * (p[1] * p[1] + p[2] * p[2]);
+        A[1][1] += m * (p[0] * p[0] + p[2] * p[2]);
+        A[2][2] += m * (p[0] * p[0] + p[1] * p[1]);
+        A[0][1] -= m * (p[0] * p[1]);
+        A[1][2] -= m * (p[1] * p[2]);
+        A[2][0] -= m * (p[2] * p[0]);
+      }
+      A[1][0] = A[0][1];
+      A[2][1] = A[1][2];
+      A[0][2] = A[2][0];
+      /* matrix_inverse_3d's determinant (common_template_functions.h:168-172) */
+      const double det = (A[0][0] * (A[1][1] * A[2][2] - A[2][1] * A[1][2])) -
+                         (A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])) +
+                         (A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]));
+      if (det != 0.0)
+        return 0.0;
     }
-    for (int d = 0; d < 3; ++d)
-      vcm[d] = f[d] / msum;
   }
   double e = 0.0;
   for (int i = 0; i < s->n; ++i) {

@@ -97,7 +97,7 @@ void orc_set_thermo(orc_sys *s, double k_b, int n_r_df, int temperature_process,
 /* Md_simulator::current_step of the next orc_step (thermostats act when it is
  * a multiple of their step). */
 void orc_set_step_index(orc_sys *s, long step);
-double orc_kinetic_energy(const orc_sys *s);   /* NAN for n_r_df 4..6 */
+double orc_kinetic_energy(const orc_sys *s);   /* 0 for n_r_df 4..6 unless I is singular */
 double orc_temperature(const orc_sys *s);      /* last finalized (0 until then) */
 double orc_pressure(const orc_sys *s);
 /* constraint berendsen (berendsen.cpp:33-146) and nve (nve.cpp:33-178). */

@@ -572,6 +572,23 @@ def thermo_cases():
     c.thermo = ThermoSettings(k_b=1.0, temperature_process=False)
     c.berendsen = BerendsenSettings(temperature=1.0, coupling=0.1, dt=0.002, step=1, lambda_max=1.01)
     run_case("thermo_berclamp200_run6", c, "run+thermo", run_steps=6, dt=0.002)
+    # n_r_df 4..6 (kinetic_energy.cpp:75-116): the reference tests
+    # matrix_inverse_3d's return code the wrong way round, so the kinetic
+    # energy (and the temperature) is 0 for an invertible inertia tensor.  A
+    # slab box (boundary 1 1 0, so n_r_df 4) with the pressure and berendsen
+    # (lambda clamped to lambda_max at T = 0), and n_r_df 5 set explicitly.
+    d = configs.ions(n=300, kmax=3, seed=107)
+    d.bc = np.array([1, 1, 0], dtype=np.int32)
+    d.ewald_k.kz_max = 6
+    d.ewald_k.slab_normal_axis = 2
+    d.ewald_k.slab_lz = 3.0 * float(d.size[2])
+    d.thermo = ThermoSettings(k_b=1.0, temperature_process=True, pressure_process=True)
+    d.berendsen = BerendsenSettings(temperature=1.2, coupling=0.02, dt=0.002, step=3)
+    run_case("thermo_slabrot300_setup", d, "setup+thermo")
+    run_case("thermo_slabrot300_run12", d, "run+thermo", run_steps=12, dt=0.002)
+    e = configs.ions(n=240, kmax=3, seed=109)
+    e.thermo = ThermoSettings(k_b=1.0, n_r_df=5, temperature_process=True, pressure_process=True)
+    run_case("thermo_rdf5_240_run12", e, "run+thermo", run_steps=12, dt=0.002)
 
 
 CASES = {"original": original_cases, "c2": c2_cases, "slab": slab_cases, "ljsr": ljsr_cases,

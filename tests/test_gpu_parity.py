@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import fixtures as F
-from caviar_amd import CaviarError, Engine, NaNAccelerationError, configs
+from caviar_amd import Batch, CaviarError, Engine, NaNAccelerationError, configs
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -139,6 +139,71 @@ def test_mixed_lattice_shapes(gpu, k):
     rms = np.sqrt(((a32 - a64) ** 2).sum(axis=1).mean() / (a64 ** 2).sum(axis=1).mean())
     assert rms <= 1e-5, (k, rms)
     assert abs(out["mixed"][1] - out["fp64"][1]) <= 1e-6 * abs(out["fp64"][1]), k
+
+
+@pytest.mark.parametrize("k", [(2, 2, 32), (3, 1, 40), (2, 3, 70), (120, 120, 2)])
+def test_wide_lattices_against_oracle(gpu, k):
+    """k lattices past one force-pass K slab (|nz| > 31: 2 and 3 slabs of up
+    to 32 K-steps), past 8 S-pass column tiles (2 and 3 launches of column
+    groups), and with [X | Y | Z] records too long for 16-atom S-pass stages
+    (kx = ky = 120: 8-atom stages).  FP64 accelerations, S(k), the k-space
+    energy and point fields against the oracle (1e-12); the mixed form, which
+    takes the FP32 tiles here, within 1e-5 force RMS of FP64."""
+    kx, ky, kz = k
+    s = configs.ions(n=300 if kx > 100 else 900, rho=0.3, kmax=1, seed=7 + kz)
+    s.ewald_k.kx_max, s.ewald_k.ky_max, s.ewald_k.kz_max = kx, ky, kz
+    print(f"[case] wide lattice {k}", flush=True)
+    o = Oracle(s)
+    o.setup()
+    e = Engine(s.copy())
+    e.setup()
+    a64 = e.get("acc")
+    assert F.rel_err(a64, o.get("acc")) <= 1e-12, k
+    sk, osk = e.sk(), o.sk()
+    assert F.rel_err(np.c_[sk.real, sk.imag], np.c_[osk.real, osk.imag]) <= 1e-12, k
+    ek = e.energy(lj=False)[2]
+    assert abs(ek - o.energy_k()) <= 1e-12 * abs(o.energy_k()), k
+    pts = e.get("pos")[:50] + np.array([0.1, -0.2, 0.3])
+    f, p = e.point_field("k", pts)
+    fo, po = o.point_k(pts)
+    assert F.rel_err(f, fo) <= 1e-12, k
+    assert F.rel_err(p, po) <= 1e-12, k
+    em = Engine(s.copy(), precision="mixed")
+    em.setup()
+    a32 = em.get("acc")
+    rms = np.sqrt(((a32 - a64) ** 2).sum(axis=1).mean() / (a64 ** 2).sum(axis=1).mean())
+    assert rms <= 1e-5, (k, rms)
+
+
+def test_wide_lattice_batch(gpu):
+    """Batched replicas with kz = 40 (two K slabs, two column groups): FP64
+    against single-system runs (1e-12), the mixed form (FP32 tiles) within
+    1e-5 force RMS, and the per-replica k-space field of both forms."""
+    reps = []
+    for r, n in enumerate((300, 900, 500)):
+        s = configs.ions(n=n, rho=0.3, kmax=1, seed=40 + r)
+        s.ewald_k.kx_max, s.ewald_k.ky_max, s.ewald_k.kz_max = 2, 3, 40
+        reps.append(s)
+    print("[case] batch of 3, lattice (2, 3, 40)", flush=True)
+    bd = Batch(reps)
+    bd.setup()
+    bm = Batch(reps, precision="mixed")
+    bm.setup()
+    for r, s in enumerate(reps):
+        e = Engine(s.copy())
+        e.setup()
+        a = bd.get(r, "acc")
+        assert F.rel_err(a, e.get("acc")) <= 1e-12, r
+        am = bm.get(r, "acc")
+        rms = np.sqrt(((am - a) ** 2).sum(axis=1).mean() / (a ** 2).sum(axis=1).mean())
+        assert rms <= 1e-5, (r, rms)
+    bd.step(0.002, 5)
+    bm.step(0.002, 5)
+    for r, s in enumerate(reps):
+        e = Engine(s.copy())
+        e.setup()
+        e.step(0.002, 5)
+        assert F.rel_err(bd.get(r, "acc"), e.get("acc")) <= 1e-9, r
 
 
 def test_fp64_is_bitwise_reproducible(gpu):

@@ -929,8 +929,10 @@ struct FArgsH {
 
 // Force pass of the split form: A = the atoms' Z (kappa = 2 nz + re/im, from
 // the split records), B = the operand blocks staged in LDS per row block and
-// shared by the workgroup's waves.  The workgroup's blocked [X | Y] rows are staged
-// in LDS once, so the epilogue's per-(atom, row) phase factors are LDS reads.
+// shared by the workgroup's waves.  The workgroup's blocked X rows and Y rows
+// of ny >= 0 are staged in LDS once (Y(-ny) = conj Y(ny)), so the epilogue's
+// per-(atom, row) phase factors are LDS reads.  Staging a third fewer rows
+// leaves LDS on the CU for list-build waves when both run concurrently.
 template <int NK, int NT, int W>
 __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
 {
@@ -941,7 +943,9 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
   const int nwaves = blockDim.x >> 6;
   const int blk_bytes = split_blk_bytes(NK);
   const int wg_atoms = 16 * NT * nwaves;  // a multiple of kTabAtoms
-  const int xy_bytes = wg_atoms / kTabAtoms * a.TS * kTabRow;
+  // staged rows per block: X(0..kx) and Y(0..ky); Y(-ny) = conj Y(ny)
+  const int TSs = a.nX + a.ky + 1;
+  const int xy_bytes = wg_atoms / kTabAtoms * TSs * kTabRow;
   // operand buffers at smem_raw + xy_bytes + {0, blk_bytes}, addressed from the
   // __shared__ array itself so the compiler emits LDS (not flat) accesses
   const Cx<float> *lxy = reinterpret_cast<const Cx<float> *>(smem_raw);
@@ -967,8 +971,15 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
     // the workgroup's table blocks (its first atom starts a block)
     const long cnt = min((long)wg_atoms, a.n - wg_base);
     const int nb = (int)((cnt + kTabAtoms - 1) / kTabAtoms);
-    copy_to_lds_async(tab + (wg_base + toff) / kTabAtoms * a.TS * kTabRow, smem_raw,
-                      nb * a.TS * kTabRow, wave, nwaves, lane);
+    // (the X rows and the Y rows of ny >= 0 of each block)
+    const unsigned char *src = tab + (wg_base + toff) / kTabAtoms * a.TS * kTabRow;
+    for (int bk = 0; bk < nb; ++bk) {
+      copy_to_lds_async(src + (size_t)bk * a.TS * kTabRow, smem_raw + bk * TSs * kTabRow,
+                        a.nX * kTabRow, wave, nwaves, lane);
+      copy_to_lds_async(src + ((size_t)bk * a.TS + a.nX + a.ky) * kTabRow,
+                        smem_raw + (bk * TSs + a.nX) * kTabRow, (a.ky + 1) * kTabRow, wave,
+                        nwaves, lane);
+    }
   }
 
   // A fragments: atom l%16 of tile tau, kappa = 32 s + 8 (l/16) + i, i.e.
@@ -1010,7 +1021,7 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
 #pragma unroll
   for (int tau = 0; tau < NT; ++tau) {
     const int rel = wave * 16 * NT + tau * 16 + 4 * lk;
-    xoff[tau] = (rel / kTabAtoms) * a.TS * (kTabRow / 8) + rel % kTabAtoms;
+    xoff[tau] = (rel / kTabAtoms) * TSs * (kTabRow / 8) + rel % kTabAtoms;
   }
   // lattice row of the lane, r = rb*16 + l%16 = nxi nY + nyi, kept incrementally;
   // rows past R have zero operands (U = 0) and read the phase factors of row 0
@@ -1041,16 +1052,17 @@ __global__ __launch_bounds__(64 * W) void k_kforce_h(FArgsH a)
     // phase factors X Y of the lane's (atom, row) pairs, formed while the
     // products below run (atoms past n give sums that are never written)
     const bool rin = rb * 16 + lr < a.R;
-    const int rx = rin ? nxi : 0, ry = rin ? nyi : 0;
-    const float wx = (float)rx, wy = (float)(ry - a.ky);
+    const int rx = rin ? nxi : 0, yk = rin ? nyi - a.ky : 0, ya = yk < 0 ? -yk : yk;
+    const float wx = (float)rx, wy = (float)yk, ys = yk < 0 ? -1.f : 1.f;
     float pr[NT][4], pi[NT][4];
 #pragma unroll
     for (int tau = 0; tau < NT; ++tau) {
       const float4 *px = reinterpret_cast<const float4 *>(lxy + xoff[tau] + rx * (kTabRow / 8));
-      const float4 *py = reinterpret_cast<const float4 *>(lxy + xoff[tau] + (offY + ry) * (kTabRow / 8));
+      const float4 *py = reinterpret_cast<const float4 *>(lxy + xoff[tau] + (offY + ya) * (kTabRow / 8));
       const float4 x01 = px[0], x23 = px[1], y01 = py[0], y23 = py[1];
       const float xr[4] = {x01.x, x01.z, x23.x, x23.z}, xi[4] = {x01.y, x01.w, x23.y, x23.w};
-      const float yr[4] = {y01.x, y01.z, y23.x, y23.z}, yi[4] = {y01.y, y01.w, y23.y, y23.w};
+      const float yr[4] = {y01.x, y01.z, y23.x, y23.z};
+      const float yi[4] = {ys * y01.y, ys * y01.w, ys * y23.y, ys * y23.w};
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         pr[tau][e] = xr[e] * yr[e] - xi[e] * yi[e];
@@ -1677,12 +1689,12 @@ static int sk_pipeline_h(cav_ctx *c)
   return reduce_partials(c, grid_for(nthr), 3);
 }
 
-// LDS of a split force-pass workgroup: two operand blocks plus the blocked
-// [X | Y] rows of its 16 nt w atoms (w waves).
+// LDS of a split force-pass workgroup: two operand blocks plus the blocked X
+// rows and Y rows of ny >= 0 of its 16 nt w atoms (w waves).
 static size_t split_force_lds_p(const EwaldKPlan &p, int nkh, int nt = 2, int w = kWavesF)
 {
   return 2 * (size_t)split_blk_bytes(nkh) +
-         (size_t)16 * nt * w / kTabAtoms * (p.nX + p.nY) * kTabRow;
+         (size_t)16 * nt * w / kTabAtoms * (p.nX + p.ky + 1) * kTabRow;
 }
 
 static size_t split_force_lds(const cav_ctx *c)

@@ -162,9 +162,11 @@ def lattice(kmax):
     nz = 2 * kmax + 1
     rpad = -(-((kmax + 1) * (2 * kmax + 1)) // 16) * 16
     sc = -(-(2 * nz) // 16) * 16
+    # split S pass: the matrix cores take the nz != 0 columns (nz = 0 is summed on the VALU)
+    sch = max(16, -(-(2 * (nz - 1)) // 16) * 16)
     kz = -(-(2 * nz) // 4) * 4
     nkh = max(1, -(-(2 * (nz - 1)) // 32))  # split force pass: z = 0 seeds the accumulators
-    return rpad, sc, kz, nkh
+    return rpad, sc, sch, kz, nkh
 
 
 def separable_flop(kmax):
@@ -189,7 +191,7 @@ def kernel_rooflines(n, kmax, ppa, lj_ppa, tm32, tm64):
     products per term and the row padding; rocprofv3's
     SQ_INSTS_VALU_MFMA_MOPS_* x 512 gives the same totals)."""
     nk = (2 * kmax) ** 3
-    rpad, sc, kz, nkh = lattice(kmax)
+    rpad, sc, sch, kz, nkh = lattice(kmax)
     sep_sk, sep_kf = separable_flop(kmax)
     traffic = {}
     if os.path.exists(TRAFFIC_FILE):
@@ -199,7 +201,7 @@ def kernel_rooflines(n, kmax, ppa, lj_ppa, tm32, tm64):
         split = form == "mixed"
         mfma = "fp16_mfma" if split else "fp64_mfma"
         exec_kf = (4 * rpad * 32 * nkh * 2 * 3 if split else 2 * 4 * rpad * kz) * n
-        exec_sk = (2 * rpad * sc * 2 * 3 if split else 2 * rpad * sc * 2) * n
+        exec_sk = (2 * rpad * sch * 2 * 3 if split else 2 * rpad * sc * 2) * n
         rows = {
             "ewald_k_force": ("mfma", mfma, sep_kf * n, FLOP_KF * nk * n, exec_kf),
             "ewald_k_sk": ("mfma", mfma, sep_sk * n, FLOP_SK * nk * n, exec_sk),

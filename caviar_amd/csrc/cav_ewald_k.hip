@@ -730,6 +730,10 @@ struct SArgsH {
 // adjacent along the reduction axis.  In the blocked layout a lane's four
 // atoms are adjacent in a row, so each operand is read with two (records) or
 // one (Z half) 16-byte LDS read at a compile-time offset from a per-lane base.
+// The matrix cores take the 2 (nZ - 1) columns of nz != 0 (SCT tiles); the
+// nz = 0 column, Z = 1 for every atom, is the plain sum of the A entries:
+// each K-step's four atoms of a lane are summed in FP32 and accumulated in
+// FP64 beside the products.
 template <int RT, int SCT>
 __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
 {
@@ -738,9 +742,9 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
   const unsigned rec_bytes = (unsigned)a.TS * kTabRow;  // [X | Y] rows of a block
   const unsigned zrow_bytes = (unsigned)a.nZ * kTabRow;  // (Zr, -Zi) rows of a block
   // per buffer: records (X already carries each atom's scaled charge); nZ
-  // (Zr, -Zi) rows then nZ (Zi, Zr) rows, padded to nZ + 8 SCT rows (rows past
-  // them are never written: their columns are masked on output)
-  const unsigned zrows = (unsigned)(a.nZ + 8 * SCT);
+  // (Zr, -Zi) rows then nZ (Zi, Zr) rows, padded to nZ + 8 ceil(2 nZ / 16)
+  // rows as split_sk_lds allocates them
+  const unsigned zrows = (unsigned)(a.nZ + 8 * ((2 * a.nZ + 15) / 16));
   // the (Zi, Zr) rows start zpad bytes later, so that they sit 32 banks away
   // from the (Zr, -Zi) rows: a column tile's even and odd lanes then read
   // disjoint banks
@@ -790,9 +794,21 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
     xo[r] = (unsigned)(ok ? row / a.nY : 0) * kTabRow + 32 * lk;
     yo[r] = (unsigned)(a.nX + (ok ? row % a.nY : 0)) * kTabRow + 32 * lk;
   }
-  // column s*16 + l%16: even columns are (Zr, -Zi) rows, odd ones (Zi, Zr)
-  const unsigned zo = rec_bytes + (lr & 1) * zpad +
-                      (unsigned)((lr & 1) * a.nZ + (lr >> 1)) * kTabRow + 16 * lk;
+  // product column s*16 + l%16: even columns are (Zr, -Zi) rows, odd ones
+  // (Zi, Zr); column pair c' = s*8 + l%16/2 is nz != 0 number c' (table row
+  // c' below kz, c' + 1 above); pairs past nZ - 1 read row 0 and are masked
+  // on output
+  const int kz = (a.nZ - 1) >> 1, ncp = a.nZ - 1;
+  unsigned zo[SCT];
+#pragma unroll
+  for (int s = 0; s < SCT; ++s) {
+    const int cp = s * 8 + (lr >> 1);
+    const int zc = cp >= ncp ? 0 : (cp < kz ? cp : cp + 1);
+    zo[s] = rec_bytes + (lr & 1) * zpad + (unsigned)((lr & 1) * a.nZ + zc) * kTabRow + 16 * lk;
+  }
+  double z0r[RT], z0i[RT];  // the lane's share of the nz = 0 column per row tile
+#pragma unroll
+  for (int r = 0; r < RT; ++r) z0r[r] = z0i[r] = 0.0;
   const bool active = rt0 < rt_end;
 
   // The last wave of the workgroup is the producer: it streams each block's
@@ -873,6 +889,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
         const float4 x01 = px[0], x23 = px[1], y01 = py[0], y23 = py[1];
         const float xr[4] = {x01.x, x01.z, x23.x, x23.z}, xi[4] = {x01.y, x01.w, x23.y, x23.w};
         const float yr[4] = {y01.x, y01.z, y23.x, y23.z}, yi[4] = {y01.y, y01.w, y23.y, y23.w};
+        float tr = 0.f, ti = 0.f;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const float pr = xr[u] * yr[u] - xi[u] * yi[u];
@@ -881,11 +898,15 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
           split_pair_mix(pr, pi, h, l);
           ahi[r][u] = h;
           alo[r][u] = l;
+          tr += pr;
+          ti += pi;
         }
+        z0r[r] += (double)tr;
+        z0i[r] += (double)ti;
       }
 #pragma unroll
       for (int s = 0; s < SCT; ++s) {
-        const unsigned char *pz = smem_raw + cur + zo + s * 8 * kTabRow + 64 * ks;
+        const unsigned char *pz = smem_raw + cur + zo[s] + 64 * ks;
         const u4v bhi = *reinterpret_cast<const u4v *>(pz);
         const u4v blo = *reinterpret_cast<const u4v *>(pz + 128);
 #pragma unroll
@@ -899,18 +920,38 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_sk_h(SArgsH a)
     __syncthreads();
     b ^= 1;
   }
+  // nz = 0 column: the four lanes of a row hold atoms 4 (l/16) .. +3 of
+  // every K-step; summed in a fixed order
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    z0r[r] += __shfl_xor(z0r[r], 16, 64);
+    z0i[r] += __shfl_xor(z0i[r], 16, 64);
+    z0r[r] += __shfl_xor(z0r[r], 32, 64);
+    z0i[r] += __shfl_xor(z0i[r], 32, 64);
+  }
+  const int nrow = a.nX * a.nY;
 #pragma unroll
   for (int r = 0; r < RT; ++r) {
     if (rt0 + r >= rt_end) break;
+    float *out = a.part + ((long)chunk * a.Rpad + (rt0 + r) * 16) * a.SC;
 #pragma unroll
     for (int s = 0; s < SCT; ++s)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = (rt0 + r) * 16 + 4 * lk + e;
-        const int col = s * 16 + lr;
-        const bool ok = row < a.nX * a.nY && (col >> 1) < a.nZ;
-        a.part[((long)chunk * a.Rpad + row) * a.SC + col] = ok ? acc[r][s][e] : 0.f;
+        const int row = 4 * lk + e;
+        const int cp = s * 8 + (lr >> 1);
+        const bool ok = (rt0 + r) * 16 + row < nrow;
+        if (cp < ncp) out[(long)row * a.SC + 2 * (cp < kz ? cp : cp + 1) + (lr & 1)] = ok ? acc[r][s][e] : 0.f;
       }
+    // lanes 0..15 write the nz = 0 pair of row l%16, lanes 16..63 clear the
+    // padding columns 2 nZ .. SC - 1
+    const bool ok = (rt0 + r) * 16 + lr < nrow;
+    if (lk == 0) {
+      out[(long)lr * a.SC + 2 * kz] = ok ? (float)z0r[r] : 0.f;
+      out[(long)lr * a.SC + 2 * kz + 1] = ok ? (float)z0i[r] : 0.f;
+    } else {
+      for (int col = 2 * a.nZ + lk - 1; col < a.SC; col += 3) out[(long)lr * a.SC + col] = 0.f;
+    }
   }
 }
 
@@ -1607,6 +1648,10 @@ static int kforce(cav_ctx *c, int npts, const void *tab, double *out4)
 // ([X | Y] rows, nZ + 8 SCT Z rows and the bank padding between the two Z
 // row sets).  (A third buffer, to request blocks two ahead, measured slower:
 // it halves the workgroups per CU.)
+// Product column tiles of the split S pass: the 2 (nZ - 1) columns of nz != 0
+// (the nz = 0 column is summed on the vector ALUs).
+static int split_sct(const EwaldKPlan &p) { return std::max(1, (2 * (p.nZ - 1) + 15) / 16); }
+
 static size_t split_sk_lds(const EwaldKPlan &p)
 {
   const int zrows = p.nZ + 8 * (roundup(2 * p.nZ, 16) / 16);
@@ -1670,13 +1715,13 @@ static int sk_pipeline_h(cav_ctx *c)
   const dim3 lg(nch, groups);
   timer_begin(c, c->t_sk);
   if (RT == 1)
-    launch_sk_h<1>(c->stream, a, lg, smem, a.SC / 16, waves);
+    launch_sk_h<1>(c->stream, a, lg, smem, split_sct(p), waves);
   else if (RT == 2)
-    launch_sk_h<2>(c->stream, a, lg, smem, a.SC / 16, waves);
+    launch_sk_h<2>(c->stream, a, lg, smem, split_sct(p), waves);
   else if (RT == 4)
-    launch_sk_h<4>(c->stream, a, lg, smem, a.SC / 16, waves);
+    launch_sk_h<4>(c->stream, a, lg, smem, split_sct(p), waves);
   else
-    launch_sk_h<3>(c->stream, a, lg, smem, a.SC / 16, waves);
+    launch_sk_h<3>(c->stream, a, lg, smem, split_sct(p), waves);
   timer_end(c, c->t_sk);
   const int nthr = p.Rpad * p.nZ;
   const long plane = (long)p.Rpad * a.SC;
@@ -2226,11 +2271,11 @@ static int batch_ewald_k_t(cav_batch *b)
     a.tiles_per_group = tpg;
     const size_t smem = split_sk_lds(p);
     if (RT == 1)
-      batch_sk_h<1>(b, a, groups, smem, SC / 16, waves);
+      batch_sk_h<1>(b, a, groups, smem, split_sct(p), waves);
     else if (RT == 2)
-      batch_sk_h<2>(b, a, groups, smem, SC / 16, waves);
+      batch_sk_h<2>(b, a, groups, smem, split_sct(p), waves);
     else
-      batch_sk_h<3>(b, a, groups, smem, SC / 16, waves);
+      batch_sk_h<3>(b, a, groups, smem, split_sct(p), waves);
   }
   const int nthr = p.Rpad * p.nZ;
   // partials: FP32 from the split S pass, FP64 from the FP32 / FP64 tiles

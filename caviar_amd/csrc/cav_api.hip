@@ -224,7 +224,7 @@ static void free_all(cav_ctx *c)
                   c->part,    c->scal,   c->tab,       c->sk_part,  c->sk,       c->bmat,
                   c->d_W,     c->kacc,      c->order,    c->ord_blk,  c->xq,
                   c->tid,     c->d_raw_eps, c->d_raw_sig, c->ztab, c->bmat_h, c->cpos, c->sk_sum,
-                  c->binpos,  c->xqs,    c->tids,       c->nll,    c->lstart, c->litems,
+                  c->binpos,  c->xqs,    c->tids,       c->nll,    c->nlb,    c->lstart, c->litems,
                   c->ekx,     c->eky,    c->ekz,        c->part_k};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -374,6 +374,7 @@ static int alloc_list(cav_ctx *c, int cap)
   c->nl_cap = cap;
   CAV_TRY(dmalloc(c, &c->nl, (size_t)cap * c->n));
   CAV_TRY(dmalloc(c, &c->nll, (size_t)cap * c->n));
+  CAV_TRY(dmalloc(c, &c->nlb, (size_t)cap * c->n));
   return CAV_OK;
 }
 
@@ -383,7 +384,10 @@ static int sync_flags(cav_ctx *c)
   CAV_HIP(hipStreamSynchronize(c->stream));
   // a conditional list build that ran decides whether the staging positions
   // match the current list
-  if (c->h_flags[F_REBUILD]) c->nll_built = c->nll_pending;
+  if (c->h_flags[F_REBUILD]) {
+    c->nll_built = c->nll_pending;
+    c->nlb_built = c->nlb_pending;
+  }
   return CAV_OK;
 }
 
@@ -505,7 +509,10 @@ static int neighbor_phase(cav_ctx *c, bool forced)
   timer_end(c, c->t_nb);
   timer_begin(c, c->t_list);
   CAV_TRY(launch_list(c, true));
-  if (forced) c->nll_built = c->nll_pending;  // the build is known to run
+  if (forced) {  // the build is known to run
+    c->nll_built = c->nll_pending;
+    c->nlb_built = c->nlb_pending;
+  }
   CAV_TRY(launch_save_old(c, true));
   timer_end(c, c->t_list);
   return CAV_OK;

@@ -194,6 +194,7 @@ struct cav_ctx {
   double4 *xqs = nullptr;  // (x, y, z, q) in bin order, refreshed every step
   int2 *tids = nullptr;    // (type, id) in bin order
   unsigned short *nll = nullptr;  // Verlet entries as positions in the row's cell staging order
+  int *nlb = nullptr;             // Verlet entries as bin-order slots of the list build
   int *ord_blk = nullptr;  // scan block sums for the compaction
 
   // list (row-major: nl[i*nl_cap + k], nn[i] entries in row i)
@@ -245,6 +246,9 @@ struct cav_ctx {
   // staging positions (nll) valid for the current list / written by the last
   // conditional build launched
   bool nll_built = false, nll_pending = false;
+  // bin-order slots (nlb) valid for the current list / written by the last
+  // conditional build launched
+  bool nlb_built = false, nlb_pending = false;
   bool tuned = false;
   double *d_W = nullptr;     // [Rpad][nZ]
   double *kacc = nullptr;    // per-atom k-space (fx, fy, fz, phi) scratch for point queries
@@ -360,6 +364,7 @@ int launch_bins(cav_ctx *c, bool conditional);
 int launch_list(cav_ctx *c, bool conditional);
 int launch_save_old(cav_ctx *c, bool conditional);
 bool cell_staged(const cav_ctx *c);
+bool sorted_rows(const cav_ctx *c);
 int launch_pairs(cav_ctx *c, bool lj, bool er, bool accumulate);
 int launch_pack(cav_ctx *c);
 int launch_pairs_list(cav_ctx *c, bool ljl, bool sr, bool accumulate, bool pack);  // packed (x, y, z, q), (type, id) of owned + ghost slots

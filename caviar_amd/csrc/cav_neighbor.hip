@@ -164,6 +164,16 @@ __device__ __forceinline__ double lane_d(double v, int src)
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// The w word of a bin-ordered candidate (cpos): the atom index in the low 32
+// bits and its bin-order slot in the high 32 bits, so the list build can
+// record either.
+__device__ __forceinline__ double cpos_tag(int j, long t)
+{
+  return __longlong_as_double((long long)((unsigned long long)(unsigned)t << 32 | (unsigned)j));
+}
+__device__ __forceinline__ int cpos_index(double w) { return (int)(__double_as_longlong(w) & 0xffffffffll); }
+__device__ __forceinline__ int cpos_slot(double w) { return (int)(__double_as_longlong(w) >> 32); }
+
 // A double that is the same in every lane, moved to scalar registers.
 __device__ __forceinline__ double uniform_d(double v)
 {
@@ -352,7 +362,8 @@ __global__ __launch_bounds__(64 * kSortWaves) void k_bin_sort(int nbins, const i
   sort_bin_wave(start[b], start[b + 1], items, v_[wv]);
 }
 
-// (x, y, z, index) in bin order, for contiguous candidate staging in the list build
+// (x, y, z, index and slot) in bin order, for contiguous candidate staging in
+// the list build
 __global__ void k_cell_pack(long cap_total, int n, const int *items, const double *x, const double *y,
                             const double *z, double4 *cpos, const int *flags, const int *cond)
 {
@@ -360,7 +371,7 @@ __global__ void k_cell_pack(long cap_total, int n, const int *items, const doubl
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= (long)n + flags[F_NGHOST] || t >= cap_total) return;
   const int j = items[t];
-  cpos[t] = make_double4(x[j], y[j], z[j], (double)j);
+  cpos[t] = make_double4(x[j], y[j], z[j], cpos_tag(j, t));
 }
 
 // Owned atoms in cell order: stable compaction of bin_items (ghost entries
@@ -464,7 +475,8 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
                                                   const int *st, const int *start,
                                                   const int *items, int *nl, int *nn, int *flags,
                                                   const int *cond, const RepDev *rep,
-                                                  const int *cell_rep, unsigned short *nll)
+                                                  const int *cell_rep, unsigned short *nll,
+                                                  int *nlb)
 {
   // Each wave builds the rows of one cell on its own (wave-local LDS, no
   // block barrier).  The wave index is read as a scalar so that everything
@@ -502,7 +514,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
   const unsigned long long below = (1ull << lane) - 1ull;
   // padding and empty owned slots sit at +-1e300: their squared distance to
   // anything is +inf, so they never pass the cutoff test
-  const double4 far_cand = make_double4(1e300, 1e300, 1e300, -1.0);
+  const double4 far_cand = make_double4(1e300, 1e300, 1e300, cpos_tag(-1, -1));
   // the stencil's cell ranges, loaded by lanes 0..ns-1 at once, and their
   // running offsets in staging order (stencil cells in order, each in bin order)
   __shared__ int s_base_[kListWaves][32], s_off_[kListWaves][33];
@@ -533,7 +545,8 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
     const int gsize = (no - og) < 64 ? (no - og) : 64;
     // lane t holds owned atom og+t (position, slot) and its row count; the
     // kAPI atoms of a group are read from those lanes, never from memory
-    const double4 mine = lane < gsize ? cpos[s0 + og + lane] : make_double4(-1e300, -1e300, -1e300, -1.0);
+    const double4 mine = lane < gsize ? cpos[s0 + og + lane]
+                                      : make_double4(-1e300, -1e300, -1e300, cpos_tag(-1, -1));
     int cnt = 0;
     for (int mbase = 0; mbase < total; mbase += kCand) {
       // stage candidates mbase .. mbase+m-1 of the flattened stencil: each
@@ -558,7 +571,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
 #pragma unroll
         for (int u = 0; u < kAPI; ++u) {
           const int src = o0 + u;  // < 64: o0 is a multiple of kAPI below 64
-          ia[u] = __builtin_amdgcn_readlane((int)mine.w, src);
+          ia[u] = __builtin_amdgcn_readlane(cpos_index(mine.w), src);
           px[u] = lane_d(mine.x, src);
           py[u] = lane_d(mine.y, src);
           pz[u] = lane_d(mine.z, src);
@@ -568,7 +581,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
         for (int cc = 0; cc < mpad; cc += 64) {
           const int idx = cc + lane;
           const double4 q = sc[idx];
-          const int j = (int)q.w;
+          const int j = cpos_index(q.w);
 #pragma unroll
           for (int u = 0; u < kAPI; ++u) {
             const double dx = xsub(px[u], q.x), dy = xsub(py[u], q.y), dz = xsub(pz[u], q.z);
@@ -585,6 +598,10 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
                   unsigned short *lp = nll + at0;
                   lp[pre] = (unsigned short)(mbase + idx);  // position in the stencil staging order
                 }
+                if (nlb) {
+                  int *bp = nlb + at0;
+                  bp[pre] = cpos_slot(q.w);  // bin-order slot of the candidate
+                }
               }
               c0[u] = __builtin_amdgcn_readfirstlane(c0[u] + (int)__popcll(mask));
             }
@@ -596,7 +613,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
       __syncwarp();
     }
     if (lane < gsize) {
-      const int i = (int)mine.w - base + obase;
+      const int i = cpos_index(mine.w) - base + obase;
       nn[i] = cnt < cap ? cnt : cap;
       // the longest row is needed only to regrow the list after an overflow:
       // one device-wide atomic per overflowing row, none otherwise
@@ -656,6 +673,11 @@ int launch_list(cav_ctx *c, bool conditional)
   const bool with_nll = c->pair_kind == CAV_PAIRS_CELLS && c->nll && !c->all_atom_test;
   c->nll_pending = with_nll;
   if (!conditional) c->nll_built = with_nll;
+  // bin-order slots of the entries, for the row kernel's gathers from the
+  // bin-ordered copies: valid on the same terms
+  const bool with_nlb = c->pair_kind == CAV_PAIRS_AUTO && c->nlb && !c->all_atom_test;
+  c->nlb_pending = with_nlb;
+  if (!conditional) c->nlb_built = with_nlb;
   if (c->all_atom_test)
     k_list_all<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->nl_cap, c->nl, c->nn, c->flags,
                                                         cond);
@@ -663,8 +685,9 @@ int launch_list(cav_ctx *c, bool conditional)
     k_list_wave<<<list_grid(c->nbins), 64 * kListWaves, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->cpos,
                                                 c->d_st_off, c->d_st, c->bin_start, c->bin_items,
                                                 c->nl, c->nn, c->flags, cond, nullptr, nullptr,
-                                                with_nll ? c->nll : nullptr);
-  if (with_nll) {
+                                                with_nll ? c->nll : nullptr,
+                                                with_nlb ? c->nlb : nullptr);
+  if (with_nll || with_nlb) {
     const long tot = (long)c->n + c->gcap;
     k_list_bins<<<grid_for(tot > c->nbins + 1 ? tot : c->nbins + 1), kBlock, 0, c->stream>>>(
         c->n, c->nbins, tot, c->bin_start, c->bin_items, c->lstart, c->litems, c->binpos, c->flags,
@@ -918,7 +941,7 @@ __global__ void kb_cell_pack(const RepDev *rep, const int *flags, const int *ite
   if (a >= (long)r.n + f[F_NGHOST]) return;
   const long t = r.base + a;
   const int j = items[t];
-  cpos[t] = make_double4(x[j], y[j], z[j], (double)j);
+  cpos[t] = make_double4(x[j], y[j], z[j], cpos_tag(j, t));
 }
 
 __global__ void kb_save_old(const RepDev *rep, const double *x, const double *y, const double *z,
@@ -971,7 +994,8 @@ int batch_neighbor(cav_batch *b, bool forced)
                                                      b->z, b->cpos);
   k_list_wave<<<list_grid(b->ncells), 64 * kListWaves, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->cpos, b->st_off,
                                                b->st, b->bin_start, b->bin_items, b->nl, b->nn,
-                                               b->flags, nullptr, b->d_rep, b->cell_rep, nullptr);
+                                               b->flags, nullptr, b->d_rep, b->cell_rep, nullptr,
+                                               nullptr);
   kb_save_old<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->x, b->y, b->z, b->xo, b->yo, b->zo,
                                                     b->flags);
   BAT_HIP(hipGetLastError());

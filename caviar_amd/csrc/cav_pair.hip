@@ -54,6 +54,13 @@ struct PairArgs {
   double *part;
   int accumulate;  // add into the acceleration instead of overwriting it
   const RepDev *rep;  // batched replicas (blockIdx.y) or nullptr
+  // rows as bin-order slots (or nullptr): neighbours are then read from the
+  // bin-ordered copies, where the neighbours of cell-ordered atoms lie close
+  // together, with the atom index from the list build's bin order
+  const int *nlb;
+  const double4 *xqs;
+  const int2 *tids;
+  const int *litems;
 };
 
 // Neighbours whose loads a lane issues together in the row-gather kernel
@@ -209,7 +216,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
     const T qi = (T)pi4.w;
     const T keqi = (T)(a.ke) * qi;
     const int cnt = a.nn[i];
-    const int *row = a.nl + (long)i * a.cap;
+    const bool bins = a.nlb != nullptr;
+    const int *row = (bins ? a.nlb : a.nl) + (long)i * a.cap;
     constexpr int U = pair_u<T>();
     for (int k0 = sub; k0 < cnt; k0 += U * kLPA) {
       int jj[U];
@@ -220,11 +228,22 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
         const int k = k0 + u * kLPA;
         jj[u] = k < cnt ? row[k] : -1;
       }
+      if (bins) {
+        // entries past the row read slot 0 and are skipped below
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int j = jj[u] < 0 ? i : jj[u];
-        pj[u] = a.xq[j];
-        tj[u] = a.tid[j];
+        for (int u = 0; u < U; ++u) {
+          const int pp = jj[u] < 0 ? 0 : jj[u];
+          pj[u] = a.xqs[pp];
+          tj[u] = a.tids[pp];
+          jj[u] = jj[u] < 0 ? -1 : a.litems[pp];
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int j = jj[u] < 0 ? i : jj[u];
+          pj[u] = a.xq[j];
+          tj[u] = a.tid[j];
+        }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -684,6 +703,10 @@ static PairArgs pair_args(cav_ctx *c)
   a.ay = c->ay;
   a.az = c->az;
   a.part = c->part;
+  a.nlb = nullptr;
+  a.xqs = nullptr;
+  a.tids = nullptr;
+  a.litems = nullptr;
   return a;
 }
 
@@ -710,6 +733,13 @@ static void launch_pairs_t(cav_ctx *c, const PairArgs &a, bool lj, bool er, bool
 // the cell-built list (its entries' staging positions), no per-molecule
 // lambda.  Otherwise, and by default, the row-gather kernel runs (faster on
 // the ion workloads measured at N=262144, rho=0.3: 0.165 vs 0.232 ms mixed).
+// The row kernel reads neighbours through the list build's bin-order slots
+// when the current list carries them (CAV_PAIRS_AUTO; lists built from cells).
+bool sorted_rows(const cav_ctx *c)
+{
+  return c->pair_kind == CAV_PAIRS_AUTO && !c->all_atom_test && c->nlb && c->nlb_built;
+}
+
 bool cell_staged(const cav_ctx *c)
 {
   return c->pair_kind == CAV_PAIRS_CELLS && c->from_cells && c->build_cells && !c->all_atom_test &&
@@ -815,9 +845,17 @@ int launch_pairs(cav_ctx *c, bool want_lj, bool want_er, bool accumulate)
   // every block writes one virial partial: the buffer must cover the grid
   if ((cells ? cell_grid(c) : pair_grid(c)) > c->npart)
     return fail(c, CAV_ERR_STATE, "partial-sum buffer smaller than the pair grid");
+  const bool bins = !cells && sorted_rows(c);
+  if (bins) {
+    a.nlb = c->nlb;
+    a.xqs = c->xqs;
+    a.tids = c->tids;
+    a.litems = c->litems;
+  }
   k_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->x, c->y, c->z, c->q, c->type,
                                                  c->id, c->flags, c->xq, c->tid,
-                                                 cells ? c->binpos : nullptr, c->xqs, c->tids);
+                                                 cells || bins ? c->binpos : nullptr, c->xqs,
+                                                 c->tids);
   timer_begin(c, c->t_pair);
   if (cells) {
     if (c->precision == CAV_FP64)
@@ -1177,6 +1215,10 @@ int batch_pairs(cav_batch *b)
     a.part = b->part;
     a.accumulate = 0;
     a.rep = b->d_rep;
+    a.nlb = nullptr;
+    a.xqs = nullptr;
+    a.tids = nullptr;
+    a.litems = nullptr;
     if (b->precision == CAV_FP64)
       batch_pairs_t<double>(b, a, g);
     else

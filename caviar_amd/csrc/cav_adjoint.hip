@@ -394,6 +394,7 @@ static int adjoint_run(cav_ctx *c, const double *w_host, const AdjWork &k, doubl
   CAV_HIP(hipMemcpyAsync(k.w, w_host, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
   for (int v = 0; v < nv + 2 * nt; ++v) hout[v] = 0.0;
   if (c->lj_on || c->er_on) {
+    CAV_TRY(launch_nl_fill(c));  // the index rows, if the last build wrote slots only
     AdjArgs a;
     a.n = n;
     a.nt = nt;

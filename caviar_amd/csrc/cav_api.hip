@@ -1373,6 +1373,7 @@ static int list_host(cav_ctx *c, std::vector<int> &nl, std::vector<int> &nn)
 {
   nl.resize((size_t)c->nl_cap * c->n);
   nn.resize(c->n);
+  CAV_TRY(launch_nl_fill(c));
   CAV_HIP(hipMemcpyAsync(nl.data(), c->nl, sizeof(int) * nl.size(), hipMemcpyDeviceToHost, c->stream));
   CAV_HIP(hipMemcpyAsync(nn.data(), c->nn, sizeof(int) * nn.size(), hipMemcpyDeviceToHost, c->stream));
   CAV_HIP(hipStreamSynchronize(c->stream));

@@ -31,6 +31,7 @@ enum Flag : int {
   F_FORCE = 7,     // host-requested rebuild
   F_GOVER = 8,     // batched replicas: ghosts exceeded the replica's ghost capacity
   F_MAXST = 9,     // largest cell stencil (candidate atoms) the cell-staged pair kernel met
+  F_NL_STALE = 10, // the last list build wrote bin-order slots only: index rows are filled on demand
   F_COUNT = 16
 };
 
@@ -365,6 +366,7 @@ int launch_list(cav_ctx *c, bool conditional);
 int launch_save_old(cav_ctx *c, bool conditional);
 bool cell_staged(const cav_ctx *c);
 bool sorted_rows(const cav_ctx *c);
+int launch_nl_fill(cav_ctx *c);
 int launch_pairs(cav_ctx *c, bool lj, bool er, bool accumulate);
 int launch_pack(cav_ctx *c);
 int launch_pairs_list(cav_ctx *c, bool ljl, bool sr, bool accumulate, bool pack);  // packed (x, y, z, q), (type, id) of owned + ghost slots

@@ -444,6 +444,9 @@ constexpr int kCand = CAV_LIST_CAND;
 // neighborlist.cpp:531-566).  The reference's half list is the subset with
 // owned j > i plus every ghost; both directions are kept so that forces are
 // gathered per atom without atomics.  Rows are row-major: nl[i*cap + k].
+// With nlb the entries are (also) written as bin-order slots; with nl null
+// only the slots are written and F_NL_STALE tells the readers of the index
+// rows to fill them first (launch_nl_fill).
 //
 // One wave per cell: the candidates of the cell's de-duplicated stencil are
 // staged in LDS (in stencil order, each cell in bin order), then the cell's
@@ -497,6 +500,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
     start += r;  // every replica's cell starts carry one closing entry
   }
   if (cond && !*cond) return;
+  if (!rep && b == 0 && threadIdx.x == 0) flags[F_NL_STALE] = nl ? 0 : 1;
   // staged candidates (x, y, z, slot) in the cell's stencil order; a stage is
   // padded to a multiple of 64 with far-away entries so every lane tests one
   __shared__ double4 sc_[kListWaves][kCand];
@@ -592,8 +596,10 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
               if (pass && c0[u] + pre < cap) {
                 // uniform row pointers plus a 32-bit lane offset
                 const long at0 = row[u] + c0[u];
-                int *rp = nl + at0;
-                rp[pre] = j - base;
+                if (nl) {
+                  int *rp = nl + at0;
+                  rp[pre] = j - base;
+                }
                 if (nll) {
                   unsigned short *lp = nll + at0;
                   lp[pre] = (unsigned short)(mbase + idx);  // position in the stencil staging order
@@ -684,7 +690,8 @@ int launch_list(cav_ctx *c, bool conditional)
   else
     k_list_wave<<<list_grid(c->nbins), 64 * kListWaves, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->cpos,
                                                 c->d_st_off, c->d_st, c->bin_start, c->bin_items,
-                                                c->nl, c->nn, c->flags, cond, nullptr, nullptr,
+                                                with_nlb ? nullptr : c->nl, c->nn, c->flags,
+                                                cond, nullptr, nullptr,
                                                 with_nll ? c->nll : nullptr,
                                                 with_nlb ? c->nlb : nullptr);
   if (with_nll || with_nlb) {
@@ -693,6 +700,33 @@ int launch_list(cav_ctx *c, bool conditional)
         c->n, c->nbins, tot, c->bin_start, c->bin_items, c->lstart, c->litems, c->binpos, c->flags,
         cond);
   }
+  CAV_HIP(hipGetLastError());
+  return CAV_OK;
+}
+
+// Index rows from the bin-order slots of the last list build, when that build
+// wrote the slots only (F_NL_STALE); the flag is cleared by a kernel of its own
+// after every entry is written.
+__global__ void k_nl_fill(long entries, int cap, long slots, const int *nlb, const int *litems,
+                          const int *nn, int *nl, const int *flags)
+{
+  if (!flags[F_NL_STALE]) return;
+  const long e = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= entries || e % cap >= nn[e / cap]) return;
+  const int t = nlb[e];
+  if (t >= 0 && t < slots) nl[e] = litems[t];
+}
+
+__global__ void k_nl_fresh(int *flags) { flags[F_NL_STALE] = 0; }
+
+int launch_nl_fill(cav_ctx *c)
+{
+  if (!c->nlb || !c->nl || c->all_atom_test) return CAV_OK;
+  const long entries = (long)c->n * c->nl_cap;
+  k_nl_fill<<<grid_for(entries), kBlock, 0, c->stream>>>(entries, c->nl_cap, (long)c->n + c->gcap,
+                                                          c->nlb, c->litems, c->nn, c->nl,
+                                                          c->flags);
+  k_nl_fresh<<<1, 1, 0, c->stream>>>(c->flags);
   CAV_HIP(hipGetLastError());
   return CAV_OK;
 }

@@ -786,6 +786,7 @@ int launch_pairs_list(cav_ctx *c, bool want_ljl, bool want_sr, bool accumulate, 
   const bool ljl = want_ljl && c->ljl_on, sr = want_sr && c->sr_on;
   if (!ljl && !sr) return CAV_OK;
   if (pack) CAV_TRY(launch_pack(c));
+  CAV_TRY(launch_nl_fill(c));
   PairArgs a = pair_args(c);
   a.accumulate = accumulate ? 1 : 0;
   ListArgs b;
@@ -851,6 +852,8 @@ int launch_pairs(cav_ctx *c, bool want_lj, bool want_er, bool accumulate)
     a.xqs = c->xqs;
     a.tids = c->tids;
     a.litems = c->litems;
+  } else {
+    CAV_TRY(launch_nl_fill(c));
   }
   k_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->x, c->y, c->z, c->q, c->type,
                                                  c->id, c->flags, c->xq, c->tid,
@@ -1120,6 +1123,7 @@ __global__ void k_energy_lj(PairArgs a, const double *u12, const double *u6, con
 
 int launch_energy_lj(cav_ctx *c, double *e)
 {
+  CAV_TRY(launch_nl_fill(c));
   const int nt2 = c->nt * c->nt;
   PairArgs a = pair_args(c);
   k_energy_lj<<<grid_for(c->n), kBlock, 0, c->stream>>>(a, c->d_eps + 3 * nt2, c->d_eps + 4 * nt2,

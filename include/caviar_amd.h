@@ -317,12 +317,16 @@ int cav_set_tiles(cav_ctx *ctx, int s_rows, int s_chunk, int f_tiles, int f_wave
  * every pass in order on one stream (per-kernel timings without contention).
  * The accelerations are the same either way. */
 int cav_set_overlap(cav_ctx *ctx, int on);
-/* Short-range pair kernel.  CAV_PAIRS_ROWS gathers every neighbour of an
- * atom's Verlet row through L2 (8 lanes per atom); CAV_PAIRS_CELLS stages each
- * cell's 27-cell stencil in LDS and reads the neighbours from there (where it
- * applies: list built from cells, no molecule lambda; otherwise rows).
- * CAV_PAIRS_AUTO (default) is the faster of the two on the measured
- * workloads: ROWS.  Both give the same forces to rounding. */
+/* Short-range pair kernel.  CAV_PAIRS_AUTO (default): the row kernel (8 lanes
+ * per atom) reads each Verlet row as the bin-order slots the list build
+ * recorded and gathers the neighbours from bin-ordered copies of the atoms,
+ * where the neighbours of cell-ordered atoms lie close together; the atom-index
+ * rows (cav_num_pairs, the other kernels) are filled from the slots on demand.
+ * CAV_PAIRS_ROWS gathers the same rows by atom index; CAV_PAIRS_CELLS stages
+ * each cell's 27-cell stencil in LDS and reads the neighbours from there
+ * (where it applies: list built from cells, no molecule lambda; otherwise
+ * rows).  AUTO and ROWS give the same forces bit for bit, CELLS to rounding.
+ * A new mode takes effect at the next list build. */
 enum cav_pair_kernel { CAV_PAIRS_AUTO = 0, CAV_PAIRS_ROWS = 1, CAV_PAIRS_CELLS = 2 };
 int cav_set_pair_kernel(cav_ctx *ctx, int kind);
 int cav_pair_kernel(const cav_ctx *ctx, int *kind_in_use);

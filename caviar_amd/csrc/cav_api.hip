@@ -505,7 +505,7 @@ static int neighbor_phase(cav_ctx *c, bool forced)
   }
   timer_begin(c, c->t_nb);
   // cells are needed on rebuild steps, and every step by the LJ cell walk
-  CAV_TRY(launch_bins(c, !c->lj_cellwalk));
+  CAV_TRY(launch_bins(c, !c->lj_cellwalk, true));
   timer_end(c, c->t_nb);
   timer_begin(c, c->t_list);
   CAV_TRY(launch_list(c, true));

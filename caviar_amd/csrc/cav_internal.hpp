@@ -361,7 +361,9 @@ void timer_end(cav_ctx *c, Timer &t);
 // launchers (defined in the .hip files)
 int launch_ghosts(cav_ctx *c);
 int launch_rebuild_check(cav_ctx *c);
-int launch_bins(cav_ctx *c, bool conditional);
+// for_list: the binning of a list build, which also keeps the bin order the
+// list refers to (litems, binpos)
+int launch_bins(cav_ctx *c, bool conditional, bool for_list = false);
 int launch_list(cav_ctx *c, bool conditional);
 int launch_save_old(cav_ctx *c, bool conditional);
 bool cell_staged(const cav_ctx *c);

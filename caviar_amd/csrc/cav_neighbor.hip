@@ -363,15 +363,21 @@ __global__ __launch_bounds__(64 * kSortWaves) void k_bin_sort(int nbins, const i
 }
 
 // (x, y, z, index and slot) in bin order, for contiguous candidate staging in
-// the list build
+// the list build; on the list build's own binning (list_cond set) also the
+// bin order the list keeps: slot -> atom (litems) and atom -> slot (binpos)
 __global__ void k_cell_pack(long cap_total, int n, const int *items, const double *x, const double *y,
-                            const double *z, double4 *cpos, const int *flags, const int *cond)
+                            const double *z, double4 *cpos, const int *flags, const int *cond,
+                            int *litems, int *binpos, const int *list_cond)
 {
   if (cond && !*cond) return;
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= (long)n + flags[F_NGHOST] || t >= cap_total) return;
   const int j = items[t];
   cpos[t] = make_double4(x[j], y[j], z[j], cpos_tag(j, t));
+  if (litems && *list_cond) {
+    litems[t] = j;
+    binpos[j] = (int)t;
+  }
 }
 
 // Owned atoms in cell order: stable compaction of bin_items (ghost entries
@@ -398,9 +404,18 @@ struct OwnedFlagBounded {
   }
 };
 
-int launch_bins(cav_ctx *c, bool conditional)
+// Whether the list build keeps its bin order (for the bin-order pair gathers
+// or the cell-staged pair kernel).
+static bool list_keeps_bins(const cav_ctx *c)
+{
+  return !c->all_atom_test && ((c->pair_kind == CAV_PAIRS_CELLS && c->nll) ||
+                               (c->pair_kind == CAV_PAIRS_AUTO && c->nlb));
+}
+
+int launch_bins(cav_ctx *c, bool conditional, bool for_list)
 {
   const int *cond = conditional ? c->flags + F_REBUILD : nullptr;
+  const bool keep = for_list && list_keeps_bins(c);
   const long tot = (long)c->n + c->gcap;
   const BinRule rule = bin_rule(c);
   k_zero_int<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->bin_cnt, c->nbins, cond);
@@ -417,7 +432,10 @@ int launch_bins(cav_ctx *c, bool conditional)
   k_bin_sort<<<(c->nbins + kSortWaves - 1) / kSortWaves, 64 * kSortWaves, 0, c->stream>>>(
       c->nbins, c->bin_start, c->bin_items, cond);
   k_cell_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(tot, c->n, c->bin_items, c->x, c->y, c->z,
-                                                      c->cpos, c->flags, cond);
+                                                      c->cpos, c->flags, cond,
+                                                      keep ? c->litems : nullptr,
+                                                      keep ? c->binpos : nullptr,
+                                                      c->flags + F_REBUILD);
   const int nbo = grid_for(tot);
   OwnedFlagBounded of{c->bin_items, c->flags, c->n};
   k_block_sums<<<nbo, kBlock, 0, c->stream>>>(tot, of, c->ord_blk, cond);
@@ -694,7 +712,9 @@ int launch_list(cav_ctx *c, bool conditional)
                                                 cond, nullptr, nullptr,
                                                 with_nll ? c->nll : nullptr,
                                                 with_nlb ? c->nlb : nullptr);
-  if (with_nll || with_nlb) {
+  // (the bin order itself, litems and binpos, was kept by the list build's
+  // binning; the cell-staged kernel also needs the cell starts)
+  if (with_nll) {
     const long tot = (long)c->n + c->gcap;
     k_list_bins<<<grid_for(tot > c->nbins + 1 ? tot : c->nbins + 1), kBlock, 0, c->stream>>>(
         c->n, c->nbins, tot, c->bin_start, c->bin_items, c->lstart, c->litems, c->binpos, c->flags,

@@ -173,7 +173,7 @@ __device__ __forceinline__ void block_virial(double vir, double *part)
 // z, q) and (type, id), before computing, so the gathers overlap.  The
 // partial forces of an atom's lanes are summed with a fixed shuffle tree, so
 // FP64 results stay reproducible.
-template <typename T, bool LJ, bool ER, bool LAM>
+template <typename T, bool LJ, bool ER, bool LAM, bool BINS>
 __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
 {
   extern __shared__ double dyn_pair[];
@@ -216,8 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
     const T qi = (T)pi4.w;
     const T keqi = (T)(a.ke) * qi;
     const int cnt = a.nn[i];
-    const bool bins = a.nlb != nullptr;
-    const int *row = (bins ? a.nlb : a.nl) + (long)i * a.cap;
+    const int *row = (BINS ? a.nlb : a.nl) + (long)i * a.cap;
     constexpr int U = pair_u<T>();
     for (int k0 = sub; k0 < cnt; k0 += U * kLPA) {
       int jj[U];
@@ -228,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
         const int k = k0 + u * kLPA;
         jj[u] = k < cnt ? row[k] : -1;
       }
-      if (bins) {
+      if constexpr (BINS) {
         // entries past the row read slot 0 and are skipped below
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -712,21 +711,31 @@ static PairArgs pair_args(cav_ctx *c)
 
 static int pair_grid(const cav_ctx *c) { return (grid_for((long)c->n * kLPA) + 7) / 8 * 8; }
 
-template <typename T>
-static void launch_pairs_t(cav_ctx *c, const PairArgs &a, bool lj, bool er, bool lam)
+template <typename T, bool BINS>
+static void launch_pairs_tb(cav_ctx *c, const PairArgs &a, bool lj, bool er, bool lam)
 {
   const int g = pair_grid(c);
   const size_t lds = 3 * sizeof(T) * a.nt * a.nt;  // LJ tables
   if (lj && er && lam)
-    k_pairs<T, true, true, true><<<g, kBlock, lds, c->stream>>>(a);
+    k_pairs<T, true, true, true, BINS><<<g, kBlock, lds, c->stream>>>(a);
   else if (lj && er)
-    k_pairs<T, true, true, false><<<g, kBlock, lds, c->stream>>>(a);
+    k_pairs<T, true, true, false, BINS><<<g, kBlock, lds, c->stream>>>(a);
   else if (lj)
-    k_pairs<T, true, false, false><<<g, kBlock, lds, c->stream>>>(a);
+    k_pairs<T, true, false, false, BINS><<<g, kBlock, lds, c->stream>>>(a);
   else if (er && lam)
-    k_pairs<T, false, true, true><<<g, kBlock, 0, c->stream>>>(a);
+    k_pairs<T, false, true, true, BINS><<<g, kBlock, 0, c->stream>>>(a);
   else
-    k_pairs<T, false, true, false><<<g, kBlock, 0, c->stream>>>(a);
+    k_pairs<T, false, true, false, BINS><<<g, kBlock, 0, c->stream>>>(a);
+}
+
+// rows as bin-order slots (a.nlb) or as atom indices, each its own kernel
+template <typename T>
+static void launch_pairs_t(cav_ctx *c, const PairArgs &a, bool lj, bool er, bool lam)
+{
+  if (a.nlb)
+    launch_pairs_tb<T, true>(c, a, lj, er, lam);
+  else
+    launch_pairs_tb<T, false>(c, a, lj, er, lam);
 }
 
 // The cell-staged kernel runs on request (CAV_PAIRS_CELLS) where it applies:
@@ -1170,11 +1179,11 @@ static void batch_pairs_t(cav_batch *b, const PairArgs &a, dim3 g)
 {
   const size_t lds = 3 * sizeof(T) * a.nt * a.nt;  // LJ tables
   if (b->lj_on && b->er_on)
-    k_pairs<T, true, true, false><<<g, kBlock, lds, b->stream>>>(a);
+    k_pairs<T, true, true, false, false><<<g, kBlock, lds, b->stream>>>(a);
   else if (b->lj_on)
-    k_pairs<T, true, false, false><<<g, kBlock, lds, b->stream>>>(a);
+    k_pairs<T, true, false, false, false><<<g, kBlock, lds, b->stream>>>(a);
   else
-    k_pairs<T, false, true, false><<<g, kBlock, 0, b->stream>>>(a);
+    k_pairs<T, false, true, false, false><<<g, kBlock, 0, b->stream>>>(a);
 }
 
 // LJ + Ewald_r accelerations of every replica (overwriting), virial per replica.

@@ -950,6 +950,8 @@ int cav_upload_atoms(cav_ctx *c, int n, const int *id, const int *type, const in
   CAV_TRY(dmalloc(c, &c->cpos, tot));
   CAV_TRY(dmalloc(c, &c->binpos, tot));
   CAV_TRY(dmalloc(c, &c->litems, tot));
+  // no list build's bin order exists for the new slot arrays yet
+  k_set_flag<<<1, 1, 0, c->stream>>>(c->flags, F_NGHOST_OLD, -1);
   CAV_TRY(dmalloc(c, &c->xqs, tot));
   CAV_TRY(dmalloc(c, &c->tids, tot));
   CAV_TRY(dmalloc(c, &c->nn, n));

@@ -258,16 +258,46 @@ __global__ void k_zero_int(int *p, long n, const int *cond)
   if (i < n) p[i] = 0;
 }
 
+// Atoms are binned in the bin order of the last list build (`border`, a
+// permutation of its n + G_old slots; atoms past them in index order; entries
+// of atoms that no longer exist skipped), so lanes next to each other mostly
+// fall into one bin and a wave adds one count per run of equal bins instead
+// of one per atom.  The bins' contents are the same in any order: the fill
+// below uses the same order and every bin is sorted by index afterwards.
+__device__ __forceinline__ long bin_order_atom(long t, long tot, long told, const int *border)
+{
+  const long a = t < told ? (long)border[t] : t;
+  return a >= 0 && a < tot ? a : -1;
+}
+
+// Run of equal bins that lane `lane` heads: its length (0 for a lane that
+// heads no run); `hm` returns the heads of the wave.
+__device__ __forceinline__ int bin_run(int b, unsigned long long &hm)
+{
+  const int lane = threadIdx.x & 63;
+  const int pb = __shfl_up(b, 1, 64);
+  const bool brk = lane == 0 || b != pb;
+  const unsigned long long bm = __ballot(brk);
+  hm = __ballot(brk && b >= 0);
+  if (!(brk && b >= 0)) return 0;
+  const unsigned long long above = bm & ~((2ull << lane) - 1ull);
+  return (above ? __ffsll((long long)above) - 1 : 64) - lane;
+}
+
 __global__ void k_bin_count(int n, long cap_total, BinRule rule, const double *x, const double *y,
                             const double *z, int *bin_of, int *bin_cnt, const int *flags,
-                            const int *cond)
+                            const int *cond, const int *border)
 {
   if (cond && !*cond) return;
-  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (a >= (long)n + flags[F_NGHOST] || a >= cap_total) return;
-  const int b = rule(x[a], y[a], z[a]);
-  bin_of[a] = b;
-  atomicAdd(bin_cnt + b, 1);
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long tot = (long)n + flags[F_NGHOST];
+  const long told = flags[F_NGHOST_OLD] < 0 ? 0 : (long)n + flags[F_NGHOST_OLD];
+  const long a = t < cap_total && t < (tot > told ? tot : told) ? bin_order_atom(t, tot, told, border) : -1;
+  const int b = a >= 0 ? rule(x[a], y[a], z[a]) : -1;
+  if (a >= 0) bin_of[a] = b;
+  unsigned long long hm;
+  const int len = bin_run(b, hm);
+  if (len) atomicAdd(bin_cnt + b, len);
 }
 
 struct CountOf {
@@ -286,13 +316,24 @@ __global__ void k_bin_start(int nbins, const int *cnt, const int *blk, int *star
 }
 
 __global__ void k_bin_fill(int n, long cap_total, const int *bin_of, const int *start, int *cur,
-                           int *items, const int *flags, const int *cond)
+                           int *items, const int *flags, const int *cond, const int *border)
 {
   if (cond && !*cond) return;
-  const long a = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (a >= (long)n + flags[F_NGHOST] || a >= cap_total) return;
-  const int b = bin_of[a];
-  items[start[b] + atomicAdd(cur + b, 1)] = (int)a;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long tot = (long)n + flags[F_NGHOST];
+  const long told = flags[F_NGHOST_OLD] < 0 ? 0 : (long)n + flags[F_NGHOST_OLD];
+  const long a = t < cap_total && t < (tot > told ? tot : told) ? bin_order_atom(t, tot, told, border) : -1;
+  const int b = a >= 0 ? bin_of[a] : -1;
+  unsigned long long hm;
+  const int len = bin_run(b, hm);
+  int base = 0;
+  if (len) base = atomicAdd(cur + b, len);
+  // every lane of a run takes the next slot after its head's
+  const int lane = threadIdx.x & 63;
+  const unsigned long long mine = hm & ((2ull << lane) - 1ull);
+  const int head = mine ? 63 - __clzll((long long)mine) : lane;
+  base = __shfl(base, head, 64);
+  if (b >= 0) items[start[b] + base + (lane - head)] = (int)a;
 }
 
 // Insertion order of update_cell_list (owned ascending, then ghosts n+g
@@ -404,23 +445,18 @@ struct OwnedFlagBounded {
   }
 };
 
-// Whether the list build keeps its bin order (for the bin-order pair gathers
-// or the cell-staged pair kernel).
-static bool list_keeps_bins(const cav_ctx *c)
-{
-  return !c->all_atom_test && ((c->pair_kind == CAV_PAIRS_CELLS && c->nll) ||
-                               (c->pair_kind == CAV_PAIRS_AUTO && c->nlb));
-}
-
 int launch_bins(cav_ctx *c, bool conditional, bool for_list)
 {
   const int *cond = conditional ? c->flags + F_REBUILD : nullptr;
-  const bool keep = for_list && list_keeps_bins(c);
+  // every list build keeps its bin order (the bin-order pair gathers and the
+  // cell-staged kernel read it, and the next binning visits atoms in it)
+  const bool keep = for_list;
   const long tot = (long)c->n + c->gcap;
   const BinRule rule = bin_rule(c);
   k_zero_int<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->bin_cnt, c->nbins, cond);
   k_bin_count<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, rule, c->x, c->y, c->z,
-                                                      c->bin_of, c->bin_cnt, c->flags, cond);
+                                                      c->bin_of, c->bin_cnt, c->flags, cond,
+                                                      c->litems);
   const int nb = grid_for(c->nbins);
   CountOf cnt{c->bin_cnt};
   k_block_sums<<<nb, kBlock, 0, c->stream>>>(c->nbins, cnt, c->scan_tmp, cond);
@@ -428,7 +464,8 @@ int launch_bins(cav_ctx *c, bool conditional, bool for_list)
   k_bin_start<<<nb, kBlock, 0, c->stream>>>(c->nbins, c->bin_cnt, c->scan_tmp, c->bin_start, cond);
   k_zero_int<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->bin_cnt, c->nbins, cond);
   k_bin_fill<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->bin_of, c->bin_start,
-                                                     c->bin_cnt, c->bin_items, c->flags, cond);
+                                                     c->bin_cnt, c->bin_items, c->flags, cond,
+                                                     c->litems);
   k_bin_sort<<<(c->nbins + kSortWaves - 1) / kSortWaves, 64 * kSortWaves, 0, c->stream>>>(
       c->nbins, c->bin_start, c->bin_items, cond);
   k_cell_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(tot, c->n, c->bin_items, c->x, c->y, c->z,

@@ -474,6 +474,44 @@ def test_pair_kernel_switch_mid_run(gpu):
     assert F.rel_err(e.get("pos"), o.get("pos")) <= 1e-9
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_slot_rows_and_index_rows(gpu, prec):
+    """The default pair mode's list builds write the entries as bin-order slots
+    only; the atom-index rows are filled from them when something reads them.
+    Two engines on the same system, one in the default mode and one gathering
+    by atom index throughout, step 12 times (every step rebuilds the list).
+    Then every reader of the index rows sees the same rows and gives the same
+    numbers: the pair list, the list statistics, the LJ + Ewald_r energy
+    (cell walk) and the adjoint; the accelerations of the run are bit-identical,
+    and so are the row kernel's accelerations on the filled rows after a
+    switch to CAV_PAIRS_ROWS without a rebuild."""
+    s = configs.ions(n=3000, rho=0.3, kmax=6, seed=5)
+    print(f"[case] slot rows vs index rows {prec}", flush=True)
+    a = Engine(s.copy(), precision=prec)
+    b = Engine(s.copy(), precision=prec)
+    b.set_pair_kernel("rows")
+    for e in (a, b):
+        e.setup()
+        e.step(0.002, 12)
+    assert np.array_equal(a.get("acc"), b.get("acc"))
+    assert np.array_equal(a.get("pos"), b.get("pos"))
+    assert np.array_equal(F.sorted_pairs(a.pairs()), F.sorted_pairs(b.pairs()))
+    assert a.list_stats() == b.list_stats()
+    ea, eb = a.energy(k=False), b.energy(k=False)
+    assert ea[0] == eb[0] and ea[1] == eb[1]
+    w = np.random.default_rng(3).standard_normal((len(s.pos), 3))
+    ga, gb = a.adjoint(w), b.adjoint(w)
+    for x, y in zip(ga, gb):
+        assert np.array_equal(x, y)
+    # the last build wrote slots only: the row kernel reads the filled rows
+    a.set_pair_kernel("rows")
+    a.reset_acceleration()
+    a.forces("lj", "ewald_r")
+    b.reset_acceleration()
+    b.forces("lj", "ewald_r")
+    assert np.array_equal(a.get("acc"), b.get("acc"))
+
+
 def test_charges_changed_after_setup(gpu):
     """New type charges set after setup reach the owned atoms and the current
     ghosts before the next force, energy and point-field call (cav_forces on

@@ -270,8 +270,8 @@ __device__ __forceinline__ long bin_order_atom(long t, long tot, long told, cons
   return a >= 0 && a < tot ? a : -1;
 }
 
-// Run of equal bins that lane `lane` heads: its length (0 for a lane that
-// heads no run); `hm` returns the heads of the wave.
+// Runs of equal bins (b >= 0) among consecutive lanes: the length of the run
+// this lane heads (0 if it heads none); `hm` returns the wave's run heads.
 __device__ __forceinline__ int bin_run(int b, unsigned long long &hm)
 {
   const int lane = threadIdx.x & 63;
@@ -556,7 +556,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
   }
   if (cond && !*cond) return;
   if (!rep && b == 0 && threadIdx.x == 0) flags[F_NL_STALE] = nl ? 0 : 1;
-  // staged candidates (x, y, z, slot) in the cell's stencil order; a stage is
+  // staged candidates (x, y, z, index and slot) in the cell's stencil order; a stage is
   // padded to a multiple of 64 with far-away entries so every lane tests one
   __shared__ double4 sc_[kListWaves][kCand];
   double4 *sc = sc_[wv];

@@ -64,8 +64,10 @@ struct PairArgs {
 };
 
 // Neighbours whose loads a lane issues together in the row-gather kernel
-// (measured at N=262144: FP32 2/4/6/8 -> 0.187/0.166/0.157/0.164 ms, FP64
-// 0.204/0.220/0.208/0.254 ms; the FP64 pair math needs the registers).
+// (measured at N=262144 with gathers by atom index: FP32 2/4/6/8 ->
+// 0.187/0.166/0.157/0.164 ms, FP64 0.204/0.220/0.208/0.254 ms, the FP64 pair
+// math needs the registers; from the bin-ordered copies, FP32 4/6/8/10 ->
+// 0.111/0.108/0.115/0.126 ms).
 template <typename T>
 constexpr int pair_u() { return sizeof(T) == 8 ? 2 : 6; }
 

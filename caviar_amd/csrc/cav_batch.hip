@@ -227,31 +227,29 @@ static int batch_layout(cav_batch *b)
     own += h.n;
     tabs += ((long)h.n + 31) / 32 * 32;
     maxn = std::max(maxn, h.n);
-    // cells and the de-duplicated periodic 27-stencil (neighborlist.cpp:581-697)
+    // cells of the reference's width (cutoff_extra) on the reference's grid
+    // (neighborlist.cpp:581-697), plus one layer of cells past each face that
+    // holds the ghosts where they lie instead of clamping them into the edge
+    // cells; the ghosts are the periodic images, so a cell's 27-stencil is
+    // its plain neighbourhood in this grid (no wrap).  Small dense replicas
+    // put several times their own atoms in the ghost shell, so clamping it
+    // into the edge cells made every edge atom test the whole shell.
     d.cbase = cells;
     int nbins = 1;
     for (int k = 0; k < 3; ++k) {
-      d.nb[k] = std::max(1, (int)std::ceil(d.L[k] * b->cinv));
+      d.nb[k] = std::max(1, (int)std::ceil(d.L[k] * b->cinv)) + 2;
       nbins *= d.nb[k];
     }
     d.nbins = nbins;
-    const int nbx = d.nb[0], nby = d.nb[1];
+    const int nbx = d.nb[0], nby = d.nb[1], nbz = d.nb[2];
     for (int k = 0; k < nbins; ++k) {
       const int cx = k % nbx, cy = (k / nbx) % nby, cz = k / (nbx * nby);
       b->h_st_off.push_back((int)b->h_st.size());
-      std::vector<int> seen;
       for (int a = cx - 1; a <= cx + 1; ++a)
         for (int c = cy - 1; c <= cy + 1; ++c)
-          for (int e = cz - 1; e <= cz + 1; ++e) {
-            int v[3] = {a, c, e};
-            for (int q = 0; q < 3; ++q) {
-              if (v[q] == d.nb[q]) v[q] = 0;
-              if (v[q] == -1) v[q] = d.nb[q] - 1;
-            }
-            const int lin = v[0] + nbx * v[1] + nbx * nby * v[2];
-            if (std::find(seen.begin(), seen.end(), lin) == seen.end()) seen.push_back(lin);
-          }
-      for (int lin : seen) b->h_st.push_back(cells + lin);
+          for (int e = cz - 1; e <= cz + 1; ++e)
+            if (a >= 0 && a < nbx && c >= 0 && c < nby && e >= 0 && e < nbz)
+              b->h_st.push_back(cells + a + nbx * c + nbx * nby * e);
       b->h_cell_rep.push_back(r);
     }
     cells += nbins;

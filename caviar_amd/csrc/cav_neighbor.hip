@@ -936,9 +936,29 @@ __global__ void kb_set_rebuild(int B, int *flags)
   if (r < B) flags[16 * r + F_REBUILD] = 1;
 }
 
-__device__ __forceinline__ BinRule bin_rule_of(const RepDev &r, double cinv)
+// A replica's cell: the reference's cell index (binlist_index, floor of the
+// offset from the box's lower corner times 1/cutoff_extra) shifted by one for
+// the ghost layer below the box, clamped to the grid of nb = (box cells + 2)
+// per axis (ghosts farther than one cell from the box, which no owned atom
+// reaches, share the outermost layer).
+struct BinRuleGhost {
+  double lox, loy, loz, cinv;
+  int nbx, nby, nbz;
+  __device__ __forceinline__ int operator()(double px, double py, double pz) const
+  {
+    int ix = (int)floor(xmul(xsub(px, lox), cinv)) + 1;
+    int iy = (int)floor(xmul(xsub(py, loy), cinv)) + 1;
+    int iz = (int)floor(xmul(xsub(pz, loz), cinv)) + 1;
+    ix = ix < 0 ? 0 : (ix >= nbx ? nbx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy >= nby ? nby - 1 : iy);
+    iz = iz < 0 ? 0 : (iz >= nbz ? nbz - 1 : iz);
+    return ix + nbx * iy + nbx * nby * iz;
+  }
+};
+
+__device__ __forceinline__ BinRuleGhost bin_rule_of(const RepDev &r, double cinv)
 {
-  BinRule b;
+  BinRuleGhost b;
   b.lox = r.lo[0];
   b.loy = r.lo[1];
   b.loz = r.lo[2];

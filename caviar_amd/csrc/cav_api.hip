@@ -218,6 +218,7 @@ static void free_all(cav_ctx *c)
                   c->aox,     c->aoy,    c->aoz,       c->snap,     c->type,     c->id,
                   c->mol,     c->q,      c->gsrc,      c->flags,    c->blk,      c->bin_of,
                   c->bin_cnt, c->bin_start, c->bin_items, c->d_st_off, c->d_st,  c->scan_tmp,
+                  c->d_xst_off, c->d_xst,
                   c->nl,      c->nn,     c->d_minv,    c->d_charge, c->d_eps,    c->d_lambda,
                   c->d_ljl,   c->d_sr_lambda, c->d_elc_k, c->d_elc_chi, c->d_elc_dip,
                   c->d_thermo, c->d_mass,
@@ -362,10 +363,35 @@ static int neighbor_init(cav_ctx *c)
   CAV_TRY(dmalloc(c, &c->d_st, c->st.size()));
   CAV_HIP(hipMemcpyAsync(c->d_st_off, c->st_off.data(), sizeof(int) * c->st_off.size(), hipMemcpyHostToDevice, c->stream));
   CAV_HIP(hipMemcpyAsync(c->d_st, c->st.data(), sizeof(int) * c->st.size(), hipMemcpyHostToDevice, c->stream));
-  CAV_TRY(dmalloc(c, &c->bin_cnt, c->nbins));
-  CAV_TRY(dmalloc(c, &c->bin_start, c->nbins + 1));
-  CAV_TRY(dmalloc(c, &c->lstart, c->nbins + 1));
-  CAV_TRY(dmalloc(c, &c->scan_tmp, grid_for(c->nbins) + 1));
+  // the list build's grid: one more layer of cells past every face, where the
+  // ghosts lie (the periodic images, so no stencil wraps) and where positions
+  // outside a non-periodic box are clamped; each cell's stencil is its 3x3x3
+  // neighbourhood in the grid
+  const int gx = nbx + 2, gy = nby + 2, gz = nbz + 2;
+  const long xnbins = (long)gx * gy * gz;
+  if (xnbins > (1L << 26)) return fail(c, CAV_ERR_ARG, "cell grid too large");
+  c->xnbins = (int)xnbins;
+  std::vector<int> xoff(xnbins + 1), xst;
+  xst.reserve(27 * (size_t)xnbins);
+  for (long k = 0; k < xnbins; ++k) {
+    const int cx = (int)(k % gx), cy = (int)((k / gx) % gy), cz = (int)(k / ((long)gx * gy));
+    xoff[k] = (int)xst.size();
+    for (int a = cx - 1; a <= cx + 1; ++a)
+      for (int b = cy - 1; b <= cy + 1; ++b)
+        for (int e = cz - 1; e <= cz + 1; ++e)
+          if (a >= 0 && a < gx && b >= 0 && b < gy && e >= 0 && e < gz)
+            xst.push_back(a + gx * b + gx * gy * e);
+  }
+  xoff[xnbins] = (int)xst.size();
+  CAV_TRY(dmalloc(c, &c->d_xst_off, xoff.size()));
+  CAV_TRY(dmalloc(c, &c->d_xst, xst.size()));
+  CAV_HIP(hipMemcpy(c->d_xst_off, xoff.data(), sizeof(int) * xoff.size(), hipMemcpyHostToDevice));
+  CAV_HIP(hipMemcpy(c->d_xst, xst.data(), sizeof(int) * xst.size(), hipMemcpyHostToDevice));
+  const int bins = std::max(c->nbins, c->xnbins);
+  CAV_TRY(dmalloc(c, &c->bin_cnt, bins));
+  CAV_TRY(dmalloc(c, &c->bin_start, bins + 1));
+  CAV_TRY(dmalloc(c, &c->lstart, bins + 1));
+  CAV_TRY(dmalloc(c, &c->scan_tmp, grid_for(bins) + 1));
   return CAV_OK;
 }
 

@@ -119,6 +119,9 @@ struct cav_ctx {
   int from_cells = 0, build_cells = 0, all_atom_test = 0, rebuild_test = 0;
   int nb[3] = {0, 0, 0}, nbins = 0;
   std::vector<int> st_off, st;
+  // the list build's grid: the reference grid plus a layer of cells past
+  // every face (nb + 2 per axis) with plain 3x3x3 stencils
+  int xnbins = 0;
 
   // force fields
   bool lj_on = false;
@@ -187,6 +190,7 @@ struct cav_ctx {
   // cells
   int *bin_of = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_items = nullptr;
   int *d_st_off = nullptr, *d_st = nullptr;
+  int *d_xst_off = nullptr, *d_xst = nullptr;  // stencils of the list build's grid
   int *scan_tmp = nullptr;
   int *order = nullptr;    // owned atoms in cell order (compaction of bin_items)
   double4 *cpos = nullptr; // (x, y, z, index) in bin order, for the list build
@@ -368,6 +372,13 @@ int launch_list(cav_ctx *c, bool conditional);
 int launch_save_old(cav_ctx *c, bool conditional);
 bool cell_staged(const cav_ctx *c);
 bool sorted_rows(const cav_ctx *c);
+// Whether list builds bin on the grid with the ghost layer: the cell walks
+// (LJ beyond the list cutoff) and the cell-staged pair kernel read the cells
+// of the reference grid.
+inline bool list_on_ghost_grid(const cav_ctx *c)
+{
+  return !c->lj_cellwalk && c->pair_kind != CAV_PAIRS_CELLS && !c->all_atom_test;
+}
 int launch_nl_fill(cav_ctx *c);
 int launch_pairs(cav_ctx *c, bool lj, bool er, bool accumulate);
 int launch_pack(cav_ctx *c);

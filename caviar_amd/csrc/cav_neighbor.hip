@@ -224,13 +224,18 @@ int launch_rebuild_check(cav_ctx *c)
 struct BinRule {
   double lox, loy, loz, cinv;
   int nbx, nby, nbz;
+  int shift;  // 0: the reference grid; 1: the grid with a ghost layer past every face
   // binlist_index + neigh_bin_index (neighborlist.cpp:408-436): floor, clamp,
-  // x-fastest linear index.
+  // x-fastest linear index.  With the ghost layer the index is the
+  // reference's plus one and the grid has two more cells per axis: ghosts
+  // then lie in their own cells instead of clamped into the edge cells, and
+  // those farther than one cell from the box, which no owned atom reaches,
+  // share the outermost layer.
   __device__ __forceinline__ int operator()(double px, double py, double pz) const
   {
-    int ix = (int)floor(xmul(xsub(px, lox), cinv));
-    int iy = (int)floor(xmul(xsub(py, loy), cinv));
-    int iz = (int)floor(xmul(xsub(pz, loz), cinv));
+    int ix = (int)floor(xmul(xsub(px, lox), cinv)) + shift;
+    int iy = (int)floor(xmul(xsub(py, loy), cinv)) + shift;
+    int iz = (int)floor(xmul(xsub(pz, loz), cinv)) + shift;
     ix = ix < 0 ? 0 : (ix >= nbx ? nbx - 1 : ix);
     iy = iy < 0 ? 0 : (iy >= nby ? nby - 1 : iy);
     iz = iz < 0 ? 0 : (iz >= nbz ? nbz - 1 : iz);
@@ -238,16 +243,17 @@ struct BinRule {
   }
 };
 
-static BinRule bin_rule(const cav_ctx *c)
+static BinRule bin_rule(const cav_ctx *c, bool ghost_layer)
 {
   BinRule b;
   b.lox = c->lo[0];
   b.loy = c->lo[1];
   b.loz = c->lo[2];
   b.cinv = c->cinv;
-  b.nbx = c->nb[0];
-  b.nby = c->nb[1];
-  b.nbz = c->nb[2];
+  b.shift = ghost_layer ? 1 : 0;
+  b.nbx = c->nb[0] + 2 * b.shift;
+  b.nby = c->nb[1] + 2 * b.shift;
+  b.nbz = c->nb[2] + 2 * b.shift;
   return b;
 }
 
@@ -452,22 +458,24 @@ int launch_bins(cav_ctx *c, bool conditional, bool for_list)
   // cell-staged kernel read it, and the next binning visits atoms in it)
   const bool keep = for_list;
   const long tot = (long)c->n + c->gcap;
-  const BinRule rule = bin_rule(c);
-  k_zero_int<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->bin_cnt, c->nbins, cond);
+  const bool xg = for_list && list_on_ghost_grid(c);
+  const BinRule rule = bin_rule(c, xg);
+  const int nbins = xg ? c->xnbins : c->nbins;
+  k_zero_int<<<grid_for(nbins), kBlock, 0, c->stream>>>(c->bin_cnt, nbins, cond);
   k_bin_count<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, rule, c->x, c->y, c->z,
                                                       c->bin_of, c->bin_cnt, c->flags, cond,
                                                       c->litems);
-  const int nb = grid_for(c->nbins);
+  const int nb = grid_for(nbins);
   CountOf cnt{c->bin_cnt};
-  k_block_sums<<<nb, kBlock, 0, c->stream>>>(c->nbins, cnt, c->scan_tmp, cond);
+  k_block_sums<<<nb, kBlock, 0, c->stream>>>(nbins, cnt, c->scan_tmp, cond);
   k_scan_blocks<<<1, kBlock, 0, c->stream>>>(nb, c->scan_tmp, c->flags + F_COUNT - 1, cond);
-  k_bin_start<<<nb, kBlock, 0, c->stream>>>(c->nbins, c->bin_cnt, c->scan_tmp, c->bin_start, cond);
-  k_zero_int<<<grid_for(c->nbins), kBlock, 0, c->stream>>>(c->bin_cnt, c->nbins, cond);
+  k_bin_start<<<nb, kBlock, 0, c->stream>>>(nbins, c->bin_cnt, c->scan_tmp, c->bin_start, cond);
+  k_zero_int<<<grid_for(nbins), kBlock, 0, c->stream>>>(c->bin_cnt, nbins, cond);
   k_bin_fill<<<grid_for(tot), kBlock, 0, c->stream>>>(c->n, tot, c->bin_of, c->bin_start,
                                                      c->bin_cnt, c->bin_items, c->flags, cond,
                                                      c->litems);
-  k_bin_sort<<<(c->nbins + kSortWaves - 1) / kSortWaves, 64 * kSortWaves, 0, c->stream>>>(
-      c->nbins, c->bin_start, c->bin_items, cond);
+  k_bin_sort<<<(nbins + kSortWaves - 1) / kSortWaves, 64 * kSortWaves, 0, c->stream>>>(
+      nbins, c->bin_start, c->bin_items, cond);
   k_cell_pack<<<grid_for(tot), kBlock, 0, c->stream>>>(tot, c->n, c->bin_items, c->x, c->y, c->z,
                                                       c->cpos, c->flags, cond,
                                                       keep ? c->litems : nullptr,
@@ -737,14 +745,17 @@ int launch_list(cav_ctx *c, bool conditional)
   // bin-order slots of the entries, for the row kernel's gathers from the
   // bin-ordered copies: valid on the same terms
   const bool with_nlb = c->pair_kind == CAV_PAIRS_AUTO && c->nlb && !c->all_atom_test;
+  // the grid the list build's binning used (launch_bins with for_list)
+  const bool xg = list_on_ghost_grid(c);
   c->nlb_pending = with_nlb;
   if (!conditional) c->nlb_built = with_nlb;
   if (c->all_atom_test)
     k_list_all<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->nl_cap, c->nl, c->nn, c->flags,
                                                         cond);
   else
-    k_list_wave<<<list_grid(c->nbins), 64 * kListWaves, 0, c->stream>>>(c->n, c->nbins, c->nl_cap, c->rce2, c->cpos,
-                                                c->d_st_off, c->d_st, c->bin_start, c->bin_items,
+    k_list_wave<<<list_grid(xg ? c->xnbins : c->nbins), 64 * kListWaves, 0, c->stream>>>(
+        c->n, xg ? c->xnbins : c->nbins, c->nl_cap, c->rce2, c->cpos,
+        xg ? c->d_xst_off : c->d_st_off, xg ? c->d_xst : c->d_st, c->bin_start, c->bin_items,
                                                 with_nlb ? nullptr : c->nl, c->nn, c->flags,
                                                 cond, nullptr, nullptr,
                                                 with_nll ? c->nll : nullptr,
@@ -936,33 +947,15 @@ __global__ void kb_set_rebuild(int B, int *flags)
   if (r < B) flags[16 * r + F_REBUILD] = 1;
 }
 
-// A replica's cell: the reference's cell index (binlist_index, floor of the
-// offset from the box's lower corner times 1/cutoff_extra) shifted by one for
-// the ghost layer below the box, clamped to the grid of nb = (box cells + 2)
-// per axis (ghosts farther than one cell from the box, which no owned atom
-// reaches, share the outermost layer).
-struct BinRuleGhost {
-  double lox, loy, loz, cinv;
-  int nbx, nby, nbz;
-  __device__ __forceinline__ int operator()(double px, double py, double pz) const
-  {
-    int ix = (int)floor(xmul(xsub(px, lox), cinv)) + 1;
-    int iy = (int)floor(xmul(xsub(py, loy), cinv)) + 1;
-    int iz = (int)floor(xmul(xsub(pz, loz), cinv)) + 1;
-    ix = ix < 0 ? 0 : (ix >= nbx ? nbx - 1 : ix);
-    iy = iy < 0 ? 0 : (iy >= nby ? nby - 1 : iy);
-    iz = iz < 0 ? 0 : (iz >= nbz ? nbz - 1 : iz);
-    return ix + nbx * iy + nbx * nby * iz;
-  }
-};
-
-__device__ __forceinline__ BinRuleGhost bin_rule_of(const RepDev &r, double cinv)
+// A replica's cells: the grid with the ghost layer (r.nb counts it).
+__device__ __forceinline__ BinRule bin_rule_of(const RepDev &r, double cinv)
 {
-  BinRuleGhost b;
+  BinRule b;
   b.lox = r.lo[0];
   b.loy = r.lo[1];
   b.loz = r.lo[2];
   b.cinv = cinv;
+  b.shift = 1;
   b.nbx = r.nb[0];
   b.nby = r.nb[1];
   b.nbz = r.nb[2];

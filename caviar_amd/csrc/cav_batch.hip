@@ -162,7 +162,8 @@ static void batch_free(cav_batch *b)
                   b->bin_start, b->bin_items, b->gblk, b->nl, b->nn, b->flags, b->d_minv, b->d_lj,
                   b->part, b->vir, b->d_W, b->tab, b->ztab, b->bmat, b->bmat_h, b->sk_part, b->sk,
                   b->epart, b->s_items, b->s_items_o, b->f_items, b->rchunks, b->cpos, b->d_raw_eps,
-                  b->d_raw_sig, b->d_charge};
+                  b->d_raw_sig, b->d_charge, b->nlb, b->xqs, b->tids, b->litems, b->binpos,
+                  b->order, b->ocnt, b->ostart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->h_flags) (void)hipHostFree(b->h_flags);
@@ -291,7 +292,15 @@ static int batch_layout(cav_batch *b)
   CAV_TRY(balloc(b, &b->bin_start, (size_t)b->ncells + b->B));
   CAV_TRY(balloc(b, &b->gblk, (size_t)b->B * b->scan_blocks));
   CAV_TRY(balloc(b, &b->nl, (size_t)own * b->cap));
+  CAV_TRY(balloc(b, &b->nlb, (size_t)own * b->cap));
   CAV_TRY(balloc(b, &b->nn, own));
+  CAV_TRY(balloc(b, &b->xqs, slots));
+  CAV_TRY(balloc(b, &b->tids, slots));
+  CAV_TRY(balloc(b, &b->litems, slots));
+  CAV_TRY(balloc(b, &b->binpos, slots));
+  CAV_TRY(balloc(b, &b->order, own));
+  CAV_TRY(balloc(b, &b->ocnt, b->ncells));
+  CAV_TRY(balloc(b, &b->ostart, b->ncells));
   CAV_TRY(balloc(b, &b->flags, 16 * (size_t)b->B));
   CAV_TRY(balloc(b, &b->part, (size_t)b->B * b->pair_blocks));
   CAV_TRY(balloc(b, &b->vir, b->B));

@@ -321,6 +321,14 @@ struct cav_batch {
   double4 *cpos = nullptr; // slots: (x, y, z, slot) in bin order
   int *gblk = nullptr;     // [B][scan_blocks] ghost-count block sums
   int *nl = nullptr, *nn = nullptr;
+  // the pair kernel's bin-order gathers: rows as replica-local bin slots
+  // (beside nl), bin-ordered copies of (x, y, z, q) and (type, id), slot ->
+  // local atom, local atom -> slot, owned atoms in cell order, and per cell
+  // the owned count and its offset within the replica
+  int *nlb = nullptr;
+  double4 *xqs = nullptr;
+  int2 *tids = nullptr;
+  int *litems = nullptr, *binpos = nullptr, *order = nullptr, *ocnt = nullptr, *ostart = nullptr;
   int cap = 0;
   int *flags = nullptr, *h_flags = nullptr;  // [B][F_COUNT]
   double *d_minv = nullptr, *d_lj = nullptr;

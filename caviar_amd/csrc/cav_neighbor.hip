@@ -669,7 +669,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
                 }
                 if (nlb) {
                   int *bp = nlb + at0;
-                  bp[pre] = cpos_slot(q.w);  // bin-order slot of the candidate
+                  bp[pre] = cpos_slot(q.w) - base;  // bin-order slot of the candidate
                 }
               }
               c0[u] = __builtin_amdgcn_readfirstlane(c0[u] + (int)__popcll(mask));
@@ -1022,9 +1022,12 @@ __global__ void kb_bin_fill(const RepDev *rep, const int *flags, const int *bin_
   items[start[c + blockIdx.y] + atomicAdd(cur + c, 1)] = (int)s;
 }
 
-__global__ __launch_bounds__(64 * kSortWaves) void kb_bin_sort(int ncells, const int *cell_rep,
+// After the sort a cell lists its owned atoms first (they have the lower
+// slots); their count goes to ocnt, taken before the sort (it is order-free).
+__global__ __launch_bounds__(64 * kSortWaves) void kb_bin_sort(const RepDev *rep, int ncells,
+                                                               const int *cell_rep,
                                                                const int *flags, const int *start,
-                                                               int *items)
+                                                               int *items, int *ocnt)
 {
   __shared__ int v_[kSortWaves][kSortWaveMax];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1032,20 +1035,61 @@ __global__ __launch_bounds__(64 * kSortWaves) void kb_bin_sort(int ncells, const
   if (c >= ncells) return;
   const int rr = cell_rep[c];
   if (!flags[16 * rr + F_REBUILD]) return;
-  sort_bin_wave(start[c + rr], start[c + rr + 1], items, v_[wv]);
+  const int s0 = start[c + rr], e0 = start[c + rr + 1];
+  const int lane = threadIdx.x & 63, own_end = rep[rr].base + rep[rr].n;
+  int no = 0;
+  for (int t = s0; t < e0; t += 64) no += __popcll(__ballot(t + lane < e0 && items[t + lane] < own_end));
+  if (lane == 0) ocnt[c] = no;
+  sort_bin_wave(s0, e0, items, v_[wv]);
 }
 
-__global__ void kb_cell_pack(const RepDev *rep, const int *flags, const int *items, const double *x,
-                             const double *y, const double *z, double4 *cpos)
+// per replica: each cell's first owned atom in cell order (exclusive scan of
+// the owned counts over the replica's cells)
+__global__ void kb_owned_start(const RepDev *rep, const int *flags, const int *ocnt, int *ostart)
 {
-  const RepDev &r = rep[blockIdx.y];
-  const int *f = flags + 16 * blockIdx.y;
+  const int rr = blockIdx.x;
+  const RepDev &r = rep[rr];
+  if (!flags[16 * rr + F_REBUILD]) return;
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int k0 = 0; k0 < r.nbins; k0 += kBlock) {
+    const int k = k0 + threadIdx.x;
+    const int v = k < r.nbins ? ocnt[r.cbase + k] : 0;
+    int tot;
+    const int ex = block_exclusive_scan(v, &tot);
+    if (k < r.nbins) ostart[r.cbase + k] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+}
+
+// (x, y, z, index and slot) in bin order for the list build, and the bin
+// order the pair kernel reads through: slot -> local atom (litems), local atom
+// -> slot (binpos), owned atoms in cell order (order)
+__global__ void kb_cell_pack(const RepDev *rep, const int *flags, const int *items,
+                             const int *bin_of, const int *start, const int *ostart,
+                             const double *x, const double *y, const double *z, double4 *cpos,
+                             int *litems, int *binpos, int *order)
+{
+  const int rr = blockIdx.y;
+  const RepDev &r = rep[rr];
+  const int *f = flags + 16 * rr;
   if (!f[F_REBUILD]) return;
   const long a = (long)blockIdx.x * kBlock + threadIdx.x;
   if (a >= (long)r.n + f[F_NGHOST]) return;
   const long t = r.base + a;
   const int j = items[t];
   cpos[t] = make_double4(x[j], y[j], z[j], cpos_tag(j, t));
+  const int lj = j - r.base;
+  litems[t] = lj;
+  binpos[j] = (int)a;
+  if (lj < r.n) {
+    // owned atoms come first in their sorted cell
+    const int c = bin_of[j];
+    order[r.obase + ostart[c] + (int)(t - start[c + rr])] = lj;
+  }
 }
 
 __global__ void kb_save_old(const RepDev *rep, const double *x, const double *y, const double *z,
@@ -1093,13 +1137,15 @@ int batch_neighbor(cav_batch *b, bool forced)
   kb_bin_fill<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_of, b->cell_rep,
                                                     b->bin_start, b->bin_cnt + b->ncells, b->bin_items);
   kb_bin_sort<<<(b->ncells + kSortWaves - 1) / kSortWaves, 64 * kSortWaves, 0, b->stream>>>(
-      b->ncells, b->cell_rep, b->flags, b->bin_start, b->bin_items);
-  kb_cell_pack<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_items, b->x, b->y,
-                                                     b->z, b->cpos);
+      b->d_rep, b->ncells, b->cell_rep, b->flags, b->bin_start, b->bin_items, b->ocnt);
+  kb_owned_start<<<b->B, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->ocnt, b->ostart);
+  kb_cell_pack<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_items, b->bin_of,
+                                                     b->bin_start, b->ostart, b->x, b->y, b->z,
+                                                     b->cpos, b->litems, b->binpos, b->order);
   k_list_wave<<<list_grid(b->ncells), 64 * kListWaves, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->cpos, b->st_off,
                                                b->st, b->bin_start, b->bin_items, b->nl, b->nn,
                                                b->flags, nullptr, b->d_rep, b->cell_rep, nullptr,
-                                               nullptr);
+                                               b->nlb);
   kb_save_old<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->x, b->y, b->z, b->xo, b->yo, b->zo,
                                                     b->flags);
   BAT_HIP(hipGetLastError());

@@ -191,6 +191,13 @@ __global__ __launch_bounds__(kBlock) void k_pairs(PairArgs a)
     a.tid += base;
     a.nl += (long)obase * a.cap;
     a.nn += obase;
+    if (a.order) a.order += obase;
+    if constexpr (BINS) {
+      a.nlb += (long)obase * a.cap;
+      a.xqs += base;
+      a.tids += base;
+      a.litems += base;
+    }
     a.ax += obase;
     a.ay += obase;
     a.az += obase;
@@ -1148,16 +1155,26 @@ int launch_energy_lj(cav_ctx *c, double *e)
 
 // ---------------------------------------------------------- batched replicas --
 
+// Packed copies of every replica's slots, by index and in the last list
+// build's bin order (binpos: slot -> replica-local bin position; a changed
+// ghost count forces a rebuild first, so every slot has one).
 __global__ void kb_pack(const RepDev *rep, const int *flags, const double *x, const double *y,
                         const double *z, const double *q, const int *type, const int *id,
-                        double4 *xq, int2 *tid)
+                        double4 *xq, int2 *tid, const int *binpos, double4 *xqs, int2 *tids)
 {
   const RepDev &r = rep[blockIdx.y];
   const long a = (long)blockIdx.x * kBlock + threadIdx.x;
   if (a >= (long)r.n + flags[16 * blockIdx.y + F_NGHOST]) return;
   const long s = r.base + a;
-  xq[s] = make_double4(x[s], y[s], z[s], q[s]);
-  tid[s] = make_int2(type[s], id[s]);
+  const double4 v = make_double4(x[s], y[s], z[s], q[s]);
+  const int2 w = make_int2(type[s], id[s]);
+  xq[s] = v;
+  tid[s] = w;
+  if (binpos) {
+    const long t = r.base + binpos[s];
+    xqs[t] = v;
+    tids[t] = w;
+  }
 }
 
 // per replica: sum of its per-block partials in a fixed order
@@ -1176,16 +1193,18 @@ __global__ void kb_reduce_rows(int nper, const double *part, double *out)
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
+// every replica's rows are written both as indices and as bin-order slots:
+// the pair kernel reads the slots, the downloads and the adjoint the indices
 template <typename T>
 static void batch_pairs_t(cav_batch *b, const PairArgs &a, dim3 g)
 {
   const size_t lds = 3 * sizeof(T) * a.nt * a.nt;  // LJ tables
   if (b->lj_on && b->er_on)
-    k_pairs<T, true, true, false, false><<<g, kBlock, lds, b->stream>>>(a);
+    k_pairs<T, true, true, false, true><<<g, kBlock, lds, b->stream>>>(a);
   else if (b->lj_on)
-    k_pairs<T, true, false, false, false><<<g, kBlock, lds, b->stream>>>(a);
+    k_pairs<T, true, false, false, true><<<g, kBlock, lds, b->stream>>>(a);
   else
-    k_pairs<T, false, true, false, false><<<g, kBlock, 0, b->stream>>>(a);
+    k_pairs<T, false, true, false, true><<<g, kBlock, 0, b->stream>>>(a);
 }
 
 // LJ + Ewald_r accelerations of every replica (overwriting), virial per replica.
@@ -1194,7 +1213,8 @@ int batch_pairs(cav_batch *b)
   int maxslots = 0;
   for (const RepDev &r : b->reps) maxslots = std::max(maxslots, r.n + r.gcap);
   kb_pack<<<dim3(grid_for(maxslots), b->B), kBlock, 0, b->stream>>>(
-      b->d_rep, b->flags, b->x, b->y, b->z, b->q, b->type, b->id, b->xq, b->tid);
+      b->d_rep, b->flags, b->x, b->y, b->z, b->q, b->type, b->id, b->xq, b->tid,
+      b->nl_on ? b->binpos : nullptr, b->xqs, b->tids);
   const dim3 g(b->pair_blocks, b->B);
   if (!b->lj_on && !b->er_on) {
     kb_zero_own<<<dim3(grid_for(b->maxn), b->B), kBlock, 0, b->stream>>>(b->d_rep, b->ax, b->ay,
@@ -1211,7 +1231,7 @@ int batch_pairs(cav_batch *b)
     a.type = a.id = a.mol = nullptr;
     a.nl = b->nl;
     a.nn = b->nn;
-    a.order = nullptr;
+    a.order = b->order;
     a.xq = b->xq;
     a.tid = b->tid;
     const int nt2 = b->nt * b->nt;
@@ -1230,10 +1250,10 @@ int batch_pairs(cav_batch *b)
     a.part = b->part;
     a.accumulate = 0;
     a.rep = b->d_rep;
-    a.nlb = nullptr;
-    a.xqs = nullptr;
-    a.tids = nullptr;
-    a.litems = nullptr;
+    a.nlb = b->nlb;
+    a.xqs = b->xqs;
+    a.tids = b->tids;
+    a.litems = b->litems;
     if (b->precision == CAV_FP64)
       batch_pairs_t<double>(b, a, g);
     else

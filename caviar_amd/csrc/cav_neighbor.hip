@@ -536,6 +536,10 @@ constexpr int kListWaves = CAV_LIST_WPB;
 
 static int list_grid(long cells) { return (int)((cells + kListWaves - 1) / kListWaves); }
 
+// The row outputs a build writes (index rows nl, staging positions nll,
+// bin-order slots nlb) are template flags, so the append path of the inner
+// loop carries only the stores the launch needs.
+template <bool NL, bool NLL, bool NLB>
 __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins, int cap, double rce2,
                                                   const double4 *cpos, const int *st_off,
                                                   const int *st, const int *start,
@@ -563,7 +567,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
     start += r;  // every replica's cell starts carry one closing entry
   }
   if (cond && !*cond) return;
-  if (!rep && b == 0 && threadIdx.x == 0) flags[F_NL_STALE] = nl ? 0 : 1;
+  if (!rep && b == 0 && threadIdx.x == 0) flags[F_NL_STALE] = NL ? 0 : 1;
   // staged candidates (x, y, z, index and slot) in the cell's stencil order; a stage is
   // padded to a multiple of 64 with far-away entries so every lane tests one
   __shared__ double4 sc_[kListWaves][kCand];
@@ -578,7 +582,6 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
   }
   if (no == 0) return;
   const int sb = st_off[b], ns = st_off[b + 1] - sb;  // <= 27 stencil cells
-  const unsigned long long below = (1ull << lane) - 1ull;
   // padding and empty owned slots sit at +-1e300: their squared distance to
   // anything is +inf, so they never pass the cutoff test
   const double4 far_cand = make_double4(1e300, 1e300, 1e300, cpos_tag(-1, -1));
@@ -653,26 +656,30 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
           for (int u = 0; u < kAPI; ++u) {
             const double dx = xsub(px[u], q.x), dy = xsub(py[u], q.y), dz = xsub(pz[u], q.z);
             const bool pass = j != ia[u] && dot_rn(dx, dy, dz) < rce2;
-            const unsigned long long mask = __ballot(pass);
+            // the ballot of the compare itself (a scalar mask), the lane's
+            // rank among the passing lanes by mbcnt, and a scalar row counter
+            const unsigned long long mask = __builtin_amdgcn_ballot_w64(pass);
             if (mask) {
-              const int pre = __popcll(mask & below);
-              if (pass && c0[u] + pre < cap) {
+              const int cu = c0[u];
+              const unsigned pre = __builtin_amdgcn_mbcnt_hi(
+                  (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+              if (pass && cu + (int)pre < cap) {
                 // uniform row pointers plus a 32-bit lane offset
-                const long at0 = row[u] + c0[u];
-                if (nl) {
+                const long at0 = row[u] + cu;
+                if constexpr (NL) {
                   int *rp = nl + at0;
                   rp[pre] = j - base;
                 }
-                if (nll) {
+                if constexpr (NLL) {
                   unsigned short *lp = nll + at0;
                   lp[pre] = (unsigned short)(mbase + idx);  // position in the stencil staging order
                 }
-                if (nlb) {
+                if constexpr (NLB) {
                   int *bp = nlb + at0;
                   bp[pre] = cpos_slot(q.w) - base;  // bin-order slot of the candidate
                 }
               }
-              c0[u] = __builtin_amdgcn_readfirstlane(c0[u] + (int)__popcll(mask));
+              c0[u] = cu + (int)__popcll(mask);
             }
           }
         }
@@ -692,6 +699,30 @@ __global__ __launch_bounds__(64 * kListWaves) void k_list_wave(int n, int nbins,
       }
     }
   }
+}
+
+// k_list_wave for the outputs that are not null (index rows, staging
+// positions, bin-order slots: the combinations the single system and the
+// batched replicas use)
+static void launch_list_wave(hipStream_t st, int n, int nbins, int cap, double rce2,
+                             const double4 *cpos, const int *st_off, const int *stc,
+                             const int *start, const int *items, int *nl, int *nn, int *flags,
+                             const int *cond, const RepDev *rep, const int *cell_rep,
+                             unsigned short *nll, int *nlb)
+{
+  const dim3 g(list_grid(nbins)), blk(64 * kListWaves);
+#define CAV_LISTW(A, B, C)                                                                      \
+  k_list_wave<A, B, C><<<g, blk, 0, st>>>(n, nbins, cap, rce2, cpos, st_off, stc, start, items, \
+                                          nl, nn, flags, cond, rep, cell_rep, nll, nlb)
+  if (nlb && !nl)
+    CAV_LISTW(false, false, true);
+  else if (nlb)
+    CAV_LISTW(true, false, true);
+  else if (nll)
+    CAV_LISTW(true, true, false);
+  else
+    CAV_LISTW(true, false, false);
+#undef CAV_LISTW
 }
 
 // all_atom_test_function (neighborlist.cpp:161-191): every owned j != i and
@@ -753,13 +784,10 @@ int launch_list(cav_ctx *c, bool conditional)
     k_list_all<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->nl_cap, c->nl, c->nn, c->flags,
                                                         cond);
   else
-    k_list_wave<<<list_grid(xg ? c->xnbins : c->nbins), 64 * kListWaves, 0, c->stream>>>(
-        c->n, xg ? c->xnbins : c->nbins, c->nl_cap, c->rce2, c->cpos,
-        xg ? c->d_xst_off : c->d_st_off, xg ? c->d_xst : c->d_st, c->bin_start, c->bin_items,
-                                                with_nlb ? nullptr : c->nl, c->nn, c->flags,
-                                                cond, nullptr, nullptr,
-                                                with_nll ? c->nll : nullptr,
-                                                with_nlb ? c->nlb : nullptr);
+    launch_list_wave(c->stream, c->n, xg ? c->xnbins : c->nbins, c->nl_cap, c->rce2, c->cpos,
+                     xg ? c->d_xst_off : c->d_st_off, xg ? c->d_xst : c->d_st, c->bin_start,
+                     c->bin_items, with_nlb ? nullptr : c->nl, c->nn, c->flags, cond, nullptr,
+                     nullptr, with_nll ? c->nll : nullptr, with_nlb ? c->nlb : nullptr);
   // (the bin order itself, litems and binpos, was kept by the list build's
   // binning; the cell-staged kernel also needs the cell starts)
   if (with_nll) {
@@ -1142,10 +1170,9 @@ int batch_neighbor(cav_batch *b, bool forced)
   kb_cell_pack<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->flags, b->bin_items, b->bin_of,
                                                      b->bin_start, b->ostart, b->x, b->y, b->z,
                                                      b->cpos, b->litems, b->binpos, b->order);
-  k_list_wave<<<list_grid(b->ncells), 64 * kListWaves, 0, b->stream>>>(0, b->ncells, b->cap, b->rce2, b->cpos, b->st_off,
-                                               b->st, b->bin_start, b->bin_items, b->nl, b->nn,
-                                               b->flags, nullptr, b->d_rep, b->cell_rep, nullptr,
-                                               b->nlb);
+  launch_list_wave(b->stream, 0, b->ncells, b->cap, b->rce2, b->cpos, b->st_off, b->st,
+                   b->bin_start, b->bin_items, b->nl, b->nn, b->flags, nullptr, b->d_rep,
+                   b->cell_rep, nullptr, b->nlb);
   kb_save_old<<<slot_grid, kBlock, 0, b->stream>>>(b->d_rep, b->x, b->y, b->z, b->xo, b->yo, b->zo,
                                                     b->flags);
   BAT_HIP(hipGetLastError());

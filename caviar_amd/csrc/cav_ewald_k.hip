@@ -659,53 +659,58 @@ __device__ __forceinline__ void put_split(unsigned char *xy, unsigned char *zt, 
   }
 }
 
-// (cos, sin) of 2 pi m u for table entry e of an atom at (px, py, pz) / box.
-__device__ __forceinline__ void split_phase(int e, int nX, int nY, int ky, int kz, double ux,
-                                            double uy, double uz, float &cs, float &sn)
+// The rows of one axis for one atom: e^{2 pi i m u} for m = 0..k, by complex
+// multiplication in FP64 from e^{2 pi i u} (u reduced to [-1/2, 1/2] first,
+// one FP64 sincospi), each entry rounded to FP32 once; the entries of -m are
+// the conjugates.  The FP64 products carry ~k ulp of FP64 phase error, far
+// below the FP32 rounding of the entries.  Axis 0 (X, m >= 0 only) carries
+// the atom's scaled charge.  e0 is the entry of m = 0; dead (padding) atoms
+// write zeros.
+__device__ __forceinline__ void split_axis_rows(unsigned char *xy, unsigned char *zt, long blk,
+                                                int j, int TXY, int nZ, int axis, int k, int e0,
+                                                double u, bool live, float qs)
 {
-  double u;
-  int m;
-  if (e < nX) {
-    m = e;
-    u = ux;
-  } else if (e < nX + nY) {
-    m = e - nX - ky;
-    u = uy;
-  } else {
-    m = e - nX - nY - kz;
-    u = uz;
+  u -= rint(u);
+  double s1, c1;
+  sincospi(2.0 * u, &s1, &c1);
+  double cr = 1.0, ci = 0.0;
+  for (int m = 0; m <= k; ++m) {
+    float cs = live ? (float)cr : 0.f, sn = live ? (float)ci : 0.f;
+    if (axis == 0) {
+      cs *= qs;
+      sn *= qs;
+    }
+    put_split(xy, zt, blk, e0 + m, j, TXY, nZ, cs, sn);
+    if (axis != 0 && m > 0) put_split(xy, zt, blk, e0 - m, j, TXY, nZ, cs, -sn);
+    const double nr = cr * c1 - ci * s1, ni = cr * s1 + ci * c1;
+    cr = nr;
+    ci = ni;
   }
-  double f = (double)m * u;
-  f -= rint(f);
-  sincospif((float)(2.0 * f), &sn, &cs);
 }
 
-// One thread per (atom, table entry); the grid's y index is the entry, so its
-// kind (X, Y or Z row) and coordinate are uniform per workgroup and no thread
-// divides by the entry count.  Consecutive threads write consecutive atoms of
-// one row.
+// One thread per (atom, axis); the grid's y index is the axis, so its kind
+// (X, Y or Z rows) and coordinate are uniform per workgroup.  Consecutive
+// threads write consecutive atoms of one row.
 __global__ void k_tables_h(long npts, int kx, int ky, int kz, double ilx, double ily, double ilz,
                            const double *x, const double *y, const double *z, const double *q,
                            float sa, unsigned char *xy, unsigned char *zt)
 {
   const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1;
-  const int e = (int)blockIdx.y;
+  const int axis = (int)blockIdx.y;
   const long nat = (npts + kTabAtoms - 1) / kTabAtoms * kTabAtoms;
   const long a = (long)blockIdx.x * kBlock + threadIdx.x;
   if (a >= nat) return;
-  const long blk = a / kTabAtoms;
-  const int j = (int)(a & (kTabAtoms - 1));
-  float cs = 0.f, sn = 0.f;
-  if (a < npts) {
-    const double u = e < nX ? x[a] * ilx : (e < nX + nY ? y[a] * ily : z[a] * ilz);
-    split_phase(e, nX, nY, ky, kz, u, u, u, cs, sn);
-    if (e < nX) {  // X carries the atom's scaled charge: q X Y in one complex product
-      const float qs = (float)q[a] * sa;
-      cs *= qs;
-      sn *= qs;
-    }
+  const bool live = a < npts;
+  double u = 0.0;
+  float qs = 0.f;
+  if (live) {
+    u = axis == 0 ? x[a] * ilx : (axis == 1 ? y[a] * ily : z[a] * ilz);
+    if (axis == 0) qs = (float)q[a] * sa;
   }
-  put_split(xy, zt, blk, e, j, nX + nY, nZ, cs, sn);
+  const int k = axis == 0 ? kx : (axis == 1 ? ky : kz);
+  const int e0 = axis == 0 ? 0 : (axis == 1 ? nX + ky : nX + nY + kz);
+  split_axis_rows(xy, zt, a / kTabAtoms, (int)(a & (kTabAtoms - 1)), nX + nY, nZ, axis, k, e0, u,
+                  live, qs);
 }
 
 __device__ __forceinline__ f4 mfma_h(u4v a, u4v b, f4 c)
@@ -1684,10 +1689,9 @@ static void launch_sk_h(hipStream_t st, SArgsH a, dim3 lgrid, size_t smem, int S
 static int sk_pipeline_h(cav_ctx *c)
 {
   const EwaldKPlan &p = c->ek;
-  const int TS = p.nX + p.nY + p.nZ;
   const long nblk = (c->n + kTabAtoms - 1) / kTabAtoms;
   timer_begin(c, c->t_tab);
-  k_tables_h<<<dim3(grid_for(nblk * kTabAtoms), TS), kBlock, 0, c->stream>>>(
+  k_tables_h<<<dim3(grid_for(nblk * kTabAtoms), 3), kBlock, 0, c->stream>>>(
       c->n, p.kx, p.ky, p.kz, 1.0 / p.lx, 1.0 / p.ly, 1.0 / p.lz, c->x, c->y, c->z, c->q,
       (float)c->sA, (unsigned char *)c->tab, (unsigned char *)c->ztab);
   timer_end(c, c->t_tab);
@@ -2044,31 +2048,29 @@ __global__ void kb_tables(const RepDev *rep, int kx, int ky, int kz, const doubl
 }
 
 // Batched split tables: replica r's atoms fill the blocks from r.tbase.  The
-// grid's y index is the replica and its z index the table entry, as in
-// k_tables_h.
+// grid's y index is the replica and its z index the axis, as in k_tables_h.
 __global__ void kb_tables_h(const RepDev *rep, int kx, int ky, int kz, const double *x,
                             const double *y, const double *z, const double *q, float sa,
                             unsigned char *xy, unsigned char *zt)
 {
   const RepDev &r = rep[blockIdx.y];
   const int nX = kx + 1, nY = 2 * ky + 1, nZ = 2 * kz + 1;
-  const int e = (int)blockIdx.z;
+  const int axis = (int)blockIdx.z;
   const long nat = (r.n + kTabAtoms - 1) / kTabAtoms * kTabAtoms;
   const long a = (long)blockIdx.x * kBlock + threadIdx.x;
   if (a >= nat) return;
-  const long blk = a / kTabAtoms, s = r.base + a;
-  const int j = (int)(a & (kTabAtoms - 1));
-  float cs = 0.f, sn = 0.f;
-  if (a < r.n) {
-    const double u = e < nX ? x[s] * r.il[0] : (e < nX + nY ? y[s] * r.il[1] : z[s] * r.il[2]);
-    split_phase(e, nX, nY, ky, kz, u, u, u, cs, sn);
-    if (e < nX) {
-      const float qs = (float)q[r.obase + a] * sa;
-      cs *= qs;
-      sn *= qs;
-    }
+  const long s = r.base + a;
+  const bool live = a < r.n;
+  double u = 0.0;
+  float qs = 0.f;
+  if (live) {
+    u = axis == 0 ? x[s] * r.il[0] : (axis == 1 ? y[s] * r.il[1] : z[s] * r.il[2]);
+    if (axis == 0) qs = (float)q[r.obase + a] * sa;
   }
-  put_split(xy, zt, r.tbase / kTabAtoms + blk, e, j, nX + nY, nZ, cs, sn);
+  const int k = axis == 0 ? kx : (axis == 1 ? ky : kz);
+  const int e0 = axis == 0 ? 0 : (axis == 1 ? nX + ky : nX + nY + kz);
+  split_axis_rows(xy, zt, r.tbase / kTabAtoms + a / kTabAtoms, (int)(a & (kTabAtoms - 1)),
+                  nX + nY, nZ, axis, k, e0, u, live, qs);
 }
 
 
@@ -2246,7 +2248,7 @@ static int batch_ewald_k_t(cav_batch *b)
     const size_t smem = 2 * (size_t)a.stage * TS * sizeof(Cx<T>) + 2 * (size_t)a.stage * sizeof(T);
     launch_sk_rt<T, kRT>(b->stream, a, dim3(b->n_sitems, groups), smem, SC / 16, waves);
   } else {
-    const dim3 hgrid(grid_for((long)(b->maxn + kTabAtoms - 1) / kTabAtoms * kTabAtoms), b->B, TS);
+    const dim3 hgrid(grid_for((long)(b->maxn + kTabAtoms - 1) / kTabAtoms * kTabAtoms), b->B, 3);
     kb_tables_h<<<hgrid, kBlock, 0, b->stream>>>(b->d_rep, p.kx, p.ky, p.kz, b->x, b->y, b->z,
                                                  b->q_o, (float)b->sA, (unsigned char *)b->tab,
                                                  (unsigned char *)b->ztab);

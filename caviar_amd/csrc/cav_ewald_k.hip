@@ -56,8 +56,10 @@ __device__ __forceinline__ int mfma_row(int lk, int e)
 
 // Per atom a record of TS complex values [X(0..kx) | Y(-ky..ky) | Z(-kz..kz)].
 // The phase fraction n*x/l is reduced to [-1/2, 1/2] in FP64 before sincospi,
-// so the FP32 tables of the mixed form carry ~1e-7 absolute phase error even
-// where n k.r reaches hundreds of radians.
+// so the FP32 tables (point fields and the adjoint of the mixed form) carry
+// ~1e-7 absolute phase error even where n k.r reaches hundreds of radians; the
+// split form's blocked tables (k_tables_h) are products of FP64 phase factors
+// rounded once, within 1 FP32 ulp.
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef unsigned u4v __attribute__((ext_vector_type(4)));

@@ -52,10 +52,14 @@ static void timer_harvest(Timer &t)
 // integrate_velocity_verlet drift (md_simulator.cpp:584-594) with the PBC wrap
 // of exchange_owned fused in (domain.cpp:418-468).  Every operation is
 // rounded as the reference's expression is: r += (v*dt + ((0.5*a)*dt)*dt).
+// snap (or nullptr): the step snapshot of the owned positions, velocities and
+// accelerations before the drift ([x | y | z | vx | vy | vz | ax | ay | az],
+// n each), written from the values the drift reads anyway.
 __global__ void k_drift(int n, double dt, const double *lo, const double *hi, const double *L,
                         int bx, int by, int bz, double *x, double *y, double *z, const double *vx,
                         const double *vy, const double *vz, const double *ax, const double *ay,
-                        const double *az, double *aox, double *aoy, double *aoz, int *flags)
+                        const double *az, double *aox, double *aoy, double *aoz, int *flags,
+                        double *snap)
 {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -63,6 +67,14 @@ __global__ void k_drift(int n, double dt, const double *lo, const double *hi, co
   if (isnan(a[0]) || isnan(a[1]) || isnan(a[2])) flags[F_NAN] = 1;
   const double v[3] = {vx[i], vy[i], vz[i]};
   double *p[3] = {x + i, y + i, z + i};
+  if (snap) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      snap[(long)d * n + i] = *p[d];
+      snap[(long)(3 + d) * n + i] = v[d];
+      snap[(long)(6 + d) * n + i] = a[d];
+    }
+  }
   const int b[3] = {bx, by, bz};
   bool wrapped = false;
 #pragma unroll
@@ -1159,12 +1171,12 @@ int cav_step(cav_ctx *c, double dt, int nsteps)
   for (int s = 0; s < nsteps; ++s) {
     bool redo = true;
     for (int attempt = 0; redo && attempt < 4; ++attempt) {
-      CAV_TRY(snapshot(c, true));
+      // the step snapshot (restored on NaN or overflow) is written by the drift
       k_begin_step<<<1, 1, 0, c->stream>>>(c->flags);
       timer_begin(c, c->t_int);
       k_drift<<<grid_for(c->n), kBlock, 0, c->stream>>>(
           c->n, dt, lo, hi, L, c->bc[0], c->bc[1], c->bc[2], c->x, c->y, c->z, c->vx, c->vy, c->vz,
-          c->ax, c->ay, c->az, c->aox, c->aoy, c->aoz, c->flags);
+          c->ax, c->ay, c->az, c->aox, c->aoy, c->aoz, c->flags, c->snap);
       timer_end(c, c->t_int);
       CAV_HIP(hipGetLastError());
       const bool side = c->overlap && c->ek_on && c->n > 0;
